@@ -1,0 +1,79 @@
+"""Builders of ``datain/`` directories for the workloads this package is measured
+and tested on (structured quad/hex meshes, mpiprocs = 1).
+
+Each builder writes app.bin, master.bin, mesh.bin and sol.bin in the layout
+text2code writes (see ``datain.py``) and returns a small dict describing the case
+(model name and sizes).  The pdeapp/pdemodel settings they encode:
+
+* ``poisson2d`` -- examples/Poisson/poisson2d (pdeapp.txt, pdemodel.txt) on an
+  n x n quad mesh of the unit square, Dirichlet on all sides.
+* ``convdiff3d`` -- steady convection-diffusion on an n^3 hex mesh of the unit
+  cube, ncu components, velocity c, diffusivity kappa, per-node uniform[-1, 1]
+  source passed as the auxiliary field v (``vdgfile``); Dirichlet u = 0 on x = 0
+  (condition 1), zero normal flux elsewhere (condition 2).
+"""
+import os
+import numpy as np
+
+from . import datain as dw
+
+PRECOND = {"bjacobi": 0, "eas": 1, "bilu0": 2}
+
+
+def _app_flags(precond, ortho):
+    return {5: ortho, 6: PRECOND[precond] if isinstance(precond, str) else precond}
+
+
+def poisson2d(path, n=16, porder=3, pgauss=14, model="poisson", kappa=1.0, tau=1.0,
+              precond="bjacobi", restart=50, gmres_tol=1e-12, gmres_maxit=1000,
+              newton_iter=1, newton_tol=1e-6, ortho=0, nd=2):
+    """Poisson on the unit square (nd=2) or cube (nd=3), all boundaries Dirichlet."""
+    os.makedirs(path, exist_ok=True)
+    m = dw.make_master(nd, porder, pgauss)
+    p, t = dw.structured_grid([n] * nd)
+    if nd == 2:
+        exprs = [lambda v: np.all(np.abs(v[:, 1]) < 1e-8), lambda v: np.all(np.abs(v[:, 0] - 1) < 1e-8),
+                 lambda v: np.all(np.abs(v[:, 1] - 1) < 1e-8), lambda v: np.all(np.abs(v[:, 0]) < 1e-8)]
+    else:
+        exprs = [lambda v, d=d, s=s: np.all(np.abs(v[:, d] - s) < 1e-8)
+                 for (d, s) in [(1, 0), (0, 1), (1, 1), (0, 0), (2, 0), (2, 1)]]
+    bnd = dw.boundary_face_tags(p, t, nd, exprs)
+    bcs = [1] * len(exprs)
+    bf = np.where(bnd >= 0, np.array(bcs)[np.maximum(bnd, 0)], 0)
+    dw.write_app(os.path.join(path, "app.bin"), nd=nd, ncu=1,
+                 flag=_app_flags(precond, ortho),
+                 problem={13: newton_iter, 14: gmres_maxit, 15: restart},
+                 physicsparam=[kappa], tau=[tau], solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
+    dw.write_master(os.path.join(path, "master.bin"), m)
+    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, bcs)
+    xdg = dw.compute_dgnodes(m, p, t)
+    dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=1, nc=nd + 1)
+    return dict(model=model, nd=nd, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=1)
+
+
+def convdiff3d(path, n=32, porder=4, pgauss=None, ncu=2, kappa=1e-2, vel=(1.0, 0.5, 0.25),
+               tau=1.0, seed=20261202, precond="bjacobi", restart=100, gmres_tol=1e-10,
+               gmres_maxit=2000, newton_iter=1, newton_tol=1e-8, ortho=0, cartgridpart=None):
+    """Steady convection-diffusion on an n^3 (or n[0] x n[1] x n[2]) hex mesh."""
+    os.makedirs(path, exist_ok=True)
+    nn = [n] * 3 if np.isscalar(n) else list(n)
+    pgauss = 2 * porder if pgauss is None else pgauss
+    m = dw.make_master(3, porder, pgauss)
+    p, t = dw.structured_grid(nn)
+    exprs = [lambda v: np.all(np.abs(v[:, 0]) < 1e-8), lambda v: True]
+    bnd = dw.boundary_face_tags(p, t, 3, exprs)
+    bcs = [1, 2]
+    bf = np.where(bnd >= 0, np.array(bcs)[np.maximum(bnd, 0)], 0)
+    cg = [] if cartgridpart is None else list(cartgridpart)
+    dw.write_app(os.path.join(path, "app.bin"), nd=3, ncu=ncu, nco=ncu,
+                 flag=_app_flags(precond, ortho),
+                 problem={13: newton_iter, 14: gmres_maxit, 15: restart},
+                 physicsparam=[kappa] + list(vel), tau=[tau],
+                 solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
+    dw.write_master(os.path.join(path, "master.bin"), m)
+    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, bcs, cartgridpart=cg)
+    xdg = dw.compute_dgnodes(m, p, t)
+    rng = np.random.default_rng(seed)
+    odg = rng.uniform(-1.0, 1.0, size=(t.shape[0], m["npe"], ncu))
+    dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=ncu, nc=4 * ncu, odg=odg)
+    return dict(model="convdiff", nd=3, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=ncu)

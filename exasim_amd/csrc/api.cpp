@@ -1,0 +1,154 @@
+// api.cpp -- the extern "C" entry points of libexahip.so (declared in
+// include/exasim_hdg.h).  Every call catches the library's exceptions and turns
+// them into an EXA_ERR_* code plus a message kept on the handle (or, when
+// there is no handle yet, in a thread-local slot).
+#include "exasim_hdg.h"
+#include "solver.hpp"
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+struct exa_handle {
+  std::unique_ptr<exa::Solver> solver;
+  std::string err;
+};
+
+namespace {
+thread_local std::string g_create_error;
+
+template <typename F>
+int guard(exa_handle* h, F&& f) {
+  if (!h || !h->solver) return EXA_ERR_ARG;
+  try {
+    f(*h->solver);
+    h->err.clear();
+    return EXA_OK;
+  } catch (const exa::Error& e) {
+    h->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    h->err = "host allocation failed";
+    return EXA_ERR_HIP;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return EXA_ERR_ARG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int exa_create(const char* datain_dir, const char* dataout_prefix, const char* model, int device,
+               exa_handle** out) {
+  if (!out || !datain_dir || !model) return EXA_ERR_ARG;
+  *out = nullptr;
+  try {
+    auto h = std::make_unique<exa_handle>();
+    h->solver = std::make_unique<exa::Solver>(datain_dir, dataout_prefix ? dataout_prefix : "", model,
+                                              device);
+    *out = h.release();
+    g_create_error.clear();
+    return EXA_OK;
+  } catch (const exa::Error& e) {
+    g_create_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return EXA_ERR_ARG;
+  }
+}
+
+void exa_destroy(exa_handle* h) { delete h; }
+
+const char* exa_last_error(const exa_handle* h) {
+  if (!h) return g_create_error.c_str();
+  return h->err.c_str();
+}
+
+int exa_sizes(const exa_handle* h, int* out, int len) {
+  if (!h || !h->solver || !out) return 0;
+  const exa::Solver& s = *h->solver;
+  const exa::Dims& d = s.d;
+  int v[16] = {d.nd, d.ncu, d.nc, d.npe, d.npf, d.nfe, d.ne, d.nf, d.n, d.m, static_cast<int>(d.N),
+               d.nge, d.ngf, d.nco, s.restart, s.precond};
+  int k = len < 16 ? len : 16;
+  for (int i = 0; i < k; ++i) out[i] = v[i];
+  return k;
+}
+
+int exa_set_option(exa_handle* h, const char* name, double value) {
+  if (!name) return EXA_ERR_ARG;
+  return guard(h, [&](exa::Solver& s) { s.set_option(name, value); });
+}
+
+int exa_solve(exa_handle* h, int* newton_iters, int* gmres_iters, double* final_norm) {
+  return guard(h, [&](exa::Solver& s) { s.solve(newton_iters, gmres_iters, final_norm); });
+}
+
+int exa_get_q(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.get_q(); }); }
+
+int exa_assemble(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.assemble(); }); }
+
+int exa_residual(exa_handle* h, double* norm_b, double* norm_ru) {
+  return guard(h, [&](exa::Solver& s) { s.residual(norm_b, norm_ru); });
+}
+
+int exa_matvec(exa_handle* h, const double* v_dev, double* w_dev) {
+  if (!v_dev || !w_dev) return EXA_ERR_ARG;
+  return guard(h, [&](exa::Solver& s) {
+    if (!s.assembled) throw exa::Error(EXA_ERR_ARG, "exa_matvec before exa_assemble");
+    s.matvec(v_dev, w_dev);
+  });
+}
+
+int exa_precond(exa_handle* h, double* x_dev) {
+  if (!x_dev) return EXA_ERR_ARG;
+  return guard(h, [&](exa::Solver& s) {
+    if (!s.assembled) throw exa::Error(EXA_ERR_ARG, "exa_precond before exa_assemble");
+    s.apply_precond(x_dev);
+  });
+}
+
+int exa_gmres(exa_handle* h, const double* b_dev, double* x_dev, int* iters) {
+  if (!b_dev || !x_dev) return EXA_ERR_ARG;
+  return guard(h, [&](exa::Solver& s) {
+    if (!s.assembled) throw exa::Error(EXA_ERR_ARG, "exa_gmres before exa_assemble");
+    int it = s.gmres(b_dev, x_dev);
+    if (iters) *iters = it;
+  });
+}
+
+long exa_get_array(exa_handle* h, const char* name, double* dst, long len) {
+  if (!name) return -EXA_ERR_ARG;
+  long out = 0;
+  int rc = guard(h, [&](exa::Solver& s) { out = s.get_array(name, dst, len); });
+  return rc == EXA_OK ? out : -rc;
+}
+
+int exa_set_array(exa_handle* h, const char* name, const double* src, long len) {
+  if (!name || !src) return EXA_ERR_ARG;
+  return guard(h, [&](exa::Solver& s) { s.set_array(name, src, len); });
+}
+
+int exa_device_ptr(exa_handle* h, const char* name, void** ptr, long* len) {
+  if (!name || !ptr) return EXA_ERR_ARG;
+  return guard(h, [&](exa::Solver& s) { s.device_ptr(name, ptr, len); });
+}
+
+int exa_write_output(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.write_output(); }); }
+
+int exa_timings(exa_handle* h, double* ms, long* counts, int len) {
+  int n = 0;
+  int rc = guard(h, [&](exa::Solver& s) { n = s.timings(ms, counts, len); });
+  return rc == EXA_OK ? n : -rc;
+}
+
+int exa_reset_timings(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.reset_timings(); }); }
+
+int exa_sync(exa_handle* h) {
+  return guard(h, [&](exa::Solver& s) { EXA_HIP(hipStreamSynchronize(s.s)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,1214 @@
+// kernels.hip -- HIP kernels of the HDG Newton-GMRES path for gfx950 (MI355X).
+//
+// Layouts (column-major, first index fastest, as in the reference):
+//   master tables  shapegt[nge, npe, nd+1], shapegw[npe, nge, nd+1],
+//                  shapfgt[ngf, npf, nd],   shapfgw[npf, ngf, nd]
+//   udg[npe, nc, ne], uh[ncu, npf, nf], perm[npf, nfe], elemcon[npf, nfe, ne]
+//   element geometry egeo, face geometry fgeo: structure of arrays over the
+//   quadrature points (field-major), so consecutive threads touch consecutive points.
+//   Per-element matrices in the condensation layouts of uequation.cpp:559-775:
+//   u-DOF r = i + npe*c, trace DOF t = c + ncu*(j + npf*l).
+//
+// Every reduction has a fixed order and no atomics are used, so results are
+// bitwise reproducible from run to run.
+#include "kernels.hpp"
+#include "models.hpp"
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace exa {
+
+namespace {
+
+inline unsigned nblocks(long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// Batched FP64 GEMM: LDS-tiled, 256 threads, each thread a TM x TN micro-tile.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int BK, int TM, int TN>
+__global__ __launch_bounds__(256) void k_gemm(Gemm g, int tilesM) {
+  const int tile = blockIdx.x % tilesM;
+  const long b = blockIdx.x / tilesM;
+  const int b0 = static_cast<int>(b % g.nb0);
+  const long t1 = b / g.nb0;
+  const int b1 = static_cast<int>(t1 % g.nb1);
+  const int b2 = static_cast<int>(t1 / g.nb1);
+  const double* A = g.A + g.sA0 * b0 + g.sA1 * b1 + g.sA2 * b2;
+  const double* B = g.B + g.sB0 * b0 + g.sB1 * b1 + g.sB2 * b2;
+  double* C = g.C + g.sC0 * b0 + g.sC1 * b1 + g.sC2 * b2;
+  const int row0 = tile * BM, col0 = blockIdx.y * BN;
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    for (int idx = threadIdx.x; idx < BM * BK; idx += 256) {
+      int r = idx % BM, kk = idx / BM;
+      int gr = row0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < g.M && gk < g.K) ? A[gr + g.lda * gk] : 0.0;
+    }
+    for (int idx = threadIdx.x; idx < BK * BN; idx += 256) {
+      int kk = idx % BK, c = idx / BK;
+      int gk = k0 + kk, gc = col0 + c;
+      Bs[kk][c] = (gk < g.K && gc < g.N) ? B[gk + g.ldb * gc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; ++kk) {
+      double a[TM], bb[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bb[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      int r = row0 + tx + 16 * i, c = col0 + ty + 16 * j;
+      if (r < g.M && c < g.N) {
+        double v = g.alpha * acc[i][j];
+        if (g.beta != 0.0) v = fma(g.beta, C[r + g.ldc * c], v);
+        C[r + g.ldc * c] = v;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked LU with partial pivoting + back substitution, one workgroup per matrix.
+// ---------------------------------------------------------------------------
+constexpr int LU_NB = 16;
+constexpr int LU_CH = 64;
+
+__global__ __launch_bounds__(256) void k_lu_solve(double* Aall, int n, int ncols, long stride,
+                                                  int* info) {
+  double* A = Aall + stride * blockIdx.x;
+  extern __shared__ double P[];  // n x LU_NB panel (ld n)
+  __shared__ double Us[LU_NB][LU_CH + 1];
+  __shared__ double redv[256];
+  __shared__ int redi[256];
+  __shared__ int piv[LU_NB];
+  const int tid = threadIdx.x;
+  int singular = 0;
+  for (int k0 = 0; k0 < n; k0 += LU_NB) {
+    const int nb = min(LU_NB, n - k0);
+    const int rows = n - k0;
+    for (int idx = tid; idx < rows * nb; idx += 256) {
+      int r = k0 + idx % rows, kk = idx / rows;
+      P[r + n * kk] = A[r + static_cast<long>(n) * (k0 + kk)];
+    }
+    __syncthreads();
+    for (int kk = 0; kk < nb; ++kk) {
+      const int col = k0 + kk;
+      double best = -1.0;
+      int bi = col;
+      for (int r = col + tid; r < n; r += 256) {
+        double v = fabs(P[r + n * kk]);
+        if (v > best) { best = v; bi = r; }
+      }
+      redv[tid] = best;
+      redi[tid] = bi;
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+          double o = redv[tid + s];
+          int oi = redi[tid + s];
+          if (o > redv[tid] || (o == redv[tid] && oi < redi[tid])) { redv[tid] = o; redi[tid] = oi; }
+        }
+        __syncthreads();
+      }
+      const int p = redi[0];
+      if (tid == 0) piv[kk] = p;
+      if (p != col && tid < nb) {
+        double t = P[col + n * tid];
+        P[col + n * tid] = P[p + n * tid];
+        P[p + n * tid] = t;
+      }
+      __syncthreads();
+      const double pv = P[col + n * kk];
+      if (pv == 0.0) singular = 1;
+      for (int r = col + 1 + tid; r < n; r += 256) P[r + n * kk] /= pv;
+      __syncthreads();
+      const int rr = n - col - 1, cc = nb - kk - 1;
+      for (int idx = tid; idx < rr * cc; idx += 256) {
+        int r = col + 1 + idx % rr, c = kk + 1 + idx / rr;
+        P[r + n * c] -= P[r + n * kk] * P[col + n * c];
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < rows * nb; idx += 256) {
+      int r = k0 + idx % rows, kk = idx / rows;
+      A[r + static_cast<long>(n) * (k0 + kk)] = P[r + n * kk];
+    }
+    for (int j = tid; j < ncols; j += 256) {
+      if (j >= k0 && j < k0 + nb) continue;
+      double* colj = A + static_cast<long>(n) * j;
+      for (int kk = 0; kk < nb; ++kk) {
+        int p = piv[kk], r = k0 + kk;
+        if (p != r) { double t = colj[r]; colj[r] = colj[p]; colj[p] = t; }
+      }
+    }
+    __syncthreads();
+    for (int j = k0 + nb + tid; j < ncols; j += 256) {
+      double* colj = A + static_cast<long>(n) * j;
+      double x[LU_NB];
+#pragma unroll
+      for (int kk = 0; kk < LU_NB; ++kk) x[kk] = (kk < nb) ? colj[k0 + kk] : 0.0;
+#pragma unroll
+      for (int kk = 1; kk < LU_NB; ++kk)
+#pragma unroll
+        for (int t = 0; t < kk; ++t)
+          if (kk < nb) x[kk] -= P[k0 + kk + n * t] * x[t];
+#pragma unroll
+      for (int kk = 0; kk < LU_NB; ++kk)
+        if (kk < nb) colj[k0 + kk] = x[kk];
+    }
+    __syncthreads();
+    const int R = n - k0 - nb;
+    if (R > 0) {
+      for (int j0 = k0 + nb; j0 < ncols; j0 += LU_CH) {
+        const int cw = min(LU_CH, ncols - j0);
+        for (int idx = tid; idx < nb * cw; idx += 256) {
+          int t = idx % nb, c = idx / nb;
+          Us[t][c] = A[k0 + t + static_cast<long>(n) * (j0 + c)];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < R * cw; idx += 256) {
+          int r = k0 + nb + idx % R, c = idx / R;
+          double* a = A + r + static_cast<long>(n) * (j0 + c);
+          double s = *a;
+          for (int t = 0; t < nb; ++t) s -= P[r + n * t] * Us[t][c];
+          *a = s;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // back substitution U X = Y for the right-hand-side columns
+  const int last = ((n - 1) / LU_NB) * LU_NB;
+  for (int k0 = last; k0 >= 0; k0 -= LU_NB) {
+    const int nb = min(LU_NB, n - k0);
+    const int rows = k0 + nb;
+    for (int idx = tid; idx < rows * nb; idx += 256) {
+      int r = idx % rows, kk = idx / rows;
+      P[r + n * kk] = A[r + static_cast<long>(n) * (k0 + kk)];
+    }
+    __syncthreads();
+    for (int j = n + tid; j < ncols; j += 256) {
+      double* colj = A + static_cast<long>(n) * j;
+      double x[LU_NB];
+#pragma unroll
+      for (int kk = 0; kk < LU_NB; ++kk) x[kk] = (kk < nb) ? colj[k0 + kk] : 0.0;
+#pragma unroll
+      for (int kk = LU_NB - 1; kk >= 0; --kk) {
+        if (kk < nb) {
+          double s = x[kk];
+#pragma unroll
+          for (int t = kk + 1; t < LU_NB; ++t)
+            if (t < nb) s -= P[k0 + kk + n * t] * x[t];
+          x[kk] = s / P[k0 + kk + n * kk];
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < LU_NB; ++kk)
+        if (kk < nb) colj[k0 + kk] = x[kk];
+    }
+    __syncthreads();
+    if (k0 > 0) {
+      for (int j0 = n; j0 < ncols; j0 += LU_CH) {
+        const int cw = min(LU_CH, ncols - j0);
+        for (int idx = tid; idx < nb * cw; idx += 256) {
+          int t = idx % nb, c = idx / nb;
+          Us[t][c] = A[k0 + t + static_cast<long>(n) * (j0 + c)];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < k0 * cw; idx += 256) {
+          int r = idx % k0, c = idx / k0;
+          double* a = A + r + static_cast<long>(n) * (j0 + c);
+          double s = *a;
+          for (int t = 0; t < nb; ++t) s -= P[r + n * t] * Us[t][c];
+          *a = s;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (tid == 0 && info) info[blockIdx.x] = singular;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry and setup
+// ---------------------------------------------------------------------------
+__device__ inline void adjugate(int nd, const double J[3][3], double* jac, double Xx[9]) {
+  if (nd == 2) {
+    *jac = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    Xx[0 + 2 * 0] = J[1][1];
+    Xx[1 + 2 * 0] = -J[1][0];
+    Xx[0 + 2 * 1] = -J[0][1];
+    Xx[1 + 2 * 1] = J[0][0];
+    return;
+  }
+  for (int j = 0; j < 3; ++j)
+    for (int m = 0; m < 3; ++m)
+      Xx[j + 3 * m] = J[(m + 1) % 3][(j + 1) % 3] * J[(m + 2) % 3][(j + 2) % 3] -
+                      J[(m + 1) % 3][(j + 2) % 3] * J[(m + 2) % 3][(j + 1) % 3];
+  *jac = J[0][0] * Xx[0 + 3 * 0] + J[0][1] * Xx[1 + 3 * 0] + J[0][2] * Xx[2 + 3 * 0];
+}
+
+// ElemGeomBlock (geometry.cpp:44-85, kokkosimpl.h:2329-2360): egeo fields
+// xg[d] (d < nd), Xx[j + nd*m] at nd + j + nd*m, jac at nd + nd*nd.
+__global__ void k_elem_geom(Dims d, const double* shapegt, const double* xdg, double* egeo) {
+  const long P = static_cast<long>(d.ne) * d.nge;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const int e = static_cast<int>(p / d.nge), g = static_cast<int>(p % d.nge);
+  const int nd = d.nd, npe = d.npe, nge = d.nge;
+  double xg[3] = {0, 0, 0}, J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < npe; ++i) {
+    const double phi = shapegt[g + nge * i];
+    for (int b = 0; b < nd; ++b) {
+      const double x = xdg[i + npe * (b + nd * static_cast<long>(e))];
+      xg[b] += phi * x;
+      for (int a = 0; a < nd; ++a) J[a][b] += shapegt[g + nge * (i + npe * (a + 1))] * x;
+    }
+  }
+  double jac, Xx[9];
+  adjugate(nd, J, &jac, Xx);
+  for (int b = 0; b < nd; ++b) egeo[b * P + p] = xg[b];
+  for (int k = 0; k < nd * nd; ++k) egeo[(nd + k) * P + p] = Xx[k];
+  egeo[(nd + nd * nd) * P + p] = jac;
+}
+
+// ElemFaceGeomBlock (geometry.cpp:168-205, kokkosimpl.h:2386-2416): fgeo fields
+// xg[d], nl[d] at nd + d, jac at 2*nd, at point p = (e*nfe + l)*ngf + g.
+__global__ void k_face_geom(Dims d, const double* shapfgt, const int* perm, const double* xdg,
+                            double* fgeo) {
+  const long P = static_cast<long>(d.ne) * d.nfe * d.ngf;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const int g = static_cast<int>(p % d.ngf);
+  const long el = p / d.ngf;
+  const int l = static_cast<int>(el % d.nfe);
+  const long e = el / d.nfe;
+  const int nd = d.nd, npe = d.npe, npf = d.npf, ngf = d.ngf;
+  double xg[3] = {0, 0, 0}, T0[3] = {0, 0, 0}, T1[3] = {0, 0, 0};
+  for (int j = 0; j < npf; ++j) {
+    const int node = perm[j + npf * l];
+    const double phi = shapfgt[g + ngf * j];
+    const double d0 = shapfgt[g + ngf * (j + npf * 1)];
+    const double d1 = (nd == 3) ? shapfgt[g + ngf * (j + npf * 2)] : 0.0;
+    for (int b = 0; b < nd; ++b) {
+      const double x = xdg[node + npe * (b + nd * e)];
+      xg[b] += phi * x;
+      T0[b] += d0 * x;
+      T1[b] += d1 * x;
+    }
+  }
+  double nl[3], jac;
+  if (nd == 2) {
+    jac = sqrt(T0[0] * T0[0] + T0[1] * T0[1]);
+    nl[0] = T0[1] / jac;
+    nl[1] = -T0[0] / jac;
+  } else {
+    nl[0] = T0[1] * T1[2] - T0[2] * T1[1];
+    nl[1] = T0[2] * T1[0] - T0[0] * T1[2];
+    nl[2] = T0[0] * T1[1] - T0[1] * T1[0];
+    jac = sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
+    nl[0] /= jac;
+    nl[1] /= jac;
+    nl[2] /= jac;
+  }
+  for (int b = 0; b < nd; ++b) {
+    fgeo[b * P + p] = xg[b];
+    fgeo[(nd + b) * P + p] = nl[b];
+  }
+  fgeo[(2 * nd) * P + p] = jac;
+}
+
+__global__ void k_interp_elem(Dims d, int ncomp, const double* shapegt, const double* src, double* dst) {
+  const long P = static_cast<long>(d.ne) * d.nge;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const long e = p / d.nge;
+  const int g = static_cast<int>(p % d.nge);
+  for (int c = 0; c < ncomp; ++c) {
+    double s = 0.0;
+    for (int i = 0; i < d.npe; ++i) s += shapegt[g + d.nge * i] * src[i + d.npe * (c + ncomp * e)];
+    dst[c * P + p] = s;
+  }
+}
+
+__global__ void k_interp_face(Dims d, int ncomp, const double* shapfgt, const int* perm,
+                              const double* src, double* dst) {
+  const long P = static_cast<long>(d.ne) * d.nfe * d.ngf;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const int g = static_cast<int>(p % d.ngf);
+  const long el = p / d.ngf;
+  const int l = static_cast<int>(el % d.nfe);
+  const long e = el / d.nfe;
+  for (int c = 0; c < ncomp; ++c) {
+    double s = 0.0;
+    for (int j = 0; j < d.npf; ++j)
+      s += shapfgt[g + d.ngf * j] * src[perm[j + d.npf * l] + d.npe * (c + ncomp * e)];
+    dst[c * P + p] = s;
+  }
+}
+
+// qEquationElem (qequation.cpp:66-165): Mass[r, j] = sum w phi_r phi_j jac and
+// C_d[r, i] = sum_g sum_m w dphi_r/dxi_m phi_i Xx[d, m], written into the
+// augmented matrix [Mass | C_1 .. C_nd | E_1 .. E_nd] (ld npe).
+__global__ void k_qsetup_mc(Dims d, int e0, int eb, const double* shapegt, const double* shapegw,
+                            const double* egeo, double* aug) {
+  const int npe = d.npe, nge = d.nge, nd = d.nd;
+  const long ncq = static_cast<long>(npe) + npe * nd + static_cast<long>(d.ndf) * nd;
+  const long per = static_cast<long>(npe) * (npe + npe * nd);
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const int e = static_cast<int>(q / per);
+  const long rc = q % per;
+  const int r = static_cast<int>(rc % npe), col = static_cast<int>(rc / npe);
+  const long P = static_cast<long>(d.ne) * nge;
+  const long pe = static_cast<long>(e0 + e) * nge;
+  double s = 0.0;
+  if (col < npe) {
+    for (int g = 0; g < nge; ++g)
+      s += shapegw[r + npe * g] * shapegt[g + nge * col] * egeo[(nd + nd * nd) * P + pe + g];
+  } else {
+    const int cc = col - npe, dd = cc / npe, i = cc % npe;
+    for (int g = 0; g < nge; ++g) {
+      double t = 0.0;
+      for (int m = 0; m < nd; ++m)
+        t += shapegw[r + npe * (g + nge * (m + 1))] * egeo[(nd + dd + nd * m) * P + pe + g];
+      s += t * shapegt[g + nge * i];
+    }
+  }
+  aug[e * (npe * ncq) + r + static_cast<long>(npe) * col] = s;
+}
+
+// qEquationElemFaceBlock (qequation.cpp:258-340): E_d[perm[k,l], i + npf*l] = sum w phi_k phi_i n_d jac.
+__global__ void k_qsetup_e(Dims d, int e0, int eb, const double* shapfgt, const double* shapfgw,
+                           const int* perm, const double* fgeo, double* aug) {
+  const int npe = d.npe, npf = d.npf, ngf = d.ngf, nd = d.nd, nfe = d.nfe, ndf = d.ndf;
+  const long ncq = static_cast<long>(npe) + npe * nd + static_cast<long>(ndf) * nd;
+  const long per = static_cast<long>(nfe) * npf * npf * nd;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const int e = static_cast<int>(q / per);
+  long r = q % per;
+  const int k = static_cast<int>(r % npf); r /= npf;
+  const int i = static_cast<int>(r % npf); r /= npf;
+  const int l = static_cast<int>(r % nfe);
+  const int dd = static_cast<int>(r / nfe);
+  const long P = static_cast<long>(d.ne) * nfe * ngf;
+  const long pf = (static_cast<long>(e0 + e) * nfe + l) * ngf;
+  double s = 0.0;
+  for (int g = 0; g < ngf; ++g)
+    s += shapfgw[k + npf * g] * shapfgt[g + ngf * i] * fgeo[(nd + dd) * P + pf + g] *
+         fgeo[(2 * nd) * P + pf + g];
+  const long col = npe + static_cast<long>(npe) * nd + (i + npf * l) + static_cast<long>(ndf) * dd;
+  aug[e * (npe * ncq) + perm[k + npf * l] + npe * col] = s;
+}
+
+// MinvC stacked MCs[(k + npe*d) + npe*nd*j], MinvE stacked MEs[(k + npe*d) + npe*nd*t].
+__global__ void k_qpack(Dims d, int e0, int eb, const double* aug, double* MCs, double* MEs,
+                        double* Mass) {
+  const int npe = d.npe, nd = d.nd, ndf = d.ndf;
+  const long ncq = static_cast<long>(npe) + npe * nd + static_cast<long>(ndf) * nd;
+  const long per = static_cast<long>(npe) * nd * (npe + ndf);
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const int e = static_cast<int>(q / per);
+  long r = q % per;
+  const int kd = static_cast<int>(r % (npe * nd));
+  const int col = static_cast<int>(r / (npe * nd));
+  const int k = kd % npe, dd = kd / npe;
+  const double* X = aug + e * (npe * ncq);
+  const long E = e0 + e;
+  if (col < npe) {
+    MCs[E * npe * nd * npe + kd + static_cast<long>(npe) * nd * col] =
+        X[k + static_cast<long>(npe) * (npe + col + npe * dd)];
+  } else {
+    const int t = col - npe;
+    MEs[E * npe * nd * ndf + kd + static_cast<long>(npe) * nd * t] =
+        X[k + static_cast<long>(npe) * (npe + npe * nd + t + static_cast<long>(ndf) * dd)];
+  }
+}
+
+// hdgGetQ (qequation.cpp:372-487): q_d = MinvC_d u - MinvE_d uh, stored in udg.
+__global__ void k_getq(Dims d, const double* MCs, const double* MEs, const int* elemcon,
+                       const double* uh, double* udg) {
+  const int npe = d.npe, nd = d.nd, ncu = d.ncu, nc = d.nc, ndf = d.ndf;
+  const long per = static_cast<long>(npe) * nd * ncu;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * d.ne) return;
+  const long e = q / per;
+  long r = q % per;
+  const int kd = static_cast<int>(r % (npe * nd));
+  const int c = static_cast<int>(r / (npe * nd));
+  const int k = kd % npe, dd = kd / npe;
+  const double* mc = MCs + e * npe * nd * npe;
+  const double* me = MEs + e * npe * nd * ndf;
+  const double* u = udg + npe * (c + nc * e);
+  double s = 0.0;
+  for (int j = 0; j < npe; ++j) s += mc[kd + static_cast<long>(npe) * nd * j] * u[j];
+  for (int t = 0; t < ndf; ++t)
+    s -= me[kd + static_cast<long>(npe) * nd * t] * uh[c + ncu * static_cast<long>(elemcon[t + ndf * e])];
+  udg[k + npe * (ncu + c + ncu * dd) + npe * nc * e] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Element volume terms: uEquationElemBlock (uequation.cpp:45-203)
+// ---------------------------------------------------------------------------
+// rv[(dp*NCU + k)*P + p]       : s*jac (dp = 0), sum_j f[k, j] Xx[j, m] (dp = m+1)
+// rJ[((dp*NCU + k)*NC + c)*P + p]: the same for the Jacobians (ApplyXxJac, kokkosimpl.h:2105-2158)
+template <class M, bool JAC>
+__global__ void k_vol_points(Dims d, AsmArgs a) {
+  constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
+  const long P = static_cast<long>(a.eb) * d.nge;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const long e = p / d.nge;
+  const int g = static_cast<int>(p % d.nge);
+  const long E = a.e0 + e;
+  const long PG = static_cast<long>(d.ne) * d.nge;
+  const long pg = E * d.nge + g;
+  double uq[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) uq[c] = 0.0;
+  for (int i = 0; i < d.npe; ++i) {
+    const double phi = a.shapegt[g + d.nge * i];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) uq[c] += phi * a.udg[i + d.npe * (c + NC * E)];
+  }
+  double x[ND], Xx[ND * ND], v[8];
+#pragma unroll
+  for (int b = 0; b < ND; ++b) x[b] = a.egeo[b * PG + pg];
+#pragma unroll
+  for (int k = 0; k < ND * ND; ++k) Xx[k] = a.egeo[(ND + k) * PG + pg];
+  const double jac = a.egeo[(ND + ND * ND) * PG + pg];
+  for (int c = 0; c < d.nco && c < 8; ++c) v[c] = a.og[c * PG + pg];
+  double f[NCU * ND], s[NCU];
+  double f_uq[JAC ? NCU * ND * NC : 1], s_uq[JAC ? NCU * NC : 1];
+  M::template flux<JAC>(x, uq, v, a.param, a.time, f, f_uq);
+  M::template source<JAC>(x, uq, v, a.param, a.time, s, s_uq);
+#pragma unroll
+  for (int k = 0; k < NCU; ++k) {
+    a.rv[(0 * NCU + k) * P + p] = s[k] * jac;
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) t += f[k + NCU * j] * Xx[j + ND * m];
+      a.rv[((m + 1) * NCU + k) * P + p] = t;
+    }
+  }
+  if (JAC) {
+#pragma unroll
+    for (int k = 0; k < NCU; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        a.rJ[((0 * NCU + k) * NC + c) * P + p] = s_uq[k + NCU * c] * jac;
+#pragma unroll
+        for (int m = 0; m < ND; ++m) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) t += f_uq[(k + NCU * j) + NCU * ND * c] * Xx[j + ND * m];
+          a.rJ[(((m + 1) * NCU + k) * NC + c) * P + p] = t;
+        }
+      }
+  }
+}
+
+// Ru[i + npe*k] = sum_dp sum_g shapegw[i, g, dp] * rv  (Gauss2Node with shapegw)
+__global__ void k_vol_res(Dims d, AsmArgs a, double* out, long ldo) {
+  const int npe = d.npe, nge = d.nge, ncu = d.ncu, nd = d.nd;
+  const long per = static_cast<long>(npe) * ncu;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * a.eb) return;
+  const long e = q / per;
+  const int r = static_cast<int>(q % per);
+  const int i = r % npe, k = r / npe;
+  const long P = static_cast<long>(a.eb) * nge;
+  double s = 0.0;
+  for (int dp = 0; dp <= nd; ++dp) {
+    const double* rv = a.rv + (dp * ncu + k) * P + e * nge;
+    const double* w = a.shapegw + static_cast<long>(npe) * nge * dp;
+    for (int g = 0; g < nge; ++g) s += w[i + npe * g] * rv[g];
+  }
+  out[e * ldo + r] = s;
+}
+
+// W[(e*NC + c)*NCU + k][i + npe*g] = sum_dp shapegw[i, g, dp] * rJ[dp, k, c]
+__global__ void k_vol_W(Dims d, AsmArgs a) {
+  const int npe = d.npe, nge = d.nge, ncu = d.ncu, nc = d.nc, nd = d.nd;
+  const long blk = static_cast<long>(npe) * nge;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= blk * ncu * nc * a.eb) return;
+  const long bidx = q / blk;
+  const int ig = static_cast<int>(q % blk);
+  const int i = ig % npe, g = ig / npe;
+  const int k = static_cast<int>(bidx % ncu);
+  const long t = bidx / ncu;
+  const int c = static_cast<int>(t % nc);
+  const long e = t / nc;
+  const long P = static_cast<long>(a.eb) * nge;
+  double s = 0.0;
+  for (int dp = 0; dp <= nd; ++dp)
+    s += a.shapegw[i + npe * (g + nge * dp)] * a.rJ[((dp * ncu + k) * nc + c) * P + e * nge + g];
+  a.W[q] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Face terms: uEquationElemFaceBlock (uequation.cpp:206-557) with FhatDriver
+// (Model/KokkosDrivers.cpp:874-934) and FbouDriver for boundary faces.
+// fpt fields (JAC): fhI[NCU], fhI_uq[NCU*NC], fhI_uh[NCU*NCU], then the same
+// with boundary faces replaced by FbouHdg ("B"); residual-only: fhI[NCU], fhB[NCU].
+// ---------------------------------------------------------------------------
+template <class M, bool JAC>
+__global__ void k_face_points(Dims d, AsmArgs a) {
+  constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
+  constexpr int NF1 = JAC ? (NCU + NCU * NC + NCU * NCU) : NCU;
+  const long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const int g = static_cast<int>(p % d.ngf);
+  const long el = p / d.ngf;
+  const int l = static_cast<int>(el % d.nfe);
+  const long E = a.e0 + el / d.nfe;
+  const long PF = static_cast<long>(d.ne) * d.nfe * d.ngf;
+  const long pf = (E * d.nfe + l) * d.ngf + g;
+  double uh[NCU], uq[NC], usave[NCU];
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) uh[c] = 0.0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) uq[c] = 0.0;
+  for (int j = 0; j < d.npf; ++j) {
+    const double phi = a.shapfgt[g + d.ngf * j];
+    const long gn = a.elemcon[j + d.npf * (l + d.nfe * E)];
+    const int node = a.perm[j + d.npf * l];
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) uh[c] += phi * a.uh[c + NCU * gn];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) uq[c] += phi * a.udg[node + d.npe * (c + NC * E)];
+  }
+  double x[ND], nl[ND], v[8];
+#pragma unroll
+  for (int b = 0; b < ND; ++b) {
+    x[b] = a.fgeo[b * PF + pf];
+    nl[b] = a.fgeo[(ND + b) * PF + pf];
+  }
+  const double jac = a.fgeo[(2 * ND) * PF + pf];
+  for (int c = 0; c < d.nco && c < 8; ++c) v[c] = a.ogf[c * PF + pf];
+  const double tau0 = a.tau0;
+  // interior formula: flux at (uhat, q), dotted with n, plus tau*(u - uhat)
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) {
+    usave[c] = uq[c];
+    uq[c] = uh[c];
+  }
+  double f[NCU * ND], f_uq[JAC ? NCU * ND * NC : 1];
+  M::template flux<JAC>(x, uq, v, a.param, a.time, f, f_uq);
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) uq[c] = usave[c];
+#pragma unroll
+  for (int k = 0; k < NCU; ++k) {
+    double fh = 0.0;
+#pragma unroll
+    for (int b = 0; b < ND; ++b) fh += f[k + NCU * b] * nl[b];
+    fh += tau0 * (usave[k] - uh[k]);
+    a.fpt[k * P + p] = fh * jac;
+  }
+  if (JAC) {
+#pragma unroll
+    for (int k = 0; k < NCU; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < ND; ++b) t += f_uq[(k + NCU * b) + NCU * ND * c] * nl[b];
+        if (c < NCU) {
+          a.fpt[(NCU + NCU * NC + k + NCU * c) * P + p] = (t - (c == k ? tau0 : 0.0)) * jac;
+          a.fpt[(NCU + k + NCU * c) * P + p] = (c == k ? tau0 : 0.0) * jac;
+        } else {
+          a.fpt[(NCU + k + NCU * c) * P + p] = t * jac;
+        }
+      }
+  }
+  const int ib = a.bf[l + d.nfe * E];
+  if (ib > 0) {
+    double fb[NCU], fb_uq[JAC ? NCU * NC : 1], fb_uh[JAC ? NCU * NCU : 1];
+    M::template fbou<JAC>(ib, x, uq, v, uh, nl, a.tau, a.param, a.time, fb, fb_uq, fb_uh);
+#pragma unroll
+    for (int k = 0; k < NCU; ++k) a.fpt[(NF1 + k) * P + p] = fb[k] * jac;
+    if (JAC) {
+#pragma unroll
+      for (int i = 0; i < NCU * NC; ++i) a.fpt[(NF1 + NCU + i) * P + p] = fb_uq[i] * jac;
+#pragma unroll
+      for (int i = 0; i < NCU * NCU; ++i) a.fpt[(NF1 + NCU + NCU * NC + i) * P + p] = fb_uh[i] * jac;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NF1; ++i) a.fpt[(NF1 + i) * P + p] = a.fpt[i * P + p];
+  }
+}
+
+// Residual parts of face l: Ru[perm[a,l] + npe*k] -= int phi_a fhI; Rh = -int phi_a fhB
+// (assembleRu, kokkosimpl.h:1650-1659; uequation.cpp:426-427).
+__global__ void k_face_res(Dims d, AsmArgs a, int l, int nf1, double* ru, long ldru, double* Rh) {
+  const int npf = d.npf, ngf = d.ngf, ncu = d.ncu;
+  const long per = static_cast<long>(npf) * ncu;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * a.eb) return;
+  const long e = q / per;
+  const int r = static_cast<int>(q % per);
+  const int aa = r % npf, k = r / npf;
+  const long P = static_cast<long>(a.eb) * d.nfe * ngf;
+  const long pb = (e * d.nfe + l) * ngf;
+  double sI = 0.0, sB = 0.0;
+  for (int g = 0; g < ngf; ++g) {
+    const double w = a.shapfgw[aa + npf * g];
+    sI += w * a.fpt[k * P + pb + g];
+    sB += w * a.fpt[(nf1 + k) * P + pb + g];
+  }
+  ru[e * ldru + a.perm[aa + npf * l] + d.npe * k] -= sI;
+  Rh[(a.e0 + e) * d.m + k + ncu * (aa + npf * l)] = -sB;
+}
+
+// Jacobian parts of face l: D/B += int phi_a phi_b fhI_uq, F += int phi_a phi_b fhI_uh,
+// K/G = int phi_a phi_b fhB_uq, H = int phi_a phi_b fhB_uh
+// (assembleMatrixBD/F/GK/H, kokkosimpl.h:1661-1730).
+__global__ void k_face_jac(Dims d, AsmArgs a, int l, double* aug, double* Bfull, double* Kt,
+                           double* Gfull, double* H) {
+  const int npf = d.npf, ngf = d.ngf, ncu = d.ncu, nc = d.nc, npe = d.npe, nd = d.nd;
+  const int n = d.n, m = d.m;
+  const int ncol = nc + ncu;
+  const long per = static_cast<long>(npf) * npf * ncu * ncol;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * a.eb) return;
+  const long e = q / per;
+  long r = q % per;
+  const int aa = static_cast<int>(r % npf); r /= npf;
+  const int bb = static_cast<int>(r % npf); r /= npf;
+  const int k = static_cast<int>(r % ncu);
+  const int c = static_cast<int>(r / ncu);
+  const long P = static_cast<long>(a.eb) * d.nfe * ngf;
+  const long pb = (e * d.nfe + l) * ngf;
+  const int nf1 = ncu + ncu * nc + ncu * ncu;
+  const int fI = (c < nc) ? (ncu + k + ncu * c) : (ncu + ncu * nc + k + ncu * (c - nc));
+  double sI = 0.0, sB = 0.0;
+  for (int g = 0; g < ngf; ++g) {
+    const double w = a.shapfgw[aa + npf * g] * a.shapfgt[g + ngf * bb];
+    sI += w * a.fpt[fI * P + pb + g];
+    sB += w * a.fpt[(nf1 + fI) * P + pb + g];
+  }
+  const int pa = a.perm[aa + npf * l], pbn = a.perm[bb + npf * l];
+  const long augs = static_cast<long>(n) * (n + m + 1);
+  const long trow = k + ncu * (aa + npf * l);
+  if (c < ncu) {
+    aug[e * augs + pa + npe * k + static_cast<long>(n) * (pbn + npe * c)] += sI;
+    Kt[e * m * static_cast<long>(n) + trow + static_cast<long>(m) * (pbn + npe * c)] += sB;
+  } else if (c < nc) {
+    const int cq = c - ncu, cp = cq % ncu, dd = cq / ncu;
+    const long col = pbn + npe * dd + static_cast<long>(npe) * nd * cp;
+    Bfull[e * n * static_cast<long>(n) * nd + pa + npe * k + n * col] += sI;
+    Gfull[e * m * static_cast<long>(n) * nd + trow + m * col] += sB;
+  } else {
+    const int c2 = c - nc;
+    aug[e * augs + pa + npe * k + static_cast<long>(n) * (n + c2 + ncu * (bb + npf * l))] += sI;
+    H[(a.e0 + e) * static_cast<long>(m) * m + trow + static_cast<long>(m) * (c2 + ncu * (bb + npf * l))] = sB;
+  }
+}
+
+// After the solve: DinvF = -X_F, Ru = X_Ru (uequation.cpp:700-711).
+__global__ void k_post_schur(Dims d, int e0, int eb, const double* aug, double* DinvF, double* RuS) {
+  const int n = d.n, m = d.m;
+  const long per = static_cast<long>(n) * (m + 1);
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const long e = q / per;
+  const long r = q % per;
+  const double v = aug[e * static_cast<long>(n) * (n + m + 1) + static_cast<long>(n) * n + r];
+  const long E = e0 + e;
+  if (r < static_cast<long>(n) * m) DinvF[E * n * static_cast<long>(m) + r] = -v;
+  else RuS[E * n + (r - static_cast<long>(n) * m)] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Global system
+// ---------------------------------------------------------------------------
+// PutElementFaceNodes twice (kokkosimpl.h:1032-1064) as one gather per face node.
+__global__ void k_face_gather(long nnodes, int ncu, const long* src1, const long* src2,
+                              const double* we, double* w) {
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= nnodes * ncu) return;
+  const long g = q / ncu;
+  const int c = static_cast<int>(q % ncu);
+  double s = we[src1[g] + c];
+  const long s2 = src2[g];
+  if (s2 >= 0) s += we[s2 + c];
+  w[q] = s;
+}
+
+// BlockJacobi (kokkosimpl.h:1067-1111) into an augmented [BE | I] for inversion.
+__global__ void k_bj_assemble(Dims d, const int* f2e, const int* elemcon, const double* H, double* aug) {
+  const int ncf = d.ncf, ncu = d.ncu, npf = d.npf, nfe = d.nfe, m = d.m;
+  const long per = static_cast<long>(ncf) * 2 * ncf;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * d.nf) return;
+  const long f = q / per;
+  const int rs = static_cast<int>(q % per);
+  const int r = rs % ncf, s = rs / ncf;
+  double v;
+  if (s >= ncf) {
+    v = (s - ncf == r) ? 1.0 : 0.0;
+  } else {
+    const int e1 = f2e[4 * f], l1 = f2e[4 * f + 1], e2 = f2e[4 * f + 2], l2 = f2e[4 * f + 3];
+    v = H[static_cast<long>(e1) * m * m + (r + ncf * l1) + static_cast<long>(m) * (s + ncf * l1)];
+    if (e2 >= 0) {
+      const int ar = r % ncu, br = r / ncu, as = s % ncu, bs = s / ncu;
+      const long base = npf * (l2 + static_cast<long>(nfe) * e2);
+      const int dr = ar + ncu * static_cast<int>(elemcon[br + base] - npf * f);
+      const int ds = as + ncu * static_cast<int>(elemcon[bs + base] - npf * f);
+      v += H[static_cast<long>(e2) * m * m + (dr + ncf * l2) + static_cast<long>(m) * (ds + ncf * l2)];
+    }
+  }
+  aug[f * per + rs] = v;
+}
+
+__global__ void k_bj_extract(Dims d, const double* aug, double* BE) {
+  const long per = static_cast<long>(d.ncf) * d.ncf;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * d.nf) return;
+  const long f = q / per;
+  BE[q] = aug[f * 2 * per + per + q % per];
+}
+
+// x_f <- BE_f x_f (preconditioner.cpp:282-292); one thread per row.
+__global__ void k_bj_apply(Dims d, const double* BE, const double* x, double* y) {
+  const int ncf = d.ncf;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(ncf) * d.nf) return;
+  const long f = q / ncf;
+  const int r = static_cast<int>(q % ncf);
+  const double* B = BE + f * ncf * static_cast<long>(ncf);
+  const double* xf = x + f * ncf;
+  double s = 0.0;
+  for (int c = 0; c < ncf; ++c) s += B[r + static_cast<long>(ncf) * c] * xf[c];
+  y[q] = s;
+}
+
+// hdgMatVec element part (matvec.cpp:377-380): we_e = H_e * v_e with v_e gathered
+// through elemcon.  One workgroup per element; each of the 4 waves owns a
+// quarter of the columns, lanes own rows (coalesced column reads of H), and the
+// four partial sums are added in LDS in a fixed order.
+constexpr int MV_MAXRB = 16;  // m <= 1024
+__global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const double* H, const int* elemcon,
+                                                     const double* v, double* we) {
+  const int m = d.m, ncu = d.ncu, ndf = d.ndf;
+  const long e = blockIdx.x;
+  extern __shared__ double sm[];
+  double* ve = sm;          // m
+  double* red = sm + m;     // 4*m
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int t = tid; t < m; t += 256)
+    ve[t] = v[(t % ncu) + ncu * static_cast<long>(elemcon[t / ncu + ndf * e])];
+  __syncthreads();
+  const int cper = (m + 3) / 4;
+  const int c0 = wave * cper, c1 = min(m, c0 + cper);
+  const int nrb = (m + 63) / 64;
+  double acc[MV_MAXRB];
+#pragma unroll
+  for (int rb = 0; rb < MV_MAXRB; ++rb) acc[rb] = 0.0;
+  const double* He = H + e * static_cast<long>(m) * m;
+  for (int c = c0; c < c1; ++c) {
+    const double vc = ve[c];
+    const double* col = He + static_cast<long>(m) * c;
+#pragma unroll
+    for (int rb = 0; rb < MV_MAXRB; ++rb) {
+      const int r = lane + 64 * rb;
+      if (rb < nrb && r < m) acc[rb] = fma(col[r], vc, acc[rb]);
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < MV_MAXRB; ++rb) {
+    const int r = lane + 64 * rb;
+    if (rb < nrb && r < m) red[wave * m + r] = acc[rb];
+  }
+  __syncthreads();
+  for (int r = tid; r < m; r += 256)
+    we[e * m + r] = ((red[r] + red[m + r]) + red[2 * m + r]) + red[3 * m + r];
+}
+
+// hdgGetDUDG (matvec.cpp:219-238): du_e = Ru_e + DinvF_e * x_e.
+__global__ void k_dudg(Dims d, const double* RuS, const double* DinvF, const int* elemcon,
+                       const double* x, double* du) {
+  const int n = d.n, m = d.m, ncu = d.ncu, ndf = d.ndf;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(n) * d.ne) return;
+  const long e = q / n;
+  const int r = static_cast<int>(q % n);
+  const double* F = DinvF + e * n * static_cast<long>(m);
+  double s = RuS[q];
+  for (int t = 0; t < m; ++t)
+    s += F[r + static_cast<long>(n) * t] * x[(t % ncu) + ncu * static_cast<long>(elemcon[t / ncu + ndf * e])];
+  du[q] = s;
+}
+
+// UpdateUDG (kokkosimpl.h:322-338) on the u components.
+__global__ void k_update_udg(Dims d, const double* du, double alpha, double* udg) {
+  const int npe = d.npe, ncu = d.ncu, nc = d.nc;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(npe) * ncu * d.ne) return;
+  const long e = q / (npe * ncu);
+  const int r = static_cast<int>(q % (npe * ncu));
+  const int i = r % npe, c = r / npe;
+  udg[i + npe * (c + nc * e)] += alpha * du[q];
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels (fixed-order reductions)
+// ---------------------------------------------------------------------------
+constexpr int DOT_BS = 256;
+constexpr int DOT_MAXB = 1024;
+
+__global__ __launch_bounds__(DOT_BS) void k_dot_partial(const double* x, const double* y, long n,
+                                                        double* partial) {
+  __shared__ double sh[DOT_BS];
+  double s = 0.0;
+  for (long i = blockIdx.x * static_cast<long>(DOT_BS) + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * DOT_BS)
+    s = fma(x[i], y[i], s);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = DOT_BS / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// partial is [nvec][nb]; out[v] = sum_b partial[v][b] (sqrt when sqrt_out)
+__global__ __launch_bounds__(DOT_BS) void k_reduce(const double* partial, int nb, double* out,
+                                                   int sqrt_out) {
+  __shared__ double sh[DOT_BS];
+  const double* pv = partial + static_cast<long>(blockIdx.x) * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += DOT_BS) s += pv[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = DOT_BS / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt_out ? sqrt(sh[0]) : sh[0];
+}
+
+// partial[v][b] for v < k: dot(V_v, w) -- blockIdx.y = v
+__global__ __launch_bounds__(DOT_BS) void k_multidot_partial(const double* V, long ldv,
+                                                             const double* w, long n, double* partial) {
+  __shared__ double sh[DOT_BS];
+  const double* x = V + ldv * blockIdx.y;
+  double s = 0.0;
+  for (long i = blockIdx.x * static_cast<long>(DOT_BS) + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * DOT_BS)
+    s = fma(x[i], w[i], s);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = DOT_BS / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[static_cast<long>(blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ void k_axpy_dev(double* y, const double* x, const double* coef, double scale, long n) {
+  const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (i < n) y[i] = fma(scale * coef[0], x[i], y[i]);
+}
+
+__global__ void k_scale_inv_dev(double* x, const double* coef, long n) {
+  const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (i < n) x[i] = x[i] * (1.0 / coef[0]);
+}
+
+__global__ void k_axpby(double* y, const double* x, double a, double b, long n) {
+  const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void k_copy(double* y, const double* x, long n) {
+  const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+
+// x += alpha * V[:, 0:k] y   (y on the device)
+__global__ void k_gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha,
+                            long n) {
+  const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < k; ++j) s = fma(V[i + ldv * j], y[j], s);
+  x[i] = fma(alpha, s, x[i]);
+}
+
+template <class M>
+void launch_points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) {
+  const int bs = 256;
+  if (vol) {
+    long P = static_cast<long>(a.eb) * d.nge;
+    if (jac) k_vol_points<M, true><<<nblocks(P, bs), bs, 0, s>>>(d, a);
+    else k_vol_points<M, false><<<nblocks(P, bs), bs, 0, s>>>(d, a);
+  } else {
+    long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
+    if (jac) k_face_points<M, true><<<nblocks(P, bs), bs, 0, s>>>(d, a);
+    else k_face_points<M, false><<<nblocks(P, bs), bs, 0, s>>>(d, a);
+  }
+  EXA_HIP(hipGetLastError());
+}
+
+template <template <int, int> class MT>
+void dispatch_points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) {
+  if (d.nd == 2 && d.ncu == 1) return launch_points<MT<2, 1>>(d, a, jac, vol, s);
+  if (d.nd == 3 && d.ncu == 1) return launch_points<MT<3, 1>>(d, a, jac, vol, s);
+  if (d.nd == 2 && d.ncu == 2) return launch_points<MT<2, 2>>(d, a, jac, vol, s);
+  if (d.nd == 3 && d.ncu == 2) return launch_points<MT<3, 2>>(d, a, jac, vol, s);
+  throw Error(5, "model instantiated for nd in {2,3}, ncu in {1,2} only");
+}
+
+void points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) {
+  if (d.nco > 8) throw Error(5, "at most 8 auxiliary components");
+  switch (a.model) {
+    case MODEL_POISSON: return dispatch_points<Poisson>(d, a, jac, vol, s);
+    case MODEL_POISSON_POLY: return dispatch_points<PoissonPoly>(d, a, jac, vol, s);
+    case MODEL_CONVDIFF: return dispatch_points<ConvDiff>(d, a, jac, vol, s);
+    default: throw Error(1, "unknown model");
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+void gemm(const Gemm& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  const long batch = static_cast<long>(g.nb0) * g.nb1 * g.nb2;
+  if (batch <= 0) return;
+  if (g.M <= 32 && g.N <= 32) {
+    int tm = (g.M + 31) / 32, tn = (g.N + 31) / 32;
+    dim3 grid(static_cast<unsigned>(tm * batch), tn);
+    k_gemm<32, 32, 16, 2, 2><<<grid, 256, 0, s>>>(g, tm);
+  } else {
+    int tm = (g.M + 63) / 64, tn = (g.N + 63) / 64;
+    dim3 grid(static_cast<unsigned>(tm * batch), tn);
+    k_gemm<64, 64, 16, 4, 4><<<grid, 256, 0, s>>>(g, tm);
+  }
+  EXA_HIP(hipGetLastError());
+}
+
+void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s) {
+  if (batch <= 0) return;
+  // dynamic LDS panel n x LU_NB doubles next to ~12 KB of static LDS, kept within 64 KB
+  if (n > 400) throw Error(5, "dense block larger than 400 rows");
+  size_t shm = static_cast<size_t>(n) * LU_NB * sizeof(double);
+  k_lu_solve<<<batch, 256, shm, s>>>(A, n, ncols, stride, info);
+  EXA_HIP(hipGetLastError());
+}
+
+void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s) {
+  long P = static_cast<long>(d.ne) * d.nge;
+  k_elem_geom<<<nblocks(P, 256), 256, 0, s>>>(d, shapegt, xdg, egeo);
+  EXA_HIP(hipGetLastError());
+}
+
+void face_geom(const Dims& d, const double* shapfgt, const int* perm, const double* xdg, double* fgeo,
+               hipStream_t s) {
+  long P = static_cast<long>(d.ne) * d.nfe * d.ngf;
+  k_face_geom<<<nblocks(P, 256), 256, 0, s>>>(d, shapfgt, perm, xdg, fgeo);
+  EXA_HIP(hipGetLastError());
+}
+
+void interp_elem(const Dims& d, int ncomp, const double* shapegt, const double* src, double* dst,
+                 hipStream_t s) {
+  if (ncomp <= 0) return;
+  long P = static_cast<long>(d.ne) * d.nge;
+  k_interp_elem<<<nblocks(P, 256), 256, 0, s>>>(d, ncomp, shapegt, src, dst);
+  EXA_HIP(hipGetLastError());
+}
+
+void interp_face(const Dims& d, int ncomp, const double* shapfgt, const int* perm, const double* src,
+                 double* dst, hipStream_t s) {
+  if (ncomp <= 0) return;
+  long P = static_cast<long>(d.ne) * d.nfe * d.ngf;
+  k_interp_face<<<nblocks(P, 256), 256, 0, s>>>(d, ncomp, shapfgt, perm, src, dst);
+  EXA_HIP(hipGetLastError());
+}
+
+void qsetup(const Dims& d, int e0, int eb, const double* shapegt, const double* shapegw,
+            const double* shapfgt, const double* shapfgw, const int* perm, const double* egeo,
+            const double* fgeo, double* aug, hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * (d.npe + d.npe * d.nd) * eb;
+  k_qsetup_mc<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, shapegt, shapegw, egeo, aug);
+  long n2 = static_cast<long>(d.nfe) * d.npf * d.npf * d.nd * eb;
+  k_qsetup_e<<<nblocks(n2, 256), 256, 0, s>>>(d, e0, eb, shapfgt, shapfgw, perm, fgeo, aug);
+  EXA_HIP(hipGetLastError());
+}
+
+void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double* MEs, double* Mass,
+           hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * d.nd * (d.npe + d.ndf) * eb;
+  k_qpack<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, MCs, MEs, Mass);
+  EXA_HIP(hipGetLastError());
+}
+
+void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
+          double* udg, hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * d.nd * d.ncu * d.ne;
+  k_getq<<<nblocks(n1, 256), 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg);
+  EXA_HIP(hipGetLastError());
+}
+
+void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s) { points(d, a, jac, true, s); }
+void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s) { points(d, a, jac, false, s); }
+
+void vol_res(const Dims& d, const AsmArgs& a, double* out, long ldo, hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * d.ncu * a.eb;
+  k_vol_res<<<nblocks(n1, 256), 256, 0, s>>>(d, a, out, ldo);
+  EXA_HIP(hipGetLastError());
+}
+
+void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * d.nge * d.ncu * d.nc * a.eb;
+  k_vol_W<<<nblocks(n1, 256), 256, 0, s>>>(d, a);
+  EXA_HIP(hipGetLastError());
+}
+
+void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long ldru, double* Rh,
+              hipStream_t s) {
+  int nf1 = jac ? (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) : d.ncu;
+  long n1 = static_cast<long>(d.npf) * d.ncu * a.eb;
+  k_face_res<<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, nf1, ru, ldru, Rh);
+  EXA_HIP(hipGetLastError());
+}
+
+void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull, double* Kt,
+              double* Gfull, double* H, hipStream_t s) {
+  long n1 = static_cast<long>(d.npf) * d.npf * d.ncu * (d.nc + d.ncu) * a.eb;
+  k_face_jac<<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, aug, Bfull, Kt, Gfull, H);
+  EXA_HIP(hipGetLastError());
+}
+
+void post_schur(const Dims& d, int e0, int eb, const double* aug, double* DinvF, double* RuS,
+                hipStream_t s) {
+  long n1 = static_cast<long>(d.n) * (d.m + 1) * eb;
+  k_post_schur<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, DinvF, RuS);
+  EXA_HIP(hipGetLastError());
+}
+
+void face_gather(long nnodes, int ncu, const long* src1, const long* src2, const double* we, double* w,
+                 hipStream_t s) {
+  k_face_gather<<<nblocks(nnodes * ncu, 256), 256, 0, s>>>(nnodes, ncu, src1, src2, we, w);
+  EXA_HIP(hipGetLastError());
+}
+
+void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double* H, double* aug,
+                 hipStream_t s) {
+  long n1 = static_cast<long>(d.ncf) * 2 * d.ncf * d.nf;
+  k_bj_assemble<<<nblocks(n1, 256), 256, 0, s>>>(d, f2e, elemcon, H, aug);
+  EXA_HIP(hipGetLastError());
+}
+
+void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s) {
+  long n1 = static_cast<long>(d.ncf) * d.ncf * d.nf;
+  k_bj_extract<<<nblocks(n1, 256), 256, 0, s>>>(d, aug, BE);
+  EXA_HIP(hipGetLastError());
+}
+
+void bj_apply(const Dims& d, const double* BE, const double* x, double* y, hipStream_t s) {
+  long n1 = static_cast<long>(d.ncf) * d.nf;
+  k_bj_apply<<<nblocks(n1, 256), 256, 0, s>>>(d, BE, x, y);
+  EXA_HIP(hipGetLastError());
+}
+
+void matvec_elem(const Dims& d, const double* H, const int* elemcon, const double* v, double* we,
+                 hipStream_t s) {
+  if (d.m > 64 * MV_MAXRB) throw Error(5, "element trace block larger than 1024");
+  size_t shm = static_cast<size_t>(5) * d.m * sizeof(double);
+  k_matvec_elem<<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
+  EXA_HIP(hipGetLastError());
+}
+
+void dudg(const Dims& d, const double* RuS, const double* DinvF, const int* elemcon, const double* x,
+          double* du, hipStream_t s) {
+  long n1 = static_cast<long>(d.n) * d.ne;
+  k_dudg<<<nblocks(n1, 256), 256, 0, s>>>(d, RuS, DinvF, elemcon, x, du);
+  EXA_HIP(hipGetLastError());
+}
+
+void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * d.ncu * d.ne;
+  k_update_udg<<<nblocks(n1, 256), 256, 0, s>>>(d, du, alpha, udg);
+  EXA_HIP(hipGetLastError());
+}
+
+int dot_partials_count(long n) {
+  long b = (n + DOT_BS - 1) / DOT_BS;
+  return static_cast<int>(std::max(1L, std::min<long>(DOT_MAXB, b)));
+}
+
+void dot(const double* x, const double* y, long n, double* partial, double* out, int sqrt_out,
+         hipStream_t s) {
+  int nb = dot_partials_count(n);
+  k_dot_partial<<<nb, DOT_BS, 0, s>>>(x, y, n, partial);
+  k_reduce<<<1, DOT_BS, 0, s>>>(partial, nb, out, sqrt_out);
+  EXA_HIP(hipGetLastError());
+}
+
+void multidot(const double* V, long ldv, int k, const double* w, long n, double* partial, double* out,
+              hipStream_t s) {
+  int nb = dot_partials_count(n);
+  dim3 grid(nb, k);
+  k_multidot_partial<<<grid, DOT_BS, 0, s>>>(V, ldv, w, n, partial);
+  k_reduce<<<k, DOT_BS, 0, s>>>(partial, nb, out, 0);
+  EXA_HIP(hipGetLastError());
+}
+
+void axpy_dev(double* y, const double* x, const double* coef, double scale, long n, hipStream_t s) {
+  k_axpy_dev<<<nblocks(n, 256), 256, 0, s>>>(y, x, coef, scale, n);
+  EXA_HIP(hipGetLastError());
+}
+
+void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s) {
+  k_scale_inv_dev<<<nblocks(n, 256), 256, 0, s>>>(x, coef, n);
+  EXA_HIP(hipGetLastError());
+}
+
+void axpby(double* y, const double* x, double a, double b, long n, hipStream_t s) {
+  k_axpby<<<nblocks(n, 256), 256, 0, s>>>(y, x, a, b, n);
+  EXA_HIP(hipGetLastError());
+}
+
+void copy(double* y, const double* x, long n, hipStream_t s) {
+  k_copy<<<nblocks(n, 256), 256, 0, s>>>(y, x, n);
+  EXA_HIP(hipGetLastError());
+}
+
+void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
+               hipStream_t s) {
+  k_gemv_cols<<<nblocks(n, 256), 256, 0, s>>>(x, V, ldv, k, y, alpha, n);
+  EXA_HIP(hipGetLastError());
+}
+
+}  // namespace exa

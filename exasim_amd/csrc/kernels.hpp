@@ -1,0 +1,100 @@
+// kernels.hpp -- launch wrappers of the HIP kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace exa {
+
+struct Dims {
+  int nd, ncu, nc, nco, npe, npf, nfe, nge, ngf, ne, nf;
+  int n, m, ncf, ndf;
+  long N;  // trace unknowns ncu*npf*nf
+};
+
+// Column-major batched GEMM  C_b = alpha * A_b * B_b + beta * C_b  (no transposes).
+// Batch index b = b0 + nb0*(b1 + nb1*b2); each operand offset is s0*b0 + s1*b1 + s2*b2.
+struct Gemm {
+  int M, N, K;
+  double alpha, beta;
+  const double* A; long lda, sA0, sA1, sA2;
+  const double* B; long ldb, sB0, sB1, sB2;
+  double* C;       long ldc, sC0, sC1, sC2;
+  int nb0, nb1, nb2;
+};
+void gemm(const Gemm& g, hipStream_t s);
+
+// In-place blocked LU with partial pivoting of the leading n x n block of each
+// augmented matrix A_b (n x ncols, ld n, batch stride `stride`), then back
+// substitution so that columns n..ncols-1 hold D^{-1} * RHS.  info[b] = 1 when a
+// zero pivot is met.
+void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s);
+
+// ---- setup ----
+void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s);
+void face_geom(const Dims& d, const double* shapfgt, const int* perm, const double* xdg, double* fgeo,
+               hipStream_t s);
+void interp_elem(const Dims& d, int ncomp, const double* shapegt, const double* src, double* dst,
+                 hipStream_t s);
+void interp_face(const Dims& d, int ncomp, const double* shapfgt, const int* perm, const double* src,
+                 double* dst, hipStream_t s);
+// Augmented [Mass | C_1..C_nd | E_1..E_nd] for elements [e0, e0+eb).
+void qsetup(const Dims& d, int e0, int eb, const double* shapegt, const double* shapegw,
+            const double* shapfgt, const double* shapfgw, const int* perm, const double* egeo,
+            const double* fgeo, double* aug, hipStream_t s);
+void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double* MEs, double* Mass,
+           hipStream_t s);
+void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
+          double* udg, hipStream_t s);
+
+// ---- assembly (model-templated inside) ----
+struct AsmArgs {
+  int model;
+  double time;
+  const double* param;  // device
+  const double* tau;    // device
+  double tau0;
+  int e0, eb;
+  const double *shapegt, *shapegw, *shapfgt, *shapfgw;
+  const int *perm, *elemcon, *bf;
+  const double *udg, *uh, *egeo, *fgeo, *og, *ogf;
+  double *rv, *rJ, *W, *fpt;
+};
+void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
+void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
+// Ru[e*ldo + i + npe*k] = sum phiw * rv   (Ru block of element e0+e at out + e*ldo)
+void vol_res(const Dims& d, const AsmArgs& a, double* out, long ldo, hipStream_t s);
+void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s);
+void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long ldru, double* Rh,
+              hipStream_t s);
+void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull, double* Kt,
+              double* Gfull, double* H, hipStream_t s);
+void post_schur(const Dims& d, int e0, int eb, const double* aug, double* DinvF, double* RuS,
+                hipStream_t s);
+
+// ---- global system ----
+void face_gather(long nnodes, int ncu, const long* src1, const long* src2, const double* we, double* w,
+                 hipStream_t s);
+void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double* H, double* aug,
+                 hipStream_t s);
+void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s);
+void bj_apply(const Dims& d, const double* BE, const double* x, double* y, hipStream_t s);
+void matvec_elem(const Dims& d, const double* H, const int* elemcon, const double* v, double* we,
+                 hipStream_t s);
+void dudg(const Dims& d, const double* RuS, const double* DinvF, const int* elemcon, const double* x,
+          double* du, hipStream_t s);
+void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipStream_t s);
+
+// ---- vector kernels (results stay on the device) ----
+int dot_partials_count(long n);
+void dot(const double* x, const double* y, long n, double* partial, double* out, int sqrt_out,
+         hipStream_t s);
+void axpy_dev(double* y, const double* x, const double* coef, double scale, long n, hipStream_t s);
+void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s);
+void axpby(double* y, const double* x, double a, double b, long n, hipStream_t s);
+void copy(double* y, const double* x, long n, hipStream_t s);
+void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
+               hipStream_t s);
+void multidot(const double* V, long ldv, int k, const double* w, long n, double* partial, double* out,
+              hipStream_t s);
+
+}  // namespace exa

@@ -1,0 +1,712 @@
+// solver.cpp -- setup and host control of the HDG Newton-GMRES path.
+//
+// Setup follows the CDiscretization constructor (backend/Discretization/
+// discretization.cpp:170-412): read datain, buildConn, geometry, qEquation,
+// initial uhat.  assemble() is hdgAssembleLinearSystem (discretization.cpp:483-512)
+// with uEquationHDG's element-block loop (uequation.cpp:778-787); gmres() is the
+// HDG GMRES of backend/Solver/gmres.cpp:361-597; solve() is NewtonSolver
+// (backend/Solution/solution.cpp:125-242).  All device work runs on one stream
+// owned by the handle; the host only reads back the scalars GMRES and Newton
+// branch on.
+#include "solver.hpp"
+#include "models.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace exa {
+
+namespace {
+int model_id(const std::string& name) {
+  if (name == "poisson") return MODEL_POISSON;
+  if (name == "poisson_poly") return MODEL_POISSON_POLY;
+  if (name == "convdiff") return MODEL_CONVDIFF;
+  throw Error(1, "unknown model '" + name + "'");
+}
+
+// cpuApplyGivensRotation (backend/Common/cpuimpl.h:167-197)
+void givens(double* Hc, double* s, double* cs, double* sn, int i) {
+  for (int k = 0; k < i; ++k) {
+    double t = cs[k] * Hc[k] + sn[k] * Hc[k + 1];
+    Hc[k + 1] = -sn[k] * Hc[k] + cs[k] * Hc[k + 1];
+    Hc[k] = t;
+  }
+  if (Hc[i + 1] == 0.0) {
+    cs[i] = 1.0;
+    sn[i] = 0.0;
+  } else if (std::fabs(Hc[i + 1]) > std::fabs(Hc[i])) {
+    double t = Hc[i] / Hc[i + 1];
+    sn[i] = 1.0 / std::sqrt(1.0 + t * t);
+    cs[i] = t * sn[i];
+  } else {
+    double t = Hc[i + 1] / Hc[i];
+    cs[i] = 1.0 / std::sqrt(1.0 + t * t);
+    sn[i] = t * cs[i];
+  }
+  double t = cs[i] * s[i];
+  s[i + 1] = -sn[i] * s[i];
+  s[i] = t;
+  Hc[i] = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
+  Hc[i + 1] = 0.0;
+}
+
+// cpuBackSolve (cpuimpl.h:199-208): Hm column-major with leading dimension ld.
+void back_solve(double* y, const double* Hm, const double* s, int i, int ld) {
+  for (int j = i; j >= 0; --j) y[j] = s[j];
+  for (int j = i; j >= 0; --j) {
+    y[j] /= Hm[j + ld * j];
+    for (int k = j - 1; k >= 0; --k) y[k] -= Hm[k + ld * j] * y[j];
+  }
+}
+}  // namespace
+
+Solver::Solver(const std::string& din, const std::string& dout, const std::string& mname, int dev)
+    : datain(din), dataout(dout), model(model_id(mname)), device(dev) {
+  setup();
+}
+
+Solver::~Solver() {
+  for (auto& v : ev)
+    for (auto& p : v) {
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+  if (s) (void)hipStreamSynchronize(s);
+  if (hpin) (void)hipHostFree(hpin);
+  if (s) (void)hipStreamDestroy(s);
+}
+
+void Solver::setup() {
+  app = read_app(datain + "/app.bin");
+  master = read_master(datain + "/master.bin");
+  mesh = read_mesh(datain + "/mesh.bin");
+  if (master.nd != mesh.nd) throw Error(2, "master.bin and mesh.bin disagree on nd");
+  d.nd = master.nd;
+  d.nc = static_cast<int>(app.ndims[5]);
+  d.ncu = static_cast<int>(app.ndims[6]);
+  d.npe = master.npe;
+  d.npf = master.npf;
+  d.nge = master.nge;
+  d.ngf = master.ngf;
+  d.ne = mesh.ne;
+  d.nfe = mesh.nfe;
+  if (d.nc != d.ncu * (d.nd + 1))
+    throw Error(5, "only the HDG (u, q) formulation with nc = ncu*(nd+1) is supported");
+  SolData sol = read_sol(datain + "/sol.bin", d.npe, d.nc, d.ncu, d.ne);
+  if (static_cast<long>(sol.xdg.size()) != static_cast<long>(d.npe) * d.nd * d.ne)
+    throw Error(2, "xdg must have ncx = nd components");
+  d.nco = sol.nco;
+  conn = build_conn(master, mesh);
+  d.nf = conn.nf;
+  d.n = d.npe * d.ncu;
+  d.ndf = d.npf * d.nfe;
+  d.m = d.ndf * d.ncu;
+  d.ncf = d.npf * d.ncu;
+  d.N = static_cast<long>(d.ncf) * d.nf;
+
+  // inputs are read and checked before the device is touched
+  if (device >= 0) EXA_HIP(hipSetDevice(device));
+  EXA_HIP(hipGetDevice(&device));
+  EXA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  EXA_HIP(hipHostMalloc(reinterpret_cast<void**>(&hpin), 4096 * sizeof(double), hipHostMallocDefault));
+
+  std::vector<double>& fl = app.flag;
+  std::vector<double>& pr = app.problem;
+  if (static_cast<int>(fl[0]) != 0) throw Error(5, "time-dependent problems are not supported yet");
+  precond = static_cast<int>(fl[6]);
+  ortho = static_cast<int>(fl[5]);
+  newton_maxit = static_cast<int>(pr[13]);
+  gmres_maxit = static_cast<int>(pr[14]);
+  restart = static_cast<int>(pr[15]);
+  newton_tol = app.solversparam[0];
+  gmres_tol = app.solversparam[1];
+  if (static_cast<int>(pr[27]) > 1) throw Error(5, "polynomial preconditioner (ppdegree > 1) is not supported");
+  if (static_cast<int>(pr[16]) > 0) throw Error(5, "reduced-basis initial guess (RBdim > 0) is not supported");
+
+  // master tables, mesh, fields
+  shapegt.upload(master.shapegt, s);
+  shapegw.upload(master.shapegw, s);
+  shapfgt.upload(master.shapfgt, s);
+  shapfgw.upload(master.shapfgw, s);
+  param.upload(app.physicsparam.empty() ? std::vector<double>{0.0} : app.physicsparam, s);
+  tau.upload(app.tau, s);
+  perm.upload(mesh.perm, s);
+  elemcon.upload(conn.elemcon, s);
+  bf.upload(mesh.bf, s);
+  f2e.upload(conn.f2e, s);
+  xdg.upload(sol.xdg, s);
+
+  // udg with all nc components (readsolstruct: missing q components are zero;
+  // a missing udg is Initu, which is zero for the models here)
+  std::vector<double> hudg(static_cast<size_t>(d.npe) * d.nc * d.ne, 0.0);
+  if (!sol.udg.empty()) {
+    int k = sol.udg_ncomp;
+    for (long e = 0; e < d.ne; ++e)
+      for (int c = 0; c < k; ++c)
+        for (int i = 0; i < d.npe; ++i)
+          hudg[i + d.npe * (c + d.nc * e)] = sol.udg[i + d.npe * (c + k * e)];
+  }
+  udg.upload(hudg, s);
+  // initial uhat (GetFaceNodes from the first element of each face, kokkosimpl.h:901-915)
+  std::vector<double> huh(static_cast<size_t>(d.N), 0.0);
+  if (!sol.uh.empty()) {
+    if (static_cast<long>(sol.uh.size()) != d.N) throw Error(2, "uhat in sol.bin has the wrong size");
+    huh = sol.uh;
+  } else {
+    for (long f = 0; f < d.nf; ++f) {
+      int e1 = conn.f2e[4 * f], l1 = conn.f2e[4 * f + 1];
+      for (int j = 0; j < d.npf; ++j)
+        for (int c = 0; c < d.ncu; ++c)
+          huh[c + d.ncu * (j + d.npf * f)] =
+              hudg[mesh.perm[j + d.npf * l1] + d.npe * (c + static_cast<long>(d.nc) * e1)];
+    }
+  }
+  uh.upload(huh, s);
+
+  // face-node gather sources for the atomic-free element -> face assembly
+  long nn = static_cast<long>(d.npf) * d.nf;
+  std::vector<long> h1(nn), h2(nn);
+  for (long f = 0; f < d.nf; ++f) {
+    int e1 = conn.f2e[4 * f], l1 = conn.f2e[4 * f + 1], e2 = conn.f2e[4 * f + 2], l2 = conn.f2e[4 * f + 3];
+    for (int j = 0; j < d.npf; ++j) {
+      long g = j + d.npf * f;
+      h1[g] = static_cast<long>(e1) * d.m + d.ncu * (j + d.npf * l1);
+      if (e2 >= 0) {
+        long ii = conn.elemcon[j + d.npf * (l2 + static_cast<long>(d.nfe) * e2)] - d.npf * f;
+        h2[g] = static_cast<long>(e2) * d.m + d.ncu * (ii + d.npf * l2);
+      } else {
+        h2[g] = -1;
+      }
+    }
+  }
+  src1.upload(h1, s);
+  src2.upload(h2, s);
+
+  // geometry
+  egeo.alloc(static_cast<size_t>(d.ne) * d.nge * (d.nd + d.nd * d.nd + 1));
+  fgeo.alloc(static_cast<size_t>(d.ne) * d.nfe * d.ngf * (2 * d.nd + 1));
+  elem_geom(d, shapegt.p, xdg.p, egeo.p, s);
+  face_geom(d, shapfgt.p, perm.p, xdg.p, fgeo.p, s);
+  if (d.nco > 0) {
+    odg.upload(sol.odg, s);
+    og.alloc(static_cast<size_t>(d.ne) * d.nge * d.nco);
+    ogf.alloc(static_cast<size_t>(d.ne) * d.nfe * d.ngf * d.nco);
+    interp_elem(d, d.nco, shapegt.p, odg.p, og.p, s);
+    interp_face(d, d.nco, shapfgt.p, perm.p, odg.p, ogf.p, s);
+  }
+
+  // qEquation: per element solve Mass X = [Mass | C_1..C_nd | E_1..E_nd]
+  MCs.alloc(static_cast<size_t>(d.ne) * d.npe * d.nd * d.npe);
+  MEs.alloc(static_cast<size_t>(d.ne) * d.npe * d.nd * d.ndf);
+  Mass.alloc(static_cast<size_t>(d.ne) * d.npe * d.npe);
+  {
+    long ncq = static_cast<long>(d.npe) + d.npe * d.nd + static_cast<long>(d.ndf) * d.nd;
+    long per = d.npe * ncq;
+    int qb = static_cast<int>(std::max(1L, std::min<long>(d.ne, static_cast<long>(1.0e9 / 8 / per))));
+    DevBuf<double> qaug;
+    qaug.alloc(static_cast<size_t>(qb) * per);
+    info.alloc(std::max(d.ne, d.nf) + 1);
+    for (int e0 = 0; e0 < d.ne; e0 += qb) {
+      int eb = std::min(qb, d.ne - e0);
+      EXA_HIP(hipMemsetAsync(qaug.p, 0, static_cast<size_t>(eb) * per * sizeof(double), s));
+      qsetup(d, e0, eb, shapegt.p, shapegw.p, shapfgt.p, shapfgw.p, perm.p, egeo.p, fgeo.p, qaug.p, s);
+      // keep the mass matrix before it is factorised
+      for (int e = 0; e < eb; ++e)
+        EXA_HIP(hipMemcpyAsync(Mass.p + static_cast<long>(e0 + e) * d.npe * d.npe, qaug.p + e * per,
+                               sizeof(double) * d.npe * d.npe, hipMemcpyDeviceToDevice, s));
+      lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s);
+      qpack(d, e0, eb, qaug.p, MCs.p, MEs.p, Mass.p, s);
+    }
+    EXA_HIP(hipStreamSynchronize(s));
+  }
+
+  // persistent system arrays
+  H.alloc(static_cast<size_t>(d.ne) * d.m * d.m);
+  DinvF.alloc(static_cast<size_t>(d.ne) * d.n * d.m);
+  RuS.alloc(static_cast<size_t>(d.ne) * d.n);
+  RuR.alloc(static_cast<size_t>(d.ne) * d.n);
+  Rh.alloc(static_cast<size_t>(d.ne) * d.m);
+  b.alloc(d.N);
+  bres.alloc(d.N);
+  we.alloc(static_cast<size_t>(d.ne) * d.m);
+  du.alloc(static_cast<size_t>(d.ne) * d.n);
+  BE.alloc(static_cast<size_t>(d.nf) * d.ncf * d.ncf);
+  bjaug.alloc(static_cast<size_t>(d.nf) * d.ncf * 2 * d.ncf);
+  scal.alloc(8);
+  x.alloc(d.N);
+  r.alloc(d.N);
+  tmp.alloc(d.N);
+  wv.alloc(d.N);
+  alloc_workspace();
+}
+
+void Solver::alloc_workspace() {
+  const long n = d.n, m = d.m;
+  long per = n * (n + m + 1) + n * n * d.nd + m * n + m * n * d.nd +
+             static_cast<long>(d.nd + 1) * d.ncu * d.nge + static_cast<long>(d.nd + 1) * d.ncu * d.nc * d.nge +
+             static_cast<long>(d.npe) * d.nge * d.ncu * d.nc +
+             2L * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf;
+  long cap = static_cast<long>(ws_budget_bytes / 8.0 / per);
+  neb = static_cast<int>(std::max(1L, std::min<long>(d.ne, cap)));
+  aug.alloc(static_cast<size_t>(neb) * n * (n + m + 1));
+  Bfull.alloc(static_cast<size_t>(neb) * n * n * d.nd);
+  Kt.alloc(static_cast<size_t>(neb) * m * n);
+  Gfull.alloc(static_cast<size_t>(neb) * m * n * d.nd);
+  rv.alloc(static_cast<size_t>(neb) * (d.nd + 1) * d.ncu * d.nge);
+  rJ.alloc(static_cast<size_t>(neb) * (d.nd + 1) * d.ncu * d.nc * d.nge);
+  W.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
+  fpt.alloc(static_cast<size_t>(neb) * 2 * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf);
+  V.alloc(static_cast<size_t>(restart + 1) * d.N);
+  hcol.alloc(restart + 2);
+  ydev.alloc(restart + 2);
+  partial.alloc(static_cast<size_t>(dot_partials_count(d.N)) * (restart + 2));
+}
+
+void Solver::set_option(const std::string& name, double v) {
+  if (name == "gmres_tol") gmres_tol = v;
+  else if (name == "gmres_restart") {
+    if (v < 1 || v > 2000) throw Error(1, "gmres_restart out of range");
+    restart = static_cast<int>(v);
+    V.alloc(static_cast<size_t>(restart + 1) * d.N);
+    hcol.alloc(restart + 2);
+    ydev.alloc(restart + 2);
+    partial.alloc(static_cast<size_t>(dot_partials_count(d.N)) * (restart + 2));
+  } else if (name == "gmres_maxit") gmres_maxit = static_cast<int>(v);
+  else if (name == "newton_tol") newton_tol = v;
+  else if (name == "newton_maxit") newton_maxit = static_cast<int>(v);
+  else if (name == "preconditioner") {
+    if (static_cast<int>(v) != 0) throw Error(5, "only block-Jacobi (0) is implemented");
+    precond = static_cast<int>(v);
+  } else if (name == "ortho") ortho = static_cast<int>(v);
+  else if (name == "verbose") verbose = static_cast<int>(v);
+  else if (name == "timing") timing = v != 0.0;
+  else if (name == "workspace_bytes") {
+    ws_budget_bytes = v;
+    alloc_workspace();
+  } else throw Error(1, "unknown option '" + name + "'");
+}
+
+AsmArgs Solver::asm_args(int e0, int eb) {
+  AsmArgs a{};
+  a.model = model;
+  a.time = 0.0;
+  a.param = param.p;
+  a.tau = tau.p;
+  a.tau0 = app.tau[0];
+  a.e0 = e0;
+  a.eb = eb;
+  a.shapegt = shapegt.p;
+  a.shapegw = shapegw.p;
+  a.shapfgt = shapfgt.p;
+  a.shapfgw = shapfgw.p;
+  a.perm = perm.p;
+  a.elemcon = elemcon.p;
+  a.bf = bf.p;
+  a.udg = udg.p;
+  a.uh = uh.p;
+  a.egeo = egeo.p;
+  a.fgeo = fgeo.p;
+  a.og = og.p;
+  a.ogf = ogf.p;
+  a.rv = rv.p;
+  a.rJ = rJ.p;
+  a.W = W.p;
+  a.fpt = fpt.p;
+  return a;
+}
+
+void Solver::tic(int cls, hipEvent_t* a) {
+  if (!timing) return;
+  EXA_HIP(hipEventCreate(a));
+  EXA_HIP(hipEventRecord(*a, s));
+  (void)cls;
+}
+
+void Solver::toc(int cls, hipEvent_t a) {
+  if (!timing) return;
+  hipEvent_t b;
+  EXA_HIP(hipEventCreate(&b));
+  EXA_HIP(hipEventRecord(b, s));
+  ev[cls].push_back({a, b});
+}
+
+void Solver::collect_timings() {
+  EXA_HIP(hipStreamSynchronize(s));
+  for (int c = 0; c < T_NCLASS; ++c) {
+    for (auto& p : ev[c]) {
+      float t = 0.f;
+      EXA_HIP(hipEventElapsedTime(&t, p.first, p.second));
+      ms[c] += t;
+      cnt[c] += 1;
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+    ev[c].clear();
+  }
+}
+
+int Solver::timings(double* out_ms, long* counts, int len) {
+  collect_timings();
+  int k = std::min(len, static_cast<int>(T_NCLASS));
+  for (int c = 0; c < k; ++c) {
+    if (out_ms) out_ms[c] = ms[c];
+    if (counts) counts[c] = cnt[c];
+  }
+  return T_NCLASS;
+}
+
+void Solver::reset_timings() {
+  collect_timings();
+  ms.fill(0.0);
+  cnt.fill(0);
+}
+
+double Solver::norm(const double* v, long n) {
+  dot(v, v, n, partial.p, scal.p, 1, s);
+  EXA_HIP(hipMemcpyAsync(hpin, scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  return hpin[0];
+}
+
+void Solver::get_q() { getq(d, MCs.p, MEs.p, elemcon.p, uh.p, udg.p, s); }
+
+void Solver::assemble() {
+  const long n = d.n, m = d.m, nd = d.nd, ncu = d.ncu, npe = d.npe, ndf = d.ndf;
+  const long augs = n * (n + m + 1);
+  const long blk = static_cast<long>(npe) * d.nge;
+  hipEvent_t t0{};
+  for (int e0 = 0; e0 < d.ne; e0 += neb) {
+    const int eb = std::min(neb, d.ne - e0);
+    AsmArgs a = asm_args(e0, eb);
+    tic(T_ASSEMBLY, &t0);
+    EXA_HIP(hipMemsetAsync(aug.p, 0, sizeof(double) * augs * eb, s));
+    EXA_HIP(hipMemsetAsync(Kt.p, 0, sizeof(double) * m * n * eb, s));
+    EXA_HIP(hipMemsetAsync(Gfull.p, 0, sizeof(double) * m * n * nd * eb, s));
+    EXA_HIP(hipMemsetAsync(H.p + e0 * m * m, 0, sizeof(double) * m * m * eb, s));
+    // volume terms
+    vol_points(d, a, true, s);
+    vol_res(d, a, aug.p + n * (n + m), augs, s);
+    vol_W(d, a, s);
+    {  // D block: A_kc = -W_kc * Phi for the u components
+      Gemm g{};
+      g.M = npe; g.N = npe; g.K = d.nge; g.alpha = -1.0; g.beta = 0.0;
+      g.A = W.p; g.lda = npe; g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
+      g.B = shapegt.p; g.ldb = d.nge; g.sB0 = g.sB1 = g.sB2 = 0;
+      g.C = aug.p; g.ldc = n; g.sC0 = npe; g.sC1 = n * npe; g.sC2 = augs;
+      g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
+      gemm(g, s);
+    }
+    for (int dd = 0; dd < nd; ++dd) {  // B_d blocks for the q components
+      Gemm g{};
+      g.M = npe; g.N = npe; g.K = d.nge; g.alpha = -1.0; g.beta = 0.0;
+      g.A = W.p + (ncu + ncu * dd) * ncu * blk; g.lda = npe;
+      g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
+      g.B = shapegt.p; g.ldb = d.nge;
+      g.C = Bfull.p + n * npe * dd; g.ldc = n; g.sC0 = npe; g.sC1 = n * npe * nd; g.sC2 = n * n * nd;
+      g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
+      gemm(g, s);
+    }
+    // face terms, one local face at a time (no two writes of a launch collide)
+    face_points(d, a, true, s);
+    for (int l = 0; l < d.nfe; ++l) {
+      face_res(d, a, l, true, aug.p + n * (n + m), augs, Rh.p, s);
+      face_jac(d, a, l, aug.p, Bfull.p, Kt.p, Gfull.p, H.p, s);
+    }
+    // eliminate q: D += B*MinvC, F -= B*MinvE, K += G*MinvC, H -= G*MinvE (uequation.cpp:614-681)
+    {
+      Gemm g{};
+      g.M = n; g.N = npe; g.K = npe * nd; g.alpha = 1.0; g.beta = 1.0;
+      g.A = Bfull.p; g.lda = n; g.sA0 = n * npe * nd; g.sA1 = n * n * nd;
+      g.B = MCs.p + e0 * npe * nd * npe; g.ldb = npe * nd; g.sB0 = 0; g.sB1 = npe * nd * npe;
+      g.C = aug.p; g.ldc = n; g.sC0 = n * npe; g.sC1 = augs;
+      g.nb0 = static_cast<int>(ncu); g.nb1 = eb; g.nb2 = 1;
+      gemm(g, s);
+      g.N = static_cast<int>(ndf); g.alpha = -1.0;
+      g.B = MEs.p + e0 * npe * nd * ndf; g.sB1 = npe * nd * ndf;
+      g.C = aug.p + n * n; g.ldc = n * ncu; g.sC0 = n; g.sC1 = augs;
+      gemm(g, s);
+      Gemm k{};
+      k.M = m; k.N = npe; k.K = npe * nd; k.alpha = 1.0; k.beta = 1.0;
+      k.A = Gfull.p; k.lda = m; k.sA0 = m * npe * nd; k.sA1 = m * n * nd;
+      k.B = MCs.p + e0 * npe * nd * npe; k.ldb = npe * nd; k.sB0 = 0; k.sB1 = npe * nd * npe;
+      k.C = Kt.p; k.ldc = m; k.sC0 = m * npe; k.sC1 = m * n;
+      k.nb0 = static_cast<int>(ncu); k.nb1 = eb; k.nb2 = 1;
+      gemm(k, s);
+      k.N = static_cast<int>(ndf); k.alpha = -1.0;
+      k.B = MEs.p + e0 * npe * nd * ndf; k.sB1 = npe * nd * ndf;
+      k.C = H.p + e0 * m * m; k.ldc = m * ncu; k.sC0 = m; k.sC1 = m * m;
+      gemm(k, s);
+    }
+    toc(T_ASSEMBLY, t0);
+    // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru]
+    tic(T_LU, &t0);
+    lu_solve(aug.p, static_cast<int>(n), static_cast<int>(n + m + 1), augs, eb, info.p, s);
+    toc(T_LU, t0);
+    tic(T_ASSEMBLY, &t0);
+    {  // H += K * DinvF (= -K X_F), Rh -= K * (D^{-1} Ru)
+      Gemm g{};
+      g.M = m; g.N = m; g.K = n; g.alpha = -1.0; g.beta = 1.0;
+      g.A = Kt.p; g.lda = m; g.sA0 = m * n;
+      g.B = aug.p + n * n; g.ldb = n; g.sB0 = augs;
+      g.C = H.p + e0 * m * m; g.ldc = m; g.sC0 = m * m;
+      g.nb0 = eb; g.nb1 = 1; g.nb2 = 1;
+      gemm(g, s);
+      g.N = 1;
+      g.B = aug.p + n * (n + m);
+      g.C = Rh.p + e0 * m; g.sC0 = m;
+      gemm(g, s);
+    }
+    post_schur(d, e0, eb, aug.p, DinvF.p, RuS.p, s);
+    toc(T_ASSEMBLY, t0);
+    EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+    const int* hinfo = reinterpret_cast<const int*>(hpin);
+    for (int e = 0; e < std::min(eb, 8192); ++e)
+      if (hinfo[e]) throw Error(4, "singular element matrix D at element " + std::to_string(e0 + e));
+  }
+  // hdgAssembleRHS (matvec.cpp:103-117)
+  face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
+  // hdgBlockJacobi (matvec.cpp:190-217)
+  tic(T_LU, &t0);
+  bj_assemble(d, f2e.p, elemcon.p, H.p, bjaug.p, s);
+  lu_solve(bjaug.p, d.ncf, 2 * d.ncf, 2L * d.ncf * d.ncf, d.nf, info.p, s);
+  bj_extract(d, bjaug.p, BE.p, s);
+  toc(T_LU, t0);
+  assembled = true;
+}
+
+void Solver::residual(double* nb, double* nru) {
+  const long n = d.n;
+  for (int e0 = 0; e0 < d.ne; e0 += neb) {
+    const int eb = std::min(neb, d.ne - e0);
+    AsmArgs a = asm_args(e0, eb);
+    vol_points(d, a, false, s);
+    vol_res(d, a, RuR.p + e0 * n, n, s);
+    face_points(d, a, false, s);
+    for (int l = 0; l < d.nfe; ++l) face_res(d, a, l, false, RuR.p + e0 * n, n, Rh.p, s);
+  }
+  face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, bres.p, s);
+  double a1 = norm(bres.p, d.N);
+  double a2 = norm(RuR.p, static_cast<long>(d.ne) * n);
+  if (nb) *nb = a1;
+  if (nru) *nru = a2;
+}
+
+void Solver::matvec(const double* v, double* w) {
+  hipEvent_t t0{};
+  tic(T_MATVEC, &t0);
+  matvec_elem(d, H.p, elemcon.p, v, we.p, s);
+  toc(T_MATVEC, t0);
+  tic(T_OTHER, &t0);
+  face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, w, s);
+  toc(T_OTHER, t0);
+}
+
+void Solver::apply_precond(double* xv) {
+  hipEvent_t t0{};
+  tic(T_PRECOND, &t0);
+  copy(tmp.p, xv, d.N, s);
+  bj_apply(d, BE.p, tmp.p, xv, s);
+  toc(T_PRECOND, t0);
+}
+
+int Solver::gmres(const double* bv, double* xv) {
+  const long N = d.N;
+  double nrmb = norm(bv, N);
+  EXA_HIP(hipMemsetAsync(xv, 0, sizeof(double) * N, s));
+  if (nrmb < newton_tol) return 0;
+  copy(r.p, bv, N, s);
+  double nrmr = nrmb;
+  double tol = std::min(0.1, gmres_tol * nrmb / nrmr);
+  apply_precond(r.p);
+  nrmb = norm(r.p, N);
+  nrmr = nrmb;
+  const int n1 = restart + 1;
+  std::vector<double> Hm(static_cast<size_t>(n1) * n1, 0.0), sv(n1 + 1), cs(n1 + 1), sn(n1 + 1),
+      y(n1 + 1), Hc(n1 + 1);
+  int j = 0;
+  hipEvent_t t0{};
+  while (j < gmres_maxit) {
+    axpby(V.p, r.p, 1.0 / nrmr, 0.0, N, s);
+    std::fill(sv.begin(), sv.end(), 0.0);
+    sv[0] = nrmr;
+    int i = 0;
+    for (; i < restart && j < gmres_maxit; ++i, ++j) {
+      double* vi = V.p + N * i;
+      double* vm = V.p + N * (i + 1);
+      matvec(vi, vm);
+      apply_precond(vm);
+      tic(T_ORTHO, &t0);
+      if (ortho == 0) {  // MGS (getpoly.cpp:24-42)
+        for (int k = 0; k <= i; ++k) {
+          dot(vm, V.p + N * k, N, partial.p, hcol.p + k, 0, s);
+          axpy_dev(vm, V.p + N * k, hcol.p + k, -1.0, N, s);
+        }
+      } else {  // CGS (gmres.cpp:81-88)
+        multidot(V.p, N, i + 1, vm, N, partial.p, hcol.p, s);
+        gemv_cols(vm, V.p, N, i + 1, hcol.p, -1.0, N, s);
+      }
+      dot(vm, vm, N, partial.p, hcol.p + i + 1, 1, s);
+      scale_inv_dev(vm, hcol.p + i + 1, N, s);
+      toc(T_ORTHO, t0);
+      EXA_HIP(hipMemcpyAsync(hpin, hcol.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
+      EXA_HIP(hipStreamSynchronize(s));
+      for (int k = 0; k < i + 2; ++k) Hc[k] = hpin[k];
+      givens(Hc.data(), sv.data(), cs.data(), sn.data(), i);
+      for (int k = 0; k < n1; ++k) Hm[k + static_cast<size_t>(n1) * i] = (k < i + 2) ? Hc[k] : 0.0;
+      double rel = std::fabs(sv[i + 1]) / nrmb;
+      if (verbose > 1) std::printf("GMRES %d rel %g tol %g\n", j, rel, tol);
+      if (rel < tol) {
+        back_solve(y.data(), Hm.data(), sv.data(), i, n1);
+        std::memcpy(hpin, y.data(), sizeof(double) * (i + 1));
+        EXA_HIP(hipMemcpyAsync(ydev.p, hpin, sizeof(double) * (i + 1), hipMemcpyHostToDevice, s));
+        gemv_cols(xv, V.p, N, i + 1, ydev.p, 1.0, N, s);
+        EXA_HIP(hipStreamSynchronize(s));
+        return j + 1;
+      }
+    }
+    back_solve(y.data(), Hm.data(), sv.data(), i - 1, n1);
+    std::memcpy(hpin, y.data(), sizeof(double) * i);
+    EXA_HIP(hipMemcpyAsync(ydev.p, hpin, sizeof(double) * i, hipMemcpyHostToDevice, s));
+    gemv_cols(xv, V.p, N, i, ydev.p, 1.0, N, s);
+    matvec(xv, r.p);
+    axpby(r.p, bv, 1.0, -1.0, N, s);  // r = b - A x
+    apply_precond(r.p);
+    nrmr = norm(r.p, N);
+    if (nrmr / nrmb < tol) return j;
+  }
+  if (verbose) std::printf("Warning: GMRES(%d) does not converge to the tolerance %g within %d iterations\n",
+                           restart, tol, gmres_maxit);
+  return j;
+}
+
+void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
+  const long N = d.N, nU = static_cast<long>(d.n) * d.ne;
+  get_q();
+  double nb = 0, nru = 0;
+  residual(&nb, &nru);
+  double nrmr = nb + nru;
+  if (verbose) std::printf("Newton Iteration: 0,  Residual Norm: %g\n", nrmr);
+  int it = 0, gtot = 0;
+  DevBuf<double> u;
+  u.alloc(N);
+  copy(u.p, uh.p, N, s);
+  for (it = 0; it < newton_maxit; ++it) {
+    assemble();
+    int gi = gmres(b.p, x.p);
+    gtot += gi;
+    double alpha = 1.0;
+    axpby(u.p, x.p, 1.0, 1.0, N, s);  // u += x
+    copy(uh.p, u.p, N, s);
+    dudg(d, RuS.p, DinvF.p, elemcon.p, x.p, du.p, s);
+    update_udg(d, du.p, alpha, udg.p, s);
+    get_q();
+    double nrm0 = nrmr;
+    residual(&nb, &nru);
+    nrmr = nb + nru;
+    if (nrmr > nrm0 && nrmr > 1e6) throw Error(4, "Residual norm increases more than 1e6");
+    while ((nrmr > nrm0 && alpha > 0.1) || std::isnan(nrmr)) {
+      if (verbose) std::printf("Newton Iteration: %d, Alpha: %g, Original Norm: %g,  Updated Norm: %g\n",
+                               it + 1, alpha, nrm0, nrmr);
+      alpha /= 2.0;
+      axpby(u.p, x.p, -alpha, 1.0, N, s);
+      copy(uh.p, u.p, N, s);
+      update_udg(d, du.p, -alpha, udg.p, s);
+      get_q();
+      residual(&nb, &nru);
+      nrmr = nb + nru;
+    }
+    if (std::isnan(nrmr)) throw Error(4, "NaN residual");
+    if (verbose)
+      std::printf("Newton Iteration: %d, Alpha: %g, Original Norm: %g,  Updated Norm: %g  (GMRES %d)\n",
+                  it + 1, alpha, nrm0, nrmr, gi);
+    if (nrmr < newton_tol) {
+      ++it;
+      break;
+    }
+  }
+  (void)nU;
+  EXA_HIP(hipStreamSynchronize(s));
+  if (newton_iters) *newton_iters = it;
+  if (gmres_total) *gmres_total = gtot;
+  if (final_norm) *final_norm = nrmr;
+}
+
+void Solver::write_output() {
+  if (dataout.empty()) throw Error(1, "no dataout prefix was given");
+  std::vector<double> hu(static_cast<size_t>(d.npe) * d.nc * d.ne), hh(d.N);
+  EXA_HIP(hipMemcpyAsync(hu.data(), udg.p, hu.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipMemcpyAsync(hh.data(), uh.p, hh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  ::exa::write_output(dataout + "udg_np0.bin", d.npe, d.nc, d.ne, hu);
+  ::exa::write_output(dataout + "uhat_np0.bin", d.ncu, d.npf, d.nf, hh);
+}
+
+long Solver::get_array(const std::string& name, double* dst, long len) {
+  const double* src = nullptr;
+  long size = 0;
+  std::vector<double> host;
+  auto from_int = [&](const std::vector<int>& v) {
+    host.assign(v.begin(), v.end());
+    size = static_cast<long>(host.size());
+  };
+  if (name == "udg") { src = udg.p; size = static_cast<long>(udg.n); }
+  else if (name == "uh") { src = uh.p; size = static_cast<long>(uh.n); }
+  else if (name == "H") { src = H.p; size = static_cast<long>(H.n); }
+  else if (name == "DinvF") { src = DinvF.p; size = static_cast<long>(DinvF.n); }
+  else if (name == "Ru") { src = RuS.p; size = static_cast<long>(RuS.n); }
+  else if (name == "Rh") { src = Rh.p; size = static_cast<long>(Rh.n); }
+  else if (name == "b") { src = b.p; size = static_cast<long>(b.n); }
+  else if (name == "BE") { src = BE.p; size = static_cast<long>(BE.n); }
+  else if (name == "Mass") { src = Mass.p; size = static_cast<long>(Mass.n); }
+  else if (name == "MinvC") { src = MCs.p; size = static_cast<long>(MCs.n); }
+  else if (name == "MinvE") { src = MEs.p; size = static_cast<long>(MEs.n); }
+  else if (name == "res_b") { src = bres.p; size = static_cast<long>(bres.n); }
+  else if (name == "res_Ru") { src = RuR.p; size = static_cast<long>(RuR.n); }
+  else if (name == "x") { src = x.p; size = static_cast<long>(x.n); }
+  else if (name == "w") { src = wv.p; size = static_cast<long>(wv.n); }
+  else if (name == "egeo") { src = egeo.p; size = static_cast<long>(egeo.n); }
+  else if (name == "fgeo") { src = fgeo.p; size = static_cast<long>(fgeo.n); }
+  else if (name == "elemcon") from_int(conn.elemcon);
+  else if (name == "f2e") from_int(conn.f2e);
+  else throw Error(1, "unknown array '" + name + "'");
+  if (!dst) return size;
+  long k = std::min(len, size);
+  if (src) {
+    EXA_HIP(hipMemcpyAsync(dst, src, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+  } else {
+    std::memcpy(dst, host.data(), sizeof(double) * k);
+  }
+  return size;
+}
+
+void Solver::set_array(const std::string& name, const double* src, long len) {
+  DevBuf<double>* t = nullptr;
+  if (name == "udg") t = &udg;
+  else if (name == "uh") t = &uh;
+  else if (name == "x") t = &x;
+  else if (name == "w") t = &wv;
+  else throw Error(1, "array '" + name + "' cannot be set");
+  if (len != static_cast<long>(t->n)) throw Error(1, "wrong length for '" + name + "'");
+  EXA_HIP(hipMemcpyAsync(t->p, src, sizeof(double) * len, hipMemcpyHostToDevice, s));
+  EXA_HIP(hipStreamSynchronize(s));
+}
+
+void Solver::device_ptr(const std::string& name, void** p, long* len) {
+  DevBuf<double>* t = nullptr;
+  if (name == "H") t = &H;
+  else if (name == "DinvF") t = &DinvF;
+  else if (name == "uh") t = &uh;
+  else if (name == "udg") t = &udg;
+  else if (name == "b") t = &b;
+  else if (name == "x") t = &x;
+  else if (name == "w") t = &wv;
+  else throw Error(1, "no device pointer for '" + name + "'");
+  *p = t->p;
+  if (len) *len = static_cast<long>(t->n);
+}
+
+}  // namespace exa

@@ -1,0 +1,83 @@
+// solver.hpp -- the solver handle behind the C ABI: device-resident HDG state of
+// one problem and the host control of the Newton-GMRES loop.
+#pragma once
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <array>
+#include <string>
+#include <vector>
+
+namespace exa {
+
+enum TimerClass { T_MATVEC = 0, T_PRECOND = 1, T_ORTHO = 2, T_ASSEMBLY = 3, T_LU = 4, T_OTHER = 5, T_NCLASS = 6 };
+
+struct Solver {
+  // problem description
+  std::string datain, dataout;
+  int model = 0;
+  int device = 0;
+  Dims d{};
+  AppData app;
+  MasterData master;
+  MeshData mesh;
+  Conn conn;
+  hipStream_t s = nullptr;
+
+  // options (app.bin defaults, overridable)
+  int precond = 0, ortho = 0, restart = 50, gmres_maxit = 200, newton_maxit = 20, verbose = 0;
+  double gmres_tol = 1e-8, newton_tol = 1e-6;
+  bool timing = false;
+  double ws_budget_bytes = 4.0e9;
+
+  // device data
+  DevBuf<double> shapegt, shapegw, shapfgt, shapfgw, param, tau;
+  DevBuf<int> perm, elemcon, bf, f2e, info;
+  DevBuf<long> src1, src2;
+  DevBuf<double> xdg, egeo, fgeo, odg, og, ogf;
+  DevBuf<double> udg, uh, MCs, MEs, Mass;
+  DevBuf<double> H, DinvF, RuS, Rh, b, BE, bjaug, we, du, RuR, bres;
+  // per-block workspace
+  int neb = 0;
+  DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
+  // GMRES
+  DevBuf<double> V, x, r, tmp, wv, partial, hcol, ydev, scal;
+  double* hpin = nullptr;  // pinned host scratch for small device->host copies
+  bool assembled = false;
+
+  // timing
+  std::array<std::vector<std::pair<hipEvent_t, hipEvent_t>>, T_NCLASS> ev;
+  std::array<double, T_NCLASS> ms{};
+  std::array<long, T_NCLASS> cnt{};
+
+  Solver(const std::string& datain, const std::string& dataout, const std::string& model, int device);
+  ~Solver();
+
+  void set_option(const std::string& name, double value);
+  // stages
+  void get_q();
+  void assemble();
+  void residual(double* nb, double* nru);
+  void matvec(const double* v, double* w);
+  void apply_precond(double* x);
+  int gmres(const double* bvec, double* xvec);
+  void solve(int* newton_iters, int* gmres_iters, double* final_norm);
+  void write_output();
+  long get_array(const std::string& name, double* dst, long len);
+  void set_array(const std::string& name, const double* src, long len);
+  void device_ptr(const std::string& name, void** p, long* len);
+  int timings(double* out_ms, long* counts, int len);
+  void reset_timings();
+
+ private:
+  void setup();
+  void alloc_workspace();
+  AsmArgs asm_args(int e0, int eb);
+  double norm(const double* x, long n);
+  void tic(int cls, hipEvent_t* a);
+  void toc(int cls, hipEvent_t a);
+  void collect_timings();
+};
+
+}  // namespace exa

@@ -1,0 +1,112 @@
+/*
+ * exasim_hdg.h -- C ABI of libexahip.so, the MI355X-native HDG Newton-GMRES path.
+ *
+ * The library reads the reference's datain/ layout (app.bin, master.bin,
+ * mesh.bin, sol.bin), runs the HDG Newton-GMRES solve on one GPU and writes
+ * the reference's dataout/ layout (out{udg,uhat}_np0.bin).  All arguments are
+ * plain pointers and sizes; host buffers are host memory, device buffers are
+ * device memory of the handle's GPU.  Every entry point returns 0 on success
+ * and a non-zero EXA_ERR_* code on failure; exa_last_error() then describes
+ * the failure.  Calls on one handle must not overlap; different handles are
+ * independent.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   exa_create     backend/Main/exasim.hpp:281-370 (t2c parse + CSolution ctor):
+ *                  readInput (backend/Discretization/readbinaryfiles.cpp:625-699),
+ *                  buildConn (backend/Discretization/connectivity.cpp:1212-1385),
+ *                  compGeometry + qEquation (backend/Discretization/discretization.cpp:170-412)
+ *   exa_solve      backend/Solution/solution.cpp:702-720 (SolveProblem -> SteadyProblem
+ *                  -> NewtonSolver, solution.cpp:125-242)
+ *   exa_write_output backend/Solution/solution.cpp:726-760 (SaveSolutions)
+ *   exa_assemble   backend/Discretization/discretization.cpp:483-512 (hdgAssembleLinearSystem)
+ *   exa_residual   backend/Discretization/discretization.cpp:576-591 (hdgAssembleResidual)
+ *   exa_matvec     backend/Discretization/matvec.cpp:240-385 (hdgMatVec)
+ *   exa_precond    backend/Preconditioning/preconditioner.cpp:273-325 (ApplyPreconditioner)
+ *   exa_gmres      backend/Solver/gmres.cpp:361-597 (GMRES, HDG overload)
+ *   exa_get_array  the debugmode=1 stage dumps (uequation.cpp:182-202, 542-556, 683-774;
+ *                  qequation.cpp:152-165; ptcsolver.cpp:363-371)
+ *   model names    the generated model plugin (backend/Model/libpdemodel.hpp:1-41) is
+ *                  selected by name; see exasim_amd/csrc/models.hpp.
+ */
+#ifndef EXASIM_HDG_H
+#define EXASIM_HDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct exa_handle exa_handle;
+
+enum {
+  EXA_OK = 0,
+  EXA_ERR_ARG = 1,        /* bad argument (null pointer, wrong size, unknown name) */
+  EXA_ERR_IO = 2,         /* datain/dataout file missing or malformed */
+  EXA_ERR_HIP = 3,        /* a HIP runtime call failed */
+  EXA_ERR_NUMERIC = 4,    /* singular element/face block, NaN residual, divergence */
+  EXA_ERR_UNSUPPORTED = 5 /* option the library does not implement */
+};
+
+/* Build a solver from datain_dir/{app,master,mesh,sol}.bin for the named model
+ * ("poisson", "poisson_poly", "convdiff").  dataout_prefix is the prefix of the
+ * output files (e.g. "dataout/out"); it may be NULL.  device < 0 selects the
+ * current HIP device. */
+int exa_create(const char* datain_dir, const char* dataout_prefix, const char* model,
+               int device, exa_handle** out);
+void exa_destroy(exa_handle* h);
+const char* exa_last_error(const exa_handle* h);
+
+/* Sizes: writes up to 16 ints: nd, ncu, nc, npe, npf, nfe, ne, nf, n, m, N (= ndof uhat),
+ * nge, ngf, nco, restart, preconditioner. Returns the count written. */
+int exa_sizes(const exa_handle* h, int* out, int len);
+
+/* Options (name, value): "gmres_tol", "gmres_restart", "gmres_maxit", "newton_tol",
+ * "newton_maxit", "preconditioner" (0 block-Jacobi), "ortho" (0 MGS, 1 CGS),
+ * "verbose", "timing" (1: record per-kernel HIP events). */
+int exa_set_option(exa_handle* h, const char* name, double value);
+
+/* Full steady solve (NewtonSolver).  newton_iters/gmres_iters/final_norm may be NULL. */
+int exa_solve(exa_handle* h, int* newton_iters, int* gmres_iters, double* final_norm);
+
+/* Stage entry points. */
+int exa_get_q(exa_handle* h);                 /* hdgGetQ on the current (udg, uh) */
+int exa_assemble(exa_handle* h);              /* H, DinvF, Ru, Rh, b and the preconditioner */
+int exa_residual(exa_handle* h, double* norm_b, double* norm_ru);
+/* w = A v (device pointers, length N). */
+int exa_matvec(exa_handle* h, const double* v_dev, double* w_dev);
+/* x = P x in place (device pointer, length N). */
+int exa_precond(exa_handle* h, double* x_dev);
+/* Solve A x = b with the handle's assembled system (device pointers); x starts at 0. */
+int exa_gmres(exa_handle* h, const double* b_dev, double* x_dev, int* iters);
+
+/* Copy a named array to host memory (len = number of doubles available at dst).
+ * Names: "udg" [npe,nc,ne], "uh" [ncu,npf,nf], "H" [m,m,ne], "DinvF" [n,m,ne],
+ * "Ru" [n,ne], "Rh" [m,ne], "b" [N], "BE" (block-Jacobi inverse) [ncf,ncf,nf],
+ * "Mass" [npe,npe,ne], "MinvC" [npe*nd,npe,ne], "MinvE" [npe*nd,npe*... see docs],
+ * "elemcon" [npf*nfe*ne] (as doubles), "f2e" [4*nf] (as doubles), "res_b" [N],
+ * "res_Ru" [n,ne].  Returns the number of doubles of the array (copies min(len, size)). */
+long exa_get_array(exa_handle* h, const char* name, double* dst, long len);
+/* Overwrite "udg" or "uh" from host memory. */
+int exa_set_array(exa_handle* h, const char* name, const double* src, long len);
+/* Device pointer of a named persistent device array (H, DinvF, uh, udg, b). */
+int exa_device_ptr(exa_handle* h, const char* name, void** ptr, long* len);
+
+/* dataout/out{udg,uhat}_np0.bin with the reference's 3-double headers. */
+int exa_write_output(exa_handle* h);
+
+/* Per-kernel-class timings recorded with HIP events on the handle's stream
+ * while the "timing" option is on: for class index k, total milliseconds and
+ * launch count.  Classes: 0 matvec, 1 precond apply, 2 orthogonalisation,
+ * 3 assembly, 4 condensation (LU), 5 other.  Returns the number of classes. */
+int exa_timings(exa_handle* h, double* ms, long* counts, int len);
+int exa_reset_timings(exa_handle* h);
+
+/* Synchronise the handle's stream. */
+int exa_sync(exa_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

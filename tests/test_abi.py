@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every entry point include/exasim_hdg.h declares
+(no compute calls: these run without a GPU)."""
+import os
+import re
+
+import pytest
+
+from exasim_amd import hdg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "exasim_hdg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(exa_[a-z_]+)\s*\(", text)))
+
+
+def test_header_matches_ctypes_table():
+    assert declared_symbols() == sorted(n for n, _, _ in hdg.SIGNATURES)
+
+
+def test_library_exports_every_symbol():
+    if not os.path.exists(hdg.LIB_PATH):
+        pytest.skip("libexahip.so not built (run python -m exasim_amd.build)")
+    lib = hdg.load()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_create_reports_missing_input(tmp_path):
+    """exa_create fails with EXA_ERR_IO before touching a device when app.bin is missing."""
+    if not os.path.exists(hdg.LIB_PATH):
+        pytest.skip("libexahip.so not built")
+    lib = hdg.load()
+    import ctypes
+    h = ctypes.c_void_p()
+    rc = lib.exa_create(str(tmp_path).encode(), None, b"poisson", -2, ctypes.byref(h))
+    assert rc != 0
+    assert h.value is None

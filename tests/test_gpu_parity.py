@@ -1,0 +1,202 @@
+"""GPU parity: libexahip.so (through the C ABI) against the numpy oracle on the
+same datain inputs.
+
+Tolerances (FP64 path): integer connectivity bit-exact; element matrices and
+vectors 1e-10 relative to the array's max-norm; converged solutions 1e-10
+relative (GMRES tolerance 1e-12 on both sides).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import hdg as ora
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def gpu_solver(path, model):
+    from exasim_amd.hdg import HDGSolver
+    return HDGSolver(path, model)
+
+
+def as_elem_mats(flat, rows, cols, ne):
+    """[rows, cols, ne] column-major -> [e, row, col]."""
+    return flat.reshape(ne, cols, rows).transpose(0, 2, 1)
+
+
+CASES = {
+    "c1": ("poisson2d", dict(n=16, porder=3, pgauss=14), "poisson"),
+    "c1small": ("poisson2d", dict(n=4, porder=2, pgauss=6), "poisson"),
+    "poly2d": ("poisson2d", dict(n=3, porder=2, pgauss=6, model="poisson_poly", newton_iter=2), "poisson_poly"),
+    "poly3d": ("poisson2d", dict(n=2, porder=2, pgauss=6, model="poisson_poly", newton_iter=2, nd=3),
+               "poisson_poly"),
+    "c2r": ("convdiff3d", dict(n=2, porder=4, ncu=2, restart=100, gmres_tol=1e-12), "convdiff"),
+}
+
+
+@pytest.fixture(scope="module")
+def built(case_dir):
+    out = {}
+
+    def get(name):
+        if name not in out:
+            builder, kw, model = CASES[name]
+            path, info = case_dir(name, builder, **kw)
+            out[name] = (path, model)
+        return out[name]
+    return get
+
+
+@pytest.mark.parametrize("name", ["c1small", "c2r"])
+def test_connectivity_bit_exact(built, name):
+    path, model = built(name)
+    say("case %s: connectivity" % name)
+    P = ora.HDGProblem(path, model)
+    with gpu_solver(path, model) as S:
+        ne, nf = S.sizes["ne"], S.sizes["nf"]
+        f2e = S.get("f2e").astype(np.int64).reshape(nf, 4)
+        ec = S.get("elemcon").astype(np.int64).reshape(ne, S.sizes["nfe"], S.sizes["npf"])
+    assert np.array_equal(f2e, P.conn["f2e"])
+    assert np.array_equal(ec, P.elemcon)
+
+
+@pytest.mark.parametrize("name", ["c1small", "c2r"])
+def test_assembly_matches_oracle(built, name):
+    path, model = built(name)
+    say("case %s: assembly" % name)
+    P = ora.HDGProblem(path, model)
+    P.get_q(P.udg, P.uh)
+    E, S_, b = P.assemble()
+    BE = P.block_jacobi(S_["H"])
+    with gpu_solver(path, model) as S:
+        z = S.sizes
+        ne, n, m, nf, ncf = z["ne"], z["n"], z["m"], z["nf"], z["ncu"] * z["npf"]
+        S.get_q()
+        S.assemble()
+        Mass = as_elem_mats(S.get("Mass"), z["npe"], z["npe"], ne)
+        H = as_elem_mats(S.get("H"), m, m, ne)
+        DinvF = as_elem_mats(S.get("DinvF"), n, m, ne)
+        Ru = S.get("Ru").reshape(ne, n)
+        Rh = S.get("Rh").reshape(ne, m)
+        bg = S.get("b")
+        BEg = as_elem_mats(S.get("BE"), ncf, ncf, nf)
+        udg = S.get("udg").reshape(ne, z["nc"], z["npe"]).transpose(0, 2, 1)
+    assert rel(Mass, P.Mass) < 1e-12
+    assert rel(udg, P.udg) < RTOL          # q from hdgGetQ
+    assert rel(H, S_["H"]) < RTOL
+    assert rel(DinvF, S_["DinvF"]) < RTOL
+    assert rel(Ru, S_["Ru"]) < RTOL
+    assert rel(Rh, S_["Rh"]) < RTOL
+    assert rel(bg, b) < RTOL
+    assert rel(BEg, BE) < RTOL
+
+
+def test_assembly_is_bitwise_reproducible(built):
+    path, model = built("c1small")
+    say("case c1small: determinism")
+    with gpu_solver(path, model) as S:
+        S.get_q()
+        S.assemble()
+        H1, b1 = S.get("H"), S.get("b")
+        S.assemble()
+        H2, b2 = S.get("H"), S.get("b")
+    assert np.array_equal(H1, H2) and np.array_equal(b1, b2)
+
+
+@pytest.mark.parametrize("name", ["c1small", "c2r"])
+def test_matvec_precond_gmres(built, name):
+    path, model = built(name)
+    say("case %s: matvec/precond/gmres" % name)
+    P = ora.HDGProblem(path, model)
+    P.get_q(P.udg, P.uh)
+    E, S_, b = P.assemble()
+    BE = P.block_jacobi(S_["H"])
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal(b.size)
+    with gpu_solver(path, model) as S:
+        S.get_q()
+        S.assemble()
+        px, _ = S.device_ptr("x")
+        pw, _ = S.device_ptr("w")
+        S.set("x", v)
+        S.matvec(px, pw)
+        w = S.get("w")
+        S.set("w", v)
+        S.precond(pw)
+        pv = S.get("w")
+        pb, _ = S.device_ptr("b")
+        its = S.gmres(pb, px)
+        xg = S.get("x")
+    assert rel(w, P.matvec(S_["H"], v)) < RTOL
+    assert rel(pv, P.apply_block_jacobi(BE, v)) < RTOL
+    xo, _ = ora.gmres(lambda q: P.matvec(S_["H"], q), lambda r: P.apply_block_jacobi(BE, r), b,
+                      P.restart, P.gmres_maxit, P.gmres_tol, P.newton_tol, P.ortho)
+    assert its > 0
+    assert rel(xg, xo) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["c1", "c2r"])
+def test_solve_matches_oracle(built, name):
+    path, model = built(name)
+    say("case %s: full Newton solve" % name)
+    P = ora.HDGProblem(path, model)
+    ora.newton_solve(P)
+    with gpu_solver(path, model) as S:
+        it, gits, nrm = S.solve()
+        z = S.sizes
+        uh = S.get("uh")
+        udg = S.get("udg").reshape(z["ne"], z["nc"], z["npe"]).transpose(0, 2, 1)
+    assert rel(uh, P.uh.reshape(-1)) < RTOL
+    assert rel(udg, P.udg) < RTOL
+    if name == "c1":
+        xe = P.xdg.reshape(-1, 2)
+        assert np.abs(udg[:, :, 0].reshape(-1) - P.model.exact(xe)).max() < 1e-5
+
+
+@pytest.mark.parametrize("name", ["poly2d", "poly3d"])
+def test_polynomial_solution_is_exact(built, name):
+    """HDG with p = 2 reproduces u = |x|^2 (and q = -grad u) to round-off."""
+    path, model = built(name)
+    say("case %s: polynomial exactness" % name)
+    with gpu_solver(path, model) as S:
+        S.solve()
+        z = S.sizes
+        udg = S.get("udg").reshape(z["ne"], z["nc"], z["npe"]).transpose(0, 2, 1)
+    P = ora.HDGProblem(path, model)
+    nd = P.nd
+    x = P.xdg.reshape(-1, nd)
+    assert np.abs(udg[:, :, 0].reshape(-1) - np.sum(x * x, 1)).max() < 1e-10
+    for d in range(nd):
+        assert np.abs(udg[:, :, 1 + d].reshape(-1) + 2 * x[:, d]).max() < 1e-9
+
+
+def test_output_files(built, tmp_path):
+    path, model = built("c1small")
+    say("case c1small: dataout files")
+    from exasim_amd.datain import read_output
+    from exasim_amd.hdg import HDGSolver
+    prefix = str(tmp_path / "out")
+    with HDGSolver(path, model, dataout=prefix) as S:
+        S.solve()
+        S.write_output()
+        z = S.sizes
+        uh = S.get("uh")
+    hdr, data = read_output(prefix + "uhat_np0.bin")
+    assert list(hdr) == [z["ncu"], z["npf"], z["nf"]]
+    assert np.array_equal(data, uh)
+    hdr, data = read_output(prefix + "udg_np0.bin")
+    assert list(hdr) == [z["npe"], z["nc"], z["ne"]]

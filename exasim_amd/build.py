@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexahip.so")
-SOURCES = ["kernels.hip", "datain.cpp", "conn.cpp", "solver.cpp", "api.cpp"]
+SOURCES = ["kernels.hip", "datain.cpp", "conn.cpp", "bilu0.cpp", "solver.cpp", "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -87,6 +87,8 @@ int exa_solve(exa_handle* h, int* newton_iters, int* gmres_iters, double* final_
   return guard(h, [&](exa::Solver& s) { s.solve(newton_iters, gmres_iters, final_norm); });
 }
 
+int exa_reset(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.reset(); }); }
+
 int exa_get_q(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.get_q(); }); }
 
 int exa_assemble(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.assemble(); }); }

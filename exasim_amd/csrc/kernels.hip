@@ -878,6 +878,215 @@ __global__ void k_update_udg(Dims d, const double* du, double alpha, double* udg
 }
 
 // ---------------------------------------------------------------------------
+// Superelement block-ILU(0): AssembleBlockILU0 (kokkosimpl.h:1264-1334),
+// hdgBlockILU0 (matvec.cpp:119-166), ApplyBlockILU0 (preconditioner.cpp:184-270).
+// BE[((t*nse) + n)*ncf*ncf + r + ncf*c]: CRS entry t of superelement n.
+// ---------------------------------------------------------------------------
+__global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon, const double* H,
+                                double* BE) {
+  const int ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf, m = d.m;
+  const long per = static_cast<long>(ncf) * ncf;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * I.nse * I.nnz) return;
+  const int rc = static_cast<int>(q % per);
+  const long tn = q / per;
+  const int n = static_cast<int>(tn % I.nse);
+  const int t = static_cast<int>(tn / I.nse);
+  const int rr = rc % ncf, cc = rc / ncf;
+  const int am = rr % ncu, bm = rr / ncu, an = cc % ncu, bn = cc / ncu;
+  const int i = I.row_of[t];
+  const int fi = I.face[n + I.nse * i];
+  const int e1 = f2e[4 * fi], l1 = f2e[4 * fi + 1], e2 = f2e[4 * fi + 2], l2 = f2e[4 * fi + 3];
+  const int nfi = npf * fi;
+  double v;
+  if (t == I.row_ptr[i]) {
+    int m1 = am + ncu * (elemcon[bm + npf * l1 + ndf * e1] - nfi);
+    int n1 = an + ncu * (elemcon[bn + npf * l1 + ndf * e1] - nfi);
+    v = H[static_cast<long>(e1) * m * m + (m1 + ncf * l1) + static_cast<long>(m) * (n1 + ncf * l1)];
+    if (e2 >= 0) {
+      int m2 = am + ncu * (elemcon[bm + npf * l2 + ndf * e2] - nfi);
+      int n2 = an + ncu * (elemcon[bn + npf * l2 + ndf * e2] - nfi);
+      v += H[static_cast<long>(e2) * m * m + (m2 + ncf * l2) + static_cast<long>(m) * (n2 + ncf * l2)];
+    }
+  } else {
+    const int j = I.col_ind[t];
+    const int fj = I.face[n + I.nse * j];
+    const int je1 = f2e[4 * fj], je2 = f2e[4 * fj + 2];
+    int e, k1, k2;
+    if (je1 == e1) { e = e1; k1 = l1; k2 = f2e[4 * fj + 1]; }
+    else if (je1 == e2) { e = e2; k1 = l2; k2 = f2e[4 * fj + 1]; }
+    else if (je2 == e1) { e = e1; k1 = l1; k2 = f2e[4 * fj + 3]; }
+    else { e = e2; k1 = l2; k2 = f2e[4 * fj + 3]; }
+    int m1 = am + ncu * (elemcon[bm + npf * k1 + ndf * e] - nfi);
+    int n2 = an + ncu * (elemcon[bn + npf * k2 + ndf * e] - npf * fj);
+    v = H[static_cast<long>(e) * m * m + (m1 + ncf * k1) + static_cast<long>(m) * (n2 + ncf * k2)];
+  }
+  BE[q] = v;
+}
+
+// One workgroup per superelement: for each face slot i, invert the diagonal
+// block (Gauss-Jordan with partial pivoting in LDS), scale the blocks below it
+// and update the trailing blocks, in the order of the elimination lists.
+__global__ __launch_bounds__(256) void k_bilu_factor(Dims d, IluDev I, double* BE, int* info) {
+  const int ncf = d.ncf;
+  const int n = blockIdx.x;
+  const long per = static_cast<long>(ncf) * ncf;
+  extern __shared__ double sm[];
+  double* A = sm;                     // ncf x 2ncf, row-major: A[r*2ncf + c]
+  double* Lb = sm + 2 * per;          // ncf x ncf column-major
+  double* fac = Lb + per;             // ncf
+  __shared__ double redv[256];
+  __shared__ int redi[256];
+  const int tid = threadIdx.x;
+  const int w2 = 2 * ncf;
+  int singular = 0;
+  auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per; };
+  for (int i = 0; i < I.nfse; ++i) {
+    double* D = blk(I.diag[i]);
+    for (int idx = tid; idx < ncf * w2; idx += 256) {
+      int r = idx / w2, c = idx % w2;
+      A[idx] = (c < ncf) ? D[r + ncf * c] : ((c - ncf) == r ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int c = 0; c < ncf; ++c) {
+      double best = -1.0;
+      int bi = c;
+      for (int r = c + tid; r < ncf; r += 256) {
+        double vv = fabs(A[r * w2 + c]);
+        if (vv > best) { best = vv; bi = r; }
+      }
+      redv[tid] = best;
+      redi[tid] = bi;
+      __syncthreads();
+      for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) {
+          double o = redv[tid + st];
+          int oi = redi[tid + st];
+          if (o > redv[tid] || (o == redv[tid] && oi < redi[tid])) { redv[tid] = o; redi[tid] = oi; }
+        }
+        __syncthreads();
+      }
+      const int p = redi[0];
+      if (p != c)
+        for (int k = tid; k < w2; k += 256) {
+          double tt = A[c * w2 + k];
+          A[c * w2 + k] = A[p * w2 + k];
+          A[p * w2 + k] = tt;
+        }
+      __syncthreads();
+      const double pv = A[c * w2 + c];
+      if (pv == 0.0) singular = 1;
+      for (int k = tid; k < w2; k += 256) A[c * w2 + k] /= pv;
+      for (int r = tid; r < ncf; r += 256) fac[r] = A[r * w2 + c];
+      __syncthreads();
+      for (int idx = tid; idx < ncf * w2; idx += 256) {
+        int r = idx / w2, k = idx % w2;
+        if (r != c) A[idx] -= fac[r] * A[c * w2 + k];
+      }
+      __syncthreads();
+    }
+    // D <- D^{-1}
+    for (int idx = tid; idx < static_cast<int>(per); idx += 256) {
+      int r = idx % ncf, c = idx / ncf;
+      D[idx] = A[r * w2 + ncf + c];
+    }
+    __syncthreads();
+    for (int k = I.eptr[i]; k < I.eptr[i + 1]; ++k) {
+      double* Aji = blk(I.elim_ji[k]);
+      // L = A_ji * D^{-1}
+      for (int idx = tid; idx < static_cast<int>(per); idx += 256) {
+        int r = idx % ncf, c = idx / ncf;
+        double acc = 0.0;
+        for (int t = 0; t < ncf; ++t) acc = fma(Aji[r + ncf * t], A[t * w2 + ncf + c], acc);
+        Lb[idx] = acc;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < static_cast<int>(per); idx += 256) Aji[idx] = Lb[idx];
+      for (int u = I.uptr[k]; u < I.uptr[k + 1]; ++u) {
+        double* Ajl = blk(I.upd_jl[u]);
+        const double* Ail = blk(I.upd_il[u]);
+        for (int idx = tid; idx < static_cast<int>(per); idx += 256) {
+          int r = idx % ncf, c = idx / ncf;
+          double acc = Ajl[idx];
+          for (int t = 0; t < ncf; ++t) acc = fma(-Lb[r + ncf * t], Ail[t + ncf * c], acc);
+          Ajl[idx] = acc;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0 && info) info[n] = singular;
+}
+
+// One workgroup per superelement: forward sweep y_i = b_i - sum L_ij y_j, backward
+// sweep x_i = D_i^{-1} (y_i - sum U_il x_l).  Y, Xo are laid out [slot = n + nse*i][ncf].
+__global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const double* BE, const double* x,
+                                                    double* Y, double* Xo) {
+  const int ncf = d.ncf;
+  const int n = blockIdx.x;
+  const long per = static_cast<long>(ncf) * ncf;
+  const int tid = threadIdx.x;
+  const int nparts = max(1, 256 / ncf);
+  const int r = tid % ncf, part = tid / ncf;
+  __shared__ double red[256];
+  __shared__ double z[256];
+  auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per; };
+  for (int i = 0; i < I.nfse; ++i) {
+    double acc = 0.0;
+    if (part < nparts)
+      for (int q = I.lptr[i] + part; q < I.lptr[i + 1]; q += nparts) {
+        const double* B = blk(I.lidx[q]);
+        const double* yj = Y + (static_cast<long>(n) + static_cast<long>(I.nse) * I.lcol[q]) * ncf;
+        for (int c = 0; c < ncf; ++c) acc = fma(B[r + ncf * c], yj[c], acc);
+      }
+    red[tid] = acc;
+    __syncthreads();
+    if (tid < ncf) {
+      double s = x[static_cast<long>(I.face[n + I.nse * i]) * ncf + tid];
+      for (int p = 0; p < nparts; ++p) s -= red[p * ncf + tid];
+      Y[(static_cast<long>(n) + static_cast<long>(I.nse) * i) * ncf + tid] = s;
+    }
+    __syncthreads();
+  }
+  for (int i = I.nfse - 1; i >= 0; --i) {
+    double acc = 0.0;
+    if (part < nparts)
+      for (int q = I.uuptr[i] + part; q < I.uuptr[i + 1]; q += nparts) {
+        const double* B = blk(I.uidx[q]);
+        const double* xj = Xo + (static_cast<long>(n) + static_cast<long>(I.nse) * I.ucol[q]) * ncf;
+        for (int c = 0; c < ncf; ++c) acc = fma(B[r + ncf * c], xj[c], acc);
+      }
+    red[tid] = acc;
+    __syncthreads();
+    if (tid < ncf) {
+      double s = Y[(static_cast<long>(n) + static_cast<long>(I.nse) * i) * ncf + tid];
+      for (int p = 0; p < nparts; ++p) s -= red[p * ncf + tid];
+      z[tid] = s;
+    }
+    __syncthreads();
+    if (tid < ncf) {
+      const double* Dinv = blk(I.diag[i]);
+      double s = 0.0;
+      for (int c = 0; c < ncf; ++c) s = fma(Dinv[tid + ncf * c], z[c], s);
+      Xo[(static_cast<long>(n) + static_cast<long>(I.nse) * i) * ncf + tid] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// x[f] = Xo[slot1[f]] + Xo[slot2[f]] (faces shared by two superelements get both)
+__global__ void k_bilu_scatter(Dims d, IluDev I, const double* Xo, double* out) {
+  const int ncf = d.ncf;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(d.nf) * ncf) return;
+  const long f = q / ncf;
+  const int r = static_cast<int>(q % ncf);
+  double s = Xo[static_cast<long>(I.slot1[f]) * ncf + r];
+  if (I.slot2[f] >= 0) s += Xo[static_cast<long>(I.slot2[f]) * ncf + r];
+  out[q] = s;
+}
+
+// ---------------------------------------------------------------------------
 // Vector kernels (fixed-order reductions)
 // ---------------------------------------------------------------------------
 constexpr int DOT_BS = 256;
@@ -1160,6 +1369,28 @@ void dudg(const Dims& d, const double* RuS, const double* DinvF, const int* elem
 void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipStream_t s) {
   long n1 = static_cast<long>(d.npe) * d.ncu * d.ne;
   k_update_udg<<<nblocks(n1, 256), 256, 0, s>>>(d, du, alpha, udg);
+  EXA_HIP(hipGetLastError());
+}
+
+void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
+                   double* BE, hipStream_t s) {
+  long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse * I.nnz;
+  k_bilu_assemble<<<nblocks(n1, 256), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
+  EXA_HIP(hipGetLastError());
+}
+
+void bilu_factor(const Dims& d, const IluDev& I, double* BE, int* info, hipStream_t s) {
+  size_t shm = (3 * static_cast<size_t>(d.ncf) * d.ncf + d.ncf) * sizeof(double);
+  if (shm > 60 * 1024 || d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 50 x 50");
+  k_bilu_factor<<<I.nse, 256, shm, s>>>(d, I, BE, info);
+  EXA_HIP(hipGetLastError());
+}
+
+void bilu_apply(const Dims& d, const IluDev& I, const double* BE, const double* x, double* Y, double* Xo,
+                double* out, hipStream_t s) {
+  if (d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 256");
+  k_bilu_apply<<<I.nse, 256, 0, s>>>(d, I, BE, x, Y, Xo);
+  k_bilu_scatter<<<nblocks(static_cast<long>(d.nf) * d.ncf, 256), 256, 0, s>>>(d, I, Xo, out);
   EXA_HIP(hipGetLastError());
 }
 

@@ -84,6 +84,20 @@ void dudg(const Dims& d, const double* RuS, const double* DinvF, const int* elem
           double* du, hipStream_t s);
 void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipStream_t s);
 
+// ---- superelement block-ILU(0) (preconditioner = 2) ----
+struct IluDev {
+  int nse, nfse, nnz;
+  const int *face, *row_ptr, *col_ind, *row_of, *diag;
+  const int *eptr, *elim_ji, *uptr, *upd_jl, *upd_il;
+  const int *lptr, *lidx, *lcol, *uuptr, *uidx, *ucol;
+  const int *slot1, *slot2;
+};
+void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
+                   double* BE, hipStream_t s);
+void bilu_factor(const Dims& d, const IluDev& I, double* BE, int* info, hipStream_t s);
+void bilu_apply(const Dims& d, const IluDev& I, const double* BE, const double* x, double* Y, double* Xo,
+                double* out, hipStream_t s);
+
 // ---- vector kernels (results stay on the device) ----
 int dot_partials_count(long n);
 void dot(const double* x, const double* y, long n, double* partial, double* out, int sqrt_out,

@@ -164,6 +164,8 @@ void Solver::setup() {
     }
   }
   uh.upload(huh, s);
+  udg0.upload(hudg, s);
+  uh0.upload(huh, s);
 
   // face-node gather sources for the atomic-free element -> face assembly
   long nn = static_cast<long>(d.npf) * d.nf;
@@ -240,6 +242,46 @@ void Solver::setup() {
   tmp.alloc(d.N);
   wv.alloc(d.N);
   alloc_workspace();
+  if (precond == 2) setup_bilu0();
+}
+
+void Solver::setup_bilu0() {
+  if (ilu_ready) return;
+  ilu = build_bilu0(mesh.cartgridpart, conn, d.nfe, d.ne);
+  std::vector<int> row_of(ilu.nnz);
+  for (int i = 0; i < ilu.nfse; ++i)
+    for (int t = ilu.row_ptr[i]; t < ilu.row_ptr[i + 1]; ++t) row_of[t] = i;
+  auto up = [&](DevBuf<int>& b, const std::vector<int>& v) {
+    if (v.empty()) b.upload(std::vector<int>{0}, s);
+    else b.upload(v, s);
+    return b.p;
+  };
+  ilud.nse = ilu.nse;
+  ilud.nfse = ilu.nfse;
+  ilud.nnz = ilu.nnz;
+  ilud.face = up(iface, ilu.face);
+  ilud.row_ptr = up(irow_ptr, ilu.row_ptr);
+  ilud.col_ind = up(icol_ind, ilu.col_ind);
+  ilud.row_of = up(irow_of, row_of);
+  ilud.diag = up(idiag, ilu.diag);
+  ilud.eptr = up(ieptr, ilu.eptr);
+  ilud.elim_ji = up(ielim_ji, ilu.elim_ji);
+  ilud.uptr = up(iuptr, ilu.uptr);
+  ilud.upd_jl = up(iupd_jl, ilu.upd_jl);
+  ilud.upd_il = up(iupd_il, ilu.upd_il);
+  ilud.lptr = up(ilptr, ilu.lptr);
+  ilud.lidx = up(ilidx, ilu.lidx);
+  ilud.lcol = up(ilcol, ilu.lcol);
+  ilud.uuptr = up(iuuptr, ilu.uuptr);
+  ilud.uidx = up(iuidx, ilu.uidx);
+  ilud.ucol = up(iucol, ilu.ucol);
+  ilud.slot1 = up(islot1, ilu.slot1);
+  ilud.slot2 = up(islot2, ilu.slot2);
+  BEilu.alloc(static_cast<size_t>(ilu.nnz) * ilu.nse * d.ncf * d.ncf);
+  Yilu.alloc(static_cast<size_t>(ilu.nfse) * ilu.nse * d.ncf);
+  Xilu.alloc(static_cast<size_t>(ilu.nfse) * ilu.nse * d.ncf);
+  if (info.n < static_cast<size_t>(ilu.nse)) info.alloc(ilu.nse + 1);
+  ilu_ready = true;
 }
 
 void Solver::alloc_workspace() {
@@ -277,8 +319,11 @@ void Solver::set_option(const std::string& name, double v) {
   else if (name == "newton_tol") newton_tol = v;
   else if (name == "newton_maxit") newton_maxit = static_cast<int>(v);
   else if (name == "preconditioner") {
-    if (static_cast<int>(v) != 0) throw Error(5, "only block-Jacobi (0) is implemented");
-    precond = static_cast<int>(v);
+    int pv = static_cast<int>(v);
+    if (pv != 0 && pv != 2) throw Error(5, "preconditioners: 0 block-Jacobi, 2 superelement block-ILU(0)");
+    if (pv == 2) setup_bilu0();
+    precond = pv;
+    assembled = false;
   } else if (name == "ortho") ortho = static_cast<int>(v);
   else if (name == "verbose") verbose = static_cast<int>(v);
   else if (name == "timing") timing = v != 0.0;
@@ -370,9 +415,19 @@ double Solver::norm(const double* v, long n) {
   return hpin[0];
 }
 
+void Solver::reset() {
+  copy(udg.p, udg0.p, static_cast<long>(udg.n), s);
+  copy(uh.p, uh0.p, static_cast<long>(uh.n), s);
+  assembled = false;
+}
+
 void Solver::get_q() { getq(d, MCs.p, MEs.p, elemcon.p, uh.p, udg.p, s); }
 
 void Solver::assemble() {
+  if (precond != 0 && precond != 2)
+    throw Error(5, "preconditioner " + std::to_string(precond) +
+                       " is not implemented; block-Jacobi (0) and block-ILU(0) (2) are "
+                       "(set option \"preconditioner\")");
   const long n = d.n, m = d.m, nd = d.nd, ncu = d.ncu, npe = d.npe, ndf = d.ndf;
   const long augs = n * (n + m + 1);
   const long blk = static_cast<long>(npe) * d.nge;
@@ -468,12 +523,26 @@ void Solver::assemble() {
   }
   // hdgAssembleRHS (matvec.cpp:103-117)
   face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
-  // hdgBlockJacobi (matvec.cpp:190-217)
   tic(T_LU, &t0);
-  bj_assemble(d, f2e.p, elemcon.p, H.p, bjaug.p, s);
-  lu_solve(bjaug.p, d.ncf, 2 * d.ncf, 2L * d.ncf * d.ncf, d.nf, info.p, s);
-  bj_extract(d, bjaug.p, BE.p, s);
+  if (precond == 0) {
+    // hdgBlockJacobi (matvec.cpp:190-217)
+    bj_assemble(d, f2e.p, elemcon.p, H.p, bjaug.p, s);
+    lu_solve(bjaug.p, d.ncf, 2 * d.ncf, 2L * d.ncf * d.ncf, d.nf, info.p, s);
+    bj_extract(d, bjaug.p, BE.p, s);
+  } else {
+    // hdgBlockILU0 (matvec.cpp:119-166)
+    bilu_assemble(d, ilud, f2e.p, elemcon.p, H.p, BEilu.p, s);
+    bilu_factor(d, ilud, BEilu.p, info.p, s);
+  }
   toc(T_LU, t0);
+  EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(precond == 2 ? ilu.nse : d.nf, 8192),
+                         hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  {
+    const int* hinfo = reinterpret_cast<const int*>(hpin);
+    for (int k = 0; k < std::min(precond == 2 ? ilu.nse : d.nf, 8192); ++k)
+      if (hinfo[k]) throw Error(4, "singular preconditioner block");
+  }
   assembled = true;
 }
 
@@ -508,7 +577,8 @@ void Solver::apply_precond(double* xv) {
   hipEvent_t t0{};
   tic(T_PRECOND, &t0);
   copy(tmp.p, xv, d.N, s);
-  bj_apply(d, BE.p, tmp.p, xv, s);
+  if (precond == 2) bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
+  else bj_apply(d, BE.p, tmp.p, xv, s);
   toc(T_PRECOND, t0);
 }
 
@@ -664,6 +734,10 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
   else if (name == "MinvC") { src = MCs.p; size = static_cast<long>(MCs.n); }
   else if (name == "MinvE") { src = MEs.p; size = static_cast<long>(MEs.n); }
   else if (name == "res_b") { src = bres.p; size = static_cast<long>(bres.n); }
+  else if (name == "BEilu") { src = BEilu.p; size = static_cast<long>(BEilu.n); }
+  else if (name == "ilu_face") from_int(ilu.face);
+  else if (name == "ilu_row_ptr") from_int(ilu.row_ptr);
+  else if (name == "ilu_col_ind") from_int(ilu.col_ind);
   else if (name == "res_Ru") { src = RuR.p; size = static_cast<long>(RuR.n); }
   else if (name == "x") { src = x.p; size = static_cast<long>(x.n); }
   else if (name == "w") { src = wv.p; size = static_cast<long>(wv.n); }

@@ -2,6 +2,7 @@
 // one problem and the host control of the Newton-GMRES loop.
 #pragma once
 
+#include "bilu0.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -36,11 +37,18 @@ struct Solver {
   DevBuf<int> perm, elemcon, bf, f2e, info;
   DevBuf<long> src1, src2;
   DevBuf<double> xdg, egeo, fgeo, odg, og, ogf;
-  DevBuf<double> udg, uh, MCs, MEs, Mass;
+  DevBuf<double> udg, uh, udg0, uh0, MCs, MEs, Mass;
   DevBuf<double> H, DinvF, RuS, Rh, b, BE, bjaug, we, du, RuR, bres;
   // per-block workspace
   int neb = 0;
   DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
+  // superelement block-ILU(0)
+  Bilu0Struct ilu;
+  bool ilu_ready = false;
+  DevBuf<int> iface, irow_ptr, icol_ind, irow_of, idiag, ieptr, ielim_ji, iuptr, iupd_jl, iupd_il, ilptr,
+      ilidx, ilcol, iuuptr, iuidx, iucol, islot1, islot2;
+  DevBuf<double> BEilu, Yilu, Xilu;
+  IluDev ilud{};
   // GMRES
   DevBuf<double> V, x, r, tmp, wv, partial, hcol, ydev, scal;
   double* hpin = nullptr;  // pinned host scratch for small device->host copies
@@ -56,6 +64,7 @@ struct Solver {
 
   void set_option(const std::string& name, double value);
   // stages
+  void reset();
   void get_q();
   void assemble();
   void residual(double* nb, double* nru);
@@ -72,6 +81,7 @@ struct Solver {
 
  private:
   void setup();
+  void setup_bilu0();
   void alloc_workspace();
   AsmArgs asm_args(int e0, int eb);
   double norm(const double* x, long n);

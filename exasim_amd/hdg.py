@@ -28,6 +28,7 @@ SIGNATURES = [
     ("exa_sizes", ctypes.c_int, [_P, _I, ctypes.c_int]),
     ("exa_set_option", ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_double]),
     ("exa_solve", ctypes.c_int, [_P, _I, _I, _D]),
+    ("exa_reset", ctypes.c_int, [_P]),
     ("exa_get_q", ctypes.c_int, [_P]),
     ("exa_assemble", ctypes.c_int, [_P]),
     ("exa_residual", ctypes.c_int, [_P, _D, _D]),
@@ -120,6 +121,10 @@ class HDGSolver:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self._check(self._lib.exa_solve(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def reset(self):
+        """Restore udg/uh to the sol.bin input state on the device."""
+        self._check(self._lib.exa_reset(self._h))
 
     def get_q(self):
         self._check(self._lib.exa_get_q(self._h))

@@ -70,6 +70,10 @@ int exa_set_option(exa_handle* h, const char* name, double value);
 /* Full steady solve (NewtonSolver).  newton_iters/gmres_iters/final_norm may be NULL. */
 int exa_solve(exa_handle* h, int* newton_iters, int* gmres_iters, double* final_norm);
 
+/* Restore udg and uh to the values read from sol.bin (device-to-device copies),
+ * so that the next exa_solve starts from the input state. */
+int exa_reset(exa_handle* h);
+
 /* Stage entry points. */
 int exa_get_q(exa_handle* h);                 /* hdgGetQ on the current (udg, uh) */
 int exa_assemble(exa_handle* h);              /* H, DinvF, Ru, Rh, b and the preconditioner */

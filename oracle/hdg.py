@@ -613,6 +613,30 @@ class HDGProblem:
             np.add.at(BE, (f[:, None, None], rows[:, :, None], rows[:, None, :]), blk)
         return np.linalg.inv(BE)
 
+    def make_preconditioner(self, H):
+        """CPreconditioner for the HDG scheme: 0 block-Jacobi, 2 superelement
+        block-ILU(0) (discretization.cpp:296-311, matvec.cpp:119-217)."""
+        if self.preconditioner == 0:
+            BE = self.block_jacobi(H)
+            return lambda r: self.apply_block_jacobi(BE, r)
+        if self.preconditioner == 2:
+            from . import bilu0
+            if not hasattr(self, "_ilu"):
+                cart = self.mesh["cartgridpart"]
+                if cart.size == 0:
+                    raise ValueError("preconditioner 2 needs cartgridpart in mesh.bin")
+                elem = bilu0.gridpartition(cart)
+                row_ptr, col_ind, face = bilu0.crs_faceordering(elem, self.conn["f2e"], self.nfe)
+                ind_ii, elim, L, U = bilu0.crs_indexingilu0(row_ptr, col_ind, face.shape[1])
+                self._ilu = (row_ptr, col_ind, face, ind_ii, elim, L, U)
+            row_ptr, col_ind, face, ind_ii, elim, L, U = self._ilu
+            BE = bilu0.assemble(H, self.conn["f2e"], self.elemcon, face, row_ptr, col_ind,
+                                self.npf, self.nfe, self.ncu)
+            bilu0.factor(BE, ind_ii, elim)
+            ncf = self.ncu * self.npf
+            return lambda r: bilu0.apply(BE, ind_ii, L, U, face, r, ncf, self.nf)
+        raise NotImplementedError("oracle preconditioner %d" % self.preconditioner)
+
     def apply_block_jacobi(self, BEinv, x):
         """ApplyPreconditioner, preconditioner == 0 (preconditioner.cpp:282-292)."""
         ncf = self.ncu * self.npf
@@ -740,11 +764,7 @@ def newton_solve(prob, log=None):
     hist = dict(norms=[nrmr], gmres_iters=[], alphas=[])
     for it in range(prob.newton_maxit):
         E, S, b = prob.assemble()
-        if prob.preconditioner == 0:
-            BE = prob.block_jacobi(S["H"])
-            precond = lambda r: prob.apply_block_jacobi(BE, r)
-        else:
-            raise NotImplementedError("oracle preconditioner %d" % prob.preconditioner)
+        precond = prob.make_preconditioner(S["H"])
         mv = lambda v: prob.matvec(S["H"], v)
         x, its = gmres(mv, precond, b, prob.restart, prob.gmres_maxit, prob.gmres_tol,
                        prob.newton_tol, prob.ortho)

@@ -4,7 +4,10 @@ Writes tests/golden/text2code_tables.npz with the tensor-element blocks of the
 reference's data tables text2code/text2code/masternodes.bin and gaussnodes.bin
 (node coordinates, face-node permutations, Gauss points and weights), and
 tests/golden/poisson2d_xdg.npz with examples/Poisson/poisson2d/xdg.bin (the
-p = 3 nodes of the 16 x 16 quad mesh of configuration C1).
+p = 3 nodes of the 16 x 16 quad mesh of configuration C1), and
+tests/golden/poisson3d.npz with the reference's committed poisson3d run
+(examples/Poisson/poisson3d: datain/{app,master,mesh}.bin and grid.bin verbatim,
+plus every 8th interior face of dataout/outuhat_np0.bin).
 
 Usage:  python tests/golden/make_fixtures.py [/root/reference]
 """
@@ -51,6 +54,15 @@ def main(ref):
     np.savez_compressed(os.path.join(HERE, "text2code_tables.npz"), **out)
     xdg = np.fromfile(os.path.join(ref, "examples", "Poisson", "poisson2d", "xdg.bin"), dtype=np.float64)
     np.savez_compressed(os.path.join(HERE, "poisson2d_xdg.npz"), header=xdg[:3], xdg=xdg[3:])
+    P3 = os.path.join(ref, "examples", "Poisson", "poisson3d")
+    rd = lambda *a: np.fromfile(os.path.join(P3, *a), dtype=np.float64)
+    uhat = rd("dataout", "outuhat_np0.bin")
+    nf = int(uhat[2])
+    faces = np.arange(0, nf, 8)
+    np.savez_compressed(os.path.join(HERE, "poisson3d.npz"), app=rd("datain", "app.bin"),
+                        master=rd("datain", "master.bin"), mesh=rd("datain", "mesh.bin"),
+                        grid=rd("grid.bin"), uhat_header=uhat[:3], faces=faces,
+                        uhat_faces=uhat[3:].reshape(nf, -1)[faces])
     print("wrote", len(out), "table blocks")
 
 

@@ -26,7 +26,7 @@ def _app_flags(precond, ortho):
 
 def poisson2d(path, n=16, porder=3, pgauss=14, model="poisson", kappa=1.0, tau=1.0,
               precond="bjacobi", restart=50, gmres_tol=1e-12, gmres_maxit=1000,
-              newton_iter=1, newton_tol=1e-6, ortho=0, nd=2):
+              newton_iter=1, newton_tol=1e-6, ortho=0, nd=2, cartgridpart=None):
     """Poisson on the unit square (nd=2) or cube (nd=3), all boundaries Dirichlet."""
     os.makedirs(path, exist_ok=True)
     m = dw.make_master(nd, porder, pgauss)
@@ -45,7 +45,8 @@ def poisson2d(path, n=16, porder=3, pgauss=14, model="poisson", kappa=1.0, tau=1
                  problem={13: newton_iter, 14: gmres_maxit, 15: restart},
                  physicsparam=[kappa], tau=[tau], solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
     dw.write_master(os.path.join(path, "master.bin"), m)
-    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, bcs)
+    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, bcs,
+                  cartgridpart=[] if cartgridpart is None else list(cartgridpart))
     xdg = dw.compute_dgnodes(m, p, t)
     dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=1, nc=nd + 1)
     return dict(model=model, nd=nd, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=1)

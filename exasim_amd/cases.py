@@ -10,7 +10,7 @@ text2code writes (see ``datain.py``) and returns a small dict describing the cas
 * ``convdiff3d`` -- steady convection-diffusion on an n^3 hex mesh of the unit
   cube, ncu components, velocity c, diffusivity kappa, per-node uniform[-1, 1]
   source passed as the auxiliary field v (``vdgfile``); Dirichlet u = 0 on x = 0
-  (condition 1), zero normal flux elsewhere (condition 2).
+  (condition 1), homogeneous Neumann (zero diffusive flux) elsewhere (condition 2).
 """
 import os
 import numpy as np

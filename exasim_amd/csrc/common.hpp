@@ -7,6 +7,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -49,14 +52,50 @@ struct DevBuf {
     EXA_HIP(hipMalloc(&p, count * sizeof(T)));
     n = count;
   }
-  void upload(const std::vector<T>& h, hipStream_t s) {
+  template <class Stager>
+  void upload(const std::vector<T>& h, Stager& st, hipStream_t s) {
     alloc(h.size());
-    if (n) EXA_HIP(hipMemcpyAsync(p, h.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    if (n) st.h2d(p, h.data(), n * sizeof(T), s);
   }
   void zero(hipStream_t s) {
     if (n) EXA_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
   }
   size_t bytes() const { return n * sizeof(T); }
+};
+
+// Host <-> device copies go through one pinned staging buffer in fixed-size
+// chunks and have completed when the call returns: caller memory (pageable,
+// possibly a temporary) is never handed to an asynchronous copy.
+struct Stager {
+  char* buf = nullptr;
+  size_t cap = 0;
+  Stager() = default;
+  Stager(const Stager&) = delete;
+  Stager& operator=(const Stager&) = delete;
+  ~Stager() {
+    if (buf) (void)hipHostFree(buf);
+  }
+  void init(size_t bytes) {
+    if (buf) return;
+    EXA_HIP(hipHostMalloc(reinterpret_cast<void**>(&buf), bytes, hipHostMallocDefault));
+    cap = bytes;
+  }
+  void h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    for (size_t off = 0; off < bytes; off += cap) {
+      size_t c = std::min(cap, bytes - off);
+      std::memcpy(buf, static_cast<const char*>(src) + off, c);
+      EXA_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, buf, c, hipMemcpyHostToDevice, s));
+      EXA_HIP(hipStreamSynchronize(s));
+    }
+  }
+  void d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    for (size_t off = 0; off < bytes; off += cap) {
+      size_t c = std::min(cap, bytes - off);
+      EXA_HIP(hipMemcpyAsync(buf, static_cast<const char*>(src) + off, c, hipMemcpyDeviceToHost, s));
+      EXA_HIP(hipStreamSynchronize(s));
+      std::memcpy(static_cast<char*>(dst) + off, buf, c);
+    }
+  }
 };
 
 // ---- datain content (readbinaryfiles.cpp) ----

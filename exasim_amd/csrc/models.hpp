@@ -111,8 +111,8 @@ struct PoissonPoly : Poisson<ND_, NCU_> {
 
 // Steady convection-diffusion, per component: Flux = kappa*q + vel*u, Source = v
 // (the auxiliary field, vdgfile), FbouHdg: condition 1 Dirichlet tau*(0 - uhat),
-// any other condition zero normal flux f(uhat, q).n + tau*(u - uhat).
-// physicsparam = [kappa, vel_1 .. vel_ND].
+// any other condition homogeneous Neumann (zero diffusive flux)
+// kappa*q.n + tau*(u - uhat).  physicsparam = [kappa, vel_1 .. vel_ND].
 template <int ND_, int NCU_>
 struct ConvDiff {
   static constexpr int ND = ND_, NCU = NCU_, NC = NCU_ * (ND_ + 1);
@@ -165,12 +165,11 @@ struct ConvDiff {
 #pragma unroll
     for (int c = 0; c < NCU; ++c) {
       double acc = tau[0] * (uq[c] - uh[c]);
-      double duh = -tau[0];
+      const double duh = -tau[0];
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
-        acc += (kappa * uq[NCU + c + NCU * d] + p[1 + d] * uh[c]) * nl[d];
+        acc += kappa * uq[NCU + c + NCU * d] * nl[d];
         if (JAC) fb_uq[c + NCU * (NCU + c + NCU * d)] = kappa * nl[d];
-        duh += p[1 + d] * nl[d];
       }
       fb[c] = acc;
       if (JAC) {

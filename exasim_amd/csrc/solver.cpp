@@ -111,6 +111,7 @@ void Solver::setup() {
   EXA_HIP(hipGetDevice(&device));
   EXA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   EXA_HIP(hipHostMalloc(reinterpret_cast<void**>(&hpin), 4096 * sizeof(double), hipHostMallocDefault));
+  st.init(size_t(32) << 20);
 
   std::vector<double>& fl = app.flag;
   std::vector<double>& pr = app.problem;
@@ -120,23 +121,25 @@ void Solver::setup() {
   newton_maxit = static_cast<int>(pr[13]);
   gmres_maxit = static_cast<int>(pr[14]);
   restart = static_cast<int>(pr[15]);
+  if (restart < 1 || restart > 2000) throw Error(2, "GMRES restart in app.bin must be in [1, 2000]");
   newton_tol = app.solversparam[0];
   gmres_tol = app.solversparam[1];
   if (static_cast<int>(pr[27]) > 1) throw Error(5, "polynomial preconditioner (ppdegree > 1) is not supported");
   if (static_cast<int>(pr[16]) > 0) throw Error(5, "reduced-basis initial guess (RBdim > 0) is not supported");
 
   // master tables, mesh, fields
-  shapegt.upload(master.shapegt, s);
-  shapegw.upload(master.shapegw, s);
-  shapfgt.upload(master.shapfgt, s);
-  shapfgw.upload(master.shapfgw, s);
-  param.upload(app.physicsparam.empty() ? std::vector<double>{0.0} : app.physicsparam, s);
-  tau.upload(app.tau, s);
-  perm.upload(mesh.perm, s);
-  elemcon.upload(conn.elemcon, s);
-  bf.upload(mesh.bf, s);
-  f2e.upload(conn.f2e, s);
-  xdg.upload(sol.xdg, s);
+  shapegt.upload(master.shapegt, st, s);
+  shapegw.upload(master.shapegw, st, s);
+  shapfgt.upload(master.shapfgt, st, s);
+  shapfgw.upload(master.shapfgw, st, s);
+  if (app.physicsparam.empty()) app.physicsparam.push_back(0.0);
+  param.upload(app.physicsparam, st, s);
+  tau.upload(app.tau, st, s);
+  perm.upload(mesh.perm, st, s);
+  elemcon.upload(conn.elemcon, st, s);
+  bf.upload(mesh.bf, st, s);
+  f2e.upload(conn.f2e, st, s);
+  xdg.upload(sol.xdg, st, s);
 
   // udg with all nc components (readsolstruct: missing q components are zero;
   // a missing udg is Initu, which is zero for the models here)
@@ -148,7 +151,7 @@ void Solver::setup() {
         for (int i = 0; i < d.npe; ++i)
           hudg[i + d.npe * (c + d.nc * e)] = sol.udg[i + d.npe * (c + k * e)];
   }
-  udg.upload(hudg, s);
+  udg.upload(hudg, st, s);
   // initial uhat (GetFaceNodes from the first element of each face, kokkosimpl.h:901-915)
   std::vector<double> huh(static_cast<size_t>(d.N), 0.0);
   if (!sol.uh.empty()) {
@@ -163,9 +166,9 @@ void Solver::setup() {
               hudg[mesh.perm[j + d.npf * l1] + d.npe * (c + static_cast<long>(d.nc) * e1)];
     }
   }
-  uh.upload(huh, s);
-  udg0.upload(hudg, s);
-  uh0.upload(huh, s);
+  uh.upload(huh, st, s);
+  udg0.upload(hudg, st, s);
+  uh0.upload(huh, st, s);
 
   // face-node gather sources for the atomic-free element -> face assembly
   long nn = static_cast<long>(d.npf) * d.nf;
@@ -183,8 +186,8 @@ void Solver::setup() {
       }
     }
   }
-  src1.upload(h1, s);
-  src2.upload(h2, s);
+  src1.upload(h1, st, s);
+  src2.upload(h2, st, s);
 
   // geometry
   egeo.alloc(static_cast<size_t>(d.ne) * d.nge * (d.nd + d.nd * d.nd + 1));
@@ -192,7 +195,7 @@ void Solver::setup() {
   elem_geom(d, shapegt.p, xdg.p, egeo.p, s);
   face_geom(d, shapfgt.p, perm.p, xdg.p, fgeo.p, s);
   if (d.nco > 0) {
-    odg.upload(sol.odg, s);
+    odg.upload(sol.odg, st, s);
     og.alloc(static_cast<size_t>(d.ne) * d.nge * d.nco);
     ogf.alloc(static_cast<size_t>(d.ne) * d.nfe * d.ngf * d.nco);
     interp_elem(d, d.nco, shapegt.p, odg.p, og.p, s);
@@ -252,8 +255,8 @@ void Solver::setup_bilu0() {
   for (int i = 0; i < ilu.nfse; ++i)
     for (int t = ilu.row_ptr[i]; t < ilu.row_ptr[i + 1]; ++t) row_of[t] = i;
   auto up = [&](DevBuf<int>& b, const std::vector<int>& v) {
-    if (v.empty()) b.upload(std::vector<int>{0}, s);
-    else b.upload(v, s);
+    static const std::vector<int> zero1{0};
+    b.upload(v.empty() ? zero1 : v, st, s);
     return b.p;
   };
   ilud.nse = ilu.nse;
@@ -645,6 +648,10 @@ int Solver::gmres(const double* bv, double* xv) {
     axpby(r.p, bv, 1.0, -1.0, N, s);  // r = b - A x
     apply_precond(r.p);
     nrmr = norm(r.p, N);
+    if (verbose) {
+      std::printf("GMRES restart after %d iterations: relative residual %g (tol %g)\n", j, nrmr / nrmb, tol);
+      std::fflush(stdout);
+    }
     if (nrmr / nrmb < tol) return j;
   }
   if (verbose) std::printf("Warning: GMRES(%d) does not converge to the tolerance %g within %d iterations\n",
@@ -707,9 +714,8 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
 void Solver::write_output() {
   if (dataout.empty()) throw Error(1, "no dataout prefix was given");
   std::vector<double> hu(static_cast<size_t>(d.npe) * d.nc * d.ne), hh(d.N);
-  EXA_HIP(hipMemcpyAsync(hu.data(), udg.p, hu.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  EXA_HIP(hipMemcpyAsync(hh.data(), uh.p, hh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  EXA_HIP(hipStreamSynchronize(s));
+  st.d2h(hu.data(), udg.p, hu.size() * sizeof(double), s);
+  st.d2h(hh.data(), uh.p, hh.size() * sizeof(double), s);
   ::exa::write_output(dataout + "udg_np0.bin", d.npe, d.nc, d.ne, hu);
   ::exa::write_output(dataout + "uhat_np0.bin", d.ncu, d.npf, d.nf, hh);
 }
@@ -749,8 +755,7 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
   if (!dst) return size;
   long k = std::min(len, size);
   if (src) {
-    EXA_HIP(hipMemcpyAsync(dst, src, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-    EXA_HIP(hipStreamSynchronize(s));
+    st.d2h(dst, src, sizeof(double) * k, s);
   } else {
     std::memcpy(dst, host.data(), sizeof(double) * k);
   }
@@ -765,8 +770,7 @@ void Solver::set_array(const std::string& name, const double* src, long len) {
   else if (name == "w") t = &wv;
   else throw Error(1, "array '" + name + "' cannot be set");
   if (len != static_cast<long>(t->n)) throw Error(1, "wrong length for '" + name + "'");
-  EXA_HIP(hipMemcpyAsync(t->p, src, sizeof(double) * len, hipMemcpyHostToDevice, s));
-  EXA_HIP(hipStreamSynchronize(s));
+  st.h2d(t->p, src, sizeof(double) * len, s);
 }
 
 void Solver::device_ptr(const std::string& name, void** p, long* len) {

@@ -52,6 +52,7 @@ struct Solver {
   // GMRES
   DevBuf<double> V, x, r, tmp, wv, partial, hcol, ydev, scal;
   double* hpin = nullptr;  // pinned host scratch for small device->host copies
+  Stager st;               // pinned staging of bulk host <-> device copies
   bool assembled = false;
 
   // timing

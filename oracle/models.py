@@ -83,8 +83,8 @@ class PoissonPoly(Poisson):
 
 class ConvDiff:
     """f[c, d] = kappa*q[c, d] + vel[d]*u[c]; s[c] = v[c];
-    FbouHdg: ib == 1 Dirichlet tau*(0 - uh), otherwise zero normal flux
-    f(uh, q).n + tau*(u - uh).  physicsparam = [kappa, vel_1..vel_nd]."""
+    FbouHdg: ib == 1 Dirichlet tau*(0 - uh), otherwise homogeneous Neumann
+    (zero diffusive flux) kappa*q.n + tau*(u - uh).  physicsparam = [kappa, vel_1..vel_nd]."""
     name = "convdiff"
 
     def __init__(self, nd, ncu):
@@ -120,14 +120,12 @@ class ConvDiff:
                 fb_uh[:, c, c] = -tau[0]
             return fb, fb_uq, fb_uh
         kappa = param[0]
-        vel = param[1:1 + nd]
         for c in range(ncu):
             acc = tau[0] * (uq[:, c] - uh[:, c])
             for d in range(nd):
                 qi = ncu + c + ncu * d
-                acc = acc + (kappa * uq[:, qi] + vel[d] * uh[:, c]) * nl[:, d]
+                acc = acc + kappa * uq[:, qi] * nl[:, d]
                 fb_uq[:, c, qi] = kappa * nl[:, d]
-                fb_uh[:, c, c] += vel[d] * nl[:, d]
             fb[:, c] = acc
             fb_uq[:, c, c] = tau[0]
             fb_uh[:, c, c] -= tau[0]

@@ -1,6 +1,7 @@
-"""Build libexahip.so (C++ host code + HIP kernels for gfx950) in-tree with hipcc.
+"""Build libexahip.so (C++ host code + HIP kernels for gfx950) in-tree with hipcc,
+and the stand-alone driver exasim_amd/gput2cEXASIM that links it.
 
-    python -m exasim_amd.build            # -> exasim_amd/libexahip.so
+    python -m exasim_amd.build            # -> exasim_amd/libexahip.so, exasim_amd/gput2cEXASIM
 """
 import os
 import subprocess
@@ -10,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexahip.so")
+EXE = os.path.join(HERE, "gput2cEXASIM")
 SOURCES = ["kernels.hip", "datain.cpp", "conn.cpp", "bilu0.cpp", "solver.cpp", "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -55,6 +57,15 @@ def build(verbose=False, jobs=None):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # stand-alone driver with the command line of gput2cEXASIM (host code only)
+    tmp = EXE + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "main_t2c.cpp"), "-o", tmp, "-L" + HERE, "-lexahip",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, EXE)
     return LIB
 
 

@@ -117,6 +117,9 @@ void Solver::setup() {
   std::vector<double>& pr = app.problem;
   if (static_cast<int>(fl[0]) != 0) throw Error(5, "time-dependent problems are not supported yet");
   precond = static_cast<int>(fl[6]);
+  if (precond != 0 && precond != 2)
+    throw Error(5, "preconditioner " + std::to_string(precond) +
+                       " is not implemented (0 block-Jacobi, 2 superelement block-ILU(0))");
   ortho = static_cast<int>(fl[5]);
   newton_maxit = static_cast<int>(pr[13]);
   gmres_maxit = static_cast<int>(pr[14]);

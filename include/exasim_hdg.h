@@ -62,9 +62,12 @@ const char* exa_last_error(const exa_handle* h);
  * nge, ngf, nco, restart, preconditioner. Returns the count written. */
 int exa_sizes(const exa_handle* h, int* out, int len);
 
-/* Options (name, value): "gmres_tol", "gmres_restart", "gmres_maxit", "newton_tol",
- * "newton_maxit", "preconditioner" (0 block-Jacobi), "ortho" (0 MGS, 1 CGS),
- * "verbose", "timing" (1: record per-kernel HIP events). */
+/* Options (name, value): "gmres_tol", "gmres_restart" (1..2000), "gmres_maxit",
+ * "newton_tol", "newton_maxit", "preconditioner" (0 block-Jacobi, 2 superelement
+ * block-ILU(0), which needs cartgridpart in mesh.bin; 1 is EXA_ERR_UNSUPPORTED),
+ * "ortho" (0 MGS, 1 CGS), "verbose" (1: Newton and GMRES restart lines on stdout),
+ * "timing" (1: record per-kernel HIP events), "workspace_bytes" (element-block
+ * workspace budget of the assembly). */
 int exa_set_option(exa_handle* h, const char* name, double value);
 
 /* Full steady solve (NewtonSolver).  newton_iters/gmres_iters/final_norm may be NULL. */
@@ -88,13 +91,16 @@ int exa_gmres(exa_handle* h, const double* b_dev, double* x_dev, int* iters);
 /* Copy a named array to host memory (len = number of doubles available at dst).
  * Names: "udg" [npe,nc,ne], "uh" [ncu,npf,nf], "H" [m,m,ne], "DinvF" [n,m,ne],
  * "Ru" [n,ne], "Rh" [m,ne], "b" [N], "BE" (block-Jacobi inverse) [ncf,ncf,nf],
- * "Mass" [npe,npe,ne], "MinvC" [npe*nd,npe,ne], "MinvE" [npe*nd,npe*... see docs],
- * "elemcon" [npf*nfe*ne] (as doubles), "f2e" [4*nf] (as doubles), "res_b" [N],
- * "res_Ru" [n,ne].  Returns the number of doubles of the array (copies min(len, size)). */
+ * "BEilu" (factored block-ILU(0) blocks) [ncf,ncf,nse,nnz], "ilu_face" [nse,nfse],
+ * "ilu_row_ptr" [nfse+1], "ilu_col_ind" [nnz] (as doubles), "Mass" [npe,npe,ne],
+ * "MinvC" [npe,nd,npe,ne], "MinvE" [npe,nd,ndf,ne], "elemcon" [npf*nfe*ne] (as doubles),
+ * "f2e" [4*nf] (as doubles), "res_b" [N], "res_Ru" [n,ne], "x", "w" [N], "egeo",
+ * "fgeo" (geometry at quadrature points).  Returns the number of doubles of the array
+ * (copies min(len, size)). */
 long exa_get_array(exa_handle* h, const char* name, double* dst, long len);
-/* Overwrite "udg" or "uh" from host memory. */
+/* Overwrite "udg", "uh", "x" or "w" from host memory (len must equal the array size). */
 int exa_set_array(exa_handle* h, const char* name, const double* src, long len);
-/* Device pointer of a named persistent device array (H, DinvF, uh, udg, b). */
+/* Device pointer of a named persistent device array (H, DinvF, uh, udg, b, x, w). */
 int exa_device_ptr(exa_handle* h, const char* name, void** ptr, long* len);
 
 /* dataout/out{udg,uhat}_np0.bin with the reference's 3-double headers. */

@@ -38,3 +38,26 @@ def test_create_reports_missing_input(tmp_path):
     rc = lib.exa_create(str(tmp_path).encode(), None, b"poisson", -2, ctypes.byref(h))
     assert rc != 0
     assert h.value is None
+
+
+DRIVER = os.path.join(ROOT, "exasim_amd", "gput2cEXASIM")
+
+
+def test_driver_command_line_errors(tmp_path):
+    """The gput2cEXASIM-style driver: usage, missing pdeapp.txt, missing model name and
+    missing datain fail with a message and a non-zero status before any device work."""
+    import subprocess
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built (run python -m exasim_amd.build)")
+    r = subprocess.run([DRIVER], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = subprocess.run([DRIVER, str(tmp_path / "none.txt")], capture_output=True, text=True)
+    assert r.returncode == 1 and "does not exist" in r.stderr
+    app = tmp_path / "pdeapp.txt"
+    app.write_text('model = "ModelD";\nmodelfile = "pdemodel.txt";\nporder = 3;\n')
+    r = subprocess.run([DRIVER, str(app)], capture_output=True, text=True)
+    assert r.returncode == 1 and "hipmodel" in r.stderr
+    app.write_text('model = "ModelD";\nhipmodel = "poisson";\ndatapath = "%s";\n' % (tmp_path / "d"))
+    r = subprocess.run([DRIVER, str(app)], capture_output=True, text=True)
+    assert r.returncode == 2 and "Error" in r.stderr          # EXA_ERR_IO: no datain/app.bin
+    assert os.path.isdir(tmp_path / "d" / "dataout")

@@ -200,3 +200,32 @@ def test_output_files(built, tmp_path):
     assert np.array_equal(data, uh)
     hdr, data = read_output(prefix + "udg_np0.bin")
     assert list(hdr) == [z["npe"], z["nc"], z["ne"]]
+
+
+def test_driver_matches_oracle(built, tmp_path):
+    """gput2cEXASIM-style driver on a datain/ directory: dataout files equal the oracle's
+    converged solution (C1, 16x16 quads, p = 3)."""
+    import os
+    import shutil
+    import subprocess
+    from exasim_amd.datain import read_output
+    path, model = built("c1")
+    say("case c1: stand-alone driver")
+    root = tmp_path / "run"
+    shutil.copytree(path, root / "datain")
+    app = root / "pdeapp.txt"
+    app.write_text('model = "ModelD";\nmodelfile = "pdemodel.txt";\nhipmodel = "%s";\n'
+                   'discretization = "hdg";\nporder = 3;\n' % model)
+    drv = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "exasim_amd",
+                       "gput2cEXASIM")
+    r = subprocess.run([drv, str(app)], capture_output=True, text=True, timeout=120)
+    say(r.stdout[-500:])
+    assert r.returncode == 0, r.stderr
+    P = ora.HDGProblem(path, model)
+    ora.newton_solve(P)
+    hdr, uh = read_output(str(root / "dataout" / "outuhat_np0.bin"))
+    assert list(hdr) == [P.ncu, P.npf, P.nf]
+    assert rel(uh, P.uh.reshape(-1)) < RTOL
+    hdr, udg = read_output(str(root / "dataout" / "outudg_np0.bin"))
+    udg = udg.reshape(P.ne, P.nc, P.npe).transpose(0, 2, 1)
+    assert rel(udg, P.udg) < RTOL

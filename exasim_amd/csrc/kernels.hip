@@ -787,6 +787,53 @@ __global__ void k_bj_assemble(Dims d, const int* f2e, const int* elemcon, const 
   aug[f * per + rs] = v;
 }
 
+// Elemental additive Schwarz (hdgElementalAdditiveSchwarz, matvec.cpp:168-188;
+// kokkosimpl.h:1113-1142): element e's m x m block of H with the diagonal block of
+// each interior face replaced by the sum of both sides' blocks, written as the
+// augmented [block | I] (m x 2m) for the batched LU inversion.
+__global__ void k_eas_assemble(Dims d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
+                               double* aug) {
+  const int m = d.m, ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf;
+  const long per = static_cast<long>(m) * 2 * m;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const int le = static_cast<int>(q / per);
+  const int rs = static_cast<int>(q % per);
+  const int r = rs % m, c = rs / m;
+  const long e = e0 + le;
+  double v;
+  if (c >= m) {
+    v = (c - m == r) ? 1.0 : 0.0;
+  } else {
+    v = H[e * m * m + r + static_cast<long>(m) * c];
+    const int l = r / ncf;
+    if (c / ncf == l) {
+      const int f = elemcon[npf * l + ndf * e] / npf;
+      const int e1 = f2e[4 * f], l1 = f2e[4 * f + 1], e2 = f2e[4 * f + 2], l2 = f2e[4 * f + 3];
+      if (e2 >= 0) {
+        const int rr = r % ncf, cc = c % ncf;
+        const int am = rr % ncu, bm = rr / ncu, an = cc % ncu, bn = cc / ncu;
+        // the other side's position of face node bm: through e2's elemcon (an involution)
+        const int om = am + ncu * (elemcon[bm + npf * l2 + ndf * e2] - npf * f);
+        const int on = an + ncu * (elemcon[bn + npf * l2 + ndf * e2] - npf * f);
+        const bool first = (e == e1 && l == l1);
+        const long eo = first ? e2 : e1;
+        const int lo = first ? l2 : l1;
+        v += H[eo * m * m + (om + ncf * lo) + static_cast<long>(m) * (on + ncf * lo)];
+      }
+    }
+  }
+  aug[static_cast<long>(le) * per + rs] = v;
+}
+
+__global__ void k_eas_extract(Dims d, int eb, const double* aug, double* BEe) {
+  const long per = static_cast<long>(d.m) * d.m;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const long e = q / per;
+  BEe[q] = aug[e * 2 * per + per + q % per];
+}
+
 __global__ void k_bj_extract(Dims d, const double* aug, double* BE) {
   const long per = static_cast<long>(d.ncf) * d.ncf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1336,6 +1383,19 @@ void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double
                  hipStream_t s) {
   long n1 = static_cast<long>(d.ncf) * 2 * d.ncf * d.nf;
   k_bj_assemble<<<nblocks(n1, 256), 256, 0, s>>>(d, f2e, elemcon, H, aug);
+  EXA_HIP(hipGetLastError());
+}
+
+void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
+                  double* aug, hipStream_t s) {
+  long n1 = static_cast<long>(d.m) * 2 * d.m * eb;
+  k_eas_assemble<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, H, f2e, elemcon, aug);
+  EXA_HIP(hipGetLastError());
+}
+
+void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s) {
+  long n1 = static_cast<long>(d.m) * d.m * eb;
+  k_eas_extract<<<nblocks(n1, 256), 256, 0, s>>>(d, eb, aug, BEe);
   EXA_HIP(hipGetLastError());
 }
 

@@ -78,6 +78,11 @@ void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double
                  hipStream_t s);
 void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s);
 void bj_apply(const Dims& d, const double* BE, const double* x, double* y, hipStream_t s);
+// elemental additive Schwarz: augmented [block | I] for elements [e0, e0+eb), and the
+// inverse (columns m..2m-1 after lu_solve) into BEe (the block's first element)
+void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
+                  double* aug, hipStream_t s);
+void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s);
 void matvec_elem(const Dims& d, const double* H, const int* elemcon, const double* v, double* we,
                  hipStream_t s);
 void dudg(const Dims& d, const double* RuS, const double* DinvF, const int* elemcon, const double* x,

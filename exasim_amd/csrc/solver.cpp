@@ -117,9 +117,10 @@ void Solver::setup() {
   std::vector<double>& pr = app.problem;
   if (static_cast<int>(fl[0]) != 0) throw Error(5, "time-dependent problems are not supported yet");
   precond = static_cast<int>(fl[6]);
-  if (precond != 0 && precond != 2)
+  if (precond < 0 || precond > 2)
     throw Error(5, "preconditioner " + std::to_string(precond) +
-                       " is not implemented (0 block-Jacobi, 2 superelement block-ILU(0))");
+                       " is not implemented (0 block-Jacobi, 1 elemental additive Schwarz, "
+                       "2 superelement block-ILU(0))");
   ortho = static_cast<int>(fl[5]);
   newton_maxit = static_cast<int>(pr[13]);
   gmres_maxit = static_cast<int>(pr[14]);
@@ -249,6 +250,16 @@ void Solver::setup() {
   wv.alloc(d.N);
   alloc_workspace();
   if (precond == 2) setup_bilu0();
+  if (precond == 1) setup_eas();
+}
+
+void Solver::setup_eas() {
+  if (d.m > 400) throw Error(5, "elemental additive Schwarz: element trace block larger than 400");
+  const double per = 16.0 * d.m * d.m;  // bytes of one augmented [block | I]
+  eas_chunk = static_cast<int>(std::max(1.0, std::min<double>(d.ne, ws_budget_bytes / per)));
+  BEeas.alloc(static_cast<size_t>(d.m) * d.m * d.ne);
+  easaug.alloc(static_cast<size_t>(eas_chunk) * 2 * d.m * d.m);
+  if (info.n < static_cast<size_t>(eas_chunk)) info.alloc(eas_chunk + 1);
 }
 
 void Solver::setup_bilu0() {
@@ -326,8 +337,11 @@ void Solver::set_option(const std::string& name, double v) {
   else if (name == "newton_maxit") newton_maxit = static_cast<int>(v);
   else if (name == "preconditioner") {
     int pv = static_cast<int>(v);
-    if (pv != 0 && pv != 2) throw Error(5, "preconditioners: 0 block-Jacobi, 2 superelement block-ILU(0)");
+    if (pv < 0 || pv > 2)
+      throw Error(5, "preconditioners: 0 block-Jacobi, 1 elemental additive Schwarz, "
+                     "2 superelement block-ILU(0)");
     if (pv == 2) setup_bilu0();
+    if (pv == 1) setup_eas();
     precond = pv;
     assembled = false;
   } else if (name == "ortho") ortho = static_cast<int>(v);
@@ -430,10 +444,8 @@ void Solver::reset() {
 void Solver::get_q() { getq(d, MCs.p, MEs.p, elemcon.p, uh.p, udg.p, s); }
 
 void Solver::assemble() {
-  if (precond != 0 && precond != 2)
-    throw Error(5, "preconditioner " + std::to_string(precond) +
-                       " is not implemented; block-Jacobi (0) and block-ILU(0) (2) are "
-                       "(set option \"preconditioner\")");
+  if (precond < 0 || precond > 2)
+    throw Error(5, "preconditioner " + std::to_string(precond) + " is not implemented");
   const long n = d.n, m = d.m, nd = d.nd, ncu = d.ncu, npe = d.npe, ndf = d.ndf;
   const long augs = n * (n + m + 1);
   const long blk = static_cast<long>(npe) * d.nge;
@@ -530,7 +542,20 @@ void Solver::assemble() {
   // hdgAssembleRHS (matvec.cpp:103-117)
   face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
   tic(T_LU, &t0);
-  if (precond == 0) {
+  if (precond == 1) {
+    // hdgElementalAdditiveSchwarz (matvec.cpp:168-188), inverted in element chunks
+    for (int e0 = 0; e0 < d.ne; e0 += eas_chunk) {
+      const int eb = std::min(eas_chunk, d.ne - e0);
+      eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
+      lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s);
+      EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
+      EXA_HIP(hipStreamSynchronize(s));
+      const int* hinfo = reinterpret_cast<const int*>(hpin);
+      for (int e = 0; e < std::min(eb, 8192); ++e)
+        if (hinfo[e]) throw Error(4, "singular Schwarz block at element " + std::to_string(e0 + e));
+      eas_extract(d, eb, easaug.p, BEeas.p + static_cast<long>(e0) * d.m * d.m, s);
+    }
+  } else if (precond == 0) {
     // hdgBlockJacobi (matvec.cpp:190-217)
     bj_assemble(d, f2e.p, elemcon.p, H.p, bjaug.p, s);
     lu_solve(bjaug.p, d.ncf, 2 * d.ncf, 2L * d.ncf * d.ncf, d.nf, info.p, s);
@@ -541,12 +566,12 @@ void Solver::assemble() {
     bilu_factor(d, ilud, BEilu.p, info.p, s);
   }
   toc(T_LU, t0);
-  EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(precond == 2 ? ilu.nse : d.nf, 8192),
-                         hipMemcpyDeviceToHost, s));
-  EXA_HIP(hipStreamSynchronize(s));
-  {
+  if (precond != 1) {
+    const int nchk = std::min(precond == 2 ? ilu.nse : d.nf, 8192);
+    EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * nchk, hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
     const int* hinfo = reinterpret_cast<const int*>(hpin);
-    for (int k = 0; k < std::min(precond == 2 ? ilu.nse : d.nf, 8192); ++k)
+    for (int k = 0; k < nchk; ++k)
       if (hinfo[k]) throw Error(4, "singular preconditioner block");
   }
   assembled = true;
@@ -583,8 +608,14 @@ void Solver::apply_precond(double* xv) {
   hipEvent_t t0{};
   tic(T_PRECOND, &t0);
   copy(tmp.p, xv, d.N, s);
-  if (precond == 2) bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
-  else bj_apply(d, BE.p, tmp.p, xv, s);
+  if (precond == 2) {
+    bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
+  } else if (precond == 1) {  // x = sum_e scatter(BE_e^{-1} gather_e(x)), as hdgMatVec with BE
+    matvec_elem(d, BEeas.p, elemcon.p, tmp.p, we.p, s);
+    face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, xv, s);
+  } else {
+    bj_apply(d, BE.p, tmp.p, xv, s);
+  }
   toc(T_PRECOND, t0);
 }
 
@@ -744,6 +775,7 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
   else if (name == "MinvE") { src = MEs.p; size = static_cast<long>(MEs.n); }
   else if (name == "res_b") { src = bres.p; size = static_cast<long>(bres.n); }
   else if (name == "BEilu") { src = BEilu.p; size = static_cast<long>(BEilu.n); }
+  else if (name == "BEeas") { src = BEeas.p; size = static_cast<long>(BEeas.n); }
   else if (name == "ilu_face") from_int(ilu.face);
   else if (name == "ilu_row_ptr") from_int(ilu.row_ptr);
   else if (name == "ilu_col_ind") from_int(ilu.col_ind);

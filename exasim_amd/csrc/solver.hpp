@@ -42,6 +42,9 @@ struct Solver {
   // per-block workspace
   int neb = 0;
   DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
+  // elemental additive Schwarz: inverted element blocks, inversion scratch
+  DevBuf<double> BEeas, easaug;
+  int eas_chunk = 0;
   // superelement block-ILU(0)
   Bilu0Struct ilu;
   bool ilu_ready = false;
@@ -83,6 +86,7 @@ struct Solver {
  private:
   void setup();
   void setup_bilu0();
+  void setup_eas();
   void alloc_workspace();
   AsmArgs asm_args(int e0, int eb);
   double norm(const double* x, long n);

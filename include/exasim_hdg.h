@@ -63,8 +63,8 @@ const char* exa_last_error(const exa_handle* h);
 int exa_sizes(const exa_handle* h, int* out, int len);
 
 /* Options (name, value): "gmres_tol", "gmres_restart" (1..2000), "gmres_maxit",
- * "newton_tol", "newton_maxit", "preconditioner" (0 block-Jacobi, 2 superelement
- * block-ILU(0), which needs cartgridpart in mesh.bin; 1 is EXA_ERR_UNSUPPORTED),
+ * "newton_tol", "newton_maxit", "preconditioner" (0 block-Jacobi, 1 elemental
+ * additive Schwarz, 2 superelement block-ILU(0), which needs cartgridpart in mesh.bin),
  * "ortho" (0 MGS, 1 CGS), "verbose" (1: Newton and GMRES restart lines on stdout),
  * "timing" (1: record per-kernel HIP events), "workspace_bytes" (element-block
  * workspace budget of the assembly). */
@@ -92,6 +92,7 @@ int exa_gmres(exa_handle* h, const double* b_dev, double* x_dev, int* iters);
  * Names: "udg" [npe,nc,ne], "uh" [ncu,npf,nf], "H" [m,m,ne], "DinvF" [n,m,ne],
  * "Ru" [n,ne], "Rh" [m,ne], "b" [N], "BE" (block-Jacobi inverse) [ncf,ncf,nf],
  * "BEilu" (factored block-ILU(0) blocks) [ncf,ncf,nse,nnz], "ilu_face" [nse,nfse],
+ * "BEeas" (inverted elemental additive Schwarz blocks) [m,m,ne],
  * "ilu_row_ptr" [nfse+1], "ilu_col_ind" [nnz] (as doubles), "Mass" [npe,npe,ne],
  * "MinvC" [npe,nd,npe,ne], "MinvE" [npe,nd,ndf,ne], "elemcon" [npf*nfe*ne] (as doubles),
  * "f2e" [4*nf] (as doubles), "res_b" [N], "res_Ru" [n,ne], "x", "w" [N], "egeo",

@@ -613,12 +613,50 @@ class HDGProblem:
             np.add.at(BE, (f[:, None, None], rows[:, :, None], rows[:, None, :]), blk)
         return np.linalg.inv(BE)
 
+    def face_diagonal_blocks(self, H):
+        """Assembled diagonal ncf x ncf face blocks of the condensed matrix, in the
+        face's own node order: [nf, ncf, ncf]."""
+        ncu, npf, nfe, nf, ne = self.ncu, self.npf, self.nfe, self.nf, self.ne
+        ncf = ncu * npf
+        A = np.zeros((nf, ncf, ncf))
+        for l in range(nfe):
+            g = self.elemcon[:, l, :]
+            f = g[:, 0] // npf
+            loc = g - f[:, None] * npf
+            rows = (np.arange(ncu)[None, None, :] + ncu * loc[:, :, None]).reshape(ne, ncf)
+            src = (np.arange(ncu)[None, :] + ncu * (np.arange(npf)[:, None] + npf * l)).ravel()
+            np.add.at(A, (f[:, None, None], rows[:, :, None], rows[:, None, :]),
+                      H[:, src[:, None], src[None, :]])
+        return A
+
+    def elemental_schwarz(self, H):
+        """hdgElementalAdditiveSchwarz (matvec.cpp:168-188, kokkosimpl.h:1113-1142):
+        each element's m x m block with the diagonal block of every face replaced by
+        the assembled (both-sides) face block, inverted."""
+        ncu, npf, nfe, ne = self.ncu, self.npf, self.nfe, self.ne
+        ncf = ncu * npf
+        A = self.face_diagonal_blocks(H)
+        BE = H.copy()
+        for l in range(nfe):
+            g = self.elemcon[:, l, :]
+            f = g[:, 0] // npf
+            loc = g - f[:, None] * npf
+            rows = (np.arange(ncu)[None, None, :] + ncu * loc[:, :, None]).reshape(ne, ncf)
+            src = (np.arange(ncu)[None, :] + ncu * (np.arange(npf)[:, None] + npf * l)).ravel()
+            blk = A[f[:, None, None], rows[:, :, None], rows[:, None, :]]
+            BE[:, src[:, None], src[None, :]] = blk
+        return np.linalg.inv(BE)
+
     def make_preconditioner(self, H):
-        """CPreconditioner for the HDG scheme: 0 block-Jacobi, 2 superelement
-        block-ILU(0) (discretization.cpp:296-311, matvec.cpp:119-217)."""
+        """CPreconditioner for the HDG scheme: 0 block-Jacobi, 1 elemental additive
+        Schwarz, 2 superelement block-ILU(0) (discretization.cpp:296-320,
+        matvec.cpp:119-217, preconditioner.cpp:282-318)."""
         if self.preconditioner == 0:
             BE = self.block_jacobi(H)
             return lambda r: self.apply_block_jacobi(BE, r)
+        if self.preconditioner == 1:
+            BEinv = self.elemental_schwarz(H)
+            return lambda r: self.matvec(BEinv, r)
         if self.preconditioner == 2:
             from . import bilu0
             if not hasattr(self, "_ilu"):

@@ -229,3 +229,41 @@ def test_driver_matches_oracle(built, tmp_path):
     hdr, udg = read_output(str(root / "dataout" / "outudg_np0.bin"))
     udg = udg.reshape(P.ne, P.nc, P.npe).transpose(0, 2, 1)
     assert rel(udg, P.udg) < RTOL
+
+
+CASES["c1eas"] = ("poisson2d", dict(n=4, porder=2, pgauss=6, precond=1), "poisson")
+CASES["c2eas"] = ("convdiff3d", dict(n=2, porder=3, ncu=2, restart=100, gmres_tol=1e-12, precond=1),
+                  "convdiff")
+
+
+@pytest.mark.parametrize("name", ["c1eas", "c2eas"])
+def test_elemental_schwarz_matches_oracle(built, name):
+    """Preconditioner 1: inverted element blocks, the apply and the preconditioned solve."""
+    path, model = built(name)
+    say("case %s: elemental additive Schwarz" % name)
+    P = ora.HDGProblem(path, model)
+    P.get_q(P.udg, P.uh)
+    E, S_, b = P.assemble()
+    BEinv = P.elemental_schwarz(S_["H"])
+    v = np.random.default_rng(5).standard_normal(b.size)
+    with gpu_solver(path, model) as S:
+        z = S.sizes
+        assert z["preconditioner"] == 1
+        S.get_q()
+        S.assemble()
+        BEg = as_elem_mats(S.get("BEeas"), z["m"], z["m"], z["ne"])
+        pw, _ = S.device_ptr("w")
+        S.set("w", v)
+        S.precond(pw)
+        pv = S.get("w")
+        pb, _ = S.device_ptr("b")
+        px, _ = S.device_ptr("x")
+        its = S.gmres(pb, px)
+        xg = S.get("x")
+    assert rel(BEg, BEinv) < RTOL
+    assert rel(pv, P.matvec(BEinv, v)) < RTOL
+    xo, its_o = ora.gmres(lambda q: P.matvec(S_["H"], q), P.make_preconditioner(S_["H"]), b, P.restart,
+                          P.gmres_maxit, P.gmres_tol, P.newton_tol, P.ortho)
+    say("  gmres iterations: gpu %d oracle %d" % (its, its_o))
+    assert abs(its - its_o) <= 1
+    assert rel(xg, xo) < 1e-9

@@ -225,3 +225,29 @@ def test_c1_manufactured_solution_converges(tmp_path):
         errs.append(np.abs(P.udg[:, :, 0].reshape(-1) - P.model.exact(x)).max())
     rate = np.log2(errs[0] / errs[1])
     assert rate > 3.5, (errs, rate)
+
+
+def test_block_jacobi_and_schwarz_blocks_match_global_matrix(c1small):
+    """Block-Jacobi blocks are the inverses of the global matrix's diagonal face blocks;
+    the elemental additive Schwarz block of an element is its H_e with each face's
+    diagonal block replaced by the global one (the global matrix is built column by
+    column from the matvec)."""
+    P = ora.HDGProblem(c1small, "poisson")
+    P.get_q(P.udg, P.uh)
+    E, S, b = P.assemble()
+    H = S["H"]
+    ncf = P.ncu * P.npf
+    N = P.nf * ncf
+    Ag = np.stack([P.matvec(H, e) for e in np.eye(N)], 1)
+    BJ = P.block_jacobi(H)
+    for f in range(P.nf):
+        dof = f * ncf + np.arange(ncf)
+        assert np.abs(np.linalg.inv(BJ[f]) - Ag[np.ix_(dof, dof)]).max() < 1e-11 * np.abs(Ag).max()
+    BEs = np.linalg.inv(P.elemental_schwarz(H))
+    for e in range(P.ne):
+        dof = (np.arange(P.ncu)[None, :] + P.ncu * P.elemcon[e].reshape(-1)[:, None]).reshape(-1)
+        ref = H[e].copy()
+        for l in range(P.nfe):
+            loc = slice(l * ncf, (l + 1) * ncf)
+            ref[loc, loc] = Ag[np.ix_(dof[loc], dof[loc])]
+        assert np.abs(BEs[e] - ref).max() < 1e-10 * np.abs(ref).max()

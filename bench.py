@@ -4,19 +4,27 @@
 Workload (default): steady convection-diffusion on the unit cube, 32^3 hex mesh,
 p = 4, two components, velocity (1, 0.5, 0.25), diffusivity 1e-2, seeded
 uniform[-1, 1] nodal source, Dirichlet u = 0 on x = 0 and homogeneous Neumann
-elsewhere, GMRES(100) to 1e-10 (FP64) preconditioned by block-ILU(0) over
-2x2x2-element superelements.  One "step" is one full solve from the
-input state: hdgGetQ, residual, assembly + static condensation, GMRES,
-hdgGetDUDG, residual (one Newton step, the problem is linear), with every input
-already resident in HBM.
+elsewhere, GMRES(100) to 1e-10 preconditioned by block-ILU(0) over
+2x2x2-element superelements.  One "step" is one full solve from the input
+state: hdgGetQ, residual, assembly + static condensation, GMRES, hdgGetDUDG,
+residual (one Newton step, the problem is linear), with every input already
+resident in HBM.
+
+Two timed legs run on the same solver: the plain FP64 form and the mixed form
+(FP32 storage of the condensed operator, the local-solve factors and the
+preconditioner blocks; FP64 residuals, LU pivots and GMRES vectors; iterative
+refinement to 1e-10 of the first residual).  ``value`` is the mixed form's
+solves/s; the FP64 form, the ratio and the mixed solution's relative error
+against the FP64 one are reported beside it.
 
 Prints one JSON line (rank 0, N = 1).  ``roofline`` is the dominant kernel of
-the GMRES loop, the element matvec: algorithmic bytes per launch
-(8*m^2*ne for H, 8*m*ne for the gathered vector, 4*(m/ncu)*ne for the
-connectivity, 8*m*ne for the result) over its average launch time measured
-with HIP events on the solver's stream during the timed solves.
-``cpu_baseline`` is the numpy oracle on one core, timed on a bounded sample of
-the same workload (see ``cpu_sample``) and scaled to one full solve.
+the GMRES loop, the element matvec of the leg reported in ``value``:
+algorithmic bytes per launch (s_H*m^2*ne for H, with s_H = 4 or 8, 8*m*ne for
+the gathered vector, 4*(m/ncu)*ne for the connectivity, 8*m*ne for the result)
+over its average launch time measured with HIP events on the solver's stream
+during the timed solves.  ``cpu_baseline`` is the numpy oracle on one core,
+timed on a bounded sample of the same workload (see ``cpu_sample``) and scaled
+to one full FP64 solve.
 """
 import argparse
 import json
@@ -33,6 +41,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s + p50/p99 latency, mixed vs FP64 GPU at rel err ≤1e-8; HBM/MFMA %"
 HBM_PEAK_GBS = 8000.0
+PREC_NAME = {0: "fp64", 1: "mixed"}
+PREC_ID = {"fp64": 0, "mixed": 1}
 
 
 def build_case(path, n, porder, precond, restart, tol, maxit, se):
@@ -82,6 +92,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxit", type=int, default=20000)
     ap.add_argument("--ortho", type=int, default=0)
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
+                    help="timed legs: FP64, mixed FP32/FP64, or both (value = the mixed leg)")
     ap.add_argument("--precond", default="bilu0", choices=["bilu0", "bjacobi"])
     ap.add_argument("--se", type=int, default=2, help="superelement edge (elements) of block-ILU(0)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -125,39 +137,59 @@ def main():
     z = S.sizes
     print("# setup (read, connectivity, geometry, q-equation) %.2f s; sizes %s" % (t_setup, z),
           file=sys.stderr, flush=True)
-
-    for w in range(args.warmup):
-        S.reset()
-        it = S.solve()
-        print("# warmup %d: newton %d gmres %d final residual %.3e" % (w, *it), file=sys.stderr, flush=True)
-
-    S.sync()
-    S.set_option("timing", 1)
-    S.reset_timings()
-    lat = []
-    gm = []
-    res = []
-    t_all = time.perf_counter()
-    for k in range(args.steps):
-        S.reset()
-        t0 = time.perf_counter()
-        ni, gi, nrm = S.solve()     # synchronises at the end
-        lat.append(time.perf_counter() - t0)
-        gm.append(gi)
-        res.append(nrm)
-        print("# step %d: %.3f s  newton %d gmres %d residual %.3e" % (k, lat[-1], ni, gi, nrm),
-              file=sys.stderr, flush=True)
-    t_all = time.perf_counter() - t_all
-    tm = S.timings()
-    S.close()
-
     ne, m, n, ncu, N = z["ne"], z["m"], z["n"], z["ncu"], z["N"]
-    mv_ms, mv_cnt = tm["matvec"]
-    mv_avg_s = mv_ms / max(mv_cnt, 1) / 1e3
-    bytes_mv = 8.0 * m * m * ne + 8.0 * m * ne + 4.0 * (m // ncu) * ne + 8.0 * m * ne
-    achieved = bytes_mv / mv_avg_s / 1e9 if mv_cnt else None
-    solves_s = args.steps / t_all
-    lat_ms = np.array(lat) * 1e3
+
+    def leg(precision):
+        """W untimed + K timed solves from the input state at one precision."""
+        S.set_option("precision", precision)
+        S.set_option("timing", 0)
+        for w in range(args.warmup):
+            S.reset()
+            it = S.solve()
+            print("# %s warmup %d: newton %d gmres %d final residual %.3e"
+                  % (PREC_NAME[precision], w, *it), file=sys.stderr, flush=True)
+        S.sync()
+        S.set_option("timing", 1)
+        S.reset_timings()
+        lat, gm, res = [], [], []
+        t_all = time.perf_counter()
+        for k in range(args.steps):
+            S.reset()
+            t0 = time.perf_counter()
+            ni, gi, nrm = S.solve()     # synchronises at the end
+            lat.append(time.perf_counter() - t0)
+            gm.append(gi)
+            res.append(nrm)
+            print("# %s step %d: %.3f s  newton %d gmres %d residual %.3e"
+                  % (PREC_NAME[precision], k, lat[-1], ni, gi, nrm), file=sys.stderr, flush=True)
+        t_all = time.perf_counter() - t_all
+        tm = S.timings()
+        uh = S.get("uh")
+        mv_ms, mv_cnt = tm["matvec"]
+        mv_avg_s = mv_ms / max(mv_cnt, 1) / 1e3
+        sH = 4.0 if precision else 8.0
+        bytes_mv = sH * m * m * ne + 8.0 * m * ne + 4.0 * (m // ncu) * ne + 8.0 * m * ne
+        achieved = bytes_mv / mv_avg_s / 1e9 if mv_cnt else None
+        lat_ms = np.array(lat) * 1e3
+        return dict(solves_s=args.steps / t_all, lat_ms=lat_ms, gm=gm, res=res, tm=tm, uh=uh,
+                    roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                              "kernel": "k_matvec_elem<%s>" % ("float" if precision else "double"),
+                              "bytes_per_launch": bytes_mv, "avg_launch_ms": mv_avg_s * 1e3})
+
+    legs = {}
+    for p in ([0, 1] if args.precision == "both" else [PREC_ID[args.precision]]):
+        legs[p] = leg(p)
+    S.close()
+    head = legs[1] if 1 in legs else legs[0]
+    err = None
+    if 0 in legs and 1 in legs:
+        u64, u32 = legs[0]["uh"], legs[1]["uh"]
+        err = float(np.abs(u32 - u64).max() / np.abs(u64).max())
+    gm = head["gm"]
+    lat_ms = head["lat_ms"]
+    tm = head["tm"]
+    solves_s = head["solves_s"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -168,19 +200,20 @@ def main():
                            env=env, capture_output=True, text=True, timeout=900)
         if r.returncode == 0:
             c = json.loads(r.stdout.strip().splitlines()[-1])
-            iters = float(np.mean(gm))
+            iters = float(np.mean(legs[0]["gm"] if 0 in legs else gm))
             t_solve = (c["t_asm"] + c["t_pre"]) / c["ne"] * ne + iters * c["t_iter"] / c["N"] * N
             cpu = {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-                   "sample": "numpy oracle, 1 thread: assembly+condensation (%.2f s) and %s "
+                   "sample": "numpy oracle (FP64), 1 thread: assembly+condensation (%.2f s) and %s "
                              "set-up (%.2f s) of the %d^3 mesh (%d elements), and one GMRES "
                              "iteration (matvec, preconditioner, MGS over %d vectors; %.4f s at %d "
-                             "unknowns), scaled linearly to %d elements and %d unknowns at the GPU's "
-                             "%.0f GMRES iterations"
+                             "unknowns), scaled linearly to %d elements and %d unknowns at the GPU FP64 "
+                             "form's %.0f GMRES iterations"
                              % (c["t_asm"], args.precond, c["t_pre"], args.cpu_sample_n, c["ne"],
                                 args.restart // 2, c["t_iter"], c["N"], ne, N, iters)}
         else:
             print("# cpu baseline failed: " + r.stderr[-2000:], file=sys.stderr)
 
+    mixed = 1 in legs
     out = {
         "metric": METRIC,
         "value": solves_s,
@@ -192,27 +225,35 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
-        "config": {"workload": "C2 steady convection-diffusion 3D, %d^3 hex, p=%d, 2 components, "
-                               "kappa=1e-2, velocity (1,0.5,0.25), GMRES(%d) tol %g, FP64, %s"
+        "config": {"workload": "steady convection-diffusion 3D, %d^3 hex, p=%d, 2 components, "
+                               "kappa=1e-2, velocity (1,0.5,0.25), GMRES(%d) tol %g, %s, %s"
                                % (args.n, args.porder, args.restart, args.tol,
                                   "block-ILU(0) over %d^3-element superelements" % args.se
-                                  if args.precond == "bilu0" else "block-Jacobi face preconditioner"),
-                   "ne": ne, "trace_dofs": N, "n": n, "m": m, "precision": "fp64"},
+                                  if args.precond == "bilu0" else "block-Jacobi face preconditioner",
+                                  "mixed FP32 storage / FP64 refinement to 1e-10" if mixed else "FP64"),
+                   "ne": ne, "trace_dofs": N, "n": n, "m": m,
+                   "precision": "mixed" if mixed else "fp64"},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "dof_per_s": N * solves_s,
         "newton_steps_per_s": solves_s,
         "gmres_iterations": gm,
-        "final_residual": res,
+        "final_residual": head["res"],
         "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in tm.items()},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k_matvec_elem", "bytes_per_launch": bytes_mv,
-                     "avg_launch_ms": mv_avg_s * 1e3},
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
     }
+    if 0 in legs:
+        L = legs[0]
+        out["fp64"] = {"solves_per_s": L["solves_s"], "p50_ms": float(np.percentile(L["lat_ms"], 50)),
+                       "p99_ms": float(np.percentile(L["lat_ms"], 99)), "gmres_iterations": L["gm"],
+                       "final_residual": L["res"], "roofline": L["roofline"],
+                       "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in L["tm"].items()}}
+    if mixed and 0 in legs:
+        out["mixed_over_fp64"] = solves_s / legs[0]["solves_s"]
+        out["mixed_rel_err_vs_fp64"] = err
     print(json.dumps(out))
     return 0
 

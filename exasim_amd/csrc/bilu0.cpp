@@ -243,6 +243,7 @@ Bilu0Struct build_bilu0(const std::vector<int>& cart, const Conn& conn, int nfe,
         if (idx_jl < 0) continue;
         s.upd_jl.push_back(idx_jl);
         s.upd_il.push_back(pp);
+        s.upd_ji.push_back(idx_ji);
       }
       s.uptr.push_back(static_cast<int>(s.upd_jl.size()));
     }

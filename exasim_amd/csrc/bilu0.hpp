@@ -16,6 +16,7 @@ struct Bilu0Struct {
   // elimination lists of the factorisation (row i: entries eptr[i]..eptr[i+1]-1;
   // entry k updates upd_jl/upd_il[uptr[k]..uptr[k+1]-1])
   std::vector<int> eptr, elim_ji, uptr, upd_jl, upd_il;
+  std::vector<int> upd_ji;            // the L block (elim_ji) each update multiplies
   // lower / upper entries of each row for the triangular sweeps
   std::vector<int> lptr, lidx, lcol, uuptr, uidx, ucol;
   // each global face lies in one or two superelements (slot = n + nse*i)

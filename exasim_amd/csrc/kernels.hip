@@ -34,9 +34,10 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g, int tilesM) {
   const long t1 = b / g.nb0;
   const int b1 = static_cast<int>(t1 % g.nb1);
   const int b2 = static_cast<int>(t1 / g.nb1);
-  const double* A = g.A + g.sA0 * b0 + g.sA1 * b1 + g.sA2 * b2;
-  const double* B = g.B + g.sB0 * b0 + g.sB1 * b1 + g.sB2 * b2;
-  double* C = g.C + g.sC0 * b0 + g.sC1 * b1 + g.sC2 * b2;
+  const long a0 = g.iA ? g.iA[b0] : b0, bb0 = g.iB ? g.iB[b0] : b0, c0 = g.iC ? g.iC[b0] : b0;
+  const double* A = g.A + g.sA0 * a0 + g.sA1 * b1 + g.sA2 * b2;
+  const double* B = g.B + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
+  double* C = g.C + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
   const int row0 = tile * BM, col0 = blockIdx.y * BN;
   __shared__ double As[BK][BM + 1];
   __shared__ double Bs[BK][BN + 1];
@@ -83,6 +84,242 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g, int tilesM) {
         C[r + g.ldc * c] = v;
       }
     }
+}
+
+// FP64 batched GEMM on the matrix cores (v_mfma_f64_16x16x4f64).  Workgroup tile
+// 64 x 64, four waves of 32 x 32 (2 x 2 MFMA tiles), K staged through LDS 16 at a
+// time with the next K-tile prefetched into registers.  Lane layout of the
+// instruction: lane l supplies A[l%16][l/16] and B[l/16][l%16] of a 16x4 / 4x16
+// step; accumulator register i of lane l is D[l/16 + 4i][l%16].
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int MG_BM = 64, MG_BN = 64, MG_BK = 32;
+
+// LDS is fragment-major: element (row r, k) of the A tile sits at
+// (r/16)*16*MG_BK + k*16 + r%16, so the 64 lanes of one MFMA operand read 64
+// consecutive doubles (no bank conflicts); B likewise with its columns.
+__device__ inline int mg_off(int rc, int k) { return (rc >> 4) * (16 * MG_BK) + k * 16 + (rc & 15); }
+
+__global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
+  const int tile = blockIdx.x % tilesM;
+  const long b = blockIdx.x / tilesM;
+  const int b0 = static_cast<int>(b % g.nb0);
+  const long t1 = b / g.nb0;
+  const int b1 = static_cast<int>(t1 % g.nb1);
+  const int b2 = static_cast<int>(t1 / g.nb1);
+  const long a0 = g.iA ? g.iA[b0] : b0, bb0 = g.iB ? g.iB[b0] : b0, c0 = g.iC ? g.iC[b0] : b0;
+  const double* A = g.A + g.sA0 * a0 + g.sA1 * b1 + g.sA2 * b2;
+  const double* B = g.B + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
+  double* C = g.C + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
+  const int row0 = tile * MG_BM, col0 = blockIdx.y * MG_BN;
+  __shared__ double As[MG_BM * MG_BK];
+  __shared__ double Bs[MG_BN * MG_BK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave & 1, wc = wave >> 1;
+  // global -> register staging: A row tid%64, k = tid/64 + 4j; B k = tid%32, column tid/32 + 8j
+  const int ar = tid & 63, ak = tid >> 6;
+  const int bk = tid & 31, bc = tid >> 5;
+  constexpr int NL = MG_BM * MG_BK / 256;
+  double ra[NL], rb[NL];
+  const bool rowok = row0 + ar < g.M;
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      const int gk = k0 + ak + 4 * j;
+      ra[j] = (rowok && gk < g.K) ? A[(row0 + ar) + g.lda * gk] : 0.0;
+      const int gk2 = k0 + bk, gc = col0 + bc + 8 * j;
+      rb[j] = (gk2 < g.K && gc < g.N) ? B[gk2 + g.ldb * gc] : 0.0;
+    }
+  };
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+  load(0);
+  const int li = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < g.K; k0 += MG_BK) {
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      As[mg_off(ar, ak + 4 * j)] = ra[j];
+      Bs[mg_off(bc + 8 * j, bk)] = rb[j];
+    }
+    __syncthreads();
+    if (k0 + MG_BK < g.K) load(k0 + MG_BK);
+#pragma unroll
+    for (int kk = 0; kk < MG_BK; kk += 4) {
+      const int kr = kk + lk;
+      const double a0v = As[mg_off(32 * wr + li, kr)], a1v = As[mg_off(32 * wr + 16 + li, kr)];
+      const double b0v = Bs[mg_off(32 * wc + li, kr)], b1v = Bs[mg_off(32 * wc + 16 + li, kr)];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0v, b0v, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0v, b1v, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1v, b0v, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1v, b1v, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = row0 + 32 * wr + 16 * ti + lk + 4 * i;
+        const int c = col0 + 32 * wc + 16 * tj + li;
+        if (r < g.M && c < g.N) {
+          double v = g.alpha * acc[ti][tj][i];
+          if (g.beta != 0.0) v = fma(g.beta, C[r + g.ldc * static_cast<long>(c)], v);
+          C[r + g.ldc * static_cast<long>(c)] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Right-looking blocked LU of the augmented systems [D | RHS] (n x ncols, ld n),
+// one panel of BLU_NB columns at a time: k_lu_panel factors the panel of every
+// matrix in LDS (partial pivoting), swaps the rows of the columns right of the
+// panel and solves L11 U12 = A12; the trailing update A22 -= L21 U12 and the
+// back substitution's updates run on the MFMA GEMM.
+// ---------------------------------------------------------------------------
+constexpr int BLU_NB = 16;
+constexpr int BLU_MIN_N = 64;  // smaller blocks use the single-kernel k_lu_solve
+
+__device__ inline void argmax_wave(double& v, int& i) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_xor(v, off);
+    int oi = __shfl_xor(i, off);
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols, long stride, int k0,
+                                                  int* info) {
+  double* A = Aall + stride * blockIdx.x;
+  const int nb = min(BLU_NB, n - k0);
+  const int rows = n - k0;
+  extern __shared__ double P[];      // rows x nb, ld rows
+  __shared__ double wv[4];
+  __shared__ int wi[4];
+  __shared__ int piv[BLU_NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int idx = tid; idx < rows * nb; idx += 256) {
+    int r = idx % rows, kk = idx / rows;
+    P[idx] = A[k0 + r + static_cast<long>(n) * (k0 + kk)];
+  }
+  __syncthreads();
+  int singular = 0;
+  for (int kk = 0; kk < nb; ++kk) {
+    double best = -1.0;
+    int bi = kk;
+    for (int r = kk + tid; r < rows; r += 256) {
+      double v = fabs(P[r + rows * kk]);
+      if (v > best) { best = v; bi = r; }
+    }
+    argmax_wave(best, bi);
+    if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double b = wv[0];
+      int p = wi[0];
+      for (int w = 1; w < 4; ++w)
+        if (wv[w] > b || (wv[w] == b && wi[w] < p)) { b = wv[w]; p = wi[w]; }
+      piv[kk] = p;
+    }
+    __syncthreads();
+    const int p = piv[kk];
+    if (p != kk && tid < nb) {
+      double t = P[kk + rows * tid];
+      P[kk + rows * tid] = P[p + rows * tid];
+      P[p + rows * tid] = t;
+    }
+    __syncthreads();
+    const double pv = P[kk + rows * kk];
+    if (pv == 0.0) singular = 1;
+    const double inv = 1.0 / pv;
+    for (int r = kk + 1 + tid; r < rows; r += 256) P[r + rows * kk] *= inv;
+    __syncthreads();
+    const int rr = rows - kk - 1, cc = nb - kk - 1;
+    for (int idx = tid; idx < rr * cc; idx += 256) {
+      int r = kk + 1 + idx % rr, c = kk + 1 + idx / rr;
+      P[r + rows * c] = fma(-P[r + rows * kk], P[kk + rows * c], P[r + rows * c]);
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < rows * nb; idx += 256) {
+    int r = idx % rows, kk = idx / rows;
+    A[k0 + r + static_cast<long>(n) * (k0 + kk)] = P[idx];
+  }
+  // columns right of the panel: row swaps, then U12 = L11^{-1} A12
+  for (int j = k0 + nb + tid; j < ncols; j += 256) {
+    double* colj = A + static_cast<long>(n) * j + k0;
+    double x[BLU_NB];
+#pragma unroll
+    for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
+    // pivots are applied in order; p < BLU_NB means both rows are in the register
+    // block (a last panel narrower than BLU_NB has rows == nb)
+#pragma unroll
+    for (int kk = 0; kk < BLU_NB; ++kk) {
+      if (kk < nb) {
+        const int p = piv[kk];
+        if (p != kk) {
+          if (p < BLU_NB) {
+#pragma unroll
+            for (int q = kk + 1; q < BLU_NB; ++q)
+              if (q == p) {
+                const double t = x[q];
+                x[q] = x[kk];
+                x[kk] = t;
+              }
+          } else {
+            const double t = colj[p];
+            colj[p] = x[kk];
+            x[kk] = t;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int kk = 1; kk < BLU_NB; ++kk)
+#pragma unroll
+      for (int t = 0; t < kk; ++t)
+        if (kk < nb) x[kk] = fma(-P[kk + rows * t], x[t], x[kk]);
+#pragma unroll
+    for (int kk = 0; kk < BLU_NB; ++kk)
+      if (kk < nb) colj[kk] = x[kk];
+  }
+  if (tid == 0 && info) info[blockIdx.x] = (k0 == 0 ? 0 : info[blockIdx.x]) | singular;
+}
+
+// X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
+__global__ __launch_bounds__(256) void k_lu_back(double* Aall, int n, int ncols, long stride, int k0) {
+  double* A = Aall + stride * blockIdx.x;
+  const int nb = min(BLU_NB, n - k0);
+  __shared__ double U[BLU_NB][BLU_NB + 1];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    int r = idx % nb, c = idx / nb;
+    U[r][c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
+  }
+  __syncthreads();
+  for (int j = n + tid; j < ncols; j += 256) {
+    double* colj = A + static_cast<long>(n) * j + k0;
+    double x[BLU_NB];
+#pragma unroll
+    for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
+#pragma unroll
+    for (int kk = BLU_NB - 1; kk >= 0; --kk) {
+      if (kk < nb) {
+        double v = x[kk];
+#pragma unroll
+        for (int t = kk + 1; t < BLU_NB; ++t)
+          if (t < nb) v = fma(-U[kk][t], x[t], v);
+        x[kk] = v / U[kk][kk];
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < BLU_NB; ++kk)
+      if (kk < nb) colj[kk] = x[kk];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -714,7 +951,7 @@ __global__ void k_face_jac(Dims d, AsmArgs a, int l, double* aug, double* Bfull,
     sB += w * a.fpt[(nf1 + fI) * P + pb + g];
   }
   const int pa = a.perm[aa + npf * l], pbn = a.perm[bb + npf * l];
-  const long augs = static_cast<long>(n) * (n + m + 1);
+  const long augs = a.augs;
   const long trow = k + ncu * (aa + npf * l);
   if (c < ncu) {
     aug[e * augs + pa + npe * k + static_cast<long>(n) * (pbn + npe * c)] += sI;
@@ -732,14 +969,15 @@ __global__ void k_face_jac(Dims d, AsmArgs a, int l, double* aug, double* Bfull,
 }
 
 // After the solve: DinvF = -X_F, Ru = X_Ru (uequation.cpp:700-711).
-__global__ void k_post_schur(Dims d, int e0, int eb, const double* aug, double* DinvF, double* RuS) {
+__global__ void k_post_schur(Dims d, int e0, int eb, const double* aug, long augs, double* DinvF,
+                             double* RuS) {
   const int n = d.n, m = d.m;
   const long per = static_cast<long>(n) * (m + 1);
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= per * eb) return;
   const long e = q / per;
   const long r = q % per;
-  const double v = aug[e * static_cast<long>(n) * (n + m + 1) + static_cast<long>(n) * n + r];
+  const double v = aug[e * augs + static_cast<long>(n) * n + r];
   const long E = e0 + e;
   if (r < static_cast<long>(n) * m) DinvF[E * n * static_cast<long>(m) + r] = -v;
   else RuS[E * n + (r - static_cast<long>(n) * m)] = v;
@@ -843,16 +1081,17 @@ __global__ void k_bj_extract(Dims d, const double* aug, double* BE) {
 }
 
 // x_f <- BE_f x_f (preconditioner.cpp:282-292); one thread per row.
-__global__ void k_bj_apply(Dims d, const double* BE, const double* x, double* y) {
+template <typename T>
+__global__ void k_bj_apply(Dims d, const T* BE, const double* x, double* y) {
   const int ncf = d.ncf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= static_cast<long>(ncf) * d.nf) return;
   const long f = q / ncf;
   const int r = static_cast<int>(q % ncf);
-  const double* B = BE + f * ncf * static_cast<long>(ncf);
+  const T* B = BE + f * ncf * static_cast<long>(ncf);
   const double* xf = x + f * ncf;
   double s = 0.0;
-  for (int c = 0; c < ncf; ++c) s += B[r + static_cast<long>(ncf) * c] * xf[c];
+  for (int c = 0; c < ncf; ++c) s += static_cast<double>(B[r + static_cast<long>(ncf) * c]) * xf[c];
   y[q] = s;
 }
 
@@ -861,7 +1100,8 @@ __global__ void k_bj_apply(Dims d, const double* BE, const double* x, double* y)
 // quarter of the columns, lanes own rows (coalesced column reads of H), and the
 // four partial sums are added in LDS in a fixed order.
 constexpr int MV_MAXRB = 16;  // m <= 1024
-__global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const double* H, const int* elemcon,
+template <typename T>
+__global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const int* elemcon,
                                                      const double* v, double* we) {
   const int m = d.m, ncu = d.ncu, ndf = d.ndf;
   const long e = blockIdx.x;
@@ -878,14 +1118,14 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const double* H, co
   double acc[MV_MAXRB];
 #pragma unroll
   for (int rb = 0; rb < MV_MAXRB; ++rb) acc[rb] = 0.0;
-  const double* He = H + e * static_cast<long>(m) * m;
+  const T* He = H + e * static_cast<long>(m) * m;
   for (int c = c0; c < c1; ++c) {
     const double vc = ve[c];
-    const double* col = He + static_cast<long>(m) * c;
+    const T* col = He + static_cast<long>(m) * c;
 #pragma unroll
     for (int rb = 0; rb < MV_MAXRB; ++rb) {
       const int r = lane + 64 * rb;
-      if (rb < nrb && r < m) acc[rb] = fma(col[r], vc, acc[rb]);
+      if (rb < nrb && r < m) acc[rb] = fma(static_cast<double>(col[r]), vc, acc[rb]);
     }
   }
 #pragma unroll
@@ -899,17 +1139,18 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const double* H, co
 }
 
 // hdgGetDUDG (matvec.cpp:219-238): du_e = Ru_e + DinvF_e * x_e.
-__global__ void k_dudg(Dims d, const double* RuS, const double* DinvF, const int* elemcon,
+template <typename T>
+__global__ void k_dudg(Dims d, const double* RuS, const T* DinvF, const int* elemcon,
                        const double* x, double* du) {
   const int n = d.n, m = d.m, ncu = d.ncu, ndf = d.ndf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= static_cast<long>(n) * d.ne) return;
   const long e = q / n;
   const int r = static_cast<int>(q % n);
-  const double* F = DinvF + e * n * static_cast<long>(m);
+  const T* F = DinvF + e * n * static_cast<long>(m);
   double s = RuS[q];
   for (int t = 0; t < m; ++t)
-    s += F[r + static_cast<long>(n) * t] * x[(t % ncu) + ncu * static_cast<long>(elemcon[t / ncu + ndf * e])];
+    s += static_cast<double>(F[r + static_cast<long>(n) * t]) * x[(t % ncu) + ncu * static_cast<long>(elemcon[t / ncu + ndf * e])];
   du[q] = s;
 }
 
@@ -971,103 +1212,89 @@ __global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* ele
   BE[q] = v;
 }
 
-// One workgroup per superelement: for each face slot i, invert the diagonal
-// block (Gauss-Jordan with partial pivoting in LDS), scale the blocks below it
-// and update the trailing blocks, in the order of the elimination lists.
-__global__ __launch_bounds__(256) void k_bilu_factor(Dims d, IluDev I, double* BE, int* info) {
+// Inverse of the diagonal block of face slot i of every superelement (one
+// workgroup each): Gauss-Jordan with partial pivoting on [D | I] in LDS
+// (row-major, 2ncf wide), the inverse written back in place.
+__global__ __launch_bounds__(256) void k_bilu_inv(Dims d, IluDev I, int i, double* BE, int* info) {
   const int ncf = d.ncf;
   const int n = blockIdx.x;
   const long per = static_cast<long>(ncf) * ncf;
   extern __shared__ double sm[];
   double* A = sm;                     // ncf x 2ncf, row-major: A[r*2ncf + c]
-  double* Lb = sm + 2 * per;          // ncf x ncf column-major
-  double* fac = Lb + per;             // ncf
-  __shared__ double redv[256];
-  __shared__ int redi[256];
-  const int tid = threadIdx.x;
+  double* fac = sm + 2 * per;         // ncf
+  __shared__ double wv[4];
+  __shared__ int wi[4];
+  __shared__ int pivs;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int w2 = 2 * ncf;
-  int singular = 0;
-  auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per; };
-  for (int i = 0; i < I.nfse; ++i) {
-    double* D = blk(I.diag[i]);
-    for (int idx = tid; idx < ncf * w2; idx += 256) {
-      int r = idx / w2, c = idx % w2;
-      A[idx] = (c < ncf) ? D[r + ncf * c] : ((c - ncf) == r ? 1.0 : 0.0);
-    }
-    __syncthreads();
-    for (int c = 0; c < ncf; ++c) {
-      double best = -1.0;
-      int bi = c;
-      for (int r = c + tid; r < ncf; r += 256) {
-        double vv = fabs(A[r * w2 + c]);
-        if (vv > best) { best = vv; bi = r; }
-      }
-      redv[tid] = best;
-      redi[tid] = bi;
-      __syncthreads();
-      for (int st = 128; st > 0; st >>= 1) {
-        if (tid < st) {
-          double o = redv[tid + st];
-          int oi = redi[tid + st];
-          if (o > redv[tid] || (o == redv[tid] && oi < redi[tid])) { redv[tid] = o; redi[tid] = oi; }
-        }
-        __syncthreads();
-      }
-      const int p = redi[0];
-      if (p != c)
-        for (int k = tid; k < w2; k += 256) {
-          double tt = A[c * w2 + k];
-          A[c * w2 + k] = A[p * w2 + k];
-          A[p * w2 + k] = tt;
-        }
-      __syncthreads();
-      const double pv = A[c * w2 + c];
-      if (pv == 0.0) singular = 1;
-      for (int k = tid; k < w2; k += 256) A[c * w2 + k] /= pv;
-      for (int r = tid; r < ncf; r += 256) fac[r] = A[r * w2 + c];
-      __syncthreads();
-      for (int idx = tid; idx < ncf * w2; idx += 256) {
-        int r = idx / w2, k = idx % w2;
-        if (r != c) A[idx] -= fac[r] * A[c * w2 + k];
-      }
-      __syncthreads();
-    }
-    // D <- D^{-1}
-    for (int idx = tid; idx < static_cast<int>(per); idx += 256) {
-      int r = idx % ncf, c = idx / ncf;
-      D[idx] = A[r * w2 + ncf + c];
-    }
-    __syncthreads();
-    for (int k = I.eptr[i]; k < I.eptr[i + 1]; ++k) {
-      double* Aji = blk(I.elim_ji[k]);
-      // L = A_ji * D^{-1}
-      for (int idx = tid; idx < static_cast<int>(per); idx += 256) {
-        int r = idx % ncf, c = idx / ncf;
-        double acc = 0.0;
-        for (int t = 0; t < ncf; ++t) acc = fma(Aji[r + ncf * t], A[t * w2 + ncf + c], acc);
-        Lb[idx] = acc;
-      }
-      __syncthreads();
-      for (int idx = tid; idx < static_cast<int>(per); idx += 256) Aji[idx] = Lb[idx];
-      for (int u = I.uptr[k]; u < I.uptr[k + 1]; ++u) {
-        double* Ajl = blk(I.upd_jl[u]);
-        const double* Ail = blk(I.upd_il[u]);
-        for (int idx = tid; idx < static_cast<int>(per); idx += 256) {
-          int r = idx % ncf, c = idx / ncf;
-          double acc = Ajl[idx];
-          for (int t = 0; t < ncf; ++t) acc = fma(-Lb[r + ncf * t], Ail[t + ncf * c], acc);
-          Ajl[idx] = acc;
-        }
-      }
-      __syncthreads();
-    }
+  double* D = BE + (static_cast<long>(I.diag[i]) * I.nse + n) * per;
+  for (int idx = tid; idx < ncf * w2; idx += 256) {
+    int r = idx / w2, c = idx % w2;
+    A[idx] = (c < ncf) ? D[r + ncf * c] : ((c - ncf) == r ? 1.0 : 0.0);
   }
-  if (tid == 0 && info) info[n] = singular;
+  __syncthreads();
+  int singular = 0;
+  for (int c = 0; c < ncf; ++c) {
+    double best = -1.0;
+    int bi = c;
+    for (int r = c + tid; r < ncf; r += 256) {
+      double vv = fabs(A[r * w2 + c]);
+      if (vv > best) { best = vv; bi = r; }
+    }
+    argmax_wave(best, bi);
+    if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double b = wv[0];
+      int p = wi[0];
+      for (int w = 1; w < 4; ++w)
+        if (wv[w] > b || (wv[w] == b && wi[w] < p)) { b = wv[w]; p = wi[w]; }
+      pivs = p;
+    }
+    __syncthreads();
+    const int p = pivs;
+    if (p != c)
+      for (int k = tid; k < w2; k += 256) {
+        double tt = A[c * w2 + k];
+        A[c * w2 + k] = A[p * w2 + k];
+        A[p * w2 + k] = tt;
+      }
+    __syncthreads();
+    const double pv = A[c * w2 + c];
+    if (pv == 0.0) singular = 1;
+    const double inv = 1.0 / pv;
+    for (int k = tid; k < w2; k += 256) A[c * w2 + k] *= inv;
+    for (int r = tid; r < ncf; r += 256) fac[r] = A[r * w2 + c];
+    __syncthreads();
+    for (int idx = tid; idx < ncf * w2; idx += 256) {
+      int r = idx / w2, k = idx % w2;
+      if (r != c) A[idx] = fma(-fac[r], A[c * w2 + k], A[idx]);
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < static_cast<int>(per); idx += 256) {
+    int r = idx % ncf, c = idx / ncf;
+    D[idx] = A[r * w2 + ncf + c];
+  }
+  if (tid == 0 && info) info[n] = (i == 0 ? 0 : info[n]) | singular;
+}
+
+// BE[elim_ji[k0 + k]][n] = Ltmp[k][n]
+__global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const double* Ltmp, double* BE) {
+  const long per = static_cast<long>(d.ncf) * d.ncf;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  const long tot = per * I.nse * ne_i;
+  if (q >= tot) return;
+  const long kn = q / per;
+  const int k = static_cast<int>(kn / I.nse);
+  const long n = kn % I.nse;
+  BE[(static_cast<long>(I.elim_ji[k0 + k]) * I.nse + n) * per + q % per] = Ltmp[q];
 }
 
 // One workgroup per superelement: forward sweep y_i = b_i - sum L_ij y_j, backward
 // sweep x_i = D_i^{-1} (y_i - sum U_il x_l).  Y, Xo are laid out [slot = n + nse*i][ncf].
-__global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const double* BE, const double* x,
+template <typename T>
+__global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* BE, const double* x,
                                                     double* Y, double* Xo) {
   const int ncf = d.ncf;
   const int n = blockIdx.x;
@@ -1082,9 +1309,9 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const doub
     double acc = 0.0;
     if (part < nparts)
       for (int q = I.lptr[i] + part; q < I.lptr[i + 1]; q += nparts) {
-        const double* B = blk(I.lidx[q]);
+        const T* B = blk(I.lidx[q]);
         const double* yj = Y + (static_cast<long>(n) + static_cast<long>(I.nse) * I.lcol[q]) * ncf;
-        for (int c = 0; c < ncf; ++c) acc = fma(B[r + ncf * c], yj[c], acc);
+        for (int c = 0; c < ncf; ++c) acc = fma(static_cast<double>(B[r + ncf * c]), yj[c], acc);
       }
     red[tid] = acc;
     __syncthreads();
@@ -1099,9 +1326,9 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const doub
     double acc = 0.0;
     if (part < nparts)
       for (int q = I.uuptr[i] + part; q < I.uuptr[i + 1]; q += nparts) {
-        const double* B = blk(I.uidx[q]);
+        const T* B = blk(I.uidx[q]);
         const double* xj = Xo + (static_cast<long>(n) + static_cast<long>(I.nse) * I.ucol[q]) * ncf;
-        for (int c = 0; c < ncf; ++c) acc = fma(B[r + ncf * c], xj[c], acc);
+        for (int c = 0; c < ncf; ++c) acc = fma(static_cast<double>(B[r + ncf * c]), xj[c], acc);
       }
     red[tid] = acc;
     __syncthreads();
@@ -1112,9 +1339,9 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const doub
     }
     __syncthreads();
     if (tid < ncf) {
-      const double* Dinv = blk(I.diag[i]);
+      const T* Dinv = blk(I.diag[i]);
       double s = 0.0;
-      for (int c = 0; c < ncf; ++c) s = fma(Dinv[tid + ncf * c], z[c], s);
+      for (int c = 0; c < ncf; ++c) s = fma(static_cast<double>(Dinv[tid + ncf * c]), z[c], s);
       Xo[(static_cast<long>(n) + static_cast<long>(I.nse) * i) * ncf + tid] = s;
     }
     __syncthreads();
@@ -1131,6 +1358,55 @@ __global__ void k_bilu_scatter(Dims d, IluDev I, const double* Xo, double* out) 
   double s = Xo[static_cast<long>(I.slot1[f]) * ncf + r];
   if (I.slot2[f] >= 0) s += Xo[static_cast<long>(I.slot2[f]) * ncf + r];
   out[q] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: stored FP32 factors and the condensed RHS of a new residual
+// ---------------------------------------------------------------------------
+__global__ void k_set_identity(double* A, int n, long ld, long stride, int batch) {
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(n) * batch) return;
+  const long b = q / n;
+  const int i = static_cast<int>(q % n);
+  A[b * stride + i + ld * i] = 1.0;
+}
+
+__global__ void k_to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride,
+                           long dstride, int batch) {
+  const long per = static_cast<long>(rows) * cols;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * batch) return;
+  const long b = q / per;
+  const long rc = q % per;
+  const int r = static_cast<int>(rc % rows);
+  const long c = rc / rows;
+  dst[b * dstride + rc] = static_cast<float>(src[b * sstride + r + ld * c]);
+}
+
+// One workgroup per element: RuS = Dinv Ru (FP64 accumulation), then Rh -= K RuS.
+__global__ __launch_bounds__(256) void k_refine_rhs(Dims d, const float* Dinv, const float* K, const double* Ru,
+                                                    double* RuS, double* Rh) {
+  const int n = d.n, m = d.m;
+  const long e = blockIdx.x;
+  extern __shared__ double sm[];
+  double* ru = sm;       // n
+  double* rs = sm + n;   // n
+  for (int t = threadIdx.x; t < n; t += 256) ru[t] = Ru[e * n + t];
+  __syncthreads();
+  const float* De = Dinv + e * static_cast<long>(n) * n;
+  for (int r = threadIdx.x; r < n; r += 256) {
+    double acc = 0.0;
+    for (int c = 0; c < n; ++c) acc = fma(static_cast<double>(De[r + static_cast<long>(n) * c]), ru[c], acc);
+    rs[r] = acc;
+    RuS[e * n + r] = acc;
+  }
+  __syncthreads();
+  const float* Ke = K + e * static_cast<long>(m) * n;
+  for (int r = threadIdx.x; r < m; r += 256) {
+    double acc = 0.0;
+    for (int c = 0; c < n; ++c) acc = fma(static_cast<double>(Ke[r + static_cast<long>(m) * c]), rs[c], acc);
+    Rh[e * m + r] -= acc;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1267,15 +1543,50 @@ void gemm(const Gemm& g, hipStream_t s) {
     dim3 grid(static_cast<unsigned>(tm * batch), tn);
     k_gemm<32, 32, 16, 2, 2><<<grid, 256, 0, s>>>(g, tm);
   } else {
-    int tm = (g.M + 63) / 64, tn = (g.N + 63) / 64;
+    int tm = (g.M + MG_BM - 1) / MG_BM, tn = (g.N + MG_BN - 1) / MG_BN;
     dim3 grid(static_cast<unsigned>(tm * batch), tn);
-    k_gemm<64, 64, 16, 4, 4><<<grid, 256, 0, s>>>(g, tm);
+    k_gemm_mfma<<<grid, 256, 0, s>>>(g, tm);
   }
   EXA_HIP(hipGetLastError());
 }
 
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s) {
   if (batch <= 0) return;
+  if (n >= BLU_MIN_N && ncols > n) {
+    if (static_cast<size_t>(n) * BLU_NB * sizeof(double) > 60 * 1024)
+      throw Error(5, "dense block larger than 480 rows");
+    for (int k0 = 0; k0 < n; k0 += BLU_NB) {
+      const int nb = std::min(BLU_NB, n - k0);
+      size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(double);
+      k_lu_panel<<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info);
+      EXA_HIP(hipGetLastError());
+      if (k0 + nb < n) {  // A22 -= L21 * U12
+        Gemm g{};
+        g.M = n - k0 - nb; g.N = ncols - k0 - nb; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
+        g.A = A + (k0 + nb) + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
+        g.B = A + k0 + static_cast<long>(n) * (k0 + nb); g.ldb = n; g.sB0 = stride;
+        g.C = A + (k0 + nb) + static_cast<long>(n) * (k0 + nb); g.ldc = n; g.sC0 = stride;
+        g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
+        gemm(g, s);
+      }
+    }
+    const int last = ((n - 1) / BLU_NB) * BLU_NB;
+    for (int k0 = last; k0 >= 0; k0 -= BLU_NB) {
+      k_lu_back<<<batch, 256, 0, s>>>(A, n, ncols, stride, k0);
+      EXA_HIP(hipGetLastError());
+      if (k0 > 0) {  // X[0:k0] -= U[0:k0, k0:k0+nb] * X[k0:k0+nb]
+        const int nb = std::min(BLU_NB, n - k0);
+        Gemm g{};
+        g.M = k0; g.N = ncols - n; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
+        g.A = A + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
+        g.B = A + k0 + static_cast<long>(n) * n; g.ldb = n; g.sB0 = stride;
+        g.C = A + static_cast<long>(n) * n; g.ldc = n; g.sC0 = stride;
+        g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
+        gemm(g, s);
+      }
+    }
+    return;
+  }
   // dynamic LDS panel n x LU_NB doubles next to ~12 KB of static LDS, kept within 64 KB
   if (n > 400) throw Error(5, "dense block larger than 400 rows");
   size_t shm = static_cast<size_t>(n) * LU_NB * sizeof(double);
@@ -1366,10 +1677,32 @@ void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull
   EXA_HIP(hipGetLastError());
 }
 
-void post_schur(const Dims& d, int e0, int eb, const double* aug, double* DinvF, double* RuS,
+void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s) {
   long n1 = static_cast<long>(d.n) * (d.m + 1) * eb;
-  k_post_schur<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, DinvF, RuS);
+  k_post_schur<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
+  EXA_HIP(hipGetLastError());
+}
+
+void set_identity(double* A, int n, long ld, long stride, int batch, hipStream_t s) {
+  long n1 = static_cast<long>(n) * batch;
+  if (n1 == 0) return;
+  k_set_identity<<<nblocks(n1, 256), 256, 0, s>>>(A, n, ld, stride, batch);
+  EXA_HIP(hipGetLastError());
+}
+
+void to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
+              hipStream_t s) {
+  long n1 = static_cast<long>(rows) * cols * batch;
+  if (n1 == 0) return;
+  k_to_float<<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
+  EXA_HIP(hipGetLastError());
+}
+
+void refine_rhs(const Dims& d, const float* Dinv, const float* K, const double* Ru, double* RuS, double* Rh,
+                hipStream_t s) {
+  size_t shm = 2 * static_cast<size_t>(d.n) * sizeof(double);
+  k_refine_rhs<<<d.ne, 256, shm, s>>>(d, Dinv, K, Ru, RuS, Rh);
   EXA_HIP(hipGetLastError());
 }
 
@@ -1405,24 +1738,27 @@ void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s) {
   EXA_HIP(hipGetLastError());
 }
 
-void bj_apply(const Dims& d, const double* BE, const double* x, double* y, hipStream_t s) {
+template <typename T>
+void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_t s) {
   long n1 = static_cast<long>(d.ncf) * d.nf;
-  k_bj_apply<<<nblocks(n1, 256), 256, 0, s>>>(d, BE, x, y);
+  k_bj_apply<T><<<nblocks(n1, 256), 256, 0, s>>>(d, BE, x, y);
   EXA_HIP(hipGetLastError());
 }
 
-void matvec_elem(const Dims& d, const double* H, const int* elemcon, const double* v, double* we,
+template <typename T>
+void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we,
                  hipStream_t s) {
   if (d.m > 64 * MV_MAXRB) throw Error(5, "element trace block larger than 1024");
   size_t shm = static_cast<size_t>(5) * d.m * sizeof(double);
-  k_matvec_elem<<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
+  k_matvec_elem<T><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
   EXA_HIP(hipGetLastError());
 }
 
-void dudg(const Dims& d, const double* RuS, const double* DinvF, const int* elemcon, const double* x,
+template <typename T>
+void dudg(const Dims& d, const double* RuS, const T* DinvF, const int* elemcon, const double* x,
           double* du, hipStream_t s) {
   long n1 = static_cast<long>(d.n) * d.ne;
-  k_dudg<<<nblocks(n1, 256), 256, 0, s>>>(d, RuS, DinvF, elemcon, x, du);
+  k_dudg<T><<<nblocks(n1, 256), 256, 0, s>>>(d, RuS, DinvF, elemcon, x, du);
   EXA_HIP(hipGetLastError());
 }
 
@@ -1439,17 +1775,26 @@ void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* el
   EXA_HIP(hipGetLastError());
 }
 
-void bilu_factor(const Dims& d, const IluDev& I, double* BE, int* info, hipStream_t s) {
-  size_t shm = (3 * static_cast<size_t>(d.ncf) * d.ncf + d.ncf) * sizeof(double);
-  if (shm > 60 * 1024 || d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 50 x 50");
-  k_bilu_factor<<<I.nse, 256, shm, s>>>(d, I, BE, info);
+void bilu_invert_diag(const Dims& d, const IluDev& I, int i, double* BE, int* info, hipStream_t s) {
+  size_t shm = (2 * static_cast<size_t>(d.ncf) * d.ncf + d.ncf) * sizeof(double);
+  if (shm > 60 * 1024) throw Error(5, "block-ILU(0) face blocks larger than 61 x 61");
+  k_bilu_inv<<<I.nse, 256, shm, s>>>(d, I, i, BE, info);
   EXA_HIP(hipGetLastError());
 }
 
-void bilu_apply(const Dims& d, const IluDev& I, const double* BE, const double* x, double* Y, double* Xo,
+void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const double* Ltmp, double* BE,
+                hipStream_t s) {
+  long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse * ne_i;
+  if (n1 == 0) return;
+  k_bilu_put_l<<<nblocks(n1, 256), 256, 0, s>>>(d, I, k0, ne_i, Ltmp, BE);
+  EXA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
                 double* out, hipStream_t s) {
   if (d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 256");
-  k_bilu_apply<<<I.nse, 256, 0, s>>>(d, I, BE, x, Y, Xo);
+  k_bilu_apply<T><<<I.nse, 256, 0, s>>>(d, I, BE, x, Y, Xo);
   k_bilu_scatter<<<nblocks(static_cast<long>(d.nf) * d.ncf, 256), 256, 0, s>>>(d, I, Xo, out);
   EXA_HIP(hipGetLastError());
 }
@@ -1502,4 +1847,18 @@ void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, dou
   EXA_HIP(hipGetLastError());
 }
 
+template void bj_apply<double>(const Dims&, const double*, const double*, double*, hipStream_t);
+template void bj_apply<float>(const Dims&, const float*, const double*, double*, hipStream_t);
+template void matvec_elem<double>(const Dims&, const double*, const int*, const double*, double*, hipStream_t);
+template void matvec_elem<float>(const Dims&, const float*, const int*, const double*, double*, hipStream_t);
+template void dudg<double>(const Dims&, const double*, const double*, const int*, const double*, double*,
+                           hipStream_t);
+template void dudg<float>(const Dims&, const double*, const float*, const int*, const double*, double*,
+                          hipStream_t);
+template void bilu_apply<double>(const Dims&, const IluDev&, const double*, const double*, double*, double*,
+                                 double*, hipStream_t);
+template void bilu_apply<float>(const Dims&, const IluDev&, const float*, const double*, double*, double*,
+                                double*, hipStream_t);
+
 }  // namespace exa
+

@@ -12,7 +12,8 @@ struct Dims {
 };
 
 // Column-major batched GEMM  C_b = alpha * A_b * B_b + beta * C_b  (no transposes).
-// Batch index b = b0 + nb0*(b1 + nb1*b2); each operand offset is s0*b0 + s1*b1 + s2*b2.
+// Batch index b = b0 + nb0*(b1 + nb1*b2); each operand offset is s0*i0 + s1*b1 + s2*b2
+// with i0 = b0, or i0 = idx[b0] when that operand's gather index list is given.
 struct Gemm {
   int M, N, K;
   double alpha, beta;
@@ -20,6 +21,9 @@ struct Gemm {
   const double* B; long ldb, sB0, sB1, sB2;
   double* C;       long ldc, sC0, sC1, sC2;
   int nb0, nb1, nb2;
+  const int* iA = nullptr;
+  const int* iB = nullptr;
+  const int* iC = nullptr;
 };
 void gemm(const Gemm& g, hipStream_t s);
 
@@ -58,6 +62,7 @@ struct AsmArgs {
   const int *perm, *elemcon, *bf;
   const double *udg, *uh, *egeo, *fgeo, *og, *ogf;
   double *rv, *rJ, *W, *fpt;
+  long augs;  // per-element stride of the augmented system [D | F | Ru (| I)]
 };
 void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
 void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
@@ -68,7 +73,16 @@ void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long
               hipStream_t s);
 void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull, double* Kt,
               double* Gfull, double* H, hipStream_t s);
-void post_schur(const Dims& d, int e0, int eb, const double* aug, double* DinvF, double* RuS,
+void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
+                hipStream_t s);
+// A_b[i + ld*i] = 1 for i < n (b < batch, batch stride `stride`)
+void set_identity(double* A, int n, long ld, long stride, int batch, hipStream_t s);
+// dst_b[r + rows*c] = (float) src_b[r + ld*c]
+void to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
+              hipStream_t s);
+// Condensed right-hand side of a new residual with stored factors (mixed precision):
+// RuS_e = Dinv_e Ru_e (n x n), Rh_e -= K_e RuS_e (m x n), per element.
+void refine_rhs(const Dims& d, const float* Dinv, const float* K, const double* Ru, double* RuS, double* Rh,
                 hipStream_t s);
 
 // ---- global system ----
@@ -77,15 +91,18 @@ void face_gather(long nnodes, int ncu, const long* src1, const long* src2, const
 void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double* H, double* aug,
                  hipStream_t s);
 void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s);
-void bj_apply(const Dims& d, const double* BE, const double* x, double* y, hipStream_t s);
+template <typename T>
+void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_t s);
 // elemental additive Schwarz: augmented [block | I] for elements [e0, e0+eb), and the
 // inverse (columns m..2m-1 after lu_solve) into BEe (the block's first element)
 void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
                   double* aug, hipStream_t s);
 void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s);
-void matvec_elem(const Dims& d, const double* H, const int* elemcon, const double* v, double* we,
+template <typename T>
+void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we,
                  hipStream_t s);
-void dudg(const Dims& d, const double* RuS, const double* DinvF, const int* elemcon, const double* x,
+template <typename T>
+void dudg(const Dims& d, const double* RuS, const T* DinvF, const int* elemcon, const double* x,
           double* du, hipStream_t s);
 void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipStream_t s);
 
@@ -93,14 +110,20 @@ void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipS
 struct IluDev {
   int nse, nfse, nnz;
   const int *face, *row_ptr, *col_ind, *row_of, *diag;
-  const int *eptr, *elim_ji, *uptr, *upd_jl, *upd_il;
+  const int *eptr, *elim_ji, *uptr, *upd_jl, *upd_il, *upd_ji;
   const int *lptr, *lidx, *lcol, *uuptr, *uidx, *ucol;
   const int *slot1, *slot2;
 };
 void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
                    double* BE, hipStream_t s);
-void bilu_factor(const Dims& d, const IluDev& I, double* BE, int* info, hipStream_t s);
-void bilu_apply(const Dims& d, const IluDev& I, const double* BE, const double* x, double* Y, double* Xo,
+// factorisation steps of face slot i, batched over the superelements: invert the
+// diagonal block (info[n] |= singular), and copy the ne_i blocks L = A_ji D^{-1}
+// computed into Ltmp[k][n] back into place; the products run on gemm()
+void bilu_invert_diag(const Dims& d, const IluDev& I, int i, double* BE, int* info, hipStream_t s);
+void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const double* Ltmp, double* BE,
+                hipStream_t s);
+template <typename T>
+void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
                 double* out, hipStream_t s);
 
 // ---- vector kernels (results stay on the device) ----

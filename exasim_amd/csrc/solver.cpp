@@ -253,6 +253,50 @@ void Solver::setup() {
   if (precond == 1) setup_eas();
 }
 
+// hdgBlockILU0 (matvec.cpp:133-165) batched over the superelements: for each face
+// slot i in order, invert the diagonal block, form L_ji = A_ji D_i^{-1} for the
+// blocks below it, and update A_jl -= L_ji A_il; the products are batched GEMMs
+// whose operands are gathered through the elimination index lists.
+void Solver::factor_bilu0() {
+  const int ncf = d.ncf;
+  const long per = static_cast<long>(ncf) * ncf, slab = per * ilu.nse;
+  for (int i = 0; i < ilu.nfse; ++i) {
+    bilu_invert_diag(d, ilud, i, BEilu.p, info.p, s);
+    const int k0 = ilu.eptr[i], ne_i = ilu.eptr[i + 1] - k0;
+    if (ne_i == 0) continue;
+    Gemm g{};
+    g.M = g.N = g.K = ncf;
+    g.alpha = 1.0; g.beta = 0.0;
+    g.A = BEilu.p; g.iA = ilud.elim_ji + k0; g.lda = ncf; g.sA0 = slab; g.sA1 = per;
+    g.B = BEilu.p + ilu.diag[i] * slab; g.ldb = ncf; g.sB0 = 0; g.sB1 = per;
+    g.C = Lilu.p; g.ldc = ncf; g.sC0 = slab; g.sC1 = per;
+    g.nb0 = ne_i; g.nb1 = ilu.nse; g.nb2 = 1;
+    gemm(g, s);
+    bilu_put_l(d, ilud, k0, ne_i, Lilu.p, BEilu.p, s);
+    const int u0 = ilu.uptr[k0], nu = ilu.uptr[k0 + ne_i] - u0;
+    if (nu == 0) continue;
+    Gemm u{};
+    u.M = u.N = u.K = ncf;
+    u.alpha = -1.0; u.beta = 1.0;
+    u.A = BEilu.p; u.iA = ilud.upd_ji + u0; u.lda = ncf; u.sA0 = slab; u.sA1 = per;
+    u.B = BEilu.p; u.iB = ilud.upd_il + u0; u.ldb = ncf; u.sB0 = slab; u.sB1 = per;
+    u.C = BEilu.p; u.iC = ilud.upd_jl + u0; u.ldc = ncf; u.sC0 = slab; u.sC1 = per;
+    u.nb0 = nu; u.nb1 = ilu.nse; u.nb2 = 1;
+    gemm(u, s);
+  }
+}
+
+void Solver::alloc_mixed() {
+  const size_t ne = d.ne, n = d.n, m = d.m, ncf2 = static_cast<size_t>(d.ncf) * d.ncf;
+  H32.alloc(m * m * ne);
+  DinvF32.alloc(n * m * ne);
+  Dinv32.alloc(n * n * ne);
+  K32.alloc(m * n * ne);
+  if (precond == 0) BE32.alloc(ncf2 * d.nf);
+  if (precond == 1) BEeas32.alloc(m * m * ne);
+  if (precond == 2) BEilu32.alloc(ncf2 * ilu.nnz * ilu.nse);
+}
+
 void Solver::setup_eas() {
   if (d.m > 400) throw Error(5, "elemental additive Schwarz: element trace block larger than 400");
   const double per = 16.0 * d.m * d.m;  // bytes of one augmented [block | I]
@@ -286,6 +330,7 @@ void Solver::setup_bilu0() {
   ilud.uptr = up(iuptr, ilu.uptr);
   ilud.upd_jl = up(iupd_jl, ilu.upd_jl);
   ilud.upd_il = up(iupd_il, ilu.upd_il);
+  ilud.upd_ji = up(iupd_ji, ilu.upd_ji);
   ilud.lptr = up(ilptr, ilu.lptr);
   ilud.lidx = up(ilidx, ilu.lidx);
   ilud.lcol = up(ilcol, ilu.lcol);
@@ -297,19 +342,22 @@ void Solver::setup_bilu0() {
   BEilu.alloc(static_cast<size_t>(ilu.nnz) * ilu.nse * d.ncf * d.ncf);
   Yilu.alloc(static_cast<size_t>(ilu.nfse) * ilu.nse * d.ncf);
   Xilu.alloc(static_cast<size_t>(ilu.nfse) * ilu.nse * d.ncf);
+  int max_elim = 1;
+  for (int i = 0; i < ilu.nfse; ++i) max_elim = std::max(max_elim, ilu.eptr[i + 1] - ilu.eptr[i]);
+  Lilu.alloc(static_cast<size_t>(max_elim) * ilu.nse * d.ncf * d.ncf);
   if (info.n < static_cast<size_t>(ilu.nse)) info.alloc(ilu.nse + 1);
   ilu_ready = true;
 }
 
 void Solver::alloc_workspace() {
   const long n = d.n, m = d.m;
-  long per = n * (n + m + 1) + n * n * d.nd + m * n + m * n * d.nd +
+  long per = n * augcols() + n * n * d.nd + m * n + m * n * d.nd +
              static_cast<long>(d.nd + 1) * d.ncu * d.nge + static_cast<long>(d.nd + 1) * d.ncu * d.nc * d.nge +
              static_cast<long>(d.npe) * d.nge * d.ncu * d.nc +
              2L * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf;
   long cap = static_cast<long>(ws_budget_bytes / 8.0 / per);
   neb = static_cast<int>(std::max(1L, std::min<long>(d.ne, cap)));
-  aug.alloc(static_cast<size_t>(neb) * n * (n + m + 1));
+  aug.alloc(static_cast<size_t>(neb) * n * augcols());
   Bfull.alloc(static_cast<size_t>(neb) * n * n * d.nd);
   Kt.alloc(static_cast<size_t>(neb) * m * n);
   Gfull.alloc(static_cast<size_t>(neb) * m * n * d.nd);
@@ -346,6 +394,17 @@ void Solver::set_option(const std::string& name, double v) {
     assembled = false;
   } else if (name == "ortho") ortho = static_cast<int>(v);
   else if (name == "verbose") verbose = static_cast<int>(v);
+  else if (name == "precision") {
+    int pv = static_cast<int>(v);
+    if (pv != 0 && pv != 1) throw Error(1, "precision: 0 (FP64) or 1 (mixed FP32/FP64)");
+    if (pv != precision) {
+      precision = pv;
+      alloc_workspace();
+      assembled = false;
+    }
+  } else if (name == "refine_rtol") refine_rtol = v;
+  else if (name == "refine_tol") refine_tol = v;
+  else if (name == "refine_maxit") refine_maxit = static_cast<int>(v);
   else if (name == "timing") timing = v != 0.0;
   else if (name == "workspace_bytes") {
     ws_budget_bytes = v;
@@ -355,6 +414,7 @@ void Solver::set_option(const std::string& name, double v) {
 
 AsmArgs Solver::asm_args(int e0, int eb) {
   AsmArgs a{};
+  a.augs = static_cast<long>(d.n) * augcols();
   a.model = model;
   a.time = 0.0;
   a.param = param.p;
@@ -447,14 +507,16 @@ void Solver::assemble() {
   if (precond < 0 || precond > 2)
     throw Error(5, "preconditioner " + std::to_string(precond) + " is not implemented");
   const long n = d.n, m = d.m, nd = d.nd, ncu = d.ncu, npe = d.npe, ndf = d.ndf;
-  const long augs = n * (n + m + 1);
+  const long augs = n * augcols();
   const long blk = static_cast<long>(npe) * d.nge;
+  if (precision) alloc_mixed();
   hipEvent_t t0{};
   for (int e0 = 0; e0 < d.ne; e0 += neb) {
     const int eb = std::min(neb, d.ne - e0);
     AsmArgs a = asm_args(e0, eb);
     tic(T_ASSEMBLY, &t0);
     EXA_HIP(hipMemsetAsync(aug.p, 0, sizeof(double) * augs * eb, s));
+    if (precision) set_identity(aug.p + n * (n + m + 1), static_cast<int>(n), n, augs, eb, s);
     EXA_HIP(hipMemsetAsync(Kt.p, 0, sizeof(double) * m * n * eb, s));
     EXA_HIP(hipMemsetAsync(Gfull.p, 0, sizeof(double) * m * n * nd * eb, s));
     EXA_HIP(hipMemsetAsync(H.p + e0 * m * m, 0, sizeof(double) * m * m * eb, s));
@@ -515,7 +577,7 @@ void Solver::assemble() {
     toc(T_ASSEMBLY, t0);
     // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru]
     tic(T_LU, &t0);
-    lu_solve(aug.p, static_cast<int>(n), static_cast<int>(n + m + 1), augs, eb, info.p, s);
+    lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s);
     toc(T_LU, t0);
     tic(T_ASSEMBLY, &t0);
     {  // H += K * DinvF (= -K X_F), Rh -= K * (D^{-1} Ru)
@@ -531,7 +593,17 @@ void Solver::assemble() {
       g.C = Rh.p + e0 * m; g.sC0 = m;
       gemm(g, s);
     }
-    post_schur(d, e0, eb, aug.p, DinvF.p, RuS.p, s);
+    post_schur(d, e0, eb, aug.p, augs, DinvF.p, RuS.p, s);
+    if (precision) {  // FP32 copies of this block's H, DinvF, D'^{-1} and K'
+      const long E0 = e0;
+      to_float(H32.p + E0 * m * m, H.p + E0 * m * m, static_cast<int>(m), static_cast<int>(m), m, m * m, m * m,
+               eb, s);
+      to_float(DinvF32.p + E0 * n * m, DinvF.p + E0 * n * m, static_cast<int>(n), static_cast<int>(m), n, n * m,
+               n * m, eb, s);
+      to_float(Dinv32.p + E0 * n * n, aug.p + n * (n + m + 1), static_cast<int>(n), static_cast<int>(n), n, augs,
+               n * n, eb, s);
+      to_float(K32.p + E0 * m * n, Kt.p, static_cast<int>(m), static_cast<int>(n), m, m * n, m * n, eb, s);
+    }
     toc(T_ASSEMBLY, t0);
     EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
     EXA_HIP(hipStreamSynchronize(s));
@@ -563,9 +635,18 @@ void Solver::assemble() {
   } else {
     // hdgBlockILU0 (matvec.cpp:119-166)
     bilu_assemble(d, ilud, f2e.p, elemcon.p, H.p, BEilu.p, s);
-    bilu_factor(d, ilud, BEilu.p, info.p, s);
+    factor_bilu0();
   }
   toc(T_LU, t0);
+  if (precision) {
+    const long c2 = static_cast<long>(d.ncf) * d.ncf;
+    if (precond == 0) to_float(BE32.p, BE.p, d.ncf, d.ncf, d.ncf, c2, c2, d.nf, s);
+    if (precond == 1) {
+      const long mm = static_cast<long>(d.m) * d.m;
+      to_float(BEeas32.p, BEeas.p, d.m, d.m, d.m, mm, mm, d.ne, s);
+    }
+    if (precond == 2) to_float(BEilu32.p, BEilu.p, d.ncf, d.ncf, d.ncf, c2, c2, ilu.nnz * ilu.nse, s);
+  }
   if (precond != 1) {
     const int nchk = std::min(precond == 2 ? ilu.nse : d.nf, 8192);
     EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * nchk, hipMemcpyDeviceToHost, s));
@@ -597,7 +678,8 @@ void Solver::residual(double* nb, double* nru) {
 void Solver::matvec(const double* v, double* w) {
   hipEvent_t t0{};
   tic(T_MATVEC, &t0);
-  matvec_elem(d, H.p, elemcon.p, v, we.p, s);
+  if (precision) matvec_elem(d, H32.p, elemcon.p, v, we.p, s);
+  else matvec_elem(d, H.p, elemcon.p, v, we.p, s);
   toc(T_MATVEC, t0);
   tic(T_OTHER, &t0);
   face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, w, s);
@@ -609,24 +691,28 @@ void Solver::apply_precond(double* xv) {
   tic(T_PRECOND, &t0);
   copy(tmp.p, xv, d.N, s);
   if (precond == 2) {
-    bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
+    if (precision) bilu_apply(d, ilud, BEilu32.p, tmp.p, Yilu.p, Xilu.p, xv, s);
+    else bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
   } else if (precond == 1) {  // x = sum_e scatter(BE_e^{-1} gather_e(x)), as hdgMatVec with BE
-    matvec_elem(d, BEeas.p, elemcon.p, tmp.p, we.p, s);
+    if (precision) matvec_elem(d, BEeas32.p, elemcon.p, tmp.p, we.p, s);
+    else matvec_elem(d, BEeas.p, elemcon.p, tmp.p, we.p, s);
     face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, xv, s);
   } else {
-    bj_apply(d, BE.p, tmp.p, xv, s);
+    if (precision) bj_apply(d, BE32.p, tmp.p, xv, s);
+    else bj_apply(d, BE.p, tmp.p, xv, s);
   }
   toc(T_PRECOND, t0);
 }
 
-int Solver::gmres(const double* bv, double* xv) {
+int Solver::gmres(const double* bv, double* xv, double tol_rel) {
   const long N = d.N;
   double nrmb = norm(bv, N);
   EXA_HIP(hipMemsetAsync(xv, 0, sizeof(double) * N, s));
-  if (nrmb < newton_tol) return 0;
+  if (nrmb < newton_tol && tol_rel < 0.0) return 0;
+  if (nrmb == 0.0) return 0;
   copy(r.p, bv, N, s);
   double nrmr = nrmb;
-  double tol = std::min(0.1, gmres_tol * nrmb / nrmr);
+  double tol = std::min(0.1, (tol_rel < 0.0 ? gmres_tol : tol_rel) * nrmb / nrmr);
   apply_precond(r.p);
   nrmb = norm(r.p, N);
   nrmr = nrmb;
@@ -711,7 +797,8 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
     double alpha = 1.0;
     axpby(u.p, x.p, 1.0, 1.0, N, s);  // u += x
     copy(uh.p, u.p, N, s);
-    dudg(d, RuS.p, DinvF.p, elemcon.p, x.p, du.p, s);
+    if (precision) dudg(d, RuS.p, DinvF32.p, elemcon.p, x.p, du.p, s);
+    else dudg(d, RuS.p, DinvF.p, elemcon.p, x.p, du.p, s);
     update_udg(d, du.p, alpha, udg.p, s);
     get_q();
     double nrm0 = nrmr;
@@ -733,6 +820,28 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
     if (verbose)
       std::printf("Newton Iteration: %d, Alpha: %g, Original Norm: %g,  Updated Norm: %g  (GMRES %d)\n",
                   it + 1, alpha, nrm0, nrmr, gi);
+    // mixed precision: iterative refinement with the FP64 residual and the stored FP32
+    // factors, until the residual is refine_rtol times this iteration's first residual
+    last_refinements = 0;
+    while (precision && nrmr > refine_rtol * nrm0 && last_refinements < refine_maxit) {
+      refine_rhs(d, Dinv32.p, K32.p, RuR.p, RuS.p, Rh.p, s);
+      face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
+      gi = gmres(b.p, x.p, refine_tol);
+      gtot += gi;
+      axpby(u.p, x.p, 1.0, 1.0, N, s);
+      copy(uh.p, u.p, N, s);
+      dudg(d, RuS.p, DinvF32.p, elemcon.p, x.p, du.p, s);
+      update_udg(d, du.p, 1.0, udg.p, s);
+      get_q();
+      residual(&nb, &nru);
+      nrmr = nb + nru;
+      ++last_refinements;
+      if (std::isnan(nrmr)) throw Error(4, "NaN residual in refinement");
+      if (verbose) {
+        std::printf("  refinement %d: residual norm %g (GMRES %d)\n", last_refinements, nrmr, gi);
+        std::fflush(stdout);
+      }
+    }
     if (nrmr < newton_tol) {
       ++it;
       break;

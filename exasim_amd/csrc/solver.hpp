@@ -29,6 +29,12 @@ struct Solver {
   // options (app.bin defaults, overridable)
   int precond = 0, ortho = 0, restart = 50, gmres_maxit = 200, newton_maxit = 20, verbose = 0;
   double gmres_tol = 1e-8, newton_tol = 1e-6;
+  // precision 0: FP64 throughout; 1: FP32 storage of the condensed operator, the local
+  // solve factors and the preconditioner, with FP64 residuals and an iterative-refinement
+  // loop to refine_rtol * (first residual norm), corrections solved to refine_tol
+  int precision = 0, refine_maxit = 5;
+  double refine_rtol = 1e-10, refine_tol = 1e-4;
+  int last_refinements = 0;
   bool timing = false;
   double ws_budget_bytes = 4.0e9;
 
@@ -42,15 +48,17 @@ struct Solver {
   // per-block workspace
   int neb = 0;
   DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
+  // mixed precision: FP32 copies of H, DinvF, D'^{-1}, K' and the preconditioner blocks
+  DevBuf<float> H32, DinvF32, Dinv32, K32, BE32, BEilu32, BEeas32;
   // elemental additive Schwarz: inverted element blocks, inversion scratch
   DevBuf<double> BEeas, easaug;
   int eas_chunk = 0;
   // superelement block-ILU(0)
   Bilu0Struct ilu;
   bool ilu_ready = false;
-  DevBuf<int> iface, irow_ptr, icol_ind, irow_of, idiag, ieptr, ielim_ji, iuptr, iupd_jl, iupd_il, ilptr,
-      ilidx, ilcol, iuuptr, iuidx, iucol, islot1, islot2;
-  DevBuf<double> BEilu, Yilu, Xilu;
+  DevBuf<int> iface, irow_ptr, icol_ind, irow_of, idiag, ieptr, ielim_ji, iuptr, iupd_jl, iupd_il, iupd_ji,
+      ilptr, ilidx, ilcol, iuuptr, iuidx, iucol, islot1, islot2;
+  DevBuf<double> BEilu, Yilu, Xilu, Lilu;
   IluDev ilud{};
   // GMRES
   DevBuf<double> V, x, r, tmp, wv, partial, hcol, ydev, scal;
@@ -74,7 +82,9 @@ struct Solver {
   void residual(double* nb, double* nru);
   void matvec(const double* v, double* w);
   void apply_precond(double* x);
-  int gmres(const double* bvec, double* xvec);
+  int gmres(const double* bvec, double* xvec, double tol_rel = -1.0);
+  long augcols() const { return static_cast<long>(d.n) + d.m + 1 + (precision ? d.n : 0); }
+  void alloc_mixed();
   void solve(int* newton_iters, int* gmres_iters, double* final_norm);
   void write_output();
   long get_array(const std::string& name, double* dst, long len);
@@ -86,6 +96,7 @@ struct Solver {
  private:
   void setup();
   void setup_bilu0();
+  void factor_bilu0();
   void setup_eas();
   void alloc_workspace();
   AsmArgs asm_args(int e0, int eb);

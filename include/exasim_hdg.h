@@ -67,7 +67,11 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * additive Schwarz, 2 superelement block-ILU(0), which needs cartgridpart in mesh.bin),
  * "ortho" (0 MGS, 1 CGS), "verbose" (1: Newton and GMRES restart lines on stdout),
  * "timing" (1: record per-kernel HIP events), "workspace_bytes" (element-block
- * workspace budget of the assembly). */
+ * workspace budget of the assembly), "precision" (0 FP64; 1 mixed: the condensed
+ * operator, the local-solve factors and the preconditioner blocks are stored in FP32,
+ * residuals, LU pivots and GMRES vectors stay FP64, and each Newton step is refined
+ * with FP64 residuals until ||R|| <= "refine_rtol" (1e-10) times its first residual,
+ * at most "refine_maxit" (5) corrections, each solved by GMRES to "refine_tol" (1e-4)). */
 int exa_set_option(exa_handle* h, const char* name, double value);
 
 /* Full steady solve (NewtonSolver).  newton_iters/gmres_iters/final_norm may be NULL. */

@@ -1,0 +1,93 @@
+"""Mixed precision (option "precision" = 1): FP32 storage of the condensed operator,
+the local-solve factors and the preconditioner blocks, FP64 residuals and iterative
+refinement.  The converged solution agrees with the FP64 oracle to 1e-8 relative
+(the stated tolerance of the mixed form) and the final residual is at most
+refine_rtol (1e-10) times the first one.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import hdg as ora
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-8
+
+CASES = {
+    "cd_bj": ("convdiff3d", dict(n=2, porder=4, ncu=2, restart=100, gmres_tol=1e-12), "convdiff"),
+    "cd_ilu": ("convdiff3d", dict(n=4, porder=2, ncu=2, restart=40, gmres_tol=1e-12, precond="bilu0",
+                                  cartgridpart=[3, 4, 4, 4, 2, 2, 2, 0]), "convdiff"),
+    "p2_eas": ("poisson2d", dict(n=8, porder=3, pgauss=8, precond=1), "poisson"),
+}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def built(case_dir):
+    out = {}
+
+    def get(name):
+        if name not in out:
+            builder, kw, model = CASES[name]
+            path, info = case_dir("mixed_" + name, builder, **kw)
+            out[name] = (path, model)
+        return out[name]
+    return get
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_solve_matches_fp64_oracle(built, name):
+    path, model = built(name)
+    say("case %s: mixed-precision solve" % name)
+    from exasim_amd.hdg import HDGSolver
+    P = ora.HDGProblem(path, model)
+    r0 = None
+    ora.newton_solve(P)
+    with HDGSolver(path, model) as S:
+        S.set_option("precision", 1)
+        S.get_q()
+        nb, nru = S.residual()
+        r0 = nb + nru
+        S.reset()
+        it, gits, nrm = S.solve()
+        z = S.sizes
+        uh = S.get("uh")
+        udg = S.get("udg").reshape(z["ne"], z["nc"], z["npe"]).transpose(0, 2, 1)
+    say("  gmres %d, residual %.3e -> %.3e, err uh %.2e" % (gits, r0, nrm, rel(uh, P.uh.reshape(-1))))
+    assert nrm <= 1e-10 * r0 * 1.0001 or nrm < 1e-14
+    assert rel(uh, P.uh.reshape(-1)) < MIXED_TOL
+    assert rel(udg, P.udg) < MIXED_TOL
+
+
+def test_mixed_matvec_uses_fp32_operator(built):
+    """The mixed matvec multiplies by the FP32-rounded H with FP64 accumulation."""
+    path, model = built("cd_bj")
+    say("case cd_bj: FP32 matvec")
+    from exasim_amd.hdg import HDGSolver
+    v = np.random.default_rng(9).standard_normal(1)
+    with HDGSolver(path, model) as S:
+        S.set_option("precision", 1)
+        S.get_q()
+        S.assemble()
+        z = S.sizes
+        H = S.get("H").reshape(z["ne"], z["m"], z["m"]).transpose(0, 2, 1)
+        v = np.random.default_rng(9).standard_normal(z["N"])
+        px, _ = S.device_ptr("x")
+        pw, _ = S.device_ptr("w")
+        S.set("x", v)
+        S.matvec(px, pw)
+        w = S.get("w")
+    P = ora.HDGProblem(path, model)
+    ref = P.matvec(H.astype(np.float32).astype(np.float64), v)
+    assert rel(w, ref) < 1e-12
+    assert rel(w, P.matvec(H, v)) > 1e-10          # it is not the FP64 operator

@@ -151,7 +151,7 @@ def main():
         S.sync()
         S.set_option("timing", 1)
         S.reset_timings()
-        lat, gm, res = [], [], []
+        lat, gm, res, refs = [], [], [], []
         t_all = time.perf_counter()
         for k in range(args.steps):
             S.reset()
@@ -160,8 +160,10 @@ def main():
             lat.append(time.perf_counter() - t0)
             gm.append(gi)
             res.append(nrm)
-            print("# %s step %d: %.3f s  newton %d gmres %d residual %.3e"
-                  % (PREC_NAME[precision], k, lat[-1], ni, gi, nrm), file=sys.stderr, flush=True)
+            nref = S.state()["refinements"]
+            refs.append(nref)
+            print("# %s step %d: %.3f s  newton %d gmres %d refinements %d residual %.3e"
+                  % (PREC_NAME[precision], k, lat[-1], ni, gi, nref, nrm), file=sys.stderr, flush=True)
         t_all = time.perf_counter() - t_all
         tm = S.timings()
         uh = S.get("uh")
@@ -171,7 +173,7 @@ def main():
         bytes_mv = sH * m * m * ne + 8.0 * m * ne + 4.0 * (m // ncu) * ne + 8.0 * m * ne
         achieved = bytes_mv / mv_avg_s / 1e9 if mv_cnt else None
         lat_ms = np.array(lat) * 1e3
-        return dict(solves_s=args.steps / t_all, lat_ms=lat_ms, gm=gm, res=res, tm=tm, uh=uh,
+        return dict(solves_s=args.steps / t_all, lat_ms=lat_ms, gm=gm, res=res, tm=tm, uh=uh, refs=refs,
                     roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                               "kernel": "k_matvec_elem<%s>" % ("float" if precision else "double"),
@@ -240,6 +242,7 @@ def main():
         "dof_per_s": N * solves_s,
         "newton_steps_per_s": solves_s,
         "gmres_iterations": gm,
+        "refinements": head["refs"],
         "final_residual": head["res"],
         "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in tm.items()},
         "roofline": head["roofline"],

@@ -71,9 +71,9 @@ int exa_sizes(const exa_handle* h, int* out, int len) {
   if (!h || !h->solver || !out) return 0;
   const exa::Solver& s = *h->solver;
   const exa::Dims& d = s.d;
-  int v[16] = {d.nd, d.ncu, d.nc, d.npe, d.npf, d.nfe, d.ne, d.nf, d.n, d.m, static_cast<int>(d.N),
-               d.nge, d.ngf, d.nco, s.restart, s.precond};
-  int k = len < 16 ? len : 16;
+  int v[18] = {d.nd, d.ncu, d.nc, d.npe, d.npf, d.nfe, d.ne, d.nf, d.n, d.m, static_cast<int>(d.N),
+               d.nge, d.ngf, d.nco, s.restart, s.precond, s.precision, s.last_refinements};
+  int k = len < 18 ? len : 18;
   for (int i = 0; i < k; ++i) out[i] = v[i];
   return k;
 }

@@ -92,14 +92,28 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g, int tilesM) {
 // instruction: lane l supplies A[l%16][l/16] and B[l/16][l%16] of a 16x4 / 4x16
 // step; accumulator register i of lane l is D[l/16 + 4i][l%16].
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 constexpr int MG_BM = 64, MG_BN = 64, MG_BK = 32;
 
 // LDS is fragment-major: element (row r, k) of the A tile sits at
 // (r/16)*16*MG_BK + k*16 + r%16, so the 64 lanes of one MFMA operand read 64
-// consecutive doubles (no bank conflicts); B likewise with its columns.
+// consecutive values (no bank conflicts); B likewise with its columns.
 __device__ inline int mg_off(int rc, int k) { return (rc >> 4) * (16 * MG_BK) + k * 16 + (rc & 15); }
 
+// One 16x16x4 step.  FP64: register i of lane l holds D[l/16 + 4i][l%16];
+// FP32 (inputs rounded to FP32, FP32 accumulation): D[4(l/16) + i][l%16].
+__device__ inline dbl4 mfma_step(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline flt4 mfma_step(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ inline int mfma_row(double, int lk, int i) { return lk + 4 * i; }
+__device__ inline int mfma_row(float, int lk, int i) { return 4 * lk + i; }
+
+template <typename CT>
 __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
+  typedef CT acc4 __attribute__((ext_vector_type(4)));
   const int tile = blockIdx.x % tilesM;
   const long b = blockIdx.x / tilesM;
   const int b0 = static_cast<int>(b % g.nb0);
@@ -111,8 +125,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
   const double* B = g.B + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
   double* C = g.C + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
   const int row0 = tile * MG_BM, col0 = blockIdx.y * MG_BN;
-  __shared__ double As[MG_BM * MG_BK];
-  __shared__ double Bs[MG_BN * MG_BK];
+  __shared__ CT As[MG_BM * MG_BK];
+  __shared__ CT Bs[MG_BN * MG_BK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave & 1, wc = wave >> 1;
   // global -> register staging: A row tid%64, k = tid/64 + 4j; B k = tid%32, column tid/32 + 8j
@@ -130,30 +144,30 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
       rb[j] = (gk2 < g.K && gc < g.N) ? B[gk2 + g.ldb * gc] : 0.0;
     }
   };
-  dbl4 acc[2][2];
+  acc4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc4{0, 0, 0, 0};
   load(0);
   const int li = lane & 15, lk = lane >> 4;
   for (int k0 = 0; k0 < g.K; k0 += MG_BK) {
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-      As[mg_off(ar, ak + 4 * j)] = ra[j];
-      Bs[mg_off(bc + 8 * j, bk)] = rb[j];
+      As[mg_off(ar, ak + 4 * j)] = static_cast<CT>(ra[j]);
+      Bs[mg_off(bc + 8 * j, bk)] = static_cast<CT>(rb[j]);
     }
     __syncthreads();
     if (k0 + MG_BK < g.K) load(k0 + MG_BK);
 #pragma unroll
     for (int kk = 0; kk < MG_BK; kk += 4) {
       const int kr = kk + lk;
-      const double a0v = As[mg_off(32 * wr + li, kr)], a1v = As[mg_off(32 * wr + 16 + li, kr)];
-      const double b0v = Bs[mg_off(32 * wc + li, kr)], b1v = Bs[mg_off(32 * wc + 16 + li, kr)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0v, b0v, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0v, b1v, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1v, b0v, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1v, b1v, acc[1][1], 0, 0, 0);
+      const CT a0v = As[mg_off(32 * wr + li, kr)], a1v = As[mg_off(32 * wr + 16 + li, kr)];
+      const CT b0v = Bs[mg_off(32 * wc + li, kr)], b1v = Bs[mg_off(32 * wc + 16 + li, kr)];
+      acc[0][0] = mfma_step(a0v, b0v, acc[0][0]);
+      acc[0][1] = mfma_step(a0v, b1v, acc[0][1]);
+      acc[1][0] = mfma_step(a1v, b0v, acc[1][0]);
+      acc[1][1] = mfma_step(a1v, b1v, acc[1][1]);
     }
     __syncthreads();
   }
@@ -163,10 +177,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int r = row0 + 32 * wr + 16 * ti + lk + 4 * i;
+        const int r = row0 + 32 * wr + 16 * ti + mfma_row(CT(0), lk, i);
         const int c = col0 + 32 * wc + 16 * tj + li;
         if (r < g.M && c < g.N) {
-          double v = g.alpha * acc[ti][tj][i];
+          double v = g.alpha * static_cast<double>(acc[ti][tj][i]);
           if (g.beta != 0.0) v = fma(g.beta, C[r + g.ldc * static_cast<long>(c)], v);
           C[r + g.ldc * static_cast<long>(c)] = v;
         }
@@ -180,7 +194,6 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
 // panel and solves L11 U12 = A12; the trailing update A22 -= L21 U12 and the
 // back substitution's updates run on the MFMA GEMM.
 // ---------------------------------------------------------------------------
-constexpr int BLU_NB = 16;
 constexpr int BLU_MIN_N = 64;  // smaller blocks use the single-kernel k_lu_solve
 
 __device__ inline void argmax_wave(double& v, int& i) {
@@ -192,6 +205,7 @@ __device__ inline void argmax_wave(double& v, int& i) {
   }
 }
 
+template <int BLU_NB>
 __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols, long stride, int k0,
                                                   int* info) {
   double* A = Aall + stride * blockIdx.x;
@@ -291,35 +305,38 @@ __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols
 }
 
 // X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
+template <int BLU_NB>
 __global__ __launch_bounds__(256) void k_lu_back(double* Aall, int n, int ncols, long stride, int k0) {
   double* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
   __shared__ double U[BLU_NB][BLU_NB + 1];
+  __shared__ double dinv[BLU_NB];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < nb * nb; idx += 256) {
     int r = idx % nb, c = idx / nb;
     U[r][c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
   }
   __syncthreads();
-  for (int j = n + tid; j < ncols; j += 256) {
-    double* colj = A + static_cast<long>(n) * j + k0;
-    double x[BLU_NB];
+  if (tid < nb) dinv[tid] = 1.0 / U[tid][tid];
+  __syncthreads();
+  // one right-hand-side column per thread (blockIdx.y selects a chunk of 256)
+  const int j = n + blockIdx.y * 256 + tid;
+  if (j >= ncols) return;
+  double* colj = A + static_cast<long>(n) * j + k0;
+  double x[BLU_NB];
 #pragma unroll
-    for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
+  for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
 #pragma unroll
-    for (int kk = BLU_NB - 1; kk >= 0; --kk) {
-      if (kk < nb) {
-        double v = x[kk];
+  for (int kk = BLU_NB - 1; kk >= 0; --kk) {
+    if (kk < nb) {
+      x[kk] *= dinv[kk];
 #pragma unroll
-        for (int t = kk + 1; t < BLU_NB; ++t)
-          if (t < nb) v = fma(-U[kk][t], x[t], v);
-        x[kk] = v / U[kk][kk];
-      }
+      for (int t = 0; t < kk; ++t) x[t] = fma(-U[t][kk], x[kk], x[t]);
     }
-#pragma unroll
-    for (int kk = 0; kk < BLU_NB; ++kk)
-      if (kk < nb) colj[kk] = x[kk];
   }
+#pragma unroll
+  for (int kk = 0; kk < BLU_NB; ++kk)
+    if (kk < nb) colj[kk] = x[kk];
 }
 
 // ---------------------------------------------------------------------------
@@ -1097,12 +1114,14 @@ __global__ void k_bj_apply(Dims d, const T* BE, const double* x, double* y) {
 
 // hdgMatVec element part (matvec.cpp:377-380): we_e = H_e * v_e with v_e gathered
 // through elemcon.  One workgroup per element; each of the 4 waves owns a
-// quarter of the columns, lanes own rows (coalesced column reads of H), and the
-// four partial sums are added in LDS in a fixed order.
-constexpr int MV_MAXRB = 16;  // m <= 1024
-template <typename T>
+// quarter of the columns, lanes own VW consecutive rows (16-byte vector loads of
+// the column), accumulation is FP64, and the four partial sums are added in LDS
+// in a fixed order.
+template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const int* elemcon,
                                                      const double* v, double* we) {
+  typedef T vecT __attribute__((ext_vector_type(VW)));
+  constexpr int MAXRB = (1024 + 64 * VW - 1) / (64 * VW);  // m <= 1024
   const int m = d.m, ncu = d.ncu, ndf = d.ndf;
   const long e = blockIdx.x;
   extern __shared__ double sm[];
@@ -1114,24 +1133,32 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const i
   __syncthreads();
   const int cper = (m + 3) / 4;
   const int c0 = wave * cper, c1 = min(m, c0 + cper);
-  const int nrb = (m + 63) / 64;
-  double acc[MV_MAXRB];
+  const int nrb = (m + 64 * VW - 1) / (64 * VW);
+  double acc[MAXRB][VW];
 #pragma unroll
-  for (int rb = 0; rb < MV_MAXRB; ++rb) acc[rb] = 0.0;
+  for (int rb = 0; rb < MAXRB; ++rb)
+#pragma unroll
+    for (int q = 0; q < VW; ++q) acc[rb][q] = 0.0;
   const T* He = H + e * static_cast<long>(m) * m;
   for (int c = c0; c < c1; ++c) {
     const double vc = ve[c];
     const T* col = He + static_cast<long>(m) * c;
 #pragma unroll
-    for (int rb = 0; rb < MV_MAXRB; ++rb) {
-      const int r = lane + 64 * rb;
-      if (rb < nrb && r < m) acc[rb] = fma(static_cast<double>(col[r]), vc, acc[rb]);
+    for (int rb = 0; rb < MAXRB; ++rb) {
+      const int r = VW * (lane + 64 * rb);
+      if (rb < nrb && r < m) {
+        const vecT h = *reinterpret_cast<const vecT*>(col + r);
+#pragma unroll
+        for (int q = 0; q < VW; ++q) acc[rb][q] = fma(static_cast<double>(h[q]), vc, acc[rb][q]);
+      }
     }
   }
 #pragma unroll
-  for (int rb = 0; rb < MV_MAXRB; ++rb) {
-    const int r = lane + 64 * rb;
-    if (rb < nrb && r < m) red[wave * m + r] = acc[rb];
+  for (int rb = 0; rb < MAXRB; ++rb) {
+    const int r = VW * (lane + 64 * rb);
+    if (rb < nrb && r < m)
+#pragma unroll
+      for (int q = 0; q < VW; ++q) red[wave * m + r + q] = acc[rb][q];
   }
   __syncthreads();
   for (int r = tid; r < m; r += 256)
@@ -1545,46 +1572,54 @@ void gemm(const Gemm& g, hipStream_t s) {
   } else {
     int tm = (g.M + MG_BM - 1) / MG_BM, tn = (g.N + MG_BN - 1) / MG_BN;
     dim3 grid(static_cast<unsigned>(tm * batch), tn);
-    k_gemm_mfma<<<grid, 256, 0, s>>>(g, tm);
+    if (g.fp32) k_gemm_mfma<float><<<grid, 256, 0, s>>>(g, tm);
+    else k_gemm_mfma<double><<<grid, 256, 0, s>>>(g, tm);
   }
   EXA_HIP(hipGetLastError());
+}
+
+template <int NB>
+void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s) {
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = std::min(NB, n - k0);
+    size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(double);
+    k_lu_panel<NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info);
+    EXA_HIP(hipGetLastError());
+    if (k0 + nb < n) {  // A22 -= L21 * U12
+      Gemm g{};
+      g.M = n - k0 - nb; g.N = ncols - k0 - nb; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
+      g.A = A + (k0 + nb) + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
+      g.B = A + k0 + static_cast<long>(n) * (k0 + nb); g.ldb = n; g.sB0 = stride;
+      g.C = A + (k0 + nb) + static_cast<long>(n) * (k0 + nb); g.ldc = n; g.sC0 = stride;
+      g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
+      gemm(g, s);
+    }
+  }
+  const int last = ((n - 1) / NB) * NB;
+  for (int k0 = last; k0 >= 0; k0 -= NB) {
+    dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 255) / 256));
+    k_lu_back<NB><<<grid, 256, 0, s>>>(A, n, ncols, stride, k0);
+    EXA_HIP(hipGetLastError());
+    if (k0 > 0) {  // X[0:k0] -= U[0:k0, k0:k0+nb] * X[k0:k0+nb]
+      const int nb = std::min(NB, n - k0);
+      Gemm g{};
+      g.M = k0; g.N = ncols - n; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
+      g.A = A + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
+      g.B = A + k0 + static_cast<long>(n) * n; g.ldb = n; g.sB0 = stride;
+      g.C = A + static_cast<long>(n) * n; g.ldc = n; g.sC0 = stride;
+      g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
+      gemm(g, s);
+    }
+  }
 }
 
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s) {
   if (batch <= 0) return;
   if (n >= BLU_MIN_N && ncols > n) {
-    if (static_cast<size_t>(n) * BLU_NB * sizeof(double) > 60 * 1024)
-      throw Error(5, "dense block larger than 480 rows");
-    for (int k0 = 0; k0 < n; k0 += BLU_NB) {
-      const int nb = std::min(BLU_NB, n - k0);
-      size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(double);
-      k_lu_panel<<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info);
-      EXA_HIP(hipGetLastError());
-      if (k0 + nb < n) {  // A22 -= L21 * U12
-        Gemm g{};
-        g.M = n - k0 - nb; g.N = ncols - k0 - nb; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
-        g.A = A + (k0 + nb) + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
-        g.B = A + k0 + static_cast<long>(n) * (k0 + nb); g.ldb = n; g.sB0 = stride;
-        g.C = A + (k0 + nb) + static_cast<long>(n) * (k0 + nb); g.ldc = n; g.sC0 = stride;
-        g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
-        gemm(g, s);
-      }
-    }
-    const int last = ((n - 1) / BLU_NB) * BLU_NB;
-    for (int k0 = last; k0 >= 0; k0 -= BLU_NB) {
-      k_lu_back<<<batch, 256, 0, s>>>(A, n, ncols, stride, k0);
-      EXA_HIP(hipGetLastError());
-      if (k0 > 0) {  // X[0:k0] -= U[0:k0, k0:k0+nb] * X[k0:k0+nb]
-        const int nb = std::min(BLU_NB, n - k0);
-        Gemm g{};
-        g.M = k0; g.N = ncols - n; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
-        g.A = A + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
-        g.B = A + k0 + static_cast<long>(n) * n; g.ldb = n; g.sB0 = stride;
-        g.C = A + static_cast<long>(n) * n; g.ldc = n; g.sC0 = stride;
-        g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
-        gemm(g, s);
-      }
-    }
+    // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide
+    if (static_cast<size_t>(n) * 32 * sizeof(double) <= 64000) blocked_lu<32>(A, n, ncols, stride, batch, info, s);
+    else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000) blocked_lu<16>(A, n, ncols, stride, batch, info, s);
+    else throw Error(5, "dense block larger than 500 rows");
     return;
   }
   // dynamic LDS panel n x LU_NB doubles next to ~12 KB of static LDS, kept within 64 KB
@@ -1746,11 +1781,13 @@ void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_
 }
 
 template <typename T>
-void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we,
-                 hipStream_t s) {
-  if (d.m > 64 * MV_MAXRB) throw Error(5, "element trace block larger than 1024");
+void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we, hipStream_t s) {
+  if (d.m > 1024) throw Error(5, "element trace block larger than 1024");
   size_t shm = static_cast<size_t>(5) * d.m * sizeof(double);
-  k_matvec_elem<T><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
+  constexpr int VMAX = 16 / sizeof(T);  // 16-byte loads: 2 doubles or 4 floats per lane
+  if (d.m % VMAX == 0) k_matvec_elem<T, VMAX><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
+  else if (d.m % 2 == 0) k_matvec_elem<T, 2><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
+  else k_matvec_elem<T, 1><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
   EXA_HIP(hipGetLastError());
 }
 

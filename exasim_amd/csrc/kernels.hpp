@@ -24,6 +24,7 @@ struct Gemm {
   const int* iA = nullptr;
   const int* iB = nullptr;
   const int* iC = nullptr;
+  int fp32 = 0;  // 1: operands rounded to FP32, FP32 MFMA products and accumulation
 };
 void gemm(const Gemm& g, hipStream_t s);
 

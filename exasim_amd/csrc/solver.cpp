@@ -404,6 +404,7 @@ void Solver::set_option(const std::string& name, double v) {
     }
   } else if (name == "refine_rtol") refine_rtol = v;
   else if (name == "refine_tol") refine_tol = v;
+  else if (name == "refine_first_tol") refine_first_tol = v;
   else if (name == "refine_maxit") refine_maxit = static_cast<int>(v);
   else if (name == "timing") timing = v != 0.0;
   else if (name == "workspace_bytes") {
@@ -531,6 +532,7 @@ void Solver::assemble() {
       g.B = shapegt.p; g.ldb = d.nge; g.sB0 = g.sB1 = g.sB2 = 0;
       g.C = aug.p; g.ldc = n; g.sC0 = npe; g.sC1 = n * npe; g.sC2 = augs;
       g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
+      g.fp32 = precision;
       gemm(g, s);
     }
     for (int dd = 0; dd < nd; ++dd) {  // B_d blocks for the q components
@@ -541,6 +543,7 @@ void Solver::assemble() {
       g.B = shapegt.p; g.ldb = d.nge;
       g.C = Bfull.p + n * npe * dd; g.ldc = n; g.sC0 = npe; g.sC1 = n * npe * nd; g.sC2 = n * n * nd;
       g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
+      g.fp32 = precision;
       gemm(g, s);
     }
     // face terms, one local face at a time (no two writes of a launch collide)
@@ -557,6 +560,7 @@ void Solver::assemble() {
       g.B = MCs.p + e0 * npe * nd * npe; g.ldb = npe * nd; g.sB0 = 0; g.sB1 = npe * nd * npe;
       g.C = aug.p; g.ldc = n; g.sC0 = n * npe; g.sC1 = augs;
       g.nb0 = static_cast<int>(ncu); g.nb1 = eb; g.nb2 = 1;
+      g.fp32 = precision;
       gemm(g, s);
       g.N = static_cast<int>(ndf); g.alpha = -1.0;
       g.B = MEs.p + e0 * npe * nd * ndf; g.sB1 = npe * nd * ndf;
@@ -568,6 +572,7 @@ void Solver::assemble() {
       k.B = MCs.p + e0 * npe * nd * npe; k.ldb = npe * nd; k.sB0 = 0; k.sB1 = npe * nd * npe;
       k.C = Kt.p; k.ldc = m; k.sC0 = m * npe; k.sC1 = m * n;
       k.nb0 = static_cast<int>(ncu); k.nb1 = eb; k.nb2 = 1;
+      k.fp32 = precision;
       gemm(k, s);
       k.N = static_cast<int>(ndf); k.alpha = -1.0;
       k.B = MEs.p + e0 * npe * nd * ndf; k.sB1 = npe * nd * ndf;
@@ -587,8 +592,10 @@ void Solver::assemble() {
       g.B = aug.p + n * n; g.ldb = n; g.sB0 = augs;
       g.C = H.p + e0 * m * m; g.ldc = m; g.sC0 = m * m;
       g.nb0 = eb; g.nb1 = 1; g.nb2 = 1;
+      g.fp32 = precision;
       gemm(g, s);
       g.N = 1;
+      g.fp32 = 0;
       g.B = aug.p + n * (n + m);
       g.C = Rh.p + e0 * m; g.sC0 = m;
       gemm(g, s);
@@ -792,7 +799,9 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
   copy(u.p, uh.p, N, s);
   for (it = 0; it < newton_maxit; ++it) {
     assemble();
-    int gi = gmres(b.p, x.p);
+    // mixed precision: the FP32 operator limits the first solve to about 1e-7, so it is
+    // solved to refine_first_tol and the refinement loop below supplies the rest
+    int gi = precision ? gmres(b.p, x.p, std::max(gmres_tol, refine_first_tol)) : gmres(b.p, x.p);
     gtot += gi;
     double alpha = 1.0;
     axpby(u.p, x.p, 1.0, 1.0, N, s);  // u += x

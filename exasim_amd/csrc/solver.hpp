@@ -33,7 +33,7 @@ struct Solver {
   // solve factors and the preconditioner, with FP64 residuals and an iterative-refinement
   // loop to refine_rtol * (first residual norm), corrections solved to refine_tol
   int precision = 0, refine_maxit = 5;
-  double refine_rtol = 1e-10, refine_tol = 1e-4;
+  double refine_rtol = 1e-10, refine_tol = 1e-4, refine_first_tol = 1e-7;
   int last_refinements = 0;
   bool timing = false;
   double ws_budget_bytes = 4.0e9;

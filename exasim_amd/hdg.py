@@ -85,11 +85,17 @@ class HDGSolver:
         if rc != 0:
             raise ExaError(rc, lib.exa_last_error(None).decode())
         self._h = h
-        s = (ctypes.c_int * 16)()
-        lib.exa_sizes(h, s, 16)
-        keys = ["nd", "ncu", "nc", "npe", "npf", "nfe", "ne", "nf", "n", "m", "N", "nge", "ngf",
-                "nco", "restart", "preconditioner"]
-        self.sizes = dict(zip(keys, list(s)))
+        self.sizes = self.state()
+
+    SIZE_KEYS = ["nd", "ncu", "nc", "npe", "npf", "nfe", "ne", "nf", "n", "m", "N", "nge", "ngf",
+                 "nco", "restart", "preconditioner", "precision", "refinements"]
+
+    def state(self):
+        """exa_sizes: the sizes plus the current preconditioner, precision and the number of
+        refinement corrections of the last mixed-precision Newton step."""
+        s = (ctypes.c_int * len(self.SIZE_KEYS))()
+        self._lib.exa_sizes(self._h, s, len(self.SIZE_KEYS))
+        return dict(zip(self.SIZE_KEYS, list(s)))
 
     def _check(self, rc):
         if rc != 0:

@@ -58,8 +58,10 @@ int exa_create(const char* datain_dir, const char* dataout_prefix, const char* m
 void exa_destroy(exa_handle* h);
 const char* exa_last_error(const exa_handle* h);
 
-/* Sizes: writes up to 16 ints: nd, ncu, nc, npe, npf, nfe, ne, nf, n, m, N (= ndof uhat),
- * nge, ngf, nco, restart, preconditioner. Returns the count written. */
+/* Sizes and state: writes up to 18 ints: nd, ncu, nc, npe, npf, nfe, ne, nf, n, m,
+ * N (= ndof uhat), nge, ngf, nco, restart, preconditioner, precision, refinements
+ * (corrections made in the last Newton step of a mixed-precision solve).
+ * Returns the count written. */
 int exa_sizes(const exa_handle* h, int* out, int len);
 
 /* Options (name, value): "gmres_tol", "gmres_restart" (1..2000), "gmres_maxit",
@@ -71,7 +73,8 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * operator, the local-solve factors and the preconditioner blocks are stored in FP32,
  * residuals, LU pivots and GMRES vectors stay FP64, and each Newton step is refined
  * with FP64 residuals until ||R|| <= "refine_rtol" (1e-10) times its first residual,
- * at most "refine_maxit" (5) corrections, each solved by GMRES to "refine_tol" (1e-4)). */
+ * at most "refine_maxit" (5) corrections, each solved by GMRES to "refine_tol" (1e-4);
+ * the first solve of each Newton step goes to max(gmres_tol, "refine_first_tol" (1e-7))). */
 int exa_set_option(exa_handle* h, const char* name, double value);
 
 /* Full steady solve (NewtonSolver).  newton_iters/gmres_iters/final_norm may be NULL. */

@@ -7,6 +7,9 @@ text2code writes (see ``datain.py``) and returns a small dict describing the cas
 
 * ``poisson2d`` -- examples/Poisson/poisson2d (pdeapp.txt, pdemodel.txt) on an
   n x n quad mesh of the unit square, Dirichlet on all sides.
+* ``convdiff_unsteady`` -- time-dependent convection-diffusion (a heat equation
+  when the velocity is zero) on the unit square/cube, Dirichlet u = 0 on every
+  side, initial state prod_d sin(pi x_d), zero source, DIRK time stepping.
 * ``convdiff3d`` -- steady convection-diffusion on an n^3 hex mesh of the unit
   cube, ncu components, velocity c, diffusivity kappa, per-node uniform[-1, 1]
   source passed as the auxiliary field v (``vdgfile``); Dirichlet u = 0 on x = 0
@@ -78,3 +81,30 @@ def convdiff3d(path, n=32, porder=4, pgauss=None, ncu=2, kappa=1e-2, vel=(1.0, 0
     odg = rng.uniform(-1.0, 1.0, size=(t.shape[0], m["npe"], ncu))
     dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=ncu, nc=4 * ncu, odg=odg)
     return dict(model="convdiff", nd=3, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=ncu)
+
+
+def convdiff_unsteady(path, nd=2, n=4, porder=3, pgauss=None, ncu=1, kappa=1.0, vel=None, tau=1.0,
+                      dt=0.01, nsteps=5, torder=3, nstage=3, precond="bjacobi", restart=50,
+                      gmres_tol=1e-12, gmres_maxit=1000, newton_iter=2, newton_tol=1e-10):
+    """u_t + div(c u - kappa grad u) = 0 with u = 0 on the boundary and
+    u(x, 0) = prod_d sin(pi x_d) (exact for c = 0: exp(-nd pi^2 kappa t) u(x, 0))."""
+    os.makedirs(path, exist_ok=True)
+    vel = [0.0] * nd if vel is None else list(vel)
+    pgauss = 2 * porder if pgauss is None else pgauss
+    m = dw.make_master(nd, porder, pgauss)
+    p, t = dw.structured_grid([n] * nd)
+    bnd = dw.boundary_face_tags(p, t, nd, [lambda v: True])
+    bf = np.where(bnd >= 0, 1, 0)
+    dw.write_app(os.path.join(path, "app.bin"), nd=nd, ncu=ncu, nco=ncu,
+                 flag={0: 1, 6: PRECOND[precond] if isinstance(precond, str) else precond},
+                 problem={2: 0, 3: torder, 4: nstage, 13: newton_iter, 14: gmres_maxit, 15: restart},
+                 physicsparam=[kappa] + vel, tau=[tau], dt=[dt] * nsteps,
+                 solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
+    dw.write_master(os.path.join(path, "master.bin"), m)
+    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, [1])
+    xdg = dw.compute_dgnodes(m, p, t)
+    u0 = np.prod(np.sin(np.pi * xdg), axis=2)
+    udg = np.repeat(u0[:, :, None], ncu, axis=2)
+    odg = np.zeros((t.shape[0], m["npe"], ncu))
+    dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=ncu, nc=ncu * (nd + 1), udg=udg, odg=odg)
+    return dict(model="convdiff", nd=nd, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=ncu)

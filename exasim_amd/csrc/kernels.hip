@@ -757,6 +757,21 @@ __global__ void k_vol_points(Dims d, AsmArgs a) {
   double f_uq[JAC ? NCU * ND * NC : 1], s_uq[JAC ? NCU * NC : 1];
   M::template flux<JAC>(x, uq, v, a.param, a.time, f, f_uq);
   M::template source<JAC>(x, uq, v, a.param, a.time, s, s_uq);
+  if (a.tdep) {  // uequation.cpp:126-151, Tdfunc = 1 for the built-in models: s += sdg - dtfactor*u
+    double sg[NCU];
+#pragma unroll
+    for (int k = 0; k < NCU; ++k) sg[k] = 0.0;
+    for (int i = 0; i < d.npe; ++i) {
+      const double phi = a.shapegt[g + d.nge * i];
+#pragma unroll
+      for (int k = 0; k < NCU; ++k) sg[k] += phi * a.sdg[i + d.npe * (k + NCU * E)];
+    }
+#pragma unroll
+    for (int k = 0; k < NCU; ++k) {
+      s[k] += sg[k] - a.dtfactor * uq[k];
+      if (JAC) s_uq[k + NCU * k] -= a.dtfactor;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NCU; ++k) {
     a.rv[(0 * NCU + k) * P + p] = s[k] * jac;
@@ -1387,6 +1402,26 @@ __global__ void k_bilu_scatter(Dims d, IluDev I, const double* Xo, double* out) 
   out[q] = s;
 }
 
+// u components of udg (DIRK stage bookkeeping, updatesource.cpp / updatesolution.cpp)
+__global__ void k_u_axpby(Dims d, const double* udg, double a, double b, double* dst) {
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(d.npe) * d.ncu * d.ne) return;
+  const long e = q / (d.npe * d.ncu);
+  const int r = static_cast<int>(q % (d.npe * d.ncu));
+  const int i = r % d.npe, c = r / d.npe;
+  const double u = udg[i + d.npe * (c + static_cast<long>(d.nc) * e)];
+  dst[q] = (b == 0.0) ? a * u : fma(a, u, b * dst[q]);
+}
+
+__global__ void k_u_insert(Dims d, const double* src, double* udg) {
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(d.npe) * d.ncu * d.ne) return;
+  const long e = q / (d.npe * d.ncu);
+  const int r = static_cast<int>(q % (d.npe * d.ncu));
+  const int i = r % d.npe, c = r / d.npe;
+  udg[i + d.npe * (c + static_cast<long>(d.nc) * e)] = src[q];
+}
+
 // ---------------------------------------------------------------------------
 // Mixed precision: stored FP32 factors and the condensed RHS of a new residual
 // ---------------------------------------------------------------------------
@@ -1716,6 +1751,18 @@ void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, dou
                 hipStream_t s) {
   long n1 = static_cast<long>(d.n) * (d.m + 1) * eb;
   k_post_schur<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
+  EXA_HIP(hipGetLastError());
+}
+
+void u_axpby(const Dims& d, const double* udg, double a, double b, double* dst, hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * d.ncu * d.ne;
+  k_u_axpby<<<nblocks(n1, 256), 256, 0, s>>>(d, udg, a, b, dst);
+  EXA_HIP(hipGetLastError());
+}
+
+void u_insert(const Dims& d, const double* src, double* udg, hipStream_t s) {
+  long n1 = static_cast<long>(d.npe) * d.ncu * d.ne;
+  k_u_insert<<<nblocks(n1, 256), 256, 0, s>>>(d, src, udg);
   EXA_HIP(hipGetLastError());
 }
 

@@ -64,6 +64,10 @@ struct AsmArgs {
   const double *udg, *uh, *egeo, *fgeo, *og, *ogf;
   double *rv, *rJ, *W, *fpt;
   long augs;  // per-element stride of the augmented system [D | F | Ru (| I)]
+  // time-dependent problems: source += sdg - dtfactor*u (sdg nodal [npe, ncu, ne])
+  int tdep;
+  double dtfactor;
+  const double* sdg;
 };
 void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
 void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
@@ -106,6 +110,9 @@ template <typename T>
 void dudg(const Dims& d, const double* RuS, const T* DinvF, const int* elemcon, const double* x,
           double* du, hipStream_t s);
 void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipStream_t s);
+// u components of udg [npe, nc, ne] as [npe, ncu, ne]: dst = a*u + b*dst, and udg.u = src
+void u_axpby(const Dims& d, const double* udg, double a, double b, double* dst, hipStream_t s);
+void u_insert(const Dims& d, const double* src, double* udg, hipStream_t s);
 
 // ---- superelement block-ILU(0) (preconditioner = 2) ----
 struct IluDev {

@@ -115,7 +115,20 @@ void Solver::setup() {
 
   std::vector<double>& fl = app.flag;
   std::vector<double>& pr = app.problem;
-  if (static_cast<int>(fl[0]) != 0) throw Error(5, "time-dependent problems are not supported yet");
+  // time dependence (setstructs.cpp:126-149, 194): flag[0] tdep, flag[1] wave, flag[10] tdfunc,
+  // problem[2] temporal scheme (0 DIRK), problem[3] order, problem[4] stages
+  tdep = static_cast<int>(fl[0]);
+  if (tdep) {
+    if (static_cast<int>(fl[1]) != 0) throw Error(5, "wave problems are not supported");
+    // Tdfunc (flag[10]) is 1 for the built-in models, so the mass term is the identity
+    if (static_cast<int>(pr[2]) != 0) throw Error(5, "only DIRK time stepping is supported");
+    torder = static_cast<int>(pr[3]);
+    tstages = static_cast<int>(pr[4]);
+    dts = app.dt;
+    time0 = app.factor.empty() ? 0.0 : app.factor[0];
+    dirk_coefficients();
+  }
+  tnow = time0;
   precond = static_cast<int>(fl[6]);
   if (precond < 0 || precond > 2)
     throw Error(5, "preconditioner " + std::to_string(precond) +
@@ -251,6 +264,58 @@ void Solver::setup() {
   alloc_workspace();
   if (precond == 2) setup_bilu0();
   if (precond == 1) setup_eas();
+  if (tdep) {
+    const size_t nu = static_cast<size_t>(d.npe) * d.ncu * d.ne;
+    sdg.alloc(nu);
+    udgprev.alloc(nu);
+    utmp.alloc(nu);
+    sdg.zero(s);
+  }
+}
+
+// DIRKcoeff (timestepcoeff.cpp:35-104) computed from the Butcher tableaux of the
+// schemes it lists: d = A^{-1} (column-major d[i + S*j]), c = b^T A^{-1}, t = A 1.
+void Solver::dirk_coefficients() {
+  const int S = tstages;
+  double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, b[3] = {0, 0, 0};
+  if (S == 1 && torder == 1) {  // backward Euler
+    A[0][0] = 1.0; b[0] = 1.0;
+  } else if (S == 1 && torder == 2) {  // implicit midpoint
+    A[0][0] = 0.5; b[0] = 1.0;
+  } else if (S == 2 && torder == 2) {  // Alexander, L-stable
+    const double g = 1.0 - 1.0 / std::sqrt(2.0);
+    A[0][0] = g; A[1][0] = 1.0 - g; A[1][1] = g; b[0] = 1.0 - g; b[1] = g;
+  } else if (S == 2 && torder == 3) {  // Crouzeix
+    const double g = (3.0 + std::sqrt(3.0)) / 6.0;
+    A[0][0] = g; A[1][0] = 1.0 - 2.0 * g; A[1][1] = g; b[0] = b[1] = 0.5;
+  } else if (S == 3 && torder == 3) {  // Alexander, L-stable, stiffly accurate
+    const double g = 0.4358665215;
+    A[0][0] = g; A[1][0] = (1.0 - g) / 2.0; A[1][1] = g;
+    A[2][0] = -(6.0 * g * g - 16.0 * g + 1.0) / 4.0; A[2][1] = (6.0 * g * g - 20.0 * g + 5.0) / 4.0; A[2][2] = g;
+    b[0] = A[2][0]; b[1] = A[2][1]; b[2] = g;
+  } else if (S == 3 && torder == 4) {  // Crouzeix
+    const double g = std::cos(M_PI / 18.0) / std::sqrt(3.0) + 0.5;
+    const double dl = 1.0 / (6.0 * (2.0 * g - 1.0) * (2.0 * g - 1.0));
+    A[0][0] = g; A[1][0] = 0.5 - g; A[1][1] = g; A[2][0] = 2.0 * g; A[2][1] = 1.0 - 4.0 * g; A[2][2] = g;
+    b[0] = dl; b[1] = 1.0 - 2.0 * dl; b[2] = dl;
+  } else {
+    throw Error(5, "DIRK(" + std::to_string(S) + "," + std::to_string(torder) + ") is not implemented");
+  }
+  // inverse of the lower-triangular A by forward substitution, column by column
+  double Ai[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int j = 0; j < S; ++j)
+    for (int i = j; i < S; ++i) {
+      double v = (i == j) ? 1.0 : 0.0;
+      for (int k = j; k < i; ++k) v -= A[i][k] * Ai[k][j];
+      Ai[i][j] = v / A[i][i];
+    }
+  for (int i = 0; i < S; ++i) {
+    dirk_t[i] = 0.0;
+    for (int k = 0; k < S; ++k) dirk_t[i] += A[i][k];
+    dirk_c[i] = 0.0;
+    for (int k = 0; k < S; ++k) dirk_c[i] += b[k] * Ai[k][i];
+    for (int j = 0; j < S; ++j) dirk_d[i + S * j] = Ai[i][j];
+  }
 }
 
 // hdgBlockILU0 (matvec.cpp:133-165) batched over the superelements: for each face
@@ -416,8 +481,11 @@ void Solver::set_option(const std::string& name, double v) {
 AsmArgs Solver::asm_args(int e0, int eb) {
   AsmArgs a{};
   a.augs = static_cast<long>(d.n) * augcols();
+  a.tdep = tdep;
+  a.dtfactor = dtfactor;
+  a.sdg = sdg.p;
   a.model = model;
-  a.time = 0.0;
+  a.time = tnow;
   a.param = param.p;
   a.tau = tau.p;
   a.tau0 = app.tau[0];
@@ -499,6 +567,9 @@ double Solver::norm(const double* v, long n) {
 void Solver::reset() {
   copy(udg.p, udg0.p, static_cast<long>(udg.n), s);
   copy(uh.p, uh0.p, static_cast<long>(uh.n), s);
+  tnow = time0;
+  dtfactor = 0.0;
+  if (tdep) sdg.zero(s);
   assembled = false;
 }
 
@@ -786,7 +857,60 @@ int Solver::gmres(const double* bv, double* xv, double tol_rel) {
   return j;
 }
 
+// Steady problems: one Newton solve.  Time-dependent problems: the t2c time loop
+// (exasim.hpp:412-470) with DIRK stages -- PreviousSolutions, UpdateSourceDIRK,
+// a Newton solve per stage, UpdateSolutionDIRK (previoussolutions.cpp:23-58,
+// updatesource.cpp:32-72, updatesolution.cpp:48-70).
 void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
+  if (!tdep) {
+    newton(newton_iters, gmres_total, final_norm);
+    return;
+  }
+  const int S = tstages;
+  double csum = 0.0;
+  for (int j = 0; j < S; ++j) csum += dirk_c[j];
+  int nit = 0, gtot = 0;
+  double nrm = 0.0;
+  double t = tnow;
+  for (size_t istep = 0; istep < dts.size(); ++istep) {
+    const double dt = dts[istep];
+    u_axpby(d, udg.p, 1.0, 0.0, udgprev.p, s);          // udgprev = u
+    axpby(utmp.p, udgprev.p, 1.0 - csum, 0.0, static_cast<long>(udgprev.n), s);
+    for (int j = 0; j < S; ++j) {
+      tnow = t + dt * dirk_t[j];
+      dtfactor = dirk_d[j * S + j] / dt;
+      if (j == 0) {
+        u_axpby(d, udg.p, dirk_d[0] / dt, 0.0, sdg.p, s);
+      } else if (j == 1) {
+        u_axpby(d, udg.p, -dirk_d[1] / dt, (dirk_d[1] + dirk_d[S + 1]) / dirk_d[0], sdg.p, s);
+      } else {
+        const double sc = dirk_d[2] / dirk_d[1];
+        const double pc = dirk_d[2] + dirk_d[S + 2] + dirk_d[2 * S + 2] - sc * (dirk_d[1] + dirk_d[S + 1]);
+        u_axpby(d, udg.p, -dirk_d[S + 2] / dt, sc, sdg.p, s);
+        axpby(sdg.p, udgprev.p, pc / dt, 1.0, static_cast<long>(sdg.n), s);
+      }
+      int ni = 0, gi = 0;
+      newton(&ni, &gi, &nrm);
+      nit += ni;
+      gtot += gi;
+      u_axpby(d, udg.p, dirk_c[j], 1.0, utmp.p, s);       // utmp += c_j u
+      if (j == S - 1) u_insert(d, utmp.p, udg.p, s);
+      if (verbose) {
+        std::printf("Timestep %zu, stage %d, time %g: Newton %d, GMRES %d, residual %g\n", istep + 1, j + 1,
+                    tnow, ni, gi, nrm);
+        std::fflush(stdout);
+      }
+    }
+    t += dt;
+  }
+  tnow = t;
+  EXA_HIP(hipStreamSynchronize(s));
+  if (newton_iters) *newton_iters = nit;
+  if (gmres_total) *gmres_total = gtot;
+  if (final_norm) *final_norm = nrm;
+}
+
+void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
   const long N = d.N, nU = static_cast<long>(d.n) * d.ne;
   get_q();
   double nb = 0, nru = 0;

@@ -345,9 +345,18 @@ class HDGProblem:
         self.gmres_tol = app["solversparam"][1]
         self.tau = app["tau"]
         self.param = app["physicsparam"]
-        self.time = 0.0
+        # time dependence (setstructs.cpp:126-149): flag[0] tdep, flag[10] tdfunc,
+        # problem[3] torder, problem[4] DIRK stages; the time steps are app.dt and the
+        # initial time app.factor[0] (setstructs.cpp:194)
+        self.tdep = int(flag[0])
+        self.tdfunc = int(flag[10])
+        self.torder, self.tstages = int(prob[3]), int(prob[4])
+        self.dt = np.asarray(app["dt"], dtype=float)
+        self.time = float(app["factor"][0]) if len(app["factor"]) else 0.0
+        self.dtfactor = 0.0
         self.xdg = sol["xdg"]
         self.udg = sol["udg"] if sol["udg"] is not None else np.zeros((self.ne, self.npe, self.nc))
+        self.sdg = np.zeros((self.ne, self.npe, self.ncu))       # DIRK stage source (ncs = ncu)
         self.odg = sol["odg"] if sol["odg"] is not None else np.zeros((self.ne, self.npe, 0))
         self.xg, self.Xx, self.jac = elem_geom(master, self.xdg)
         self.xgf, self.nlf, self.jacf = elemface_geom(master, self.xdg, self.perm)
@@ -396,6 +405,14 @@ class HDGProblem:
         uq = uqg.reshape(N, nc)
         v = self.og.reshape(N, -1)
         s, s_uq = self.model.source(x, uq, v, self.param, self.time)
+        if self.tdep:
+            # uequation.cpp:126-151; Tdfunc is 1 for the models here (so tdfunc = 0 or 1
+            # give the same term): s += sdg - dtfactor*u, s_u -= dtfactor
+            sdgg = np.einsum("gi,eic->egc", phi[0], self.sdg).reshape(N, ncu)
+            s = s + sdgg - self.dtfactor * uq[:, :ncu]
+            s_uq = s_uq.copy()
+            for c in range(ncu):
+                s_uq[:, c, c] -= self.dtfactor
         f, f_uq = self.model.flux(x, uq, v, self.param, self.time)
         ng = phi.shape[1]
         s = s.reshape(ne, ng, ncu)
@@ -836,3 +853,87 @@ def newton_solve(prob, log=None):
             break
     hist["newton_iters"] = len(hist["alphas"])
     return hist
+
+
+# --------------------------------------------------------------------------
+# DIRK time stepping -- backend/Solution/timestepcoeff.cpp:35-104 (DIRKcoeff),
+# updatesource.cpp:32-102, previoussolutions.cpp:23-58, updatesolution.cpp:48-70,
+# backend/Main/exasim.hpp:412-470 (the time loop)
+# --------------------------------------------------------------------------
+
+def dirk_tableau(nstage, torder):
+    """Butcher matrix A and weights b of the DIRK schemes the reference offers:
+    backward Euler (1,1), implicit midpoint (1,2), Alexander's L-stable (2,2) and
+    (3,3), Crouzeix's (2,3) and (3,4)."""
+    if (nstage, torder) == (1, 1):
+        return np.array([[1.0]]), np.array([1.0])
+    if (nstage, torder) == (1, 2):
+        return np.array([[0.5]]), np.array([1.0])
+    if (nstage, torder) == (2, 2):
+        g = 1.0 - 1.0 / np.sqrt(2.0)
+        return np.array([[g, 0.0], [1.0 - g, g]]), np.array([1.0 - g, g])
+    if (nstage, torder) == (2, 3):
+        g = (3.0 + np.sqrt(3.0)) / 6.0
+        return np.array([[g, 0.0], [1.0 - 2.0 * g, g]]), np.array([0.5, 0.5])
+    if (nstage, torder) == (3, 3):
+        g = 0.4358665215
+        b1 = -(6.0 * g * g - 16.0 * g + 1.0) / 4.0
+        b2 = (6.0 * g * g - 20.0 * g + 5.0) / 4.0
+        A = np.array([[g, 0.0, 0.0], [(1.0 - g) / 2.0, g, 0.0], [b1, b2, g]])
+        return A, A[2].copy()
+    if (nstage, torder) == (3, 4):
+        g = np.cos(np.pi / 18.0) / np.sqrt(3.0) + 0.5
+        dl = 1.0 / (6.0 * (2.0 * g - 1.0) ** 2)
+        A = np.array([[g, 0.0, 0.0], [0.5 - g, g, 0.0], [2.0 * g, 1.0 - 4.0 * g, g]])
+        return A, np.array([dl, 1.0 - 2.0 * dl, dl])
+    raise NotImplementedError("DIRK(%d,%d)" % (nstage, torder))
+
+
+def dirk_coeffs(nstage, torder):
+    """The reference's form of the tableau: d = A^{-1} (column-major, d[i + S*j]),
+    c = b^T A^{-1}, t = A 1 (stage abscissae)."""
+    A, b = dirk_tableau(nstage, torder)
+    Ai = np.linalg.inv(A)
+    return b @ Ai, Ai.T.reshape(-1), A.sum(1)
+
+
+def update_source_dirk(P, stage, dt, d, udgprev):
+    """UpdateSourceDIRK (updatesource.cpp:32-72) for ncs = ncu."""
+    S = P.tstages
+    Rq = P.udg[:, :, :P.ncu]
+    P.dtfactor = d[stage * S + stage] / dt
+    if stage == 0:
+        P.sdg = (d[0] / dt) * Rq
+    elif stage == 1:
+        P.sdg = ((d[1] + d[S + 1]) / d[0]) * P.sdg - (d[1] / dt) * Rq
+    elif stage == 2:
+        sc = d[2] / d[1]
+        pc = d[2] + d[S + 2] + d[2 * S + 2] - sc * (d[1] + d[S + 1])
+        P.sdg = sc * P.sdg - (d[S + 2] / dt) * Rq + (pc / dt) * udgprev
+    else:
+        raise NotImplementedError("DIRK with more than 3 stages")
+
+
+def dirk_solve(P, nsteps=None):
+    """Time loop of the t2c driver (exasim.hpp:412-470) with DIRK stages: each stage
+    is a steady Newton solve with the stage source; the new solution is the
+    c-weighted combination of the stages (UpdateSolutionDIRK)."""
+    c, d, t = dirk_coeffs(P.tstages, P.torder)
+    nsteps = len(P.dt) if nsteps is None else nsteps
+    time = P.time
+    gmres_total = 0
+    for istep in range(nsteps):
+        dt = P.dt[istep]
+        udgprev = P.udg[:, :, :P.ncu].copy()
+        utmp = (1.0 - c.sum()) * udgprev
+        for j in range(P.tstages):
+            P.time = time + dt * t[j]
+            update_source_dirk(P, j, dt, d, udgprev)
+            h = newton_solve(P)
+            gmres_total += sum(h["gmres_iters"])
+            utmp = utmp + c[j] * P.udg[:, :, :P.ncu]
+            if j == P.tstages - 1:
+                P.udg[:, :, :P.ncu] = utmp
+        time += dt
+    P.time = time
+    return gmres_total

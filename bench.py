@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxit", type=int, default=20000)
     ap.add_argument("--ortho", type=int, default=0)
+    ap.add_argument("--refine-first-tol", type=float, default=None)
+    ap.add_argument("--refine-tol", type=float, default=None)
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
                     help="timed legs: FP64, mixed FP32/FP64, or both (value = the mixed leg)")
     ap.add_argument("--precond", default="bilu0", choices=["bilu0", "bjacobi"])
@@ -133,6 +135,10 @@ def main():
     t0 = time.perf_counter()
     S = HDGSolver(work, "convdiff")
     S.set_option("ortho", args.ortho)
+    if args.refine_first_tol is not None:
+        S.set_option("refine_first_tol", args.refine_first_tol)
+    if args.refine_tol is not None:
+        S.set_option("refine_tol", args.refine_tol)
     t_setup = time.perf_counter() - t0
     z = S.sizes
     print("# setup (read, connectivity, geometry, q-equation) %.2f s; sizes %s" % (t_setup, z),

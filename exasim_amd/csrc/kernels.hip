@@ -207,7 +207,7 @@ __device__ inline void argmax_wave(double& v, int& i) {
 
 template <int BLU_NB>
 __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols, long stride, int k0,
-                                                  int* info) {
+                                                  int* info, int* pivout) {
   double* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
   const int rows = n - k0;
@@ -302,6 +302,8 @@ __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols
       if (kk < nb) colj[kk] = x[kk];
   }
   if (tid == 0 && info) info[blockIdx.x] = (k0 == 0 ? 0 : info[blockIdx.x]) | singular;
+  // panel-relative pivots of this panel (row k0+kk was exchanged with row k0+piv[kk])
+  if (pivout && tid < nb) pivout[static_cast<long>(blockIdx.x) * n + k0 + tid] = piv[tid];
 }
 
 // X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
@@ -1425,14 +1427,6 @@ __global__ void k_u_insert(Dims d, const double* src, double* udg) {
 // ---------------------------------------------------------------------------
 // Mixed precision: stored FP32 factors and the condensed RHS of a new residual
 // ---------------------------------------------------------------------------
-__global__ void k_set_identity(double* A, int n, long ld, long stride, int batch) {
-  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (q >= static_cast<long>(n) * batch) return;
-  const long b = q / n;
-  const int i = static_cast<int>(q % n);
-  A[b * stride + i + ld * i] = 1.0;
-}
-
 __global__ void k_to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride,
                            long dstride, int batch) {
   const long per = static_cast<long>(rows) * cols;
@@ -1445,31 +1439,59 @@ __global__ void k_to_float(float* dst, const double* src, int rows, int cols, lo
   dst[b * dstride + rc] = static_cast<float>(src[b * sstride + r + ld * c]);
 }
 
-// One workgroup per element: RuS = Dinv Ru (FP64 accumulation), then Rh -= K RuS.
-__global__ __launch_bounds__(256) void k_refine_rhs(Dims d, const float* Dinv, const float* K, const double* Ru,
-                                                    double* RuS, double* Rh) {
+// One workgroup per element: RuS = D'^{-1} Ru from the stored FP32 LU factors of
+// D' (replaying the blocked factorisation: each panel's row exchanges, then the
+// unit-lower elimination), the upper-triangular solve, then Rh -= K' RuS.  The
+// panel width nbp is the one lu_solve used for the factorisation.
+__global__ __launch_bounds__(256) void k_refine_rhs(Dims d, int nbp, const float* LU, const int* piv,
+                                                    const float* K, const double* Ru, double* RuS,
+                                                    double* Rh) {
   const int n = d.n, m = d.m;
   const long e = blockIdx.x;
   extern __shared__ double sm[];
-  double* ru = sm;       // n
-  double* rs = sm + n;   // n
-  for (int t = threadIdx.x; t < n; t += 256) ru[t] = Ru[e * n + t];
+  double* y = sm;   // n
+  const float* Le = LU + e * static_cast<long>(n) * n;
+  const int* pe = piv + e * static_cast<long>(n);
+  for (int t = threadIdx.x; t < n; t += 256) y[t] = Ru[e * n + t];
   __syncthreads();
-  const float* De = Dinv + e * static_cast<long>(n) * n;
-  for (int r = threadIdx.x; r < n; r += 256) {
-    double acc = 0.0;
-    for (int c = 0; c < n; ++c) acc = fma(static_cast<double>(De[r + static_cast<long>(n) * c]), ru[c], acc);
-    rs[r] = acc;
-    RuS[e * n + r] = acc;
+  for (int k0 = 0; k0 < n; k0 += nbp) {
+    const int nb = min(nbp, n - k0);
+    if (threadIdx.x == 0)
+      for (int kk = 0; kk < nb; ++kk) {
+        const int p = k0 + pe[k0 + kk];
+        if (p != k0 + kk) {
+          const double t = y[k0 + kk];
+          y[k0 + kk] = y[p];
+          y[p] = t;
+        }
+      }
+    __syncthreads();
+    for (int k = k0; k < k0 + nb; ++k) {
+      const double yk = y[k];
+      for (int r = k + 1 + threadIdx.x; r < n; r += 256)
+        y[r] = fma(-static_cast<double>(Le[r + static_cast<long>(n) * k]), yk, y[r]);
+      __syncthreads();
+    }
   }
-  __syncthreads();
+  for (int k = n - 1; k >= 0; --k) {
+    if (threadIdx.x == 0) y[k] /= static_cast<double>(Le[k + static_cast<long>(n) * k]);
+    __syncthreads();
+    const double yk = y[k];
+    for (int r = threadIdx.x; r < k; r += 256)
+      y[r] = fma(-static_cast<double>(Le[r + static_cast<long>(n) * k]), yk, y[r]);
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < n; t += 256) RuS[e * n + t] = y[t];
   const float* Ke = K + e * static_cast<long>(m) * n;
   for (int r = threadIdx.x; r < m; r += 256) {
     double acc = 0.0;
-    for (int c = 0; c < n; ++c) acc = fma(static_cast<double>(Ke[r + static_cast<long>(m) * c]), rs[c], acc);
+    for (int c = 0; c < n; ++c) acc = fma(static_cast<double>(Ke[r + static_cast<long>(m) * c]), y[c], acc);
     Rh[e * m + r] -= acc;
   }
 }
+
+// panel width lu_solve uses for an n x n factorisation (see lu_solve)
+int lu_panel_width(int n) { return (static_cast<size_t>(n) * 32 * sizeof(double) <= 64000) ? 32 : 16; }
 
 // ---------------------------------------------------------------------------
 // Vector kernels (fixed-order reductions)
@@ -1614,11 +1636,11 @@ void gemm(const Gemm& g, hipStream_t s) {
 }
 
 template <int NB>
-void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s) {
+void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, int* pivout, hipStream_t s) {
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = std::min(NB, n - k0);
     size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(double);
-    k_lu_panel<NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info);
+    k_lu_panel<NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info, pivout);
     EXA_HIP(hipGetLastError());
     if (k0 + nb < n) {  // A22 -= L21 * U12
       Gemm g{};
@@ -1648,15 +1670,19 @@ void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, 
   }
 }
 
-void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s) {
+void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout) {
   if (batch <= 0) return;
-  if (n >= BLU_MIN_N && ncols > n) {
+  if ((n >= BLU_MIN_N || pivout) && ncols > n) {
     // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide
-    if (static_cast<size_t>(n) * 32 * sizeof(double) <= 64000) blocked_lu<32>(A, n, ncols, stride, batch, info, s);
-    else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000) blocked_lu<16>(A, n, ncols, stride, batch, info, s);
-    else throw Error(5, "dense block larger than 500 rows");
+    if (static_cast<size_t>(n) * 32 * sizeof(double) <= 64000)
+      blocked_lu<32>(A, n, ncols, stride, batch, info, pivout, s);
+    else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000)
+      blocked_lu<16>(A, n, ncols, stride, batch, info, pivout, s);
+    else
+      throw Error(5, "dense block larger than 500 rows");
     return;
   }
+  if (pivout) throw Error(5, "pivot output needs right-hand-side columns");
   // dynamic LDS panel n x LU_NB doubles next to ~12 KB of static LDS, kept within 64 KB
   if (n > 400) throw Error(5, "dense block larger than 400 rows");
   size_t shm = static_cast<size_t>(n) * LU_NB * sizeof(double);
@@ -1766,13 +1792,6 @@ void u_insert(const Dims& d, const double* src, double* udg, hipStream_t s) {
   EXA_HIP(hipGetLastError());
 }
 
-void set_identity(double* A, int n, long ld, long stride, int batch, hipStream_t s) {
-  long n1 = static_cast<long>(n) * batch;
-  if (n1 == 0) return;
-  k_set_identity<<<nblocks(n1, 256), 256, 0, s>>>(A, n, ld, stride, batch);
-  EXA_HIP(hipGetLastError());
-}
-
 void to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
               hipStream_t s) {
   long n1 = static_cast<long>(rows) * cols * batch;
@@ -1781,10 +1800,10 @@ void to_float(float* dst, const double* src, int rows, int cols, long ld, long s
   EXA_HIP(hipGetLastError());
 }
 
-void refine_rhs(const Dims& d, const float* Dinv, const float* K, const double* Ru, double* RuS, double* Rh,
-                hipStream_t s) {
-  size_t shm = 2 * static_cast<size_t>(d.n) * sizeof(double);
-  k_refine_rhs<<<d.ne, 256, shm, s>>>(d, Dinv, K, Ru, RuS, Rh);
+void refine_rhs(const Dims& d, const float* LU, const int* piv, const float* K, const double* Ru, double* RuS,
+                double* Rh, hipStream_t s) {
+  size_t shm = static_cast<size_t>(d.n) * sizeof(double);
+  k_refine_rhs<<<d.ne, 256, shm, s>>>(d, lu_panel_width(d.n), LU, piv, K, Ru, RuS, Rh);
   EXA_HIP(hipGetLastError());
 }
 

@@ -32,7 +32,10 @@ void gemm(const Gemm& g, hipStream_t s);
 // augmented matrix A_b (n x ncols, ld n, batch stride `stride`), then back
 // substitution so that columns n..ncols-1 hold D^{-1} * RHS.  info[b] = 1 when a
 // zero pivot is met.
-void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s);
+// pivout (optional, [n, batch]): the panel-relative row exchanges of the blocked
+// factorisation, for refine_rhs
+void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s,
+              int* pivout = nullptr);
 
 // ---- setup ----
 void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s);
@@ -80,15 +83,13 @@ void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull
               double* Gfull, double* H, hipStream_t s);
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s);
-// A_b[i + ld*i] = 1 for i < n (b < batch, batch stride `stride`)
-void set_identity(double* A, int n, long ld, long stride, int batch, hipStream_t s);
 // dst_b[r + rows*c] = (float) src_b[r + ld*c]
 void to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
               hipStream_t s);
 // Condensed right-hand side of a new residual with stored factors (mixed precision):
-// RuS_e = Dinv_e Ru_e (n x n), Rh_e -= K_e RuS_e (m x n), per element.
-void refine_rhs(const Dims& d, const float* Dinv, const float* K, const double* Ru, double* RuS, double* Rh,
-                hipStream_t s);
+// RuS_e = D'^{-1} Ru_e from the FP32 LU factors and pivots of D', Rh_e -= K'_e RuS_e.
+void refine_rhs(const Dims& d, const float* LU, const int* piv, const float* K, const double* Ru, double* RuS,
+                double* Rh, hipStream_t s);
 
 // ---- global system ----
 void face_gather(long nnodes, int ncu, const long* src1, const long* src2, const double* we, double* w,

@@ -355,7 +355,8 @@ void Solver::alloc_mixed() {
   const size_t ne = d.ne, n = d.n, m = d.m, ncf2 = static_cast<size_t>(d.ncf) * d.ncf;
   H32.alloc(m * m * ne);
   DinvF32.alloc(n * m * ne);
-  Dinv32.alloc(n * n * ne);
+  LU32.alloc(n * n * ne);
+  piv32.alloc(n * ne);
   K32.alloc(m * n * ne);
   if (precond == 0) BE32.alloc(ncf2 * d.nf);
   if (precond == 1) BEeas32.alloc(m * m * ne);
@@ -588,7 +589,6 @@ void Solver::assemble() {
     AsmArgs a = asm_args(e0, eb);
     tic(T_ASSEMBLY, &t0);
     EXA_HIP(hipMemsetAsync(aug.p, 0, sizeof(double) * augs * eb, s));
-    if (precision) set_identity(aug.p + n * (n + m + 1), static_cast<int>(n), n, augs, eb, s);
     EXA_HIP(hipMemsetAsync(Kt.p, 0, sizeof(double) * m * n * eb, s));
     EXA_HIP(hipMemsetAsync(Gfull.p, 0, sizeof(double) * m * n * nd * eb, s));
     EXA_HIP(hipMemsetAsync(H.p + e0 * m * m, 0, sizeof(double) * m * m * eb, s));
@@ -653,7 +653,8 @@ void Solver::assemble() {
     toc(T_ASSEMBLY, t0);
     // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru]
     tic(T_LU, &t0);
-    lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s);
+    lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s,
+             precision ? piv32.p + static_cast<long>(e0) * n : nullptr);
     toc(T_LU, t0);
     tic(T_ASSEMBLY, &t0);
     {  // H += K * DinvF (= -K X_F), Rh -= K * (D^{-1} Ru)
@@ -672,14 +673,13 @@ void Solver::assemble() {
       gemm(g, s);
     }
     post_schur(d, e0, eb, aug.p, augs, DinvF.p, RuS.p, s);
-    if (precision) {  // FP32 copies of this block's H, DinvF, D'^{-1} and K'
+    if (precision) {  // FP32 copies of this block's H, DinvF, LU factors of D' and K'
       const long E0 = e0;
       to_float(H32.p + E0 * m * m, H.p + E0 * m * m, static_cast<int>(m), static_cast<int>(m), m, m * m, m * m,
                eb, s);
       to_float(DinvF32.p + E0 * n * m, DinvF.p + E0 * n * m, static_cast<int>(n), static_cast<int>(m), n, n * m,
                n * m, eb, s);
-      to_float(Dinv32.p + E0 * n * n, aug.p + n * (n + m + 1), static_cast<int>(n), static_cast<int>(n), n, augs,
-               n * n, eb, s);
+      to_float(LU32.p + E0 * n * n, aug.p, static_cast<int>(n), static_cast<int>(n), n, augs, n * n, eb, s);
       to_float(K32.p + E0 * m * n, Kt.p, static_cast<int>(m), static_cast<int>(n), m, m * n, m * n, eb, s);
     }
     toc(T_ASSEMBLY, t0);
@@ -957,7 +957,7 @@ void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
     // factors, until the residual is refine_rtol times this iteration's first residual
     last_refinements = 0;
     while (precision && nrmr > refine_rtol * nrm0 && last_refinements < refine_maxit) {
-      refine_rhs(d, Dinv32.p, K32.p, RuR.p, RuS.p, Rh.p, s);
+      refine_rhs(d, LU32.p, piv32.p, K32.p, RuR.p, RuS.p, Rh.p, s);
       face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
       gi = gmres(b.p, x.p, refine_tol);
       gtot += gi;

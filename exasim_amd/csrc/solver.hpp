@@ -54,8 +54,9 @@ struct Solver {
   // per-block workspace
   int neb = 0;
   DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
-  // mixed precision: FP32 copies of H, DinvF, D'^{-1}, K' and the preconditioner blocks
-  DevBuf<float> H32, DinvF32, Dinv32, K32, BE32, BEilu32, BEeas32;
+  // mixed precision: FP32 copies of H, DinvF, the LU factors of D', K' and the preconditioner blocks
+  DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32;
+  DevBuf<int> piv32;  // row exchanges of the blocked LU of D' ([n, ne], panel-relative)
   // elemental additive Schwarz: inverted element blocks, inversion scratch
   DevBuf<double> BEeas, easaug;
   int eas_chunk = 0;
@@ -89,7 +90,7 @@ struct Solver {
   void matvec(const double* v, double* w);
   void apply_precond(double* x);
   int gmres(const double* bvec, double* xvec, double tol_rel = -1.0);
-  long augcols() const { return static_cast<long>(d.n) + d.m + 1 + (precision ? d.n : 0); }
+  long augcols() const { return static_cast<long>(d.n) + d.m + 1; }
   void alloc_mixed();
   void solve(int* newton_iters, int* gmres_iters, double* final_norm);
   void newton(int* newton_iters, int* gmres_iters, double* final_norm);

@@ -1335,60 +1335,84 @@ __global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const double* L
   BE[(static_cast<long>(I.elim_ji[k0 + k]) * I.nse + n) * per + q % per] = Ltmp[q];
 }
 
-// One workgroup per superelement: forward sweep y_i = b_i - sum L_ij y_j, backward
-// sweep x_i = D_i^{-1} (y_i - sum U_il x_l).  Y, Xo are laid out [slot = n + nse*i][ncf].
+// One workgroup per superelement (ApplyBlockILU0, preconditioner.cpp:184-270):
+// forward sweep y_i = b_i - sum L_ij y_j, backward sweep x_i = D_i^{-1} (y_i - sum
+// U_il x_l).  The superelement's vector lives in LDS (y_i is overwritten by x_i).
+// Thread (r, part) accumulates row r over the part-th chunk of the columns of every
+// block of the current row, so all 256 threads issue loads in every step; the
+// chunks are added in a fixed order.  Xo is written as [slot = n + nse*i][ncf].
 template <typename T>
 __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* BE, const double* x,
-                                                    double* Y, double* Xo) {
-  const int ncf = d.ncf;
-  const int n = blockIdx.x;
+                                                    double* Xo) {
+  const int ncf = d.ncf, nfse = I.nfse;
+  const long n = blockIdx.x;
   const long per = static_cast<long>(ncf) * ncf;
   const int tid = threadIdx.x;
-  const int nparts = max(1, 256 / ncf);
+  const int nparts = 256 / ncf;
   const int r = tid % ncf, part = tid / ncf;
-  __shared__ double red[256];
-  __shared__ double z[256];
-  auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per; };
-  for (int i = 0; i < I.nfse; ++i) {
+  const bool active = part < nparts;
+  const int cw = (ncf + nparts - 1) / nparts;
+  const int c0 = part * cw, c1 = min(ncf, c0 + cw);
+  extern __shared__ double sm[];
+  double* v = sm;                          // nfse * ncf
+  double* red = sm + static_cast<long>(nfse) * ncf;   // 256
+  double* z = red + 256;                   // ncf
+  auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per + r; };
+  for (int idx = tid; idx < nfse * ncf; idx += 256) {
+    const int i = idx / ncf, rr = idx % ncf;
+    v[idx] = x[static_cast<long>(I.face[n + static_cast<long>(I.nse) * i]) * ncf + rr];
+  }
+  __syncthreads();
+  for (int i = 0; i < nfse; ++i) {
     double acc = 0.0;
-    if (part < nparts)
-      for (int q = I.lptr[i] + part; q < I.lptr[i + 1]; q += nparts) {
+    if (active)
+      for (int q = I.lptr[i]; q < I.lptr[i + 1]; ++q) {
         const T* B = blk(I.lidx[q]);
-        const double* yj = Y + (static_cast<long>(n) + static_cast<long>(I.nse) * I.lcol[q]) * ncf;
-        for (int c = 0; c < ncf; ++c) acc = fma(static_cast<double>(B[r + ncf * c]), yj[c], acc);
+        const double* yj = v + I.lcol[q] * ncf;
+        for (int c = c0; c < c1; ++c) acc = fma(static_cast<double>(B[ncf * c]), yj[c], acc);
       }
     red[tid] = acc;
     __syncthreads();
     if (tid < ncf) {
-      double s = x[static_cast<long>(I.face[n + I.nse * i]) * ncf + tid];
+      double s = v[i * ncf + tid];
       for (int p = 0; p < nparts; ++p) s -= red[p * ncf + tid];
-      Y[(static_cast<long>(n) + static_cast<long>(I.nse) * i) * ncf + tid] = s;
+      v[i * ncf + tid] = s;
     }
     __syncthreads();
   }
-  for (int i = I.nfse - 1; i >= 0; --i) {
+  for (int i = nfse - 1; i >= 0; --i) {
     double acc = 0.0;
-    if (part < nparts)
-      for (int q = I.uuptr[i] + part; q < I.uuptr[i + 1]; q += nparts) {
+    if (active)
+      for (int q = I.uuptr[i]; q < I.uuptr[i + 1]; ++q) {
         const T* B = blk(I.uidx[q]);
-        const double* xj = Xo + (static_cast<long>(n) + static_cast<long>(I.nse) * I.ucol[q]) * ncf;
-        for (int c = 0; c < ncf; ++c) acc = fma(static_cast<double>(B[r + ncf * c]), xj[c], acc);
+        const double* xj = v + I.ucol[q] * ncf;
+        for (int c = c0; c < c1; ++c) acc = fma(static_cast<double>(B[ncf * c]), xj[c], acc);
       }
     red[tid] = acc;
     __syncthreads();
     if (tid < ncf) {
-      double s = Y[(static_cast<long>(n) + static_cast<long>(I.nse) * i) * ncf + tid];
+      double s = v[i * ncf + tid];
       for (int p = 0; p < nparts; ++p) s -= red[p * ncf + tid];
       z[tid] = s;
     }
     __syncthreads();
-    if (tid < ncf) {
+    acc = 0.0;
+    if (active) {
       const T* Dinv = blk(I.diag[i]);
+      for (int c = c0; c < c1; ++c) acc = fma(static_cast<double>(Dinv[ncf * c]), z[c], acc);
+    }
+    red[tid] = acc;
+    __syncthreads();
+    if (tid < ncf) {
       double s = 0.0;
-      for (int c = 0; c < ncf; ++c) s = fma(static_cast<double>(Dinv[tid + ncf * c]), z[c], s);
-      Xo[(static_cast<long>(n) + static_cast<long>(I.nse) * i) * ncf + tid] = s;
+      for (int p = 0; p < nparts; ++p) s += red[p * ncf + tid];
+      v[i * ncf + tid] = s;
     }
     __syncthreads();
+  }
+  for (int idx = tid; idx < nfse * ncf; idx += 256) {
+    const int i = idx / ncf, rr = idx % ncf;
+    Xo[(n + static_cast<long>(I.nse) * i) * ncf + rr] = v[idx];
   }
 }
 
@@ -1896,8 +1920,14 @@ void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const double* 
 template <typename T>
 void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
                 double* out, hipStream_t s) {
+  (void)Y;
   if (d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 256");
-  k_bilu_apply<T><<<I.nse, 256, 0, s>>>(d, I, BE, x, Y, Xo);
+  const size_t shm = (static_cast<size_t>(I.nfse) * d.ncf + 256 + d.ncf) * sizeof(double);
+  if (shm > 160 * 1024) throw Error(5, "block-ILU(0) superelement vector larger than the LDS");
+  if (shm > 64 * 1024)
+    EXA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bilu_apply<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
+  k_bilu_apply<T><<<I.nse, 256, shm, s>>>(d, I, BE, x, Xo);
   k_bilu_scatter<<<nblocks(static_cast<long>(d.nf) * d.ncf, 256), 256, 0, s>>>(d, I, Xo, out);
   EXA_HIP(hipGetLastError());
 }

@@ -20,6 +20,9 @@ ARCH = "gfx950"
 def flags():
     return ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            # MFMA accumulators in VGPRs: without it the loop-carried tiles are copied
+            # between VGPRs and AGPRs around every K-step of the GEMM
+            "-mllvm", "-amdgpu-mfma-vgpr-form",
             "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 

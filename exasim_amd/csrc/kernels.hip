@@ -111,11 +111,22 @@ __device__ inline flt4 mfma_step(float a, float b, flt4 c) {
 __device__ inline int mfma_row(double, int lk, int i) { return lk + 4 * i; }
 __device__ inline int mfma_row(float, int lk, int i) { return 4 * lk + i; }
 
+// Workgroup w is dispatched to XCD w % 8 (each XCD has its own L2).  The logical
+// tile id L = (w % 8) * (nwg / 8) + w / 8 gives every XCD a contiguous range of L,
+// and L runs over the M/N tiles of one batch entry before the next, so the tiles
+// that share an operand run together on one XCD and reuse it from its L2.
+constexpr int NXCD = 8;
+
 template <typename CT>
-__global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
+__global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tilesN, long ntiles) {
   typedef CT acc4 __attribute__((ext_vector_type(4)));
-  const int tile = blockIdx.x % tilesM;
-  const long b = blockIdx.x / tilesM;
+  const long nwg = static_cast<long>(gridDim.x);   // a multiple of NXCD
+  const long L = (blockIdx.x % NXCD) * (nwg / NXCD) + blockIdx.x / NXCD;
+  if (L >= ntiles) return;
+  const int tpb = tilesM * tilesN;
+  const long b = L / tpb;
+  const int tt = static_cast<int>(L % tpb);
+  const int tile = tt % tilesM, tcol = tt / tilesM;
   const int b0 = static_cast<int>(b % g.nb0);
   const long t1 = b / g.nb0;
   const int b1 = static_cast<int>(t1 % g.nb1);
@@ -124,7 +135,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
   const double* A = g.A + g.sA0 * a0 + g.sA1 * b1 + g.sA2 * b2;
   const double* B = g.B + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
   double* C = g.C + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
-  const int row0 = tile * MG_BM, col0 = blockIdx.y * MG_BN;
+  const int row0 = tile * MG_BM, col0 = tcol * MG_BN;
   __shared__ CT As[MG_BM * MG_BK];
   __shared__ CT Bs[MG_BN * MG_BK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -134,14 +145,20 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
   const int bk = tid & 31, bc = tid >> 5;
   constexpr int NL = MG_BM * MG_BK / 256;
   double ra[NL], rb[NL];
+  bool ma[NL], mb[NL];
+  // out-of-range elements read a clamped (valid) address and are zeroed when stored to
+  // LDS, so the loads need no branches and stay in flight through the MFMA loop
   const bool rowok = row0 + ar < g.M;
+  const long arow = min(row0 + ar, g.M - 1);
   auto load = [&](int k0) {
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
       const int gk = k0 + ak + 4 * j;
-      ra[j] = (rowok && gk < g.K) ? A[(row0 + ar) + g.lda * gk] : 0.0;
+      ra[j] = A[arow + g.lda * min(gk, g.K - 1)];
+      ma[j] = rowok & (gk < g.K);
       const int gk2 = k0 + bk, gc = col0 + bc + 8 * j;
-      rb[j] = (gk2 < g.K && gc < g.N) ? B[gk2 + g.ldb * gc] : 0.0;
+      rb[j] = B[min(gk2, g.K - 1) + g.ldb * static_cast<long>(min(gc, g.N - 1))];
+      mb[j] = (gk2 < g.K) & (gc < g.N);
     }
   };
   acc4 acc[2][2];
@@ -154,11 +171,11 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM) {
   for (int k0 = 0; k0 < g.K; k0 += MG_BK) {
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-      As[mg_off(ar, ak + 4 * j)] = static_cast<CT>(ra[j]);
-      Bs[mg_off(bc + 8 * j, bk)] = static_cast<CT>(rb[j]);
+      As[mg_off(ar, ak + 4 * j)] = static_cast<CT>(ma[j] ? ra[j] : 0.0);
+      Bs[mg_off(bc + 8 * j, bk)] = static_cast<CT>(mb[j] ? rb[j] : 0.0);
     }
     __syncthreads();
-    if (k0 + MG_BK < g.K) load(k0 + MG_BK);
+    load(k0 + MG_BK);  // unconditional: past the end it reads clamped, unused values
 #pragma unroll
     for (int kk = 0; kk < MG_BK; kk += 4) {
       const int kr = kk + lk;
@@ -1652,9 +1669,11 @@ void gemm(const Gemm& g, hipStream_t s) {
     k_gemm<32, 32, 16, 2, 2><<<grid, 256, 0, s>>>(g, tm);
   } else {
     int tm = (g.M + MG_BM - 1) / MG_BM, tn = (g.N + MG_BN - 1) / MG_BN;
-    dim3 grid(static_cast<unsigned>(tm * batch), tn);
-    if (g.fp32) k_gemm_mfma<float><<<grid, 256, 0, s>>>(g, tm);
-    else k_gemm_mfma<double><<<grid, 256, 0, s>>>(g, tm);
+    const long ntiles = static_cast<long>(tm) * tn * batch;
+    const long nwg = (ntiles + NXCD - 1) / NXCD * NXCD;
+    if (nwg > 0x7fffffffL) throw Error(5, "GEMM batch too large for one launch");
+    if (g.fp32) k_gemm_mfma<float><<<static_cast<unsigned>(nwg), 256, 0, s>>>(g, tm, tn, ntiles);
+    else k_gemm_mfma<double><<<static_cast<unsigned>(nwg), 256, 0, s>>>(g, tm, tn, ntiles);
   }
   EXA_HIP(hipGetLastError());
 }

@@ -28,7 +28,10 @@ def flags():
 
 
 def build(verbose=False, jobs=None):
-    """Compile every source to an object file (in parallel) and link the shared library."""
+    """Regenerate the models under models/, compile every source to an object file (in
+    parallel) and link the shared library."""
+    from . import modelgen
+    modelgen.generate_all()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []

@@ -14,6 +14,10 @@ text2code writes (see ``datain.py``) and returns a small dict describing the cas
   cube, ncu components, velocity c, diffusivity kappa, per-node uniform[-1, 1]
   source passed as the auxiliary field v (``vdgfile``); Dirichlet u = 0 on x = 0
   (condition 1), homogeneous Neumann (zero diffusive flux) elsewhere (condition 2).
+* ``navierstokes2d`` -- compressible Navier-Stokes (model ``ns2d``, a pdemodel file
+  under models/) in a [0, 10] x [0, 1] channel-like domain with a wall-normal
+  stretched quad mesh, an isothermal no-slip wall at y = 0 and a characteristic far
+  field elsewhere, free stream plus seeded Gaussian density bumps, DIRK time stepping.
 """
 import os
 import numpy as np
@@ -108,3 +112,68 @@ def convdiff_unsteady(path, nd=2, n=4, porder=3, pgauss=None, ncu=1, kappa=1.0, 
     odg = np.zeros((t.shape[0], m["npe"], ncu))
     dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=ncu, nc=ncu * (nd + 1), udg=udg, odg=odg)
     return dict(model="convdiff", nd=nd, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=ncu)
+
+
+def ns_freestream(gam=1.4, mach=0.5):
+    """(rho, rho u, rho v, rho E) of the nondimensional free stream: rho = 1, velocity (1, 0),
+    temperature 1, so p = 1/(gam M^2)."""
+    p = 1.0 / (gam * mach * mach)
+    return np.array([1.0, 1.0, 0.0, p / (gam - 1.0) + 0.5])
+
+
+def navierstokes2d(path, nx=256, ny=32, porder=4, pgauss=None, length=(10.0, 1.0), stretch=1.05,
+                   gam=1.4, mach=0.5, re=1000.0, pr=0.72, twall=1.0, nbumps=8, amp=1e-3, width=0.2,
+                   seed=20261202, dt=1e-2, nsteps=20, torder=3, nstage=3, tau=1.0, precond="bilu0",
+                   restart=100, gmres_tol=1e-8, gmres_maxit=2000, newton_iter=10, newton_tol=1e-8,
+                   ortho=0, cartgridpart=True, se=2):
+    """Compressible Navier-Stokes (model ``ns2d``) on [0, Lx] x [0, Ly], nx x ny quads with the
+    cell heights growing geometrically by ``stretch`` away from the wall y = 0.
+
+    Condition 1 (y = 0): no-slip isothermal wall at temperature ``twall``; condition 2
+    (every other side): characteristic far field.  The initial state is the free stream
+    (rho, u, v, T) = (1, 1, 0, 1) with ``nbumps`` Gaussian density bumps
+    amp * exp(-|x - x_k|^2 / width^2) at seeded uniform centres x_k, at constant velocity and
+    pressure; q = -grad u of that state.  DIRK(torder, nstage) with nsteps steps of dt.
+    ``cartgridpart`` = True writes the element grid (nx, ny) with se x se superelements for the
+    block-ILU(0) preconditioner.
+    """
+    os.makedirs(path, exist_ok=True)
+    pgauss = 2 * porder if pgauss is None else pgauss
+    m = dw.make_master(2, porder, pgauss)
+    p, t = dw.structured_grid([nx, ny], lo=[0.0, 0.0], hi=list(length), stretch=[None, stretch])
+    exprs = [lambda v: np.all(np.abs(v[:, 1]) < 1e-10), lambda v: True]
+    bnd = dw.boundary_face_tags(p, t, 2, exprs)
+    bcs = [1, 2]
+    bf = np.where(bnd >= 0, np.array(bcs)[np.maximum(bnd, 0)], 0)
+    uinf = ns_freestream(gam, mach)
+    param = [gam, re, pr, mach] + list(uinf) + [twall]
+    dw.write_app(os.path.join(path, "app.bin"), nd=2, ncu=4,
+                 flag={0: 1, 5: ortho, 6: PRECOND[precond] if isinstance(precond, str) else precond},
+                 problem={2: 0, 3: torder, 4: nstage, 13: newton_iter, 14: gmres_maxit, 15: restart},
+                 physicsparam=param, tau=[tau], dt=[dt] * nsteps,
+                 solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
+    dw.write_master(os.path.join(path, "master.bin"), m)
+    if cartgridpart is True:
+        if nx % se or ny % se:
+            raise ValueError("superelement size %d must divide the grid %d x %d" % (se, nx, ny))
+        cg = [2, nx, ny, se, se, 0]
+    else:
+        cg = [] if not cartgridpart else list(cartgridpart)
+    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, bcs, cartgridpart=cg)
+    xdg = dw.compute_dgnodes(m, p, t)                     # [ne, npe, 2]
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform([0.0, 0.0], list(length), size=(nbumps, 2))
+    rho = np.ones(xdg.shape[:2])
+    drho = np.zeros(xdg.shape)                            # grad rho
+    for c in centres:
+        dx = xdg - c
+        gk = amp * np.exp(-np.sum(dx * dx, axis=2) / width ** 2)
+        rho += gk
+        drho += gk[:, :, None] * (-2.0 * dx / width ** 2)
+    pres = 1.0 / (gam * mach * mach)
+    u = np.stack([rho, rho, 0.0 * rho, pres / (gam - 1.0) + 0.5 * rho], axis=2)
+    du = np.stack([drho, drho, 0.0 * drho, 0.5 * drho], axis=2)     # [ne, npe, 4, 2]
+    q = -du
+    udg = np.concatenate([u, q[..., 0], q[..., 1]], axis=2)          # (u, q_x, q_y)
+    dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=4, nc=12, udg=udg)
+    return dict(model="ns2d", nd=2, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=4)

@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "models.hpp"
 #include "common.hpp"
+#include "generated/registry.hpp"
 
 #include <algorithm>
 
@@ -747,7 +748,7 @@ __global__ void k_getq(Dims d, const double* MCs, const double* MEs, const int* 
 // rv[(dp*NCU + k)*P + p]       : s*jac (dp = 0), sum_j f[k, j] Xx[j, m] (dp = m+1)
 // rJ[((dp*NCU + k)*NC + c)*P + p]: the same for the Jacobians (ApplyXxJac, kokkosimpl.h:2105-2158)
 template <class M, bool JAC>
-__global__ void k_vol_points(Dims d, AsmArgs a) {
+__global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
   constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
   const long P = static_cast<long>(a.eb) * d.nge;
   const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -864,20 +865,15 @@ __global__ void k_vol_W(Dims d, AsmArgs a) {
 // fpt fields (JAC): fhI[NCU], fhI_uq[NCU*NC], fhI_uh[NCU*NCU], then the same
 // with boundary faces replaced by FbouHdg ("B"); residual-only: fhI[NCU], fhB[NCU].
 // ---------------------------------------------------------------------------
-template <class M, bool JAC>
-__global__ void k_face_points(Dims d, AsmArgs a) {
+// Traced state at face point g of local face l of element E: uh, uq = (u, q) from the
+// element, x, the normal nl, the face Jacobian jac and the auxiliary field v.
+template <class M>
+__device__ __forceinline__ void face_state(const Dims& d, const AsmArgs& a, long E, int l, int g,
+                                           double* uh, double* uq, double* x, double* nl, double* v,
+                                           double& jac) {
   constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
-  constexpr int NF1 = JAC ? (NCU + NCU * NC + NCU * NCU) : NCU;
-  const long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
-  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (p >= P) return;
-  const int g = static_cast<int>(p % d.ngf);
-  const long el = p / d.ngf;
-  const int l = static_cast<int>(el % d.nfe);
-  const long E = a.e0 + el / d.nfe;
   const long PF = static_cast<long>(d.ne) * d.nfe * d.ngf;
   const long pf = (E * d.nfe + l) * d.ngf + g;
-  double uh[NCU], uq[NC], usave[NCU];
 #pragma unroll
   for (int c = 0; c < NCU; ++c) uh[c] = 0.0;
 #pragma unroll
@@ -891,14 +887,28 @@ __global__ void k_face_points(Dims d, AsmArgs a) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) uq[c] += phi * a.udg[node + d.npe * (c + NC * E)];
   }
-  double x[ND], nl[ND], v[8];
 #pragma unroll
   for (int b = 0; b < ND; ++b) {
     x[b] = a.fgeo[b * PF + pf];
     nl[b] = a.fgeo[(ND + b) * PF + pf];
   }
-  const double jac = a.fgeo[(2 * ND) * PF + pf];
+  jac = a.fgeo[(2 * ND) * PF + pf];
   for (int c = 0; c < d.nco && c < 8; ++c) v[c] = a.ogf[c * PF + pf];
+}
+
+template <class M, bool JAC>
+__global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
+  constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
+  constexpr int NF1 = JAC ? (NCU + NCU * NC + NCU * NCU) : NCU;
+  const long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const int g = static_cast<int>(p % d.ngf);
+  const long el = p / d.ngf;
+  const int l = static_cast<int>(el % d.nfe);
+  const long E = a.e0 + el / d.nfe;
+  double uh[NCU], uq[NC], usave[NCU], x[ND], nl[ND], v[8], jac;
+  face_state<M>(d, a, E, l, g, uh, uq, x, nl, v, jac);
   const double tau0 = a.tau0;
   // interior formula: flux at (uhat, q), dotted with n, plus tau*(u - uhat)
 #pragma unroll
@@ -934,21 +944,43 @@ __global__ void k_face_points(Dims d, AsmArgs a) {
         }
       }
   }
-  const int ib = a.bf[l + d.nfe * E];
-  if (ib > 0) {
-    double fb[NCU], fb_uq[JAC ? NCU * NC : 1], fb_uh[JAC ? NCU * NCU : 1];
-    M::template fbou<JAC>(ib, x, uq, v, uh, nl, a.tau, a.param, a.time, fb, fb_uq, fb_uh);
-#pragma unroll
-    for (int k = 0; k < NCU; ++k) a.fpt[(NF1 + k) * P + p] = fb[k] * jac;
-    if (JAC) {
-#pragma unroll
-      for (int i = 0; i < NCU * NC; ++i) a.fpt[(NF1 + NCU + i) * P + p] = fb_uq[i] * jac;
-#pragma unroll
-      for (int i = 0; i < NCU * NCU; ++i) a.fpt[(NF1 + NCU + NCU * NC + i) * P + p] = fb_uh[i] * jac;
-    }
-  } else {
+  // boundary faces get their "B" fields from k_face_bou_points
+  if (a.bf[l + d.nfe * E] == 0) {
 #pragma unroll
     for (int i = 0; i < NF1; ++i) a.fpt[(NF1 + i) * P + p] = a.fpt[i * P + p];
+  }
+}
+
+// "B" fields of the boundary face points of the block: FbouHdg (HdgFbou, the
+// FbouDriver of Model/KokkosDrivers.cpp) at every point of the faces
+// bface[bf0 .. bf0 + nbf) (face ids E*nfe + l, ascending, all inside the block).
+// A kernel of its own, so the model's boundary code (often far larger than the
+// flux, e.g. characteristic far-field conditions) does not weigh on the interior
+// face kernel, with 64-lane blocks so it may use up to 512 VGPRs.
+template <class M, bool JAC>
+__global__ __launch_bounds__(64) void k_face_bou_points(Dims d, AsmArgs a) {
+  constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
+  constexpr int NF1 = JAC ? (NCU + NCU * NC + NCU * NCU) : NCU;
+  const long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long>(a.nbf) * d.ngf) return;
+  const int g = static_cast<int>(t % d.ngf);
+  const long face = a.bface[a.bf0 + t / d.ngf];
+  const long E = face / d.nfe;
+  const int l = static_cast<int>(face % d.nfe);
+  const long p = ((E - a.e0) * d.nfe + l) * d.ngf + g;
+  double uh[NCU], uq[NC], x[ND], nl[ND], v[8], jac;
+  face_state<M>(d, a, E, l, g, uh, uq, x, nl, v, jac);
+  const int ib = a.bf[l + d.nfe * E];
+  double fb[NCU], fb_uq[JAC ? NCU * NC : 1], fb_uh[JAC ? NCU * NCU : 1];
+  M::template fbou<JAC>(ib, x, uq, v, uh, nl, a.tau, a.param, a.time, fb, fb_uq, fb_uh);
+#pragma unroll
+  for (int k = 0; k < NCU; ++k) a.fpt[(NF1 + k) * P + p] = fb[k] * jac;
+  if (JAC) {
+#pragma unroll
+    for (int i = 0; i < NCU * NC; ++i) a.fpt[(NF1 + NCU + i) * P + p] = fb_uq[i] * jac;
+#pragma unroll
+    for (int i = 0; i < NCU * NCU; ++i) a.fpt[(NF1 + NCU + NCU * NC + i) * P + p] = fb_uh[i] * jac;
   }
 }
 
@@ -1631,6 +1663,11 @@ void launch_points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStrea
     long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
     if (jac) k_face_points<M, true><<<nblocks(P, bs), bs, 0, s>>>(d, a);
     else k_face_points<M, false><<<nblocks(P, bs), bs, 0, s>>>(d, a);
+    if (a.nbf > 0) {
+      long Pb = static_cast<long>(a.nbf) * d.ngf;
+      if (jac) k_face_bou_points<M, true><<<nblocks(Pb, 64), 64, 0, s>>>(d, a);
+      else k_face_bou_points<M, false><<<nblocks(Pb, 64), 64, 0, s>>>(d, a);
+    }
   }
   EXA_HIP(hipGetLastError());
 }
@@ -1650,6 +1687,11 @@ void points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) 
     case MODEL_POISSON: return dispatch_points<Poisson>(d, a, jac, vol, s);
     case MODEL_POISSON_POLY: return dispatch_points<PoissonPoly>(d, a, jac, vol, s);
     case MODEL_CONVDIFF: return dispatch_points<ConvDiff>(d, a, jac, vol, s);
+#define EXA_GEN_CASE(ID, NAME, G) \
+  case ID:                        \
+    return launch_points<G>(d, a, jac, vol, s);
+      EXA_GENERATED_MODELS(EXA_GEN_CASE)
+#undef EXA_GEN_CASE
     default: throw Error(1, "unknown model");
   }
 }
@@ -1659,6 +1701,19 @@ void points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) 
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+int generated_model(const std::string& name, int* nd, int* ncu, int* nv) {
+#define EXA_GEN_FIND(ID, NAME, G) \
+  if (name == NAME) {             \
+    *nd = G::ND;                  \
+    *ncu = G::NCU;                \
+    *nv = G::NV;                  \
+    return ID;                    \
+  }
+  EXA_GENERATED_MODELS(EXA_GEN_FIND)
+#undef EXA_GEN_FIND
+  return -1;
+}
+
 void gemm(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
   const long batch = static_cast<long>(g.nb0) * g.nb1 * g.nb2;

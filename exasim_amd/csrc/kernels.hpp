@@ -3,7 +3,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 namespace exa {
+
+// Models generated from pdemodel.txt files (modelgen.py -> generated/registry.hpp):
+// the model id for `name` and its nd, ncu and auxiliary-field width nv, or -1.
+int generated_model(const std::string& name, int* nd, int* ncu, int* nv);
 
 struct Dims {
   int nd, ncu, nc, nco, npe, npf, nfe, nge, ngf, ne, nf;
@@ -64,6 +70,9 @@ struct AsmArgs {
   int e0, eb;
   const double *shapegt, *shapegw, *shapfgt, *shapfgw;
   const int *perm, *elemcon, *bf;
+  // boundary faces of the block: bface[bf0 .. bf0 + nbf), face ids E*nfe + l in ascending order
+  const int* bface;
+  int bf0, nbf;
   const double *udg, *uh, *egeo, *fgeo, *og, *ogf;
   double *rv, *rJ, *W, *fpt;
   long augs;  // per-element stride of the augmented system [D | F | Ru (| I)]

@@ -23,6 +23,9 @@ int model_id(const std::string& name) {
   if (name == "poisson") return MODEL_POISSON;
   if (name == "poisson_poly") return MODEL_POISSON_POLY;
   if (name == "convdiff") return MODEL_CONVDIFF;
+  int nd, ncu, nv;
+  const int id = generated_model(name, &nd, &ncu, &nv);
+  if (id >= 0) return id;
   throw Error(1, "unknown model '" + name + "'");
 }
 
@@ -63,7 +66,7 @@ void back_solve(double* y, const double* Hm, const double* s, int i, int ld) {
 }  // namespace
 
 Solver::Solver(const std::string& din, const std::string& dout, const std::string& mname, int dev)
-    : datain(din), dataout(dout), model(model_id(mname)), device(dev) {
+    : datain(din), dataout(dout), model_name(mname), model(model_id(mname)), device(dev) {
   setup();
 }
 
@@ -98,6 +101,17 @@ void Solver::setup() {
   if (static_cast<long>(sol.xdg.size()) != static_cast<long>(d.npe) * d.nd * d.ne)
     throw Error(2, "xdg must have ncx = nd components");
   d.nco = sol.nco;
+  if (model >= 100) {  // a generated model is compiled for one (nd, ncu)
+    int nd, ncu, nv;
+    generated_model(model_name, &nd, &ncu, &nv);
+    if (nd != d.nd || ncu != d.ncu)
+      throw Error(1, "model '" + model_name + "' is for nd = " + std::to_string(nd) + ", ncu = " +
+                         std::to_string(ncu) + " but datain has nd = " + std::to_string(d.nd) +
+                         ", ncu = " + std::to_string(d.ncu));
+    if (nv > d.nco)
+      throw Error(1, "model '" + model_name + "' reads " + std::to_string(nv) +
+                         " auxiliary components (vdg) but sol.bin has " + std::to_string(d.nco));
+  }
   conn = build_conn(master, mesh);
   d.nf = conn.nf;
   d.n = d.npe * d.ncu;
@@ -155,6 +169,10 @@ void Solver::setup() {
   perm.upload(mesh.perm, st, s);
   elemcon.upload(conn.elemcon, st, s);
   bf.upload(mesh.bf, st, s);
+  bface_h.clear();
+  for (long f = 0; f < static_cast<long>(d.ne) * d.nfe; ++f)
+    if (mesh.bf[f] > 0) bface_h.push_back(static_cast<int>(f));
+  if (!bface_h.empty()) bface.upload(bface_h, st, s);
   f2e.upload(conn.f2e, st, s);
   xdg.upload(sol.xdg, st, s);
 
@@ -499,6 +517,15 @@ AsmArgs Solver::asm_args(int e0, int eb) {
   a.perm = perm.p;
   a.elemcon = elemcon.p;
   a.bf = bf.p;
+  a.bface = bface.p;
+  {
+    const int lo = static_cast<int>(std::lower_bound(bface_h.begin(), bface_h.end(), e0 * d.nfe) -
+                                    bface_h.begin());
+    const int hi = static_cast<int>(std::lower_bound(bface_h.begin(), bface_h.end(), (e0 + eb) * d.nfe) -
+                                    bface_h.begin());
+    a.bf0 = lo;
+    a.nbf = hi - lo;
+  }
   a.udg = udg.p;
   a.uh = uh.p;
   a.egeo = egeo.p;

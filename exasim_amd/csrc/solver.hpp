@@ -16,7 +16,7 @@ enum TimerClass { T_MATVEC = 0, T_PRECOND = 1, T_ORTHO = 2, T_ASSEMBLY = 3, T_LU
 
 struct Solver {
   // problem description
-  std::string datain, dataout;
+  std::string datain, dataout, model_name;
   int model = 0;
   int device = 0;
   Dims d{};
@@ -47,6 +47,8 @@ struct Solver {
   // device data
   DevBuf<double> shapegt, shapegw, shapfgt, shapfgw, param, tau;
   DevBuf<int> perm, elemcon, bf, f2e, info;
+  std::vector<int> bface_h;  // boundary faces E*nfe + l (bf > 0), ascending
+  DevBuf<int> bface;
   DevBuf<long> src1, src2;
   DevBuf<double> xdg, egeo, fgeo, odg, og, ogf;
   DevBuf<double> udg, uh, udg0, uh0, MCs, MEs, Mass;

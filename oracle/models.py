@@ -139,4 +139,11 @@ def make_model(name, nd, ncu):
         return PoissonPoly(nd, ncu)
     if name == "convdiff":
         return ConvDiff(nd, ncu)
+    # a model defined by a pdemodel file: exasim_amd/models/<name>.txt, interpreted
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "exasim_amd", "models", name + ".txt")
+    if os.path.exists(path):
+        from .dslmodel import DSLModel
+        return DSLModel(path, nd, ncu)
     raise ValueError("unknown model " + name)

@@ -177,3 +177,62 @@ def navierstokes2d(path, nx=256, ny=32, porder=4, pgauss=None, length=(10.0, 1.0
     udg = np.concatenate([u, q[..., 0], q[..., 1]], axis=2)          # (u, q_x, q_y)
     dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=4, nc=12, udg=udg)
     return dict(model="ns2d", nd=2, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=4)
+
+
+def main(argv=None):
+    """python -m exasim_amd.cases BUILDER DIR [key=value ...] [--solve] [--precision 0|1]
+    [--precond N] [--no-verbose]: write a datain directory with one of the builders above
+    and optionally solve it on the GPU, printing the Newton log, the wall time and the
+    per-class kernel timings.  Values are parsed as Python literals (e.g. nx=256 dt=0.01
+    length=(10.0,1.0) precond='bilu0')."""
+    import argparse
+    import ast
+    import json
+    import sys
+    import time
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n")[0])
+    ap.add_argument("builder")
+    ap.add_argument("dir")
+    ap.add_argument("params", nargs="*")
+    ap.add_argument("--solve", action="store_true")
+    ap.add_argument("--model", default=None)
+    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--precond", type=int, default=None)
+    ap.add_argument("--no-verbose", action="store_true")
+    ap.add_argument("--save", default=None, help="write the final uh and udg to this .npz file")
+    args = ap.parse_args(argv)
+    fn = globals().get(args.builder)
+    if fn is None or args.builder.startswith("_") or args.builder in ("main", "ns_freestream"):
+        ap.error("unknown builder " + args.builder)
+    kw = {}
+    for p in args.params:
+        k, v = p.split("=", 1)
+        try:
+            kw[k] = ast.literal_eval(v)
+        except (ValueError, SyntaxError):
+            kw[k] = v
+    t0 = time.perf_counter()
+    info = fn(args.dir, **kw)
+    print("datain %s written in %.2f s: %s" % (args.dir, time.perf_counter() - t0, info), flush=True)
+    if not args.solve:
+        return 0
+    from .hdg import HDGSolver
+    with HDGSolver(args.dir, args.model or info["model"]) as S:
+        S.set_option("verbose", 0 if args.no_verbose else 1)
+        S.set_option("precision", args.precision)
+        if args.precond is not None:
+            S.set_option("preconditioner", args.precond)
+        S.set_option("timing", 1)
+        t0 = time.perf_counter()
+        newton, gmres, norm = S.solve()
+        wall = time.perf_counter() - t0
+        print(json.dumps({"newton": newton, "gmres": gmres, "residual": norm, "wall_s": wall,
+                          "sizes": S.sizes, "timings_ms": S.timings()}), flush=True)
+        if args.save:
+            np.savez(args.save, uh=S.get("uh"), udg=S.get("udg"))
+    sys.stdout.flush()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -236,3 +236,48 @@ def main(argv=None):
 
 if __name__ == "__main__":
     raise SystemExit(main())
+
+
+def convdiff2d_param(path, n=16, porder=3, pgauss=None, kappa=0.1, angle=0.0, k=1, tau=1.0,
+                     precond="eas", restart=50, gmres_tol=1e-10, gmres_maxit=2000, newton_iter=1,
+                     newton_tol=1e-8, ortho=0):
+    """One parametrized 2D convection-diffusion instance (model ``cd2d``): n x n quads of the
+    unit square, velocity (cos angle, sin angle), diffusivity kappa, source
+    sin(k pi x) cos(k pi y), u = 0 on the boundary."""
+    os.makedirs(path, exist_ok=True)
+    pgauss = 2 * porder if pgauss is None else pgauss
+    m = dw.make_master(2, porder, pgauss)
+    p, t = dw.structured_grid([n, n])
+    bnd = dw.boundary_face_tags(p, t, 2, [lambda v: True])
+    bf = np.where(bnd >= 0, 1, 0)
+    dw.write_app(os.path.join(path, "app.bin"), nd=2, ncu=1, flag=_app_flags(precond, ortho),
+                 problem={13: newton_iter, 14: gmres_maxit, 15: restart},
+                 physicsparam=[kappa, np.cos(angle), np.sin(angle), float(k)], tau=[tau],
+                 solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
+    dw.write_master(os.path.join(path, "master.bin"), m)
+    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, [1])
+    xdg = dw.compute_dgnodes(m, p, t)
+    dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=1, nc=3)
+    return dict(model="cd2d", nd=2, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=1, n=n, porder=porder,
+                kappa=kappa, angle=angle, k=k)
+
+
+def convdiff2d_batch(root, count=256, sizes=(8, 16, 24, 32, 48, 64), orders=(2, 3, 4, 5), seed=20261202,
+                     **kw):
+    """``count`` parametrized instances of ``convdiff2d_param`` under root/inst####: n uniform
+    over ``sizes``, p uniform over ``orders``, kappa log-uniform in [1e-3, 1], velocity angle
+    uniform in [0, 2 pi) (magnitude 1), k uniform in {1, 2, 3, 4}.  Returns the instance
+    descriptions (with their datain paths), in instance order."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice(sizes))
+        porder = int(rng.choice(orders))
+        kappa = float(10.0 ** rng.uniform(-3.0, 0.0))
+        angle = float(rng.uniform(0.0, 2.0 * np.pi))
+        k = int(rng.integers(1, 5))
+        path = os.path.join(root, "inst%04d" % i)
+        info = convdiff2d_param(path, n=n, porder=porder, kappa=kappa, angle=angle, k=k, **kw)
+        info["path"] = path
+        out.append(info)
+    return out

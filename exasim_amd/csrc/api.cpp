@@ -9,6 +9,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 struct exa_handle {
   std::unique_ptr<exa::Solver> solver;
@@ -58,6 +59,59 @@ int exa_create(const char* datain_dir, const char* dataout_prefix, const char* m
     g_create_error = e.what();
     return EXA_ERR_ARG;
   }
+}
+
+int exa_batch_create(int ninst, const char* const* datain_dirs, const char* model, int device,
+                     exa_handle** out) {
+  if (!out || !datain_dirs || !model || ninst < 1) return EXA_ERR_ARG;
+  *out = nullptr;
+  try {
+    std::vector<std::string> dirs;
+    for (int i = 0; i < ninst; ++i) {
+      if (!datain_dirs[i]) {
+        g_create_error = "datain_dirs[" + std::to_string(i) + "] is null";
+        return EXA_ERR_ARG;
+      }
+      dirs.emplace_back(datain_dirs[i]);
+    }
+    auto h = std::make_unique<exa_handle>();
+    h->solver = std::make_unique<exa::Solver>(dirs, model, device);
+    *out = h.release();
+    g_create_error.clear();
+    return EXA_OK;
+  } catch (const exa::Error& e) {
+    g_create_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return EXA_ERR_ARG;
+  }
+}
+
+int exa_batch_size(const exa_handle* h) {
+  if (!h || !h->solver) return 0;
+  return h->solver->ninst;
+}
+
+int exa_batch_stats(exa_handle* h, int* newton_iters, int* gmres_iters, double* final_norm, int n) {
+  return guard(h, [&](exa::Solver& s) {
+    if (n < s.ninst) throw exa::Error(EXA_ERR_ARG, "exa_batch_stats: arrays shorter than the batch");
+    for (int i = 0; i < s.ninst; ++i) {
+      const bool have = static_cast<int>(s.inst_newton.size()) == s.ninst;
+      if (newton_iters) newton_iters[i] = have ? s.inst_newton[i] : 0;
+      if (gmres_iters) gmres_iters[i] = have ? s.inst_gmres[i] : 0;
+      if (final_norm) final_norm[i] = have ? s.inst_norm[i] : 0.0;
+    }
+  });
+}
+
+long exa_batch_get(exa_handle* h, int inst, const char* name, double* dst, long len) {
+  long out = -1;
+  int rc = guard(h, [&](exa::Solver& s) {
+    if (!name) throw exa::Error(EXA_ERR_ARG, "null name");
+    out = s.get_inst_array(inst, name, dst, len);
+  });
+  return rc == EXA_OK ? out : -rc;
 }
 
 void exa_destroy(exa_handle* h) { delete h; }

@@ -1,0 +1,528 @@
+// batch.cpp -- independent problems solved together: a batch of instances with one
+// element shape (nd, p, quadrature, ncu, model) is merged into one disconnected mesh, so
+// that assembly, condensation, the matvec and the block-Jacobi / Schwarz preconditioners
+// run as single launches over every instance's elements and faces.  The Krylov and
+// Newton scalars stay per instance: dot products, norms, Givens rotations, tolerances,
+// damping and convergence are those of each instance alone (segmented reductions over the
+// instance's own contiguous slice of each vector), so an instance's iterates and result do
+// not depend on what else is in the batch.
+//
+// Per-instance algorithm: GMRES as backend/Solver/gmres.cpp:361-597 and NewtonSolver as
+// backend/Solution/solution.cpp:125-242 (see solver.cpp for the single-instance forms).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace exa {
+
+namespace {
+
+bool same_master(const MasterData& a, const MasterData& b) {
+  return a.nd == b.nd && a.elemtype == b.elemtype && a.porder == b.porder && a.npe == b.npe &&
+         a.npf == b.npf && a.nge == b.nge && a.ngf == b.ngf && a.shapegt == b.shapegt &&
+         a.shapegw == b.shapegw && a.shapfgt == b.shapfgt && a.shapfgw == b.shapfgw;
+}
+
+template <typename T>
+void append(std::vector<T>& dst, const std::vector<T>& src) {
+  dst.insert(dst.end(), src.begin(), src.end());
+}
+
+}  // namespace
+
+Solver::Solver(const std::vector<std::string>& dins, const std::string& mname, int dev)
+    : datain(dins.empty() ? std::string() : dins[0]), model_name(mname), model(0), device(dev) {
+  if (dins.empty()) throw Error(1, "a batch needs at least one datain directory");
+  model = resolve_model(mname);
+  batch = true;
+  ninst = static_cast<int>(dins.size());
+  datains = dins;
+  SolData sol;
+  std::vector<std::vector<double>> uhs;
+  int npmax = 1, ntmax = 1;
+  std::vector<AppData> apps;
+  for (int i = 0; i < ninst; ++i) {
+    const std::string& din = dins[i];
+    AppData a = read_app(din + "/app.bin");
+    MasterData ma = read_master(din + "/master.bin");
+    MeshData me = read_mesh(din + "/mesh.bin");
+    if (ma.nd != me.nd) throw Error(2, din + ": master.bin and mesh.bin disagree on nd");
+    const int nc = static_cast<int>(a.ndims[5]), ncu = static_cast<int>(a.ndims[6]);
+    if (i == 0) {
+      app = a;
+      master = ma;
+      mesh = me;
+      set_dims();
+    } else {
+      if (!same_master(master, ma))
+        throw Error(1, din + ": a batch needs one master element (same nd, p and quadrature)");
+      if (nc != d.nc || ncu != d.ncu || me.nfe != d.nfe || me.perm != mesh.perm)
+        throw Error(1, din + ": a batch needs one (nc, ncu) and one face numbering");
+      if (a.flag[0] != app.flag[0] || a.flag[6] != app.flag[6] || a.problem[2] != app.problem[2] ||
+          a.problem[3] != app.problem[3] || a.problem[4] != app.problem[4] || a.dt != app.dt)
+        throw Error(1, din + ": a batch needs one time scheme, time steps and preconditioner");
+    }
+    SolData so = read_sol(din + "/sol.bin", ma.npe, nc, ncu, me.ne);
+    if (i > 0 && so.nco != sol.nco) throw Error(1, din + ": a batch needs one auxiliary-field width");
+    Conn c = build_conn(ma, me);
+    const long e0 = i == 0 ? 0 : eoff.back(), f0 = i == 0 ? 0 : foff.back();
+    if (i == 0) {
+      eoff = {0};
+      foff = {0};
+      sol.nco = so.nco;
+      sol.udg_ncomp = d.nc;
+      mesh.bf.clear();
+      conn = Conn{};
+    }
+    eoff.push_back(e0 + me.ne);
+    foff.push_back(f0 + c.nf);
+    // element data: [npe, *, ne] arrays concatenate over elements; udg gets all nc components
+    append(sol.xdg, so.xdg);
+    std::vector<double> u(static_cast<size_t>(ma.npe) * d.nc * me.ne, 0.0);
+    if (!so.udg.empty()) {
+      const int k = so.udg_ncomp;
+      for (long e = 0; e < me.ne; ++e)
+        for (int cc = 0; cc < k; ++cc)
+          for (int j = 0; j < ma.npe; ++j)
+            u[j + ma.npe * (cc + static_cast<long>(d.nc) * e)] = so.udg[j + ma.npe * (cc + static_cast<long>(k) * e)];
+    }
+    append(sol.udg, u);
+    append(sol.odg, so.odg);
+    uhs.push_back(so.uh);
+    append(mesh.bf, me.bf);
+    // connectivity with element, face and face-node offsets
+    for (int f = 0; f < c.nf; ++f) {
+      conn.f2e.push_back(c.f2e[4 * f] + static_cast<int>(e0));
+      conn.f2e.push_back(c.f2e[4 * f + 1]);
+      conn.f2e.push_back(c.f2e[4 * f + 2] >= 0 ? c.f2e[4 * f + 2] + static_cast<int>(e0) : -1);
+      conn.f2e.push_back(c.f2e[4 * f + 3]);
+    }
+    for (int v : c.elemcon) conn.elemcon.push_back(v + static_cast<int>(f0) * ma.npf);
+    conn.nif += c.nif;
+    npmax = std::max<int>(npmax, static_cast<int>(a.physicsparam.size()));
+    ntmax = std::max<int>(ntmax, static_cast<int>(a.tau.size()));
+    apps.push_back(std::move(a));
+  }
+  conn.nf = static_cast<int>(foff.back());
+  mesh.ne = static_cast<int>(eoff.back());
+  mesh.cartgridpart.clear();
+  mesh.t.clear();
+  d.ne = mesh.ne;
+  // initial traces: all given, or all taken from the elements (init() does that)
+  const bool any_uh = std::any_of(uhs.begin(), uhs.end(), [](const std::vector<double>& v) { return !v.empty(); });
+  const bool all_uh = std::all_of(uhs.begin(), uhs.end(), [](const std::vector<double>& v) { return !v.empty(); });
+  if (any_uh && !all_uh) throw Error(1, "a batch needs uhat in every sol.bin or in none");
+  if (all_uh)
+    for (auto& v : uhs) append(sol.uh, v);
+  // per-instance physicsparam and tau, zero-padded to the longest
+  nparam = npmax;
+  ntau = ntmax;
+  inst_param.assign(static_cast<size_t>(nparam) * ninst, 0.0);
+  inst_tau.assign(static_cast<size_t>(ntau) * ninst, 0.0);
+  for (int i = 0; i < ninst; ++i) {
+    std::copy(apps[i].physicsparam.begin(), apps[i].physicsparam.end(), inst_param.begin() + static_cast<long>(i) * nparam);
+    std::copy(apps[i].tau.begin(), apps[i].tau.end(), inst_tau.begin() + static_cast<long>(i) * ntau);
+  }
+  if (static_cast<int>(app.flag[6]) == 2)
+    throw Error(5, "block-ILU(0) (preconditioner 2) needs one structured grid; a batch uses "
+                   "preconditioner 0 or 1");
+  init(sol);
+}
+
+// Block maps of the segmented vector kernels: blocks of at most SEG_CHUNK entries that start at
+// each segment's first entry.
+void Solver::build_seg(SegDev& sd, const std::vector<long>& off) {
+  const int ns = static_cast<int>(off.size()) - 1;
+  std::vector<int> bseg, sbptr(ns + 1, 0);
+  std::vector<long> bbeg, bend;
+  for (int sg = 0; sg < ns; ++sg) {
+    sbptr[sg] = static_cast<int>(bseg.size());
+    for (long i = off[sg]; i < off[sg + 1]; i += SEG_CHUNK) {
+      bseg.push_back(sg);
+      bbeg.push_back(i);
+      bend.push_back(std::min(off[sg + 1], i + SEG_CHUNK));
+    }
+  }
+  sbptr[ns] = static_cast<int>(bseg.size());
+  if (bseg.empty()) {  // all segments empty: keep valid device pointers
+    bseg.push_back(0);
+    bbeg.push_back(0);
+    bend.push_back(0);
+  }
+  sd.bseg.upload(bseg, st, s);
+  sd.sbptr.upload(sbptr, st, s);
+  sd.bbeg.upload(bbeg, st, s);
+  sd.bend.upload(bend, st, s);
+  sd.map.nseg = ns;
+  sd.map.nblk = sbptr[ns] > 0 ? static_cast<int>(bseg.size()) : 0;
+  sd.map.bseg = sd.bseg.p;
+  sd.map.bbeg = sd.bbeg.p;
+  sd.map.bend = sd.bend.p;
+  sd.map.sbptr = sd.sbptr.p;
+}
+
+void Solver::seg_setup() {
+  std::vector<long> ot(ninst + 1), ou(ninst + 1);
+  for (int i = 0; i <= ninst; ++i) {
+    ot[i] = foff[i] * d.ncf;
+    ou[i] = eoff[i] * d.n;
+  }
+  build_seg(segT, ot);
+  build_seg(segU, ou);
+  if (!batch) return;
+  dus.alloc(static_cast<size_t>(d.n) * d.ne);
+  seg_alloc();
+}
+
+// Buffers of the segmented GMRES, sized by the restart length (called again when it changes,
+// with the stream idle).  Device: partial sums for up to restart+1 vectors, the Hessenberg
+// column, y, norms and coefficients ([0, 2ns) norms and GMRES coefficients, [3ns, 4ns) Newton
+// damping), masks ([0, ns) GMRES, [ns, 2ns) Newton).  Host (pinned): disjoint regions for the
+// column pulled back each step and for each kind of upload, so an upload still in flight is
+// never overwritten by the next one.
+void Solver::seg_alloc() {
+  const long ns = ninst, n1 = restart + 1;
+  const long nb = std::max(segT.map.nblk, segU.map.nblk);
+  EXA_HIP(hipStreamSynchronize(s));
+  segpart.alloc(static_cast<size_t>(std::max(nb, 1L)) * (n1 + 1));
+  hcol.alloc(static_cast<size_t>(n1 + 1) * ns);
+  segy.alloc(static_cast<size_t>(n1) * ns);
+  segcoef.alloc(static_cast<size_t>(4 * ns));
+  segmask.alloc(static_cast<size_t>(2 * ns));
+  const long need = (n1 + 1) * ns + 2 * ns + n1 * ns + 2 * ns;
+  if (need > hseg_len) {
+    if (hseg) EXA_HIP(hipHostFree(hseg));
+    hseg = nullptr;
+    EXA_HIP(hipHostMalloc(reinterpret_cast<void**>(&hseg), sizeof(double) * need, hipHostMallocDefault));
+    hseg_len = need;
+  }
+}
+
+double* Solver::host_region(int which) {
+  const long ns = ninst, n1 = restart + 1;
+  switch (which) {
+    case 0: return hseg;                                   // pulls, (n1+1)*ns
+    case 1: return hseg + (n1 + 1) * ns;                   // GMRES mask / coefficients, 2*ns
+    case 2: return hseg + (n1 + 1) * ns + 2 * ns;          // GMRES y, n1*ns
+    default: return hseg + (n1 + 1) * ns + 2 * ns + n1 * ns;  // Newton mask / coefficients, 2*ns
+  }
+}
+
+void Solver::seg_norms(const SegDev& sd, const double* v, double* out_dev) {
+  seg_multidot(sd.map, v, 0, 1, v, segpart.p, out_dev, 1, s);
+}
+
+// per instance ||b|| + ||Ru|| of the current state (hdgAssembleResidual, Newton's norm)
+std::vector<double> Solver::residual_seg() {
+  residual_fields();
+  seg_norms(segT, bres.p, segcoef.p);
+  seg_norms(segU, RuR.p, segcoef.p + ninst);
+  double* h = host_region(0);
+  EXA_HIP(hipMemcpyAsync(h, segcoef.p, sizeof(double) * 2 * ninst, hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  std::vector<double> out(ninst);
+  for (int i = 0; i < ninst; ++i) out[i] = h[i] + h[ninst + i];
+  return out;
+}
+
+// Restarted left-preconditioned GMRES, one Krylov recurrence per instance over shared
+// launches.  iters[i] = Arnoldi steps instance i took; returns the largest.
+int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<int>& iters) {
+  const long N = d.N;
+  const int ns = ninst;
+  const SegMap& m = segT.map;
+  const int n1 = restart + 1;
+  iters.assign(ns, 0);
+  if (static_cast<long>(hcol.n) < static_cast<long>(n1 + 1) * ns) seg_alloc();
+  double* h = host_region(0);        // pulled columns and norms
+  double* hm = host_region(1);       // [0, ns) mask, [ns, 2ns) coefficients
+  double* hy = host_region(2);       // y
+  // every host-side write to a region below comes after a stream synchronisation (a pull)
+  // that followed the region's previous upload
+  auto pull = [&](const double* src, long cnt) {
+    EXA_HIP(hipMemcpyAsync(h, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+  };
+  auto push_mask = [&](const std::vector<char>& act) {
+    for (int i = 0; i < ns; ++i) hm[i] = act[i] ? 1.0 : 0.0;
+    EXA_HIP(hipMemcpyAsync(segmask.p, hm, sizeof(double) * ns, hipMemcpyHostToDevice, s));
+  };
+
+  EXA_HIP(hipMemsetAsync(xv, 0, sizeof(double) * N, s));
+  seg_norms(segT, bv, segcoef.p);
+  pull(segcoef.p, ns);
+  std::vector<char> act(ns, 0);
+  int nact = 0;
+  for (int i = 0; i < ns; ++i) {
+    const double nb = h[i];
+    act[i] = !(nb == 0.0 || (nb < newton_tol && tol_rel < 0.0));
+    nact += act[i];
+  }
+  if (!nact) return 0;
+  const double tol = std::min(0.1, tol_rel < 0.0 ? gmres_tol : tol_rel);
+  copy(r.p, bv, N, s);
+  apply_precond(r.p);
+  push_mask(act);
+  seg_scale(m, r.p, r.p, segmask.p, 0, s);
+  seg_norms(segT, r.p, segcoef.p);
+  pull(segcoef.p, ns);
+  std::vector<double> nrmb(h, h + ns), nrmr = nrmb;
+  std::vector<double> Hm(static_cast<size_t>(ns) * n1 * n1, 0.0), sv(static_cast<size_t>(ns) * (n1 + 1)),
+      cs(static_cast<size_t>(ns) * (n1 + 1)), sn(static_cast<size_t>(ns) * (n1 + 1)), y(n1 + 1), Hc(n1 + 1);
+  auto Hs = [&](int sg) { return Hm.data() + static_cast<size_t>(sg) * n1 * n1; };
+  int j = 0;
+  hipEvent_t t0{};
+  while (j < gmres_maxit && nact > 0) {
+    // V_0 = r / ||r|| per active instance, zero elsewhere
+    for (int i = 0; i < ns; ++i) hm[ns + i] = act[i] ? nrmr[i] : 0.0;
+    EXA_HIP(hipMemcpyAsync(segcoef.p + ns, hm + ns, sizeof(double) * ns, hipMemcpyHostToDevice, s));
+    seg_scale(m, V.p, r.p, segcoef.p + ns, 1, s);
+    std::fill(sv.begin(), sv.end(), 0.0);
+    for (int i = 0; i < ns; ++i) sv[static_cast<size_t>(i) * (n1 + 1)] = act[i] ? nrmr[i] : 0.0;
+    int i = 0;
+    for (; i < restart && j < gmres_maxit; ++i, ++j) {
+      double* vi = V.p + N * i;
+      double* vm = V.p + N * (i + 1);
+      matvec(vi, vm);
+      apply_precond(vm);
+      tic(T_ORTHO, &t0);
+      if (ortho == 0) {  // MGS, per instance
+        for (int k = 0; k <= i; ++k) {
+          seg_multidot(m, V.p + N * k, N, 1, vm, segpart.p, hcol.p + static_cast<long>(k) * ns, 0, s);
+          seg_axpy(m, vm, V.p + N * k, hcol.p + static_cast<long>(k) * ns, -1.0, s);
+        }
+      } else {  // CGS, per instance
+        seg_multidot(m, V.p, N, i + 1, vm, segpart.p, hcol.p, 0, s);
+        seg_gemv_cols(m, vm, V.p, N, i + 1, hcol.p, -1.0, s);
+      }
+      seg_multidot(m, vm, N, 1, vm, segpart.p, hcol.p + static_cast<long>(i + 1) * ns, 1, s);
+      seg_scale(m, vm, vm, hcol.p + static_cast<long>(i + 1) * ns, 1, s);
+      toc(T_ORTHO, t0);
+      pull(hcol.p, static_cast<long>(i + 2) * ns);
+      bool any = false;
+      std::vector<double> ycol(static_cast<size_t>(i + 1) * ns, 0.0);
+      for (int sg = 0; sg < ns; ++sg) {
+        if (!act[sg]) continue;
+        for (int k = 0; k < i + 2; ++k) Hc[k] = h[static_cast<long>(k) * ns + sg];
+        double* svs = sv.data() + static_cast<size_t>(sg) * (n1 + 1);
+        gmres_givens(Hc.data(), svs, cs.data() + static_cast<size_t>(sg) * (n1 + 1),
+                   sn.data() + static_cast<size_t>(sg) * (n1 + 1), i);
+        for (int k = 0; k < n1; ++k) Hs(sg)[k + static_cast<size_t>(n1) * i] = (k < i + 2) ? Hc[k] : 0.0;
+        iters[sg] = j + 1;
+        if (std::fabs(svs[i + 1]) / nrmb[sg] < tol) {
+          gmres_back_solve(y.data(), Hs(sg), svs, i, n1);
+          for (int k = 0; k <= i; ++k) ycol[static_cast<size_t>(k) * ns + sg] = y[k];
+          act[sg] = 0;
+          --nact;
+          any = true;
+        }
+      }
+      if (any) {  // x += V y for the instances that converged at this step; their basis stops here
+        std::memcpy(hy, ycol.data(), sizeof(double) * ycol.size());
+        EXA_HIP(hipMemcpyAsync(segy.p, hy, sizeof(double) * ycol.size(), hipMemcpyHostToDevice, s));
+        seg_gemv_cols(m, xv, V.p, N, i + 1, segy.p, 1.0, s);
+        push_mask(act);
+        seg_scale(m, vm, vm, segmask.p, 0, s);
+      }
+      if (nact == 0) break;
+    }
+    if (nact == 0) break;
+    // restart: x += V y for the rest, then r = M^{-1}(b - A x)
+    std::vector<double> ycol(static_cast<size_t>(i) * ns, 0.0);
+    for (int sg = 0; sg < ns; ++sg) {
+      if (!act[sg]) continue;
+      gmres_back_solve(y.data(), Hs(sg), sv.data() + static_cast<size_t>(sg) * (n1 + 1), i - 1, n1);
+      for (int k = 0; k < i; ++k) ycol[static_cast<size_t>(k) * ns + sg] = y[k];
+    }
+    std::memcpy(hy, ycol.data(), sizeof(double) * ycol.size());
+    EXA_HIP(hipMemcpyAsync(segy.p, hy, sizeof(double) * ycol.size(), hipMemcpyHostToDevice, s));
+    seg_gemv_cols(m, xv, V.p, N, i, segy.p, 1.0, s);
+    matvec(xv, r.p);
+    axpby(r.p, bv, 1.0, -1.0, N, s);
+    apply_precond(r.p);
+    push_mask(act);
+    seg_scale(m, r.p, r.p, segmask.p, 0, s);
+    seg_norms(segT, r.p, segcoef.p);
+    pull(segcoef.p, ns);
+    for (int sg = 0; sg < ns; ++sg) {
+      nrmr[sg] = h[sg];
+      if (act[sg] && nrmr[sg] / nrmb[sg] < tol) {
+        act[sg] = 0;
+        --nact;
+      }
+    }
+    if (verbose) {
+      std::printf("batch GMRES restart after %d iterations: %d of %d instances still iterating\n", j, nact, ns);
+      std::fflush(stdout);
+    }
+  }
+  if (nact > 0 && verbose)
+    std::printf("Warning: GMRES(%d) leaves %d instances above the tolerance %g after %d iterations\n", restart,
+                nact, tol, gmres_maxit);
+  return *std::max_element(iters.begin(), iters.end());
+}
+
+// Newton per instance over shared launches: an instance stops when its own residual norm is
+// below newton_tol; its damping (alpha halved while its residual grows) and, in the mixed
+// form, its refinement loop are its own.  Converged instances keep their state (their right-hand
+// sides and updates are masked to zero).
+void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm) {
+  const long N = d.N;
+  const int ns = ninst;
+  if (static_cast<int>(inst_newton.size()) != ns) {
+    inst_newton.assign(ns, 0);
+    inst_gmres.assign(ns, 0);
+    inst_norm.assign(ns, 0.0);
+  }
+  // Newton's own device mask (segmask + ns) and damping coefficients (segcoef + 3 ns), uploaded from
+  // their own host region; each upload is followed by a synchronising pull before the next
+  double* mask_d = segmask.p + ns;
+  double* coef_d = segcoef.p + 3L * ns;
+  auto push = [&](double* dst, const std::vector<double>& v) {
+    double* hh = host_region(3) + (dst == mask_d ? 0 : ns);
+    std::memcpy(hh, v.data(), sizeof(double) * v.size());
+    EXA_HIP(hipMemcpyAsync(dst, hh, sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
+  };
+  auto mask_of = [&](const std::vector<char>& a) {
+    std::vector<double> v(ns);
+    for (int i = 0; i < ns; ++i) v[i] = a[i] ? 1.0 : 0.0;
+    return v;
+  };
+  get_q();
+  std::vector<double> nr = residual_seg();
+  if (verbose) {
+    double mx = *std::max_element(nr.begin(), nr.end());
+    std::printf("batch Newton Iteration: 0, largest residual norm %g over %d instances\n", mx, ns);
+  }
+  DevBuf<double> u;
+  u.alloc(N);
+  copy(u.p, uh.p, N, s);
+  std::vector<char> act(ns, 1);
+  std::vector<int> its;
+  int it = 0, gtot = 0;
+  for (it = 0; it < newton_maxit; ++it) {
+    if (std::none_of(act.begin(), act.end(), [](char c) { return c != 0; })) break;
+    assemble();
+    push(mask_d, mask_of(act));
+    seg_scale(segT.map, b.p, b.p, mask_d, 0, s);
+    gtot += gmres_seg(b.p, x.p, precision ? std::max(gmres_tol, refine_first_tol) : -1.0, its);
+    for (int i = 0; i < ns; ++i)
+      if (act[i]) {
+        inst_gmres[i] += its[i];
+        ++inst_newton[i];
+      }
+    axpby(u.p, x.p, 1.0, 1.0, N, s);
+    copy(uh.p, u.p, N, s);
+    if (precision) dudg(d, RuS.p, DinvF32.p, elemcon.p, x.p, du.p, s);
+    else dudg(d, RuS.p, DinvF.p, elemcon.p, x.p, du.p, s);
+    seg_scale(segU.map, du.p, du.p, mask_d, 0, s);
+    update_udg(d, du.p, 1.0, udg.p, s);
+    get_q();
+    std::vector<double> nrm0 = nr;
+    nr = residual_seg();
+    std::vector<double> alpha(ns, 1.0);
+    for (int i = 0; i < ns; ++i)
+      if (act[i] && nr[i] > nrm0[i] && nr[i] > 1e6)
+        throw Error(4, "instance " + std::to_string(i) + ": residual norm increases more than 1e6");
+    // damping, per instance: alpha halved while its residual grows (or is NaN)
+    for (;;) {
+      std::vector<double> coef(ns, 0.0);
+      bool any = false;
+      for (int i = 0; i < ns; ++i)
+        if (act[i] && ((nr[i] > nrm0[i] && alpha[i] > 0.1) || (std::isnan(nr[i]) && alpha[i] > 1e-6))) {
+          alpha[i] /= 2.0;
+          coef[i] = -alpha[i];
+          any = true;
+        }
+      if (!any) break;
+      push(coef_d, coef);
+      seg_axpy(segT.map, u.p, x.p, coef_d, 1.0, s);
+      copy(uh.p, u.p, N, s);
+      seg_scale(segU.map, dus.p, du.p, coef_d, 0, s);
+      update_udg(d, dus.p, 1.0, udg.p, s);
+      get_q();
+      std::vector<double> nn = residual_seg();
+      for (int i = 0; i < ns; ++i)
+        if (coef[i] != 0.0) nr[i] = nn[i];
+    }
+    for (int i = 0; i < ns; ++i)
+      if (act[i] && std::isnan(nr[i])) throw Error(4, "instance " + std::to_string(i) + ": NaN residual");
+    // mixed form: refinement with the FP64 residual, per instance
+    last_refinements = 0;
+    if (precision) {
+      for (;;) {
+        std::vector<char> ref(ns, 0);
+        bool any = false;
+        for (int i = 0; i < ns; ++i) {
+          ref[i] = act[i] && nr[i] > refine_rtol * nrm0[i];
+          any = any || ref[i];
+        }
+        if (!any || last_refinements >= refine_maxit) break;
+        refine_rhs(d, LU32.p, piv32.p, K32.p, RuR.p, RuS.p, Rh.p, s);
+        face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
+        push(mask_d, mask_of(ref));
+        seg_scale(segT.map, b.p, b.p, mask_d, 0, s);
+        gtot += gmres_seg(b.p, x.p, refine_tol, its);
+        for (int i = 0; i < ns; ++i)
+          if (ref[i]) inst_gmres[i] += its[i];
+        axpby(u.p, x.p, 1.0, 1.0, N, s);
+        copy(uh.p, u.p, N, s);
+        dudg(d, RuS.p, DinvF32.p, elemcon.p, x.p, du.p, s);
+        seg_scale(segU.map, du.p, du.p, mask_d, 0, s);
+        update_udg(d, du.p, 1.0, udg.p, s);
+        get_q();
+        std::vector<double> nn = residual_seg();
+        for (int i = 0; i < ns; ++i)
+          if (ref[i]) nr[i] = nn[i];
+        ++last_refinements;
+        for (int i = 0; i < ns; ++i)
+          if (ref[i] && std::isnan(nr[i])) throw Error(4, "instance " + std::to_string(i) + ": NaN in refinement");
+      }
+    }
+    for (int i = 0; i < ns; ++i)
+      if (act[i]) {
+        inst_norm[i] = nr[i];
+        if (nr[i] < newton_tol) act[i] = 0;
+      }
+    if (verbose) {
+      int left = static_cast<int>(std::count(act.begin(), act.end(), 1));
+      double mx = *std::max_element(nr.begin(), nr.end());
+      std::printf("batch Newton Iteration: %d, largest residual norm %g, %d instances still iterating\n", it + 1,
+                  mx, left);
+      std::fflush(stdout);
+    }
+  }
+  EXA_HIP(hipStreamSynchronize(s));
+  if (newton_iters) *newton_iters = it;
+  if (gmres_total) *gmres_total = gtot;
+  if (final_norm) *final_norm = *std::max_element(nr.begin(), nr.end());
+}
+
+long Solver::get_inst_array(int inst, const std::string& name, double* dst, long len) {
+  if (inst < 0 || inst >= ninst) throw Error(1, "instance index out of range");
+  long off = 0, cnt = 0;
+  const double* src = nullptr;
+  if (name == "uh") {
+    off = foff[inst] * d.ncf;
+    cnt = (foff[inst + 1] - foff[inst]) * d.ncf;
+    src = uh.p;
+  } else if (name == "udg") {
+    off = eoff[inst] * static_cast<long>(d.npe) * d.nc;
+    cnt = (eoff[inst + 1] - eoff[inst]) * static_cast<long>(d.npe) * d.nc;
+    src = udg.p;
+  } else {
+    throw Error(1, "per-instance arrays: uh, udg");
+  }
+  if (!dst) return cnt;
+  if (len < cnt) throw Error(1, "destination too small for " + name);
+  std::vector<double> tmpv(cnt);
+  EXA_HIP(hipStreamSynchronize(s));
+  st.d2h(tmpv.data(), src + off, cnt * sizeof(double), s);
+  std::memcpy(dst, tmpv.data(), cnt * sizeof(double));
+  return cnt;
+}
+
+}  // namespace exa

@@ -743,6 +743,14 @@ __global__ void k_getq(Dims d, const double* MCs, const double* MEs, const int* 
 }
 
 // ---------------------------------------------------------------------------
+// physicsparam and tau of the instance element E belongs to (one instance: a.einst is null)
+__device__ __forceinline__ const double* inst_param(const AsmArgs& a, long E) {
+  return a.einst ? a.param + static_cast<long>(a.einst[E]) * a.nparam : a.param;
+}
+__device__ __forceinline__ const double* inst_tau(const AsmArgs& a, long E) {
+  return a.einst ? a.tau + static_cast<long>(a.einst[E]) * a.ntau : a.tau;
+}
+
 // Element volume terms: uEquationElemBlock (uequation.cpp:45-203)
 // ---------------------------------------------------------------------------
 // rv[(dp*NCU + k)*P + p]       : s*jac (dp = 0), sum_j f[k, j] Xx[j, m] (dp = m+1)
@@ -775,8 +783,9 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
   for (int c = 0; c < d.nco && c < 8; ++c) v[c] = a.og[c * PG + pg];
   double f[NCU * ND], s[NCU];
   double f_uq[JAC ? NCU * ND * NC : 1], s_uq[JAC ? NCU * NC : 1];
-  M::template flux<JAC>(x, uq, v, a.param, a.time, f, f_uq);
-  M::template source<JAC>(x, uq, v, a.param, a.time, s, s_uq);
+  const double* prm = inst_param(a, E);
+  M::template flux<JAC>(x, uq, v, prm, a.time, f, f_uq);
+  M::template source<JAC>(x, uq, v, prm, a.time, s, s_uq);
   if (a.tdep) {  // uequation.cpp:126-151, Tdfunc = 1 for the built-in models: s += sdg - dtfactor*u
     double sg[NCU];
 #pragma unroll
@@ -909,7 +918,7 @@ __global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
   const long E = a.e0 + el / d.nfe;
   double uh[NCU], uq[NC], usave[NCU], x[ND], nl[ND], v[8], jac;
   face_state<M>(d, a, E, l, g, uh, uq, x, nl, v, jac);
-  const double tau0 = a.tau0;
+  const double tau0 = inst_tau(a, E)[0];
   // interior formula: flux at (uhat, q), dotted with n, plus tau*(u - uhat)
 #pragma unroll
   for (int c = 0; c < NCU; ++c) {
@@ -917,7 +926,7 @@ __global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
     uq[c] = uh[c];
   }
   double f[NCU * ND], f_uq[JAC ? NCU * ND * NC : 1];
-  M::template flux<JAC>(x, uq, v, a.param, a.time, f, f_uq);
+  M::template flux<JAC>(x, uq, v, inst_param(a, E), a.time, f, f_uq);
 #pragma unroll
   for (int c = 0; c < NCU; ++c) uq[c] = usave[c];
 #pragma unroll
@@ -973,7 +982,7 @@ __global__ __launch_bounds__(64) void k_face_bou_points(Dims d, AsmArgs a) {
   face_state<M>(d, a, E, l, g, uh, uq, x, nl, v, jac);
   const int ib = a.bf[l + d.nfe * E];
   double fb[NCU], fb_uq[JAC ? NCU * NC : 1], fb_uh[JAC ? NCU * NCU : 1];
-  M::template fbou<JAC>(ib, x, uq, v, uh, nl, a.tau, a.param, a.time, fb, fb_uq, fb_uh);
+  M::template fbou<JAC>(ib, x, uq, v, uh, nl, inst_tau(a, E), inst_param(a, E), a.time, fb, fb_uq, fb_uh);
 #pragma unroll
   for (int k = 0; k < NCU; ++k) a.fpt[(NF1 + k) * P + p] = fb[k] * jac;
   if (JAC) {
@@ -1634,7 +1643,8 @@ __global__ void k_scale_inv_dev(double* x, const double* coef, long n) {
 
 __global__ void k_axpby(double* y, const double* x, double a, double b, long n) {
   const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (i < n) y[i] = a * x[i] + b * y[i];
+  // b == 0: y is not read (it may be fresh memory holding any bit pattern, NaN included)
+  if (i < n) y[i] = (b == 0.0) ? a * x[i] : a * x[i] + b * y[i];
 }
 
 __global__ void k_copy(double* y, const double* x, long n) {
@@ -2051,6 +2061,108 @@ void copy(double* y, const double* x, long n, hipStream_t s) {
 void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
                hipStream_t s) {
   k_gemv_cols<<<nblocks(n, 256), 256, 0, s>>>(x, V, ldv, k, y, alpha, n);
+  EXA_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Vectors split into per-instance segments (batched solves).  Every block covers
+// entries [bbeg, bend) of one segment, so a segment's sums and updates depend on
+// its own entries and length only.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int SEG_BS = 256;
+
+// partial[v*nblk + b] = sum over block b of V_v[i] * w[i]   (blockIdx.y = v)
+__global__ __launch_bounds__(SEG_BS) void k_seg_dot_partial(SegMap m, const double* V, long ldv,
+                                                            const double* w, double* partial) {
+  __shared__ double sh[SEG_BS];
+  const int b = blockIdx.x;
+  const double* x = V + ldv * blockIdx.y;
+  double acc = 0.0;
+  for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) acc = fma(x[i], w[i], acc);
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = SEG_BS / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[static_cast<long>(blockIdx.y) * m.nblk + b] = sh[0];
+}
+
+// out[v*nseg + s] = sum of segment s's partials (fixed order), sqrt if asked  (blockIdx.y = v)
+__global__ __launch_bounds__(SEG_BS) void k_seg_reduce(SegMap m, const double* partial, double* out,
+                                                       int sqrt_out) {
+  __shared__ double sh[SEG_BS];
+  const int sg = blockIdx.x;
+  const double* pv = partial + static_cast<long>(blockIdx.y) * m.nblk;
+  double acc = 0.0;
+  for (int i = m.sbptr[sg] + threadIdx.x; i < m.sbptr[sg + 1]; i += SEG_BS) acc += pv[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = SEG_BS / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[static_cast<long>(blockIdx.y) * m.nseg + sg] = sqrt_out ? sqrt(sh[0]) : sh[0];
+}
+
+// y += scale * coef[seg] * x
+__global__ __launch_bounds__(SEG_BS) void k_seg_axpy(SegMap m, double* y, const double* x, const double* coef,
+                                                     double scale) {
+  const int b = blockIdx.x;
+  const double c = scale * coef[m.bseg[b]];
+  if (c == 0.0) return;
+  for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) y[i] = fma(c, x[i], y[i]);
+}
+
+// mode 0: y = coef[seg] * x;  mode 1: y = x / coef[seg] (0 where coef[seg] == 0)
+__global__ __launch_bounds__(SEG_BS) void k_seg_scale(SegMap m, double* y, const double* x, const double* coef,
+                                                      int mode) {
+  const int b = blockIdx.x;
+  const double cv = coef[m.bseg[b]];
+  const double c = mode == 0 ? cv : (cv != 0.0 ? 1.0 / cv : 0.0);
+  for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) y[i] = c * x[i];
+}
+
+// x += alpha * sum_j V_j * y[j*nseg + seg]
+__global__ __launch_bounds__(SEG_BS) void k_seg_gemv_cols(SegMap m, double* x, const double* V, long ldv, int k,
+                                                          const double* y, double alpha) {
+  const int b = blockIdx.x;
+  const int sg = m.bseg[b];
+  for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) {
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc = fma(V[i + ldv * j], y[static_cast<long>(j) * m.nseg + sg], acc);
+    x[i] = fma(alpha, acc, x[i]);
+  }
+}
+
+}  // namespace
+
+void seg_multidot(const SegMap& m, const double* V, long ldv, int k, const double* w, double* partial,
+                  double* out, int sqrt_out, hipStream_t s) {
+  if (m.nblk == 0 || k == 0) return;
+  k_seg_dot_partial<<<dim3(m.nblk, k), SEG_BS, 0, s>>>(m, V, ldv, w, partial);
+  k_seg_reduce<<<dim3(m.nseg, k), SEG_BS, 0, s>>>(m, partial, out, sqrt_out);
+  EXA_HIP(hipGetLastError());
+}
+
+void seg_axpy(const SegMap& m, double* y, const double* x, const double* coef, double scale, hipStream_t s) {
+  if (m.nblk == 0) return;
+  k_seg_axpy<<<m.nblk, SEG_BS, 0, s>>>(m, y, x, coef, scale);
+  EXA_HIP(hipGetLastError());
+}
+
+void seg_scale(const SegMap& m, double* y, const double* x, const double* coef, int mode, hipStream_t s) {
+  if (m.nblk == 0) return;
+  k_seg_scale<<<m.nblk, SEG_BS, 0, s>>>(m, y, x, coef, mode);
+  EXA_HIP(hipGetLastError());
+}
+
+void seg_gemv_cols(const SegMap& m, double* x, const double* V, long ldv, int k, const double* y, double alpha,
+                   hipStream_t s) {
+  if (m.nblk == 0 || k == 0) return;
+  k_seg_gemv_cols<<<m.nblk, SEG_BS, 0, s>>>(m, x, V, ldv, k, y, alpha);
   EXA_HIP(hipGetLastError());
 }
 

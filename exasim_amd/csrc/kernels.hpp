@@ -64,9 +64,10 @@ void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemco
 struct AsmArgs {
   int model;
   double time;
-  const double* param;  // device
-  const double* tau;    // device
-  double tau0;
+  const double* param;  // device, [nparam, ninst]
+  const double* tau;    // device, [ntau, ninst]
+  const int* einst;     // device, instance of each element (null: one instance)
+  int nparam, ntau;
   int e0, eb;
   const double *shapegt, *shapegw, *shapfgt, *shapfgw;
   const int *perm, *elemcon, *bf;
@@ -150,11 +151,36 @@ void dot(const double* x, const double* y, long n, double* partial, double* out,
          hipStream_t s);
 void axpy_dev(double* y, const double* x, const double* coef, double scale, long n, hipStream_t s);
 void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s);
+// y = a x + b y; with b == 0 y is only written
 void axpby(double* y, const double* x, double a, double b, long n, hipStream_t s);
 void copy(double* y, const double* x, long n, hipStream_t s);
 void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
                hipStream_t s);
 void multidot(const double* V, long ldv, int k, const double* w, long n, double* partial, double* out,
               hipStream_t s);
+
+// ---- vectors split into per-instance segments (batched solves) ----
+// Segment s of a vector is entries [off[s], off[s+1]).  Blocks of at most SEG_CHUNK entries
+// never straddle two segments (block b covers [bbeg[b], bend[b]) of segment bseg[b]; the blocks of
+// segment s are sbptr[s] .. sbptr[s+1]), so a segment's reductions see only its own entries, in an
+// order fixed by its length: an instance gets the same sums alone and in a batch.
+constexpr long SEG_CHUNK = 2048;
+struct SegMap {
+  int nseg = 0, nblk = 0;
+  const int* bseg = nullptr;
+  const long* bbeg = nullptr;
+  const long* bend = nullptr;
+  const int* sbptr = nullptr;
+};
+// out[v*nseg + s] = <V_v, w> over segment s (v < k; sqrt of it with sqrt_out); partial: k*nblk
+void seg_multidot(const SegMap& m, const double* V, long ldv, int k, const double* w, double* partial,
+                  double* out, int sqrt_out, hipStream_t s);
+// y += scale * coef[s] * x on each segment s
+void seg_axpy(const SegMap& m, double* y, const double* x, const double* coef, double scale, hipStream_t s);
+// mode 0: y = coef[s] * x;  mode 1: y = x / coef[s], and 0 where coef[s] == 0
+void seg_scale(const SegMap& m, double* y, const double* x, const double* coef, int mode, hipStream_t s);
+// x += alpha * sum_{j<k} V_j * y[j*nseg + s]
+void seg_gemv_cols(const SegMap& m, double* x, const double* V, long ldv, int k, const double* y, double alpha,
+                   hipStream_t s);
 
 }  // namespace exa

@@ -18,8 +18,7 @@
 
 namespace exa {
 
-namespace {
-int model_id(const std::string& name) {
+int resolve_model(const std::string& name) {
   if (name == "poisson") return MODEL_POISSON;
   if (name == "poisson_poly") return MODEL_POISSON_POLY;
   if (name == "convdiff") return MODEL_CONVDIFF;
@@ -30,7 +29,7 @@ int model_id(const std::string& name) {
 }
 
 // cpuApplyGivensRotation (backend/Common/cpuimpl.h:167-197)
-void givens(double* Hc, double* s, double* cs, double* sn, int i) {
+void gmres_givens(double* Hc, double* s, double* cs, double* sn, int i) {
   for (int k = 0; k < i; ++k) {
     double t = cs[k] * Hc[k] + sn[k] * Hc[k + 1];
     Hc[k + 1] = -sn[k] * Hc[k] + cs[k] * Hc[k + 1];
@@ -56,17 +55,16 @@ void givens(double* Hc, double* s, double* cs, double* sn, int i) {
 }
 
 // cpuBackSolve (cpuimpl.h:199-208): Hm column-major with leading dimension ld.
-void back_solve(double* y, const double* Hm, const double* s, int i, int ld) {
+void gmres_back_solve(double* y, const double* Hm, const double* s, int i, int ld) {
   for (int j = i; j >= 0; --j) y[j] = s[j];
   for (int j = i; j >= 0; --j) {
     y[j] /= Hm[j + ld * j];
     for (int k = j - 1; k >= 0; --k) y[k] -= Hm[k + ld * j] * y[j];
   }
 }
-}  // namespace
 
 Solver::Solver(const std::string& din, const std::string& dout, const std::string& mname, int dev)
-    : datain(din), dataout(dout), model_name(mname), model(model_id(mname)), device(dev) {
+    : datain(din), dataout(dout), model_name(mname), model(resolve_model(mname)), device(dev) {
   setup();
 }
 
@@ -78,6 +76,7 @@ Solver::~Solver() {
     }
   if (s) (void)hipStreamSynchronize(s);
   if (hpin) (void)hipHostFree(hpin);
+  if (hseg) (void)hipHostFree(hseg);
   if (s) (void)hipStreamDestroy(s);
 }
 
@@ -85,6 +84,13 @@ void Solver::setup() {
   app = read_app(datain + "/app.bin");
   master = read_master(datain + "/master.bin");
   mesh = read_mesh(datain + "/mesh.bin");
+  set_dims();
+  SolData sol = read_sol(datain + "/sol.bin", d.npe, d.nc, d.ncu, d.ne);
+  conn = build_conn(master, mesh);
+  init(sol);
+}
+
+void Solver::set_dims() {
   if (master.nd != mesh.nd) throw Error(2, "master.bin and mesh.bin disagree on nd");
   d.nd = master.nd;
   d.nc = static_cast<int>(app.ndims[5]);
@@ -97,7 +103,9 @@ void Solver::setup() {
   d.nfe = mesh.nfe;
   if (d.nc != d.ncu * (d.nd + 1))
     throw Error(5, "only the HDG (u, q) formulation with nc = ncu*(nd+1) is supported");
-  SolData sol = read_sol(datain + "/sol.bin", d.npe, d.nc, d.ncu, d.ne);
+}
+
+void Solver::init(SolData& sol) {
   if (static_cast<long>(sol.xdg.size()) != static_cast<long>(d.npe) * d.nd * d.ne)
     throw Error(2, "xdg must have ncx = nd components");
   d.nco = sol.nco;
@@ -112,7 +120,6 @@ void Solver::setup() {
       throw Error(1, "model '" + model_name + "' reads " + std::to_string(nv) +
                          " auxiliary components (vdg) but sol.bin has " + std::to_string(d.nco));
   }
-  conn = build_conn(master, mesh);
   d.nf = conn.nf;
   d.n = d.npe * d.ncu;
   d.ndf = d.npf * d.nfe;
@@ -163,9 +170,24 @@ void Solver::setup() {
   shapegw.upload(master.shapegw, st, s);
   shapfgt.upload(master.shapfgt, st, s);
   shapfgw.upload(master.shapfgw, st, s);
-  if (app.physicsparam.empty()) app.physicsparam.push_back(0.0);
-  param.upload(app.physicsparam, st, s);
-  tau.upload(app.tau, st, s);
+  if (ninst == 1) {
+    if (app.physicsparam.empty()) app.physicsparam.push_back(0.0);
+    if (app.tau.empty()) app.tau.push_back(0.0);
+    nparam = static_cast<int>(app.physicsparam.size());
+    ntau = static_cast<int>(app.tau.size());
+    inst_param = app.physicsparam;
+    inst_tau = app.tau;
+    eoff = {0, d.ne};
+    foff = {0, d.nf};
+  } else {
+    std::vector<int> ei(d.ne);
+    for (int i = 0; i < ninst; ++i)
+      for (long e = eoff[i]; e < eoff[i + 1]; ++e) ei[e] = i;
+    einst.upload(ei, st, s);
+  }
+  param.upload(inst_param, st, s);
+  tau.upload(inst_tau, st, s);
+  seg_setup();
   perm.upload(mesh.perm, st, s);
   elemcon.upload(conn.elemcon, st, s);
   bf.upload(mesh.bf, st, s);
@@ -453,6 +475,7 @@ void Solver::alloc_workspace() {
   hcol.alloc(restart + 2);
   ydev.alloc(restart + 2);
   partial.alloc(static_cast<size_t>(dot_partials_count(d.N)) * (restart + 2));
+  if (batch) seg_alloc();
 }
 
 void Solver::set_option(const std::string& name, double v) {
@@ -464,6 +487,7 @@ void Solver::set_option(const std::string& name, double v) {
     hcol.alloc(restart + 2);
     ydev.alloc(restart + 2);
     partial.alloc(static_cast<size_t>(dot_partials_count(d.N)) * (restart + 2));
+    if (batch) seg_alloc();
   } else if (name == "gmres_maxit") gmres_maxit = static_cast<int>(v);
   else if (name == "newton_tol") newton_tol = v;
   else if (name == "newton_maxit") newton_maxit = static_cast<int>(v);
@@ -507,7 +531,9 @@ AsmArgs Solver::asm_args(int e0, int eb) {
   a.time = tnow;
   a.param = param.p;
   a.tau = tau.p;
-  a.tau0 = app.tau[0];
+  a.einst = einst.p;
+  a.nparam = nparam;
+  a.ntau = ntau;
   a.e0 = e0;
   a.eb = eb;
   a.shapegt = shapegt.p;
@@ -764,6 +790,14 @@ void Solver::assemble() {
 }
 
 void Solver::residual(double* nb, double* nru) {
+  residual_fields();
+  double a1 = norm(bres.p, d.N);
+  double a2 = norm(RuR.p, static_cast<long>(d.ne) * d.n);
+  if (nb) *nb = a1;
+  if (nru) *nru = a2;
+}
+
+void Solver::residual_fields() {
   const long n = d.n;
   for (int e0 = 0; e0 < d.ne; e0 += neb) {
     const int eb = std::min(neb, d.ne - e0);
@@ -774,10 +808,6 @@ void Solver::residual(double* nb, double* nru) {
     for (int l = 0; l < d.nfe; ++l) face_res(d, a, l, false, RuR.p + e0 * n, n, Rh.p, s);
   }
   face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, bres.p, s);
-  double a1 = norm(bres.p, d.N);
-  double a2 = norm(RuR.p, static_cast<long>(d.ne) * n);
-  if (nb) *nb = a1;
-  if (nru) *nru = a2;
 }
 
 void Solver::matvec(const double* v, double* w) {
@@ -852,12 +882,12 @@ int Solver::gmres(const double* bv, double* xv, double tol_rel) {
       EXA_HIP(hipMemcpyAsync(hpin, hcol.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
       EXA_HIP(hipStreamSynchronize(s));
       for (int k = 0; k < i + 2; ++k) Hc[k] = hpin[k];
-      givens(Hc.data(), sv.data(), cs.data(), sn.data(), i);
+      gmres_givens(Hc.data(), sv.data(), cs.data(), sn.data(), i);
       for (int k = 0; k < n1; ++k) Hm[k + static_cast<size_t>(n1) * i] = (k < i + 2) ? Hc[k] : 0.0;
       double rel = std::fabs(sv[i + 1]) / nrmb;
       if (verbose > 1) std::printf("GMRES %d rel %g tol %g\n", j, rel, tol);
       if (rel < tol) {
-        back_solve(y.data(), Hm.data(), sv.data(), i, n1);
+        gmres_back_solve(y.data(), Hm.data(), sv.data(), i, n1);
         std::memcpy(hpin, y.data(), sizeof(double) * (i + 1));
         EXA_HIP(hipMemcpyAsync(ydev.p, hpin, sizeof(double) * (i + 1), hipMemcpyHostToDevice, s));
         gemv_cols(xv, V.p, N, i + 1, ydev.p, 1.0, N, s);
@@ -865,7 +895,7 @@ int Solver::gmres(const double* bv, double* xv, double tol_rel) {
         return j + 1;
       }
     }
-    back_solve(y.data(), Hm.data(), sv.data(), i - 1, n1);
+    gmres_back_solve(y.data(), Hm.data(), sv.data(), i - 1, n1);
     std::memcpy(hpin, y.data(), sizeof(double) * i);
     EXA_HIP(hipMemcpyAsync(ydev.p, hpin, sizeof(double) * i, hipMemcpyHostToDevice, s));
     gemv_cols(xv, V.p, N, i, ydev.p, 1.0, N, s);
@@ -889,8 +919,14 @@ int Solver::gmres(const double* bv, double* xv, double tol_rel) {
 // a Newton solve per stage, UpdateSolutionDIRK (previoussolutions.cpp:23-58,
 // updatesource.cpp:32-72, updatesolution.cpp:48-70).
 void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
+  if (batch) {
+    inst_newton.assign(ninst, 0);
+    inst_gmres.assign(ninst, 0);
+    inst_norm.assign(ninst, 0.0);
+  }
   if (!tdep) {
-    newton(newton_iters, gmres_total, final_norm);
+    if (batch) newton_seg(newton_iters, gmres_total, final_norm);
+    else newton(newton_iters, gmres_total, final_norm);
     return;
   }
   const int S = tstages;
@@ -917,7 +953,8 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
         axpby(sdg.p, udgprev.p, pc / dt, 1.0, static_cast<long>(sdg.n), s);
       }
       int ni = 0, gi = 0;
-      newton(&ni, &gi, &nrm);
+      if (batch) newton_seg(&ni, &gi, &nrm);
+      else newton(&ni, &gi, &nrm);
       nit += ni;
       gtot += gi;
       u_axpby(d, udg.p, dirk_c[j], 1.0, utmp.p, s);       // utmp += c_j u
@@ -965,7 +1002,9 @@ void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
     residual(&nb, &nru);
     nrmr = nb + nru;
     if (nrmr > nrm0 && nrmr > 1e6) throw Error(4, "Residual norm increases more than 1e6");
-    while ((nrmr > nrm0 && alpha > 0.1) || std::isnan(nrmr)) {
+    // the reference's rule (solution.cpp:213-227), with a floor on alpha for a residual that
+    // stays NaN, which would otherwise halve alpha forever
+    while ((nrmr > nrm0 && alpha > 0.1) || (std::isnan(nrmr) && alpha > 1e-6)) {
       if (verbose) std::printf("Newton Iteration: %d, Alpha: %g, Original Norm: %g,  Updated Norm: %g\n",
                                it + 1, alpha, nrm0, nrmr);
       alpha /= 2.0;

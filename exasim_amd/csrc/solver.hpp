@@ -14,6 +14,12 @@ namespace exa {
 
 enum TimerClass { T_MATVEC = 0, T_PRECOND = 1, T_ORTHO = 2, T_ASSEMBLY = 3, T_LU = 4, T_OTHER = 5, T_NCLASS = 6 };
 
+// model id of a built-in or generated model name (Error for an unknown name)
+int resolve_model(const std::string& name);
+// cpuApplyGivensRotation / cpuBackSolve (backend/Common/cpuimpl.h:167-208) on one Hessenberg column
+void gmres_givens(double* Hc, double* s, double* cs, double* sn, int i);
+void gmres_back_solve(double* y, const double* Hm, const double* s, int i, int ld);
+
 struct Solver {
   // problem description
   std::string datain, dataout, model_name;
@@ -75,13 +81,50 @@ struct Solver {
   Stager st;               // pinned staging of bulk host <-> device copies
   bool assembled = false;
 
+  // instances: one problem, or a batch of independent problems with one element shape,
+  // merged into one disconnected mesh (instance i owns elements eoff[i] .. eoff[i+1] and faces
+  // foff[i] .. foff[i+1], each in its own standalone order) so that every element- and
+  // face-local kernel serves the whole batch in one launch
+  int ninst = 1;
+  bool batch = false;
+  int nparam = 0, ntau = 0;
+  std::vector<double> inst_param, inst_tau;  // [nparam, ninst], [ntau, ninst]
+  std::vector<long> eoff, foff;
+  std::vector<std::string> datains;
+  DevBuf<int> einst;
+  struct SegDev {
+    DevBuf<int> bseg, sbptr;
+    DevBuf<long> bbeg, bend;
+    SegMap map;
+  };
+  SegDev segT, segU;  // segments of trace vectors [N] and of element vectors [n, ne]
+  DevBuf<double> segpart, segcoef, segy, segmask, dus;
+  double* hseg = nullptr;  // pinned host scratch for per-instance scalars
+  long hseg_len = 0;
+  std::vector<int> inst_newton, inst_gmres;
+  std::vector<double> inst_norm;
+
   // timing
   std::array<std::vector<std::pair<hipEvent_t, hipEvent_t>>, T_NCLASS> ev;
   std::array<double, T_NCLASS> ms{};
   std::array<long, T_NCLASS> cnt{};
 
   Solver(const std::string& datain, const std::string& dataout, const std::string& model, int device);
+  // a batch: the datain directories of ninst instances with one element shape and model
+  Solver(const std::vector<std::string>& datains, const std::string& model, int device);
   ~Solver();
+  void set_dims();
+  void init(SolData& sol);
+  // batch (segmented) Newton-GMRES
+  void seg_setup();
+  void build_seg(SegDev& sd, const std::vector<long>& off);
+  void seg_alloc();
+  double* host_region(int which);
+  void seg_norms(const SegDev& sd, const double* v, double* out_dev);
+  std::vector<double> residual_seg();
+  int gmres_seg(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters);
+  void newton_seg(int* newton_iters, int* gmres_iters, double* final_norm);
+  long get_inst_array(int inst, const std::string& name, double* dst, long len);
 
   void set_option(const std::string& name, double value);
   // stages
@@ -89,6 +132,7 @@ struct Solver {
   void get_q();
   void assemble();
   void residual(double* nb, double* nru);
+  void residual_fields();  // RuR, Rh and the assembled trace residual bres
   void matvec(const double* v, double* w);
   void apply_precond(double* x);
   int gmres(const double* bvec, double* xvec, double tol_rel = -1.0);

@@ -42,6 +42,11 @@ SIGNATURES = [
     ("exa_timings", ctypes.c_int, [_P, _D, _L, ctypes.c_int]),
     ("exa_reset_timings", ctypes.c_int, [_P]),
     ("exa_sync", ctypes.c_int, [_P]),
+    ("exa_batch_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p,
+                                        ctypes.c_int, ctypes.POINTER(_P)]),
+    ("exa_batch_size", ctypes.c_int, [_P]),
+    ("exa_batch_stats", ctypes.c_int, [_P, _I, _I, _D, ctypes.c_int]),
+    ("exa_batch_get", ctypes.c_long, [_P, ctypes.c_int, ctypes.c_char_p, _D, ctypes.c_long]),
 ]
 TIMER_CLASSES = ["matvec", "precond", "ortho", "assembly", "lu", "other"]
 
@@ -86,6 +91,47 @@ class HDGSolver:
             raise ExaError(rc, lib.exa_last_error(None).decode())
         self._h = h
         self.sizes = self.state()
+
+    @classmethod
+    def batch(cls, datains, model, device=-1):
+        """A batch of independent instances (``exa_batch_create``): one handle, shared launches,
+        per-instance Newton/GMRES.  ``solve()`` solves them all; ``instance_stats()`` and
+        ``get_instance()`` give each instance's counts and arrays."""
+        lib = load()
+        self = cls.__new__(cls)
+        self._lib = lib
+        arr = (ctypes.c_char_p * len(datains))(*[d.encode() for d in datains])
+        h = _P()
+        rc = lib.exa_batch_create(len(datains), arr, model.encode(), device, ctypes.byref(h))
+        if rc != 0:
+            raise ExaError(rc, lib.exa_last_error(None).decode())
+        self._h = h
+        self.sizes = self.state()
+        return self
+
+    @property
+    def ninst(self):
+        return self._lib.exa_batch_size(self._h)
+
+    def instance_stats(self):
+        """(newton_iters[ninst], gmres_iters[ninst], final_norm[ninst]) of the last solve."""
+        n = self.ninst
+        a = np.zeros(n, dtype=np.int32)
+        b = np.zeros(n, dtype=np.int32)
+        c = np.zeros(n)
+        self._check(self._lib.exa_batch_stats(self._h, a.ctypes.data_as(_I), b.ctypes.data_as(_I), _dp(c), n))
+        return a, b, c
+
+    def get_instance(self, inst, name):
+        """Instance inst's "uh" or "udg" (flattened, column-major as the merged arrays)."""
+        n = self._lib.exa_batch_get(self._h, inst, name.encode(), None, 0)
+        if n < 0:
+            self._check(-n)
+        out = np.empty(n, dtype=np.float64)
+        r = self._lib.exa_batch_get(self._h, inst, name.encode(), _dp(out), n)
+        if r < 0:
+            self._check(-r)
+        return out
 
     SIZE_KEYS = ["nd", "ncu", "nc", "npe", "npf", "nfe", "ne", "nf", "n", "m", "N", "nge", "ngf",
                  "nco", "restart", "preconditioner", "precision", "refinements"]

@@ -26,7 +26,8 @@
  *   exa_get_array  the debugmode=1 stage dumps (uequation.cpp:182-202, 542-556, 683-774;
  *                  qequation.cpp:152-165; ptcsolver.cpp:363-371)
  *   model names    the generated model plugin (backend/Model/libpdemodel.hpp:1-41) is
- *                  selected by name; see exasim_amd/csrc/models.hpp.
+ *                  selected by name: built-in structs (exasim_amd/csrc/models.hpp) or
+ *                  pdemodel.txt files compiled by exasim_amd/modelgen.py.
  */
 #ifndef EXASIM_HDG_H
 #define EXASIM_HDG_H
@@ -50,7 +51,7 @@ enum {
 };
 
 /* Build a solver from datain_dir/{app,master,mesh,sol}.bin for the named model
- * ("poisson", "poisson_poly", "convdiff").  dataout_prefix is the prefix of the
+ * ("poisson", "poisson_poly", "convdiff", or a generated model such as "ns2d").  dataout_prefix is the prefix of the
  * output files (e.g. "dataout/out"); it may be NULL.  device < 0 selects the
  * current HIP device. */
 int exa_create(const char* datain_dir, const char* dataout_prefix, const char* model,
@@ -123,6 +124,29 @@ int exa_reset_timings(exa_handle* h);
 
 /* Synchronise the handle's stream. */
 int exa_sync(exa_handle* h);
+
+/* ---- batches of independent problems (no counterpart in the reference, which runs one
+ * problem per process; each instance follows exa_solve's algorithm: NewtonSolver,
+ * backend/Solution/solution.cpp:125-242, and GMRES, backend/Solver/gmres.cpp:361-597).
+ *
+ * exa_batch_create reads ninst datain directories with one element shape (same nd, p,
+ * quadrature, ncu and face numbering), one time scheme and one preconditioner (0 or 1),
+ * and any mesh sizes, physicsparam and tau.  The instances are merged into one disconnected
+ * mesh: assembly, condensation, the matvec and the preconditioner run as single launches over
+ * all of them, while every instance keeps its own Krylov recurrence (segmented dot products
+ * over its own unknowns), GMRES and Newton tolerances, damping and convergence.  An instance's
+ * result is the same alone (a batch of one) and in a batch.  The handle takes every other
+ * entry point; exa_solve solves all instances, exa_get_array returns the merged arrays. */
+int exa_batch_create(int ninst, const char* const* datain_dirs, const char* model, int device,
+                     exa_handle** out);
+/* Number of instances of the handle (1 for a handle from exa_create, 0 for NULL). */
+int exa_batch_size(const exa_handle* h);
+/* Per instance, from the last exa_solve: Newton iterations, GMRES iterations and the final
+ * residual norm ||b|| + ||Ru||; each array (may be NULL) holds at least n >= ninst entries. */
+int exa_batch_stats(exa_handle* h, int* newton_iters, int* gmres_iters, double* final_norm, int n);
+/* Instance inst's "uh" [ncu,npf,nf_i] or "udg" [npe,nc,ne_i]: copies it to dst (len >= size)
+ * and returns its size; with dst NULL returns the size only.  Negative EXA_ERR_* on failure. */
+long exa_batch_get(exa_handle* h, int inst, const char* name, double* dst, long len);
 
 #ifdef __cplusplus
 }

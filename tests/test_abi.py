@@ -61,3 +61,21 @@ def test_driver_command_line_errors(tmp_path):
     r = subprocess.run([DRIVER, str(app)], capture_output=True, text=True)
     assert r.returncode == 2 and "Error" in r.stderr          # EXA_ERR_IO: no datain/app.bin
     assert os.path.isdir(tmp_path / "d" / "dataout")
+
+
+def test_batch_create_checks_instances(tmp_path):
+    """exa_batch_create rejects instances of different element shapes (EXA_ERR_ARG) and the
+    block-ILU(0) preconditioner (EXA_ERR_UNSUPPORTED) before any device work."""
+    if not os.path.exists(hdg.LIB_PATH):
+        pytest.skip("libexahip.so not built")
+    from exasim_amd import cases
+    a, b, c = str(tmp_path / "a"), str(tmp_path / "b"), str(tmp_path / "c")
+    cases.convdiff2d_param(a, n=2, porder=2)
+    cases.convdiff2d_param(b, n=3, porder=3)
+    cases.convdiff2d_param(c, n=2, porder=2, precond="bilu0")
+    with pytest.raises(hdg.ExaError) as e:
+        hdg.HDGSolver.batch([a, b], "cd2d", device=-2)
+    assert e.value.code == 1 and "one master element" in str(e.value)
+    with pytest.raises(hdg.ExaError) as e:
+        hdg.HDGSolver.batch([c, c], "cd2d", device=-2)
+    assert e.value.code == 5

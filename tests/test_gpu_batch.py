@@ -1,0 +1,106 @@
+"""Batches of independent instances (exa_batch_create) on the GPU.
+
+* Each instance's result in a batch is bitwise the result of the same instance alone (a batch
+  of one): the shared launches are element- and face-local and every Krylov/Newton scalar is
+  the instance's own (segmented reductions over its own unknowns).
+* Against the single-instance path (exa_create) and the oracle: 1e-10 relative (GMRES 1e-12).
+* Mixed precision batch against the FP64 batch: 1e-8 relative.
+* A nonlinear batch (two Navier-Stokes instances with different Mach and Reynolds numbers,
+  DIRK(2,2) steps) equals each instance alone, bitwise.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import hdg as ora
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+INSTANCES = [dict(n=4, kappa=0.3, angle=0.4, k=1), dict(n=6, kappa=0.01, angle=2.0, k=3),
+             dict(n=8, kappa=0.002, angle=4.1, k=2), dict(n=5, kappa=1.0, angle=5.5, k=4)]
+
+
+@pytest.fixture(scope="module")
+def cd_batch(tmp_path_factory):
+    from exasim_amd import cases
+    root = tmp_path_factory.mktemp("cdbatch")
+    paths = []
+    for i, kw in enumerate(INSTANCES):
+        p = str(root / ("i%d" % i))
+        cases.convdiff2d_param(p, porder=3, restart=40, gmres_tol=1e-12, newton_tol=1e-10, **kw)
+        paths.append(p)
+    return paths
+
+
+def _solve_batch(paths, model="cd2d", precision=0):
+    from exasim_amd.hdg import HDGSolver
+    with HDGSolver.batch(paths, model) as S:
+        assert S.ninst == len(paths)
+        S.set_option("precision", precision)
+        S.solve()
+        stats = S.instance_stats()
+        out = [(S.get_instance(i, "uh"), S.get_instance(i, "udg")) for i in range(len(paths))]
+    return out, stats
+
+
+def test_batch_equals_alone_bitwise(cd_batch):
+    say("cd2d batch of 4 (p=3, n = 4..8) vs each instance alone")
+    together, stats = _solve_batch(cd_batch)
+    assert (stats[0] == 1).all() and (stats[1] > 0).all() and (stats[2] < 1e-10).all()
+    for i, p in enumerate(cd_batch):
+        alone, st1 = _solve_batch([p])
+        assert np.array_equal(together[i][0], alone[0][0]), i
+        assert np.array_equal(together[i][1], alone[0][1]), i
+        assert st1[1][0] == stats[1][i]
+
+
+def test_batch_matches_single_path_and_oracle(cd_batch):
+    say("cd2d batch vs exa_create path and vs oracle")
+    from exasim_amd.hdg import HDGSolver
+    together, _ = _solve_batch(cd_batch)
+    for i, p in enumerate(cd_batch):
+        with HDGSolver(p, "cd2d") as S:
+            S.solve()
+            uh = S.get("uh")
+        assert rel(together[i][0], uh) < 1e-10, i
+    P = ora.HDGProblem(cd_batch[0], "cd2d")
+    ora.newton_solve(P)
+    assert rel(together[0][0], P.uh.reshape(-1)) < 1e-10
+
+
+def test_batch_mixed_precision(cd_batch):
+    say("cd2d batch: mixed precision vs FP64")
+    f64, _ = _solve_batch(cd_batch)
+    mix, stats = _solve_batch(cd_batch, precision=1)
+    for i in range(len(cd_batch)):
+        assert rel(mix[i][0], f64[i][0]) < 1e-8, i
+        assert rel(mix[i][1], f64[i][1]) < 1e-8, i
+
+
+def test_ns_batch_equals_alone(tmp_path):
+    say("ns2d batch of 2 (M = 0.5 / Re 1000, M = 0.3 / Re 400), DIRK(2,2) x 2 vs alone")
+    from exasim_amd import cases
+    paths = []
+    for i, (mach, re) in enumerate([(0.5, 1000.0), (0.3, 400.0)]):
+        p = str(tmp_path / ("ns%d" % i))
+        cases.navierstokes2d(p, nx=8 + 4 * i, ny=2, porder=3, length=(2.5, 1.0), mach=mach, re=re, nsteps=2,
+                             torder=2, nstage=2, precond="bjacobi", restart=100, gmres_tol=1e-12,
+                             newton_tol=1e-10, cartgridpart=False)
+        paths.append(p)
+    together, stats = _solve_batch(paths, "ns2d")
+    assert (stats[0] >= 4).all()
+    for i, p in enumerate(paths):
+        alone, _ = _solve_batch([p], "ns2d")
+        assert np.array_equal(together[i][0], alone[0][0]), i
+        assert np.array_equal(together[i][1], alone[0][1]), i

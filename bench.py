@@ -17,6 +17,12 @@ refinement to 1e-10 of the first residual).  ``value`` is the mixed form's
 solves/s; the FP64 form, the ratio and the mixed solution's relative error
 against the FP64 one are reported beside it.
 
+The same line carries ``batch``: 256 parametrized 2D convection-diffusion instances of uneven
+size and order (see ``run_batch``) solved as four batches (one per polynomial order, shared
+launches, per-instance Newton/GMRES), in the mixed and FP64 forms, next to the plain FP64 form
+solving them one at a time; ``--workload batch`` prints that workload's line alone, and
+``--no-batch`` leaves it out.
+
 Prints one JSON line (rank 0, N = 1).  ``roofline`` is the dominant kernel of
 the GMRES loop, the element matvec of the leg reported in ``value``:
 algorithmic bytes per launch (s_H*m^2*ne for H, with s_H = 4 or 8, 8*m*ne for
@@ -81,6 +87,136 @@ def cpu_sample(porder, restart, n_sample, precond, se):
     return dict(ne=int(P.ne), N=int(N), t_asm=float(t_asm), t_pre=float(t_pre), t_iter=float(t_it))
 
 
+def batch_cpu_sample(budget_s=20.0, nmax=32):
+    """Child process: the oracle (1 thread) on the batch workload's instances with n <= nmax, in
+    instance order, until about budget_s seconds of solves; returns per-instance seconds and
+    trace unknowns."""
+    from exasim_amd import cases
+    from oracle import hdg as ora
+    out = []
+    with tempfile.TemporaryDirectory() as d:
+        inst = cases.convdiff2d_batch(d, count=256, select=lambda i, n, p: n <= nmax and i < 96, **BATCH_KW)
+        spent = 0.0
+        for info in inst:
+            P = ora.HDGProblem(info["path"], "cd2d")
+            t0 = time.perf_counter()
+            ora.newton_solve(P)
+            t = time.perf_counter() - t0
+            spent += t
+            out.append(dict(t=t, N=int(P.nf * P.npf * P.ncu), p=info["porder"], n=info["n"]))
+            if spent > budget_s:
+                break
+    return out
+
+
+# solver settings of the batch workload's instances (GMRESortho = 1: classical Gram-Schmidt)
+BATCH_KW = dict(precond="eas", restart=100, gmres_tol=1e-10, newton_tol=1e-8, ortho=1)
+
+
+def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True):
+    """The batch workload: ``count`` parametrized 2D convection-diffusion instances (n x n quads,
+    n in {8..64}, p in {2..5}, kappa log-uniform [1e-3, 1], velocity angle uniform, source
+    sin(k pi x) cos(k pi y)), one batch handle per polynomial order, the four batches solved
+    concurrently (one host thread and one HIP stream each).  A step solves every instance from
+    its input state.  Reported: batched mixed and FP64 solves/s, the plain FP64 form (one
+    instance at a time through exa_create) on the same instances, per-instance time to solution,
+    and the mixed results' largest relative error against the plain FP64 ones."""
+    import collections
+    import threading
+    from exasim_amd import cases
+    from exasim_amd.hdg import HDGSolver
+
+    def log(msg):
+        if verbose:
+            print("# batch: " + msg, file=sys.stderr, flush=True)
+
+    work = tempfile.mkdtemp(prefix="exa_batch_")
+    t0 = time.perf_counter()
+    inst = cases.convdiff2d_batch(work, count=count, **BATCH_KW)
+    log("%d instances written in %.1f s" % (count, time.perf_counter() - t0))
+    groups = collections.defaultdict(list)
+    for i, d in enumerate(inst):
+        groups[d["porder"]].append(i)
+    # plain FP64 form: each instance alone, inputs resident (set-up outside the timed solve)
+    plain_t, uh_ref = [], {}
+    for i, d in enumerate(inst):
+        with HDGSolver(d["path"], "cd2d") as S:
+            S.sync()
+            t1 = time.perf_counter()
+            S.solve()
+            plain_t.append(time.perf_counter() - t1)
+            uh_ref[i] = S.get("uh")
+    plain_t = np.array(plain_t)
+    log("plain FP64: %.1f solves/s (sum of %d solve times %.2f s)" % (count / plain_t.sum(), count, plain_t.sum()))
+    legs = {}
+    for prec in (0, 1):
+        hs = {}
+        for p, ids in sorted(groups.items()):
+            hs[p] = HDGSolver.batch([inst[i]["path"] for i in ids], "cd2d")
+            hs[p].set_option("precision", prec)
+        lat = collections.defaultdict(list)
+        walls = []
+        for k in range(warmup + steps):
+            for h in hs.values():
+                h.reset()
+                h.sync()
+            done = {}
+            t1 = time.perf_counter()
+
+            def go(p, h):
+                h.solve()
+                done[p] = time.perf_counter() - t1
+            th = [threading.Thread(target=go, args=(p, h)) for p, h in hs.items()]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            wall = time.perf_counter() - t1
+            if k >= warmup:
+                walls.append(wall)
+                for p, ids in groups.items():
+                    lat[p].append(done[p])
+            log("%s step %d: %.3f s (%s)" % (PREC_NAME[prec], k, wall,
+                                              ", ".join("p=%d %.3f s" % (p, done[p]) for p in sorted(done))))
+        err, gm = 0.0, []
+        for p, ids in groups.items():
+            a, g, nrm = hs[p].instance_stats()
+            gm += list(g)
+            for j, i in enumerate(ids):
+                u = hs[p].get_instance(j, "uh")
+                err = max(err, float(np.abs(u - uh_ref[i]).max() / np.abs(uh_ref[i]).max()))
+        # per-instance time to solution: its batch's completion time
+        tts = np.concatenate([np.repeat(np.mean(lat[p]), len(ids)) for p, ids in groups.items()])
+        tm = {p: h.timings() for p, h in hs.items()}
+        for h in hs.values():
+            h.close()
+        legs[prec] = dict(solves_s=count / float(np.mean(walls)), wall_s=float(np.mean(walls)), err=err,
+                          gmres_mean=float(np.mean(gm)), gmres_max=int(max(gm)),
+                          p50_ms=float(np.percentile(tts, 50) * 1e3), p99_ms=float(np.percentile(tts, 99) * 1e3),
+                          group_s={int(p): float(np.mean(v)) for p, v in lat.items()})
+    cpu = None
+    if cpu_baseline:
+        env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--batch-cpu-child"], env=env,
+                           capture_output=True, text=True, timeout=900)
+        if r.returncode == 0:
+            c = json.loads(r.stdout.strip().splitlines()[-1])
+            rate = sum(x["t"] for x in c) / sum(x["N"] for x in c)      # oracle seconds per trace unknown
+            total_N = sum(int(np.sum(uh_ref[i].size)) for i in range(count))
+            cpu = {"value": count / (rate * total_N), "unit": "solves/s", "cores": 1, "kind": "port",
+                   "sample": "numpy oracle (FP64, 1 thread), full Newton-GMRES solves of the first %d "
+                             "instances with n <= 32 (%.1f s), scaled by trace unknowns to the %d instances"
+                             % (len(c), sum(x["t"] for x in c), count)}
+        else:
+            log("cpu baseline failed: " + r.stderr[-1500:])
+    plain = dict(solves_s=count / float(plain_t.sum()), p50_ms=float(np.percentile(plain_t, 50) * 1e3),
+                 p99_ms=float(np.percentile(plain_t, 99) * 1e3))
+    return dict(instances=count, groups={int(p): len(v) for p, v in groups.items()}, plain_fp64=plain,
+                batched_fp64=legs[0], batched_mixed=legs[1],
+                mixed_batched_over_plain_fp64=legs[1]["solves_s"] / plain["solves_s"],
+                fp64_batched_over_plain_fp64=legs[0]["solves_s"] / plain["solves_s"], cpu_baseline=cpu)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,10 +240,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4)
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--batch-cpu-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--workload", default="c2", choices=["c2", "batch"],
+                    help="c2: the steady 3D convection-diffusion solve (default, with the batch "
+                         "workload's results as a sub-object); batch: the batch workload alone")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch workload in the c2 line")
     args = ap.parse_args()
 
     if args.cpu_child:
         print(json.dumps(cpu_sample(args.porder, args.restart, args.cpu_sample_n, args.precond, args.se)))
+        return 0
+    if args.batch_cpu_child:
+        print(json.dumps(batch_cpu_sample()))
         return 0
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (the domain "
@@ -115,6 +259,20 @@ def main():
         return 2
 
     from exasim_amd.hdg import HDGSolver
+
+    if args.workload == "batch":
+        b = run_batch(args.steps, args.warmup, cpu_baseline=not args.no_cpu_baseline)
+        m = b["batched_mixed"]
+        print(json.dumps({
+            "metric": METRIC, "value": m["solves_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": m["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "batch of 256 parametrized 2D convection-diffusion solves (n in {8..64}, "
+                                   "p in {2..5}), one batch per p, mixed FP32/FP64 with refinement, "
+                                   "elemental Schwarz, GMRES(100) CGS tol 1e-10"},
+            "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "batch": b, "roofline": None,
+            "cpu_baseline": b["cpu_baseline"]}))
+        return 0
 
     if args.copy_calibration:
         import torch
@@ -263,6 +421,11 @@ def main():
     if mixed and 0 in legs:
         out["mixed_over_fp64"] = solves_s / legs[0]["solves_s"]
         out["mixed_rel_err_vs_fp64"] = err
+    if not args.no_batch:
+        try:
+            out["batch"] = run_batch(2, 1, cpu_baseline=not args.no_cpu_baseline)
+        except Exception as e:          # the c2 line is still printed
+            out["batch"] = {"error": repr(e)}
     print(json.dumps(out))
     return 0
 

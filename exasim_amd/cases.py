@@ -263,11 +263,12 @@ def convdiff2d_param(path, n=16, porder=3, pgauss=None, kappa=0.1, angle=0.0, k=
 
 
 def convdiff2d_batch(root, count=256, sizes=(8, 16, 24, 32, 48, 64), orders=(2, 3, 4, 5), seed=20261202,
-                     **kw):
+                     select=None, **kw):
     """``count`` parametrized instances of ``convdiff2d_param`` under root/inst####: n uniform
     over ``sizes``, p uniform over ``orders``, kappa log-uniform in [1e-3, 1], velocity angle
     uniform in [0, 2 pi) (magnitude 1), k uniform in {1, 2, 3, 4}.  Returns the instance
-    descriptions (with their datain paths), in instance order."""
+    descriptions (with their datain paths), in instance order.  ``select(index, n, porder)``, if
+    given, picks the instances to write (the others are drawn but neither written nor returned)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
@@ -276,6 +277,8 @@ def convdiff2d_batch(root, count=256, sizes=(8, 16, 24, 32, 48, 64), orders=(2, 
         kappa = float(10.0 ** rng.uniform(-3.0, 0.0))
         angle = float(rng.uniform(0.0, 2.0 * np.pi))
         k = int(rng.integers(1, 5))
+        if select is not None and not select(i, n, porder):
+            continue
         path = os.path.join(root, "inst%04d" % i)
         info = convdiff2d_param(path, n=n, porder=porder, kappa=kappa, angle=angle, k=k, **kw)
         info["path"] = path

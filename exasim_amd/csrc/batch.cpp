@@ -263,9 +263,16 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
   }
   if (!nact) return 0;
   const double tol = std::min(0.1, tol_rel < 0.0 ? gmres_tol : tol_rel);
+  // the matvec and the Schwarz preconditioner skip the elements of inactive instances (their
+  // vectors are zero); segmask holds the activity from here on
+  struct ActiveScope {
+    const double*& p;
+    ActiveScope(const double*& q, const double* v) : p(q) { p = v; }
+    ~ActiveScope() { p = nullptr; }
+  } scope(elem_active, einst.p ? segmask.p : nullptr);
+  push_mask(act);  // before the first product that reads it
   copy(r.p, bv, N, s);
   apply_precond(r.p);
-  push_mask(act);
   seg_scale(m, r.p, r.p, segmask.p, 0, s);
   seg_norms(segT, r.p, segcoef.p);
   pull(segcoef.p, ns);

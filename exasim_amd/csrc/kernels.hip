@@ -1194,11 +1194,16 @@ __global__ void k_bj_apply(Dims d, const T* BE, const double* x, double* y) {
 // in a fixed order.
 template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const int* elemcon,
-                                                     const double* v, double* we) {
+                                                     const double* v, double* we, const int* einst,
+                                                     const double* emask) {
   typedef T vecT __attribute__((ext_vector_type(VW)));
   constexpr int MAXRB = (1024 + 64 * VW - 1) / (64 * VW);  // m <= 1024
   const int m = d.m, ncu = d.ncu, ndf = d.ndf;
   const long e = blockIdx.x;
+  if (emask && einst && emask[einst[e]] == 0.0) {  // block-uniform: the element's instance is inactive
+    for (int r = threadIdx.x; r < m; r += 256) we[e * m + r] = 0.0;
+    return;
+  }
   extern __shared__ double sm[];
   double* ve = sm;          // m
   double* red = sm + m;     // 4*m
@@ -1238,6 +1243,31 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const i
   __syncthreads();
   for (int r = tid; r < m; r += 256)
     we[e * m + r] = ((red[r] + red[m + r]) + red[2 * m + r]) + red[3 * m + r];
+}
+
+// The same product for small trace blocks (m <= 96, e.g. 2D elements): one thread per row
+// of one element, the rows of consecutive elements in consecutive lanes, FP64 accumulation
+// over the columns in order.  emask/einst (optional): elements of an instance whose mask is 0
+// only write zeros (instances that have left a batched GMRES).
+template <typename T>
+__global__ __launch_bounds__(256) void k_matvec_small(Dims d, const T* H, const int* elemcon, const double* v,
+                                                      double* we, const int* einst, const double* emask) {
+  const int m = d.m, ncu = d.ncu, ndf = d.ndf;
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= static_cast<long>(m) * d.ne) return;
+  const long e = q / m;
+  const int r = static_cast<int>(q % m);
+  if (emask && einst && emask[einst[e]] == 0.0) {
+    we[q] = 0.0;
+    return;
+  }
+  const T* He = H + e * static_cast<long>(m) * m + r;
+  const int* ec = elemcon + ndf * e;
+  double acc = 0.0;
+  for (int c = 0; c < m; ++c)
+    acc = fma(static_cast<double>(He[static_cast<long>(m) * c]), v[(c % ncu) + ncu * static_cast<long>(ec[c / ncu])],
+              acc);
+  we[q] = acc;
 }
 
 // hdgGetDUDG (matvec.cpp:219-238): du_e = Ru_e + DinvF_e * x_e.
@@ -1955,13 +1985,20 @@ void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_
 }
 
 template <typename T>
-void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we, hipStream_t s) {
+void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we, hipStream_t s,
+                 const int* einst, const double* emask) {
   if (d.m > 1024) throw Error(5, "element trace block larger than 1024");
+  if (d.m <= MATVEC_SMALL_M) {
+    k_matvec_small<T><<<nblocks(static_cast<long>(d.m) * d.ne, 256), 256, 0, s>>>(d, H, elemcon, v, we, einst,
+                                                                                   emask);
+    EXA_HIP(hipGetLastError());
+    return;
+  }
   size_t shm = static_cast<size_t>(5) * d.m * sizeof(double);
   constexpr int VMAX = 16 / sizeof(T);  // 16-byte loads: 2 doubles or 4 floats per lane
-  if (d.m % VMAX == 0) k_matvec_elem<T, VMAX><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
-  else if (d.m % 2 == 0) k_matvec_elem<T, 2><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
-  else k_matvec_elem<T, 1><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we);
+  if (d.m % VMAX == 0) k_matvec_elem<T, VMAX><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+  else if (d.m % 2 == 0) k_matvec_elem<T, 2><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+  else k_matvec_elem<T, 1><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2168,8 +2205,10 @@ void seg_gemv_cols(const SegMap& m, double* x, const double* V, long ldv, int k,
 
 template void bj_apply<double>(const Dims&, const double*, const double*, double*, hipStream_t);
 template void bj_apply<float>(const Dims&, const float*, const double*, double*, hipStream_t);
-template void matvec_elem<double>(const Dims&, const double*, const int*, const double*, double*, hipStream_t);
-template void matvec_elem<float>(const Dims&, const float*, const int*, const double*, double*, hipStream_t);
+template void matvec_elem<double>(const Dims&, const double*, const int*, const double*, double*, hipStream_t,
+                                  const int*, const double*);
+template void matvec_elem<float>(const Dims&, const float*, const int*, const double*, double*, hipStream_t,
+                                 const int*, const double*);
 template void dudg<double>(const Dims&, const double*, const double*, const int*, const double*, double*,
                            hipStream_t);
 template void dudg<float>(const Dims&, const double*, const float*, const int*, const double*, double*,

@@ -115,8 +115,8 @@ void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e
                   double* aug, hipStream_t s);
 void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s);
 template <typename T>
-void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we,
-                 hipStream_t s);
+void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we, hipStream_t s,
+                 const int* einst = nullptr, const double* emask = nullptr);
 template <typename T>
 void dudg(const Dims& d, const double* RuS, const T* DinvF, const int* elemcon, const double* x,
           double* du, hipStream_t s);
@@ -145,6 +145,9 @@ template <typename T>
 void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
                 double* out, hipStream_t s);
 
+// trace blocks up to this size use the one-thread-per-row matvec
+constexpr int MATVEC_SMALL_M = 96;
+
 // ---- vector kernels (results stay on the device) ----
 int dot_partials_count(long n);
 void dot(const double* x, const double* y, long n, double* partial, double* out, int sqrt_out,
@@ -164,7 +167,7 @@ void multidot(const double* V, long ldv, int k, const double* w, long n, double*
 // never straddle two segments (block b covers [bbeg[b], bend[b]) of segment bseg[b]; the blocks of
 // segment s are sbptr[s] .. sbptr[s+1]), so a segment's reductions see only its own entries, in an
 // order fixed by its length: an instance gets the same sums alone and in a batch.
-constexpr long SEG_CHUNK = 2048;
+constexpr long SEG_CHUNK = 512;
 struct SegMap {
   int nseg = 0, nblk = 0;
   const int* bseg = nullptr;

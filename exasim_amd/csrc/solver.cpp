@@ -170,7 +170,7 @@ void Solver::init(SolData& sol) {
   shapegw.upload(master.shapegw, st, s);
   shapfgt.upload(master.shapfgt, st, s);
   shapfgw.upload(master.shapfgw, st, s);
-  if (ninst == 1) {
+  if (!batch) {
     if (app.physicsparam.empty()) app.physicsparam.push_back(0.0);
     if (app.tau.empty()) app.tau.push_back(0.0);
     nparam = static_cast<int>(app.physicsparam.size());
@@ -179,7 +179,7 @@ void Solver::init(SolData& sol) {
     inst_tau = app.tau;
     eoff = {0, d.ne};
     foff = {0, d.nf};
-  } else {
+  } else {  // every batch handle, a batch of one included, has the element -> instance map
     std::vector<int> ei(d.ne);
     for (int i = 0; i < ninst; ++i)
       for (long e = eoff[i]; e < eoff[i + 1]; ++e) ei[e] = i;
@@ -813,8 +813,8 @@ void Solver::residual_fields() {
 void Solver::matvec(const double* v, double* w) {
   hipEvent_t t0{};
   tic(T_MATVEC, &t0);
-  if (precision) matvec_elem(d, H32.p, elemcon.p, v, we.p, s);
-  else matvec_elem(d, H.p, elemcon.p, v, we.p, s);
+  if (precision) matvec_elem(d, H32.p, elemcon.p, v, we.p, s, einst.p, elem_active);
+  else matvec_elem(d, H.p, elemcon.p, v, we.p, s, einst.p, elem_active);
   toc(T_MATVEC, t0);
   tic(T_OTHER, &t0);
   face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, w, s);
@@ -829,8 +829,8 @@ void Solver::apply_precond(double* xv) {
     if (precision) bilu_apply(d, ilud, BEilu32.p, tmp.p, Yilu.p, Xilu.p, xv, s);
     else bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
   } else if (precond == 1) {  // x = sum_e scatter(BE_e^{-1} gather_e(x)), as hdgMatVec with BE
-    if (precision) matvec_elem(d, BEeas32.p, elemcon.p, tmp.p, we.p, s);
-    else matvec_elem(d, BEeas.p, elemcon.p, tmp.p, we.p, s);
+    if (precision) matvec_elem(d, BEeas32.p, elemcon.p, tmp.p, we.p, s, einst.p, elem_active);
+    else matvec_elem(d, BEeas.p, elemcon.p, tmp.p, we.p, s, einst.p, elem_active);
     face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, xv, s);
   } else {
     if (precision) bj_apply(d, BE32.p, tmp.p, xv, s);

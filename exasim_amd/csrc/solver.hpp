@@ -101,6 +101,9 @@ struct Solver {
   DevBuf<double> segpart, segcoef, segy, segmask, dus;
   double* hseg = nullptr;  // pinned host scratch for per-instance scalars
   long hseg_len = 0;
+  // during a batched GMRES: per-instance activity (1/0) read by the matvec and the Schwarz
+  // preconditioner, whose elements of inactive instances only write zeros; null otherwise
+  const double* elem_active = nullptr;
   std::vector<int> inst_newton, inst_gmres;
   std::vector<double> inst_norm;
 

@@ -234,6 +234,8 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
   const long N = d.N;
   const int ns = ninst;
   const SegMap& m = segT.map;
+  SegMap ma = m;  // the same blocks, skipping instances that have left GMRES (segmask)
+  ma.act = segmask.p;
   const int n1 = restart + 1;
   iters.assign(ns, 0);
   if (static_cast<long>(hcol.n) < static_cast<long>(n1 + 1) * ns) seg_alloc();
@@ -298,14 +300,14 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
       tic(T_ORTHO, &t0);
       if (ortho == 0) {  // MGS, per instance
         for (int k = 0; k <= i; ++k) {
-          seg_multidot(m, V.p + N * k, N, 1, vm, segpart.p, hcol.p + static_cast<long>(k) * ns, 0, s);
-          seg_axpy(m, vm, V.p + N * k, hcol.p + static_cast<long>(k) * ns, -1.0, s);
+          seg_multidot(ma, V.p + N * k, N, 1, vm, segpart.p, hcol.p + static_cast<long>(k) * ns, 0, s);
+          seg_axpy(ma, vm, V.p + N * k, hcol.p + static_cast<long>(k) * ns, -1.0, s);
         }
       } else {  // CGS, per instance
-        seg_multidot(m, V.p, N, i + 1, vm, segpart.p, hcol.p, 0, s);
-        seg_gemv_cols(m, vm, V.p, N, i + 1, hcol.p, -1.0, s);
+        seg_multidot(ma, V.p, N, i + 1, vm, segpart.p, hcol.p, 0, s);
+        seg_gemv_cols(ma, vm, V.p, N, i + 1, hcol.p, -1.0, s);
       }
-      seg_multidot(m, vm, N, 1, vm, segpart.p, hcol.p + static_cast<long>(i + 1) * ns, 1, s);
+      seg_multidot(ma, vm, N, 1, vm, segpart.p, hcol.p + static_cast<long>(i + 1) * ns, 1, s);
       seg_scale(m, vm, vm, hcol.p + static_cast<long>(i + 1) * ns, 1, s);
       toc(T_ORTHO, t0);
       pull(hcol.p, static_cast<long>(i + 2) * ns);
@@ -330,7 +332,7 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
       if (any) {  // x += V y for the instances that converged at this step; their basis stops here
         std::memcpy(hy, ycol.data(), sizeof(double) * ycol.size());
         EXA_HIP(hipMemcpyAsync(segy.p, hy, sizeof(double) * ycol.size(), hipMemcpyHostToDevice, s));
-        seg_gemv_cols(m, xv, V.p, N, i + 1, segy.p, 1.0, s);
+        seg_gemv_cols(ma, xv, V.p, N, i + 1, segy.p, 1.0, s);  // before the mask drops them
         push_mask(act);
         seg_scale(m, vm, vm, segmask.p, 0, s);
       }
@@ -346,7 +348,7 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
     }
     std::memcpy(hy, ycol.data(), sizeof(double) * ycol.size());
     EXA_HIP(hipMemcpyAsync(segy.p, hy, sizeof(double) * ycol.size(), hipMemcpyHostToDevice, s));
-    seg_gemv_cols(m, xv, V.p, N, i, segy.p, 1.0, s);
+    seg_gemv_cols(ma, xv, V.p, N, i, segy.p, 1.0, s);
     matvec(xv, r.p);
     axpby(r.p, bv, 1.0, -1.0, N, s);
     apply_precond(r.p);

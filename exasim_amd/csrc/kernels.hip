@@ -2115,6 +2115,10 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_dot_partial(SegMap m, const doub
                                                             const double* w, double* partial) {
   __shared__ double sh[SEG_BS];
   const int b = blockIdx.x;
+  if (m.act && m.act[m.bseg[b]] == 0.0) {  // block-uniform
+    if (threadIdx.x == 0) partial[static_cast<long>(blockIdx.y) * m.nblk + b] = 0.0;
+    return;
+  }
   const double* x = V + ldv * blockIdx.y;
   double acc = 0.0;
   for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) acc = fma(x[i], w[i], acc);
@@ -2148,6 +2152,7 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_reduce(SegMap m, const double* p
 __global__ __launch_bounds__(SEG_BS) void k_seg_axpy(SegMap m, double* y, const double* x, const double* coef,
                                                      double scale) {
   const int b = blockIdx.x;
+  if (m.act && m.act[m.bseg[b]] == 0.0) return;
   const double c = scale * coef[m.bseg[b]];
   if (c == 0.0) return;
   for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) y[i] = fma(c, x[i], y[i]);
@@ -2167,6 +2172,7 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_gemv_cols(SegMap m, double* x, c
                                                           const double* y, double alpha) {
   const int b = blockIdx.x;
   const int sg = m.bseg[b];
+  if (m.act && m.act[sg] == 0.0) return;
   for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) {
     double acc = 0.0;
     for (int j = 0; j < k; ++j) acc = fma(V[i + ldv * j], y[static_cast<long>(j) * m.nseg + sg], acc);

@@ -174,6 +174,9 @@ struct SegMap {
   const long* bbeg = nullptr;
   const long* bend = nullptr;
   const int* sbptr = nullptr;
+  // optional per-segment activity (device, 1/0): the dot, axpy and gemv kernels skip the blocks
+  // of inactive segments (dots over them read as 0, their entries are left unchanged)
+  const double* act = nullptr;
 };
 // out[v*nseg + s] = <V_v, w> over segment s (v < k; sqrt of it with sqrt_out); partial: k*nblk
 void seg_multidot(const SegMap& m, const double* V, long ldv, int k, const double* w, double* partial,

@@ -217,6 +217,40 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True):
                 fp64_batched_over_plain_fp64=legs[0]["solves_s"] / plain["solves_s"], cpu_baseline=cpu)
 
 
+def run_serve(seconds, rate, check_alone=4):
+    """The served workload: a Poisson stream of Navier-Stokes (70 %) and convection-diffusion
+    solve requests (n x n quads, n in {32, 64, 128}, p in {2..5}; see cases.serve_stream)
+    replayed in real time into the continuous-batching server (HBM budget 200 GB, mixed
+    precision).  Reports completed solves/s over the stream, p50/p99 time to solution (arrival
+    to result), and whether a sample of requests gets bitwise the same result alone."""
+    from exasim_amd import cases, serve
+    from exasim_amd.hdg import HDGSolver
+    work = tempfile.mkdtemp(prefix="exa_serve_")
+    t0 = time.perf_counter()
+    reqs = cases.serve_stream(work, seconds=seconds, rate=rate)
+    print("# serve: %d requests written in %.1f s" % (len(reqs), time.perf_counter() - t0), file=sys.stderr,
+          flush=True)
+    srv = serve.Server(hbm_budget_bytes=200e9, max_batch=64, workers=1, precision=1,
+                       keep_results=check_alone > 0, window=0.02,
+                       log=lambda msg: print("# serve: " + msg, file=sys.stderr, flush=True))
+    summ = srv.run_stream(reqs)
+    print("# serve: %s" % json.dumps({k: summ[k] for k in ("completed", "solves_per_s", "p50_ms", "p99_ms")}),
+          file=sys.stderr, flush=True)
+    same = []
+    for rid, path, model, t in reqs[:check_alone]:
+        if rid not in srv.results:
+            continue
+        with HDGSolver.batch([path], model) as S:
+            S.set_option("precision", srv.records[rid].get("precision", 1))
+            S.solve()
+            same.append(bool(np.array_equal(S.get_instance(0, "uh"), srv.results[rid])))
+    kinds = {}
+    for rid, path, model, t in reqs:
+        kinds[model] = kinds.get(model, 0) + 1
+    summ.update(stream_seconds=seconds, rate_per_s=rate, mix=kinds, alone_equal_sample=same)
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,7 +275,9 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=4)
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--batch-cpu-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", default="c2", choices=["c2", "batch"],
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
+    ap.add_argument("--workload", default="c2", choices=["c2", "batch", "serve"],
                     help="c2: the steady 3D convection-diffusion solve (default, with the batch "
                          "workload's results as a sub-object); batch: the batch workload alone")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch workload in the c2 line")
@@ -260,6 +296,18 @@ def main():
 
     from exasim_amd.hdg import HDGSolver
 
+    if args.workload == "serve":
+        sv = run_serve(args.serve_seconds, args.serve_rate)
+        print(json.dumps({
+            "metric": METRIC, "value": sv["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1,
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "served stream: Poisson arrivals %.0f/s for %.0f s, 70%% Navier-Stokes "
+                                   "(2 DIRK(2,2) steps of 5e-3) / 30%% convection-diffusion, n in {32,64,128}, p in "
+                                   "{2..5}, HBM budget 200 GB, mixed precision" % (args.serve_rate,
+                                                                                   args.serve_seconds)},
+            "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"], "serve": sv, "roofline": None, "cpu_baseline": None}))
+        return 0
     if args.workload == "batch":
         b = run_batch(args.steps, args.warmup, cpu_baseline=not args.no_cpu_baseline)
         m = b["batched_mixed"]

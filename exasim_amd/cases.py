@@ -284,3 +284,70 @@ def convdiff2d_batch(root, count=256, sizes=(8, 16, 24, 32, 48, 64), orders=(2, 
         info["path"] = path
         out.append(info)
     return out
+
+
+def serve_stream(root, seconds=60.0, rate=50.0, frac_ns=0.7, sizes=(32, 64, 128), orders=(2, 3, 4, 5),
+                 seed=20261202, ns_steps=2, ns_dt=5e-3):
+    """A timed stream of parametrized solve requests: Poisson arrivals at ``rate`` per second
+    for ``seconds``; each request is, with probability ``frac_ns``, a compressible
+    Navier-Stokes solve (model ``ns2d``: n x n quads of the unit square, wall at y = 0 and far
+    field elsewhere, free-stream start, Mach uniform in [0.2, 0.6], Reynolds number
+    log-uniform in [100, 5000], ``ns_steps`` DIRK(2,2) steps of ``ns_dt``, stabilization tau =
+    1 + 1/M, the free-stream wave speed |u| + c), and otherwise a
+    steady convection-diffusion solve (model ``cd2d``, kappa log-uniform in [1e-3, 1], velocity
+    angle uniform, source sin(k pi x) cos(k pi y), k uniform in 1..4); n uniform over ``sizes``,
+    p uniform over ``orders``.  Mesh, master element and node coordinates are written once per
+    (model, n, p) under root/tmpl_* and linked into each request's directory.  Returns
+    [(request id, datain path, model, arrival seconds)]."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    tmpl = {}
+    gam, pr = 1.4, 0.72
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        rid = len(out)
+        ns = rng.random() < frac_ns
+        n = int(rng.choice(sizes))
+        porder = int(rng.choice(orders))
+        path = os.path.join(root, "req%05d" % rid)
+        os.makedirs(path, exist_ok=True)
+        key = ("ns2d" if ns else "cd2d", n, porder)
+        if key not in tmpl:
+            tdir = os.path.join(root, "tmpl_%s_%d_%d" % key)
+            if ns:
+                navierstokes2d(tdir, nx=n, ny=n, porder=porder, length=(1.0, 1.0), stretch=1.0, nbumps=0,
+                               nsteps=ns_steps, torder=2, nstage=2, dt=ns_dt, precond="eas", restart=50,
+                               gmres_tol=1e-8, newton_tol=1e-8, ortho=1, cartgridpart=False)
+            else:
+                convdiff2d_param(tdir, n=n, porder=porder, precond="eas", restart=100, gmres_tol=1e-10,
+                                 newton_tol=1e-8, ortho=1)
+            m = dw.make_master(2, porder, 2 * porder)
+            p, tt = dw.structured_grid([n, n])
+            tmpl[key] = (tdir, m, dw.compute_dgnodes(m, p, tt))
+        tdir, m, xdg = tmpl[key]
+        for f in ("master.bin", "mesh.bin"):
+            os.symlink(os.path.join(tdir, f), os.path.join(path, f))
+        if ns:
+            mach = float(rng.uniform(0.2, 0.6))
+            re = float(10.0 ** rng.uniform(2.0, np.log10(5000.0)))
+            uinf = ns_freestream(gam, mach)
+            dw.write_app(os.path.join(path, "app.bin"), nd=2, ncu=4,
+                         flag={0: 1, 5: 1, 6: PRECOND["eas"]},
+                         problem={2: 0, 3: 2, 4: 2, 13: 8, 14: 600, 15: 50},
+                         physicsparam=[gam, re, pr, mach] + list(uinf) + [1.0], tau=[1.0 + 1.0 / mach],
+                         dt=[ns_dt] * ns_steps, solversparam=[1e-8, 1e-8, 1e-3, 0.0])
+            udg = np.broadcast_to(uinf, xdg.shape[:2] + (4,))
+            dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=4, nc=12, udg=np.ascontiguousarray(udg))
+        else:
+            kappa = float(10.0 ** rng.uniform(-3.0, 0.0))
+            angle = float(rng.uniform(0.0, 2.0 * np.pi))
+            k = int(rng.integers(1, 5))
+            dw.write_app(os.path.join(path, "app.bin"), nd=2, ncu=1, flag=_app_flags("eas", 1),
+                         problem={13: 1, 14: 2000, 15: 100},
+                         physicsparam=[kappa, np.cos(angle), np.sin(angle), float(k)], tau=[1.0],
+                         solversparam=[1e-8, 1e-10, 1e-3, 0.0])
+            os.symlink(os.path.join(tdir, "sol.bin"), os.path.join(path, "sol.bin"))
+        out.append((rid, path, key[0], t))
+    return out

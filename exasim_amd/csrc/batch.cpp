@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 
 namespace exa {
 
@@ -260,7 +261,7 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
   int nact = 0;
   for (int i = 0; i < ns; ++i) {
     const double nb = h[i];
-    act[i] = !(nb == 0.0 || (nb < newton_tol && tol_rel < 0.0));
+    act[i] = std::isfinite(nb) && !(nb == 0.0 || (nb < newton_tol && tol_rel < 0.0));
     nact += act[i];
   }
   if (!nact) return 0;
@@ -279,6 +280,18 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
   seg_norms(segT, r.p, segcoef.p);
   pull(segcoef.p, ns);
   std::vector<double> nrmb(h, h + ns), nrmr = nrmb;
+  bool drop = false;  // a non-finite preconditioned residual: the instance leaves GMRES (x = 0)
+  for (int i = 0; i < ns; ++i)
+    if (act[i] && !std::isfinite(nrmb[i])) {
+      act[i] = 0;
+      --nact;
+      drop = true;
+    }
+  if (drop) {
+    push_mask(act);
+    seg_scale(m, r.p, r.p, segmask.p, 0, s);
+    if (!nact) return 0;
+  }
   std::vector<double> Hm(static_cast<size_t>(ns) * n1 * n1, 0.0), sv(static_cast<size_t>(ns) * (n1 + 1)),
       cs(static_cast<size_t>(ns) * (n1 + 1)), sn(static_cast<size_t>(ns) * (n1 + 1)), y(n1 + 1), Hc(n1 + 1);
   auto Hs = [&](int sg) { return Hm.data() + static_cast<size_t>(sg) * n1 * n1; };
@@ -321,6 +334,12 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
                    sn.data() + static_cast<size_t>(sg) * (n1 + 1), i);
         for (int k = 0; k < n1; ++k) Hs(sg)[k + static_cast<size_t>(n1) * i] = (k < i + 2) ? Hc[k] : 0.0;
         iters[sg] = j + 1;
+        if (!std::isfinite(svs[i + 1])) {  // breakdown to NaN/Inf: the instance leaves GMRES
+          act[sg] = 0;
+          --nact;
+          any = true;
+          continue;
+        }
         if (std::fabs(svs[i + 1]) / nrmb[sg] < tol) {
           gmres_back_solve(y.data(), Hs(sg), svs, i, n1);
           for (int k = 0; k <= i; ++k) ycol[static_cast<size_t>(k) * ns + sg] = y[k];
@@ -410,6 +429,18 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
   u.alloc(N);
   copy(u.p, uh.p, N, s);
   std::vector<char> act(ns, 1);
+  if (static_cast<int>(inst_failed.size()) != ns) inst_failed.assign(ns, 0);
+  for (int i = 0; i < ns; ++i) act[i] = !inst_failed[i];
+  // an instance whose Newton iteration breaks down (NaN residual, growth beyond 1e6) stops:
+  // it is marked failed (final norm NaN), and its input state is put back so that the shared
+  // launches keep seeing finite data; the other instances go on
+  auto fail = [&](int i, const char* why) {
+    inst_failed[i] = 1;
+    act[i] = 0;
+    inst_norm[i] = std::numeric_limits<double>::quiet_NaN();
+    restore_instance(i, u.p);
+    if (verbose) std::printf("batch: instance %d stopped: %s\n", i, why);
+  };
   std::vector<int> its;
   int it = 0, gtot = 0;
   for (it = 0; it < newton_maxit; ++it) {
@@ -434,8 +465,7 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
     nr = residual_seg();
     std::vector<double> alpha(ns, 1.0);
     for (int i = 0; i < ns; ++i)
-      if (act[i] && nr[i] > nrm0[i] && nr[i] > 1e6)
-        throw Error(4, "instance " + std::to_string(i) + ": residual norm increases more than 1e6");
+      if (act[i] && nr[i] > nrm0[i] && nr[i] > 1e6) fail(i, "residual norm increases more than 1e6");
     // damping, per instance: alpha halved while its residual grows (or is NaN)
     for (;;) {
       std::vector<double> coef(ns, 0.0);
@@ -458,7 +488,7 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
         if (coef[i] != 0.0) nr[i] = nn[i];
     }
     for (int i = 0; i < ns; ++i)
-      if (act[i] && std::isnan(nr[i])) throw Error(4, "instance " + std::to_string(i) + ": NaN residual");
+      if (act[i] && std::isnan(nr[i])) fail(i, "NaN residual");
     // mixed form: refinement with the FP64 residual, per instance
     last_refinements = 0;
     if (precision) {
@@ -488,7 +518,7 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
           if (ref[i]) nr[i] = nn[i];
         ++last_refinements;
         for (int i = 0; i < ns; ++i)
-          if (ref[i] && std::isnan(nr[i])) throw Error(4, "instance " + std::to_string(i) + ": NaN in refinement");
+          if (ref[i] && std::isnan(nr[i])) fail(i, "NaN in refinement");
       }
     }
     for (int i = 0; i < ns; ++i)
@@ -507,7 +537,22 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
   EXA_HIP(hipStreamSynchronize(s));
   if (newton_iters) *newton_iters = it;
   if (gmres_total) *gmres_total = gtot;
-  if (final_norm) *final_norm = *std::max_element(nr.begin(), nr.end());
+  if (final_norm) {
+    double mx = 0.0;
+    for (int i = 0; i < ns; ++i)
+      if (!inst_failed[i]) mx = std::max(mx, nr[i]);
+    *final_norm = mx;
+  }
+}
+
+// Put instance i back to its input state: udg and uh (and the Newton iterate u, if given)
+void Solver::restore_instance(int i, double* u) {
+  const long ue = static_cast<long>(d.npe) * d.nc;
+  EXA_HIP(hipMemcpyAsync(udg.p + eoff[i] * ue, udg0.p + eoff[i] * ue,
+                         sizeof(double) * (eoff[i + 1] - eoff[i]) * ue, hipMemcpyDeviceToDevice, s));
+  const long f0 = foff[i] * d.ncf, fn = (foff[i + 1] - foff[i]) * d.ncf;
+  EXA_HIP(hipMemcpyAsync(uh.p + f0, uh0.p + f0, sizeof(double) * fn, hipMemcpyDeviceToDevice, s));
+  if (u) EXA_HIP(hipMemcpyAsync(u + f0, uh0.p + f0, sizeof(double) * fn, hipMemcpyDeviceToDevice, s));
 }
 
 long Solver::get_inst_array(int inst, const std::string& name, double* dst, long len) {

@@ -923,6 +923,7 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
     inst_newton.assign(ninst, 0);
     inst_gmres.assign(ninst, 0);
     inst_norm.assign(ninst, 0.0);
+    inst_failed.assign(ninst, 0);
   }
   if (!tdep) {
     if (batch) newton_seg(newton_iters, gmres_total, final_norm);

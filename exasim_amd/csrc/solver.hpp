@@ -105,6 +105,7 @@ struct Solver {
   // preconditioner, whose elements of inactive instances only write zeros; null otherwise
   const double* elem_active = nullptr;
   std::vector<int> inst_newton, inst_gmres;
+  std::vector<char> inst_failed;  // instances stopped by a Newton breakdown in this solve
   std::vector<double> inst_norm;
 
   // timing
@@ -128,6 +129,7 @@ struct Solver {
   int gmres_seg(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters);
   void newton_seg(int* newton_iters, int* gmres_iters, double* final_norm);
   long get_inst_array(int inst, const std::string& name, double* dst, long len);
+  void restore_instance(int inst, double* newton_iterate);
 
   void set_option(const std::string& name, double value);
   // stages

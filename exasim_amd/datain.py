@@ -464,3 +464,24 @@ def read_output(path):
     a = np.fromfile(path, dtype=np.float64)
     hdr = a[:3].astype(int)
     return hdr, a[3:]
+
+
+def read_sizes(path):
+    """Sizes from the headers of a datain directory's app.bin, master.bin and mesh.bin (the
+    layouts written above): nd, ncu, nc, element type, p, pgauss, npe, npf, nge, ngf, ne,
+    nfe and the GMRES restart length."""
+    a = np.fromfile(os.path.join(path, "app.bin"), dtype=np.float64)
+    na = int(a[0])
+    nsa = a[1:1 + na].astype(int)
+    o = 1 + na
+    ndims = a[o:o + nsa[0]]
+    problem = a[o + nsa[0] + nsa[1]:o + nsa[0] + nsa[1] + nsa[2]]
+    m = np.fromfile(os.path.join(path, "master.bin"), dtype=np.float64, count=64)
+    nm = int(m[0])
+    md = m[1 + nm:1 + nm + 11].astype(int)
+    h = np.fromfile(os.path.join(path, "mesh.bin"), dtype=np.float64, count=80)
+    nh = int(h[0])
+    hd = h[1 + nh:1 + nh + 20].astype(int)
+    return dict(nd=int(ndims[1]), nc=int(ndims[5]), ncu=int(ndims[6]), elemtype=int(md[1]), porder=int(md[3]),
+                pgauss=int(md[4]), npe=int(md[5]), npf=int(md[6]), nge=int(md[7]), ngf=int(md[8]),
+                ne=int(hd[1]), nfe=int(hd[4]), restart=int(problem[15]))

@@ -104,3 +104,17 @@ def test_ns_batch_equals_alone(tmp_path):
         alone, _ = _solve_batch([p], "ns2d")
         assert np.array_equal(together[i][0], alone[0][0]), i
         assert np.array_equal(together[i][1], alone[0][1]), i
+
+
+def test_batch_isolates_a_failed_instance(cd_batch, tmp_path):
+    say("cd2d batch with one instance whose diffusivity is NaN: the others are unaffected")
+    from exasim_amd import cases
+    bad = str(tmp_path / "bad")
+    cases.convdiff2d_param(bad, n=4, porder=3, kappa=float("nan"), restart=40, gmres_tol=1e-12,
+                           newton_tol=1e-10)
+    out, stats = _solve_batch([cd_batch[0], bad, cd_batch[1]])
+    norms = stats[2]
+    assert np.isnan(norms[1]) and np.isfinite(norms[0]) and np.isfinite(norms[2])
+    for k, p in ((0, cd_batch[0]), (2, cd_batch[1])):
+        alone, _ = _solve_batch([p])
+        assert np.array_equal(out[k][0], alone[0][0]), k

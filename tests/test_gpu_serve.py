@@ -1,0 +1,28 @@
+"""The served form (exasim_amd/serve.py): a timed stream of Navier-Stokes and
+convection-diffusion requests of uneven size and order, admitted by HBM budget into
+concurrent batches.  Every request completes, at least one batch holds several requests,
+and each result is bitwise the result of the same request solved alone."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_alone(tmp_path):
+    from exasim_amd import cases, serve
+    from exasim_amd.hdg import HDGSolver
+    reqs = cases.serve_stream(str(tmp_path), seconds=0.4, rate=40.0, sizes=(4, 6), orders=(2, 3), ns_steps=1)
+    print("served stream of %d requests" % len(reqs), flush=True)
+    sys.stdout.flush()
+    srv = serve.Server(hbm_budget_bytes=8e9, max_batch=4, workers=2, precision=1, keep_results=True, window=0.2)
+    summ = srv.run_stream(reqs)
+    assert summ["completed"] == len(reqs) and summ["failed"] == 0, summ
+    assert max(b[1] for b in srv.batches) > 1
+    for rid, path, model, t in reqs:
+        with HDGSolver.batch([path], model) as S:
+            S.set_option("precision", srv.records[rid]["precision"])
+            S.solve()
+            uh = S.get_instance(0, "uh")
+        assert np.array_equal(uh, srv.results[rid]), rid

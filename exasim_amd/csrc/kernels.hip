@@ -1602,7 +1602,9 @@ __global__ __launch_bounds__(256) void k_refine_rhs(Dims d, int nbp, const float
   }
 }
 
-// panel width lu_solve uses for an n x n factorisation (see lu_solve)
+// panel width lu_solve uses for an n x n factorisation (see lu_solve).  64-wide panels (up to
+// 150 KB of LDS at n = 300) were measured slower: one workgroup per CU leaves the latency-bound
+// panel factorisation without company (C2 LU 0.36 s -> 0.55 s per assembly).
 int lu_panel_width(int n) { return (static_cast<size_t>(n) * 32 * sizeof(double) <= 64000) ? 32 : 16; }
 
 // ---------------------------------------------------------------------------
@@ -1812,7 +1814,7 @@ void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hi
   if (batch <= 0) return;
   if ((n >= BLU_MIN_N || pivout) && ncols > n) {
     // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide
-    if (static_cast<size_t>(n) * 32 * sizeof(double) <= 64000)
+    if (lu_panel_width(n) == 32)
       blocked_lu<32>(A, n, ncols, stride, batch, info, pivout, s);
     else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000)
       blocked_lu<16>(A, n, ncols, stride, batch, info, pivout, s);
@@ -2131,6 +2133,32 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_dot_partial(SegMap m, const doub
   if (threadIdx.x == 0) partial[static_cast<long>(blockIdx.y) * m.nblk + b] = sh[0];
 }
 
+// partial[v*nblk + b] = <V_v, w> over block b for every v < k: one block per chunk, the chunk of
+// w staged in LDS once, wave q owns vectors q, q+4, ...; each lane sums its strided entries in
+// order and the wave adds its lanes with a fixed butterfly.
+__global__ __launch_bounds__(SEG_BS) void k_seg_multidot_partial(SegMap m, const double* V, long ldv, int k,
+                                                                 const double* w, double* partial) {
+  __shared__ double wl[SEG_CHUNK];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (m.act && m.act[m.bseg[b]] == 0.0) {  // block-uniform
+    for (int v = threadIdx.x; v < k; v += SEG_BS) partial[static_cast<long>(v) * m.nblk + b] = 0.0;
+    return;
+  }
+  const long i0 = m.bbeg[b];
+  const int len = static_cast<int>(m.bend[b] - i0);
+  for (int t = threadIdx.x; t < len; t += SEG_BS) wl[t] = w[i0 + t];
+  __syncthreads();
+  for (int v = wave; v < k; v += SEG_BS / 64) {
+    const double* x = V + ldv * v + i0;
+    double acc = 0.0;
+    for (int t = lane; t < len; t += 64) acc = fma(x[t], wl[t], acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) partial[static_cast<long>(v) * m.nblk + b] = acc;
+  }
+}
+
 // out[v*nseg + s] = sum of segment s's partials (fixed order), sqrt if asked  (blockIdx.y = v)
 __global__ __launch_bounds__(SEG_BS) void k_seg_reduce(SegMap m, const double* partial, double* out,
                                                        int sqrt_out) {
@@ -2185,7 +2213,8 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_gemv_cols(SegMap m, double* x, c
 void seg_multidot(const SegMap& m, const double* V, long ldv, int k, const double* w, double* partial,
                   double* out, int sqrt_out, hipStream_t s) {
   if (m.nblk == 0 || k == 0) return;
-  k_seg_dot_partial<<<dim3(m.nblk, k), SEG_BS, 0, s>>>(m, V, ldv, w, partial);
+  if (k >= 4) k_seg_multidot_partial<<<m.nblk, SEG_BS, 0, s>>>(m, V, ldv, k, w, partial);
+  else k_seg_dot_partial<<<dim3(m.nblk, k), SEG_BS, 0, s>>>(m, V, ldv, w, partial);
   k_seg_reduce<<<dim3(m.nseg, k), SEG_BS, 0, s>>>(m, partial, out, sqrt_out);
   EXA_HIP(hipGetLastError());
 }

@@ -4,8 +4,8 @@
 Workload (default): steady convection-diffusion on the unit cube, 32^3 hex mesh,
 p = 4, two components, velocity (1, 0.5, 0.25), diffusivity 1e-2, seeded
 uniform[-1, 1] nodal source, Dirichlet u = 0 on x = 0 and homogeneous Neumann
-elsewhere, GMRES(100) to 1e-10 preconditioned by block-ILU(0) over
-2x2x2-element superelements.  One "step" is one full solve from the input
+elsewhere, GMRES(100) to 1e-10 with classical Gram-Schmidt (GMRESortho = 1)
+preconditioned by block-ILU(0) over 2x2x2-element superelements.  One "step" is one full solve from the input
 state: hdgGetQ, residual, assembly + static condensation, GMRES, hdgGetDUDG,
 residual (one Newton step, the problem is linear), with every input already
 resident in HBM.
@@ -261,7 +261,8 @@ def main():
     ap.add_argument("--restart", type=int, default=100)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxit", type=int, default=20000)
-    ap.add_argument("--ortho", type=int, default=0)
+    ap.add_argument("--ortho", type=int, default=1,
+                    help="GMRESortho: 0 modified, 1 classical Gram-Schmidt (default; fewer passes over the basis)")
     ap.add_argument("--refine-first-tol", type=float, default=None)
     ap.add_argument("--refine-tol", type=float, default=None)
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
@@ -442,8 +443,9 @@ def main():
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": "steady convection-diffusion 3D, %d^3 hex, p=%d, 2 components, "
-                               "kappa=1e-2, velocity (1,0.5,0.25), GMRES(%d) tol %g, %s, %s"
+                               "kappa=1e-2, velocity (1,0.5,0.25), GMRES(%d) tol %g %s, %s, %s"
                                % (args.n, args.porder, args.restart, args.tol,
+                                  "CGS" if args.ortho else "MGS",
                                   "block-ILU(0) over %d^3-element superelements" % args.se
                                   if args.precond == "bilu0" else "block-Jacobi face preconditioner",
                                   "mixed FP32 storage / FP64 refinement to 1e-10" if mixed else "FP64"),

@@ -1426,81 +1426,110 @@ __global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const double* L
 // One workgroup per superelement (ApplyBlockILU0, preconditioner.cpp:184-270):
 // forward sweep y_i = b_i - sum L_ij y_j, backward sweep x_i = D_i^{-1} (y_i - sum
 // U_il x_l).  The superelement's vector lives in LDS (y_i is overwritten by x_i).
-// Thread (r, part) accumulates row r over the part-th chunk of the columns of every
-// block of the current row, so all 256 threads issue loads in every step; the
-// chunks are added in a fixed order.  Xo is written as [slot = n + nse*i][ncf].
-template <typename T>
+// Thread (rr, part) accumulates rows VW*rr .. VW*rr+VW-1 over the part-th chunk of the
+// columns of every block of the current row, with one VW-wide vector load per column (so
+// FP32 blocks move twice the bytes per load instruction of FP64 ones), and all 256 threads
+// issue loads in every step; the chunks are added in a fixed order.  Xo is written as
+// [slot = n + nse*i][ncf].
+template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* BE, const double* x,
                                                     double* Xo) {
+  typedef T vecT __attribute__((ext_vector_type(VW)));
   const int ncf = d.ncf, nfse = I.nfse;
   const long n = blockIdx.x;
   const long per = static_cast<long>(ncf) * ncf;
   const int tid = threadIdx.x;
-  const int nparts = 256 / ncf;
-  const int r = tid % ncf, part = tid / ncf;
+  const int nrow = ncf / VW;                 // row groups
+  const int nparts = 256 / nrow;
+  const int rr = tid % nrow, part = tid / nrow;
+  const int r0 = VW * rr;
   const bool active = part < nparts;
   const int cw = (ncf + nparts - 1) / nparts;
   const int c0 = part * cw, c1 = min(ncf, c0 + cw);
   extern __shared__ double sm[];
   double* v = sm;                          // nfse * ncf
-  double* red = sm + static_cast<long>(nfse) * ncf;   // 256
-  double* z = red + 256;                   // ncf
-  auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per + r; };
+  double* red = sm + static_cast<long>(nfse) * ncf;   // nparts * ncf <= 256 * VW
+  double* z = red + 256 * VW;              // ncf
+  auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per + r0; };
   for (int idx = tid; idx < nfse * ncf; idx += 256) {
-    const int i = idx / ncf, rr = idx % ncf;
-    v[idx] = x[static_cast<long>(I.face[n + static_cast<long>(I.nse) * i]) * ncf + rr];
+    const int i = idx / ncf, q = idx % ncf;
+    v[idx] = x[static_cast<long>(I.face[n + static_cast<long>(I.nse) * i]) * ncf + q];
   }
   __syncthreads();
+  double acc[VW];
   for (int i = 0; i < nfse; ++i) {
-    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < VW; ++k) acc[k] = 0.0;
     if (active)
       for (int q = I.lptr[i]; q < I.lptr[i + 1]; ++q) {
         const T* B = blk(I.lidx[q]);
         const double* yj = v + I.lcol[q] * ncf;
-        for (int c = c0; c < c1; ++c) acc = fma(static_cast<double>(B[ncf * c]), yj[c], acc);
+        for (int c = c0; c < c1; ++c) {
+          const vecT b = *reinterpret_cast<const vecT*>(B + static_cast<long>(ncf) * c);
+          const double yc = yj[c];
+#pragma unroll
+          for (int k = 0; k < VW; ++k) acc[k] = fma(static_cast<double>(b[k]), yc, acc[k]);
+        }
       }
-    red[tid] = acc;
+    if (active)
+#pragma unroll
+      for (int k = 0; k < VW; ++k) red[part * ncf + r0 + k] = acc[k];
     __syncthreads();
     if (tid < ncf) {
-      double s = v[i * ncf + tid];
-      for (int p = 0; p < nparts; ++p) s -= red[p * ncf + tid];
-      v[i * ncf + tid] = s;
+      double sacc = v[i * ncf + tid];
+      for (int p = 0; p < nparts; ++p) sacc -= red[p * ncf + tid];
+      v[i * ncf + tid] = sacc;
     }
     __syncthreads();
   }
   for (int i = nfse - 1; i >= 0; --i) {
-    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < VW; ++k) acc[k] = 0.0;
     if (active)
       for (int q = I.uuptr[i]; q < I.uuptr[i + 1]; ++q) {
         const T* B = blk(I.uidx[q]);
         const double* xj = v + I.ucol[q] * ncf;
-        for (int c = c0; c < c1; ++c) acc = fma(static_cast<double>(B[ncf * c]), xj[c], acc);
+        for (int c = c0; c < c1; ++c) {
+          const vecT b = *reinterpret_cast<const vecT*>(B + static_cast<long>(ncf) * c);
+          const double xc = xj[c];
+#pragma unroll
+          for (int k = 0; k < VW; ++k) acc[k] = fma(static_cast<double>(b[k]), xc, acc[k]);
+        }
       }
-    red[tid] = acc;
+    if (active)
+#pragma unroll
+      for (int k = 0; k < VW; ++k) red[part * ncf + r0 + k] = acc[k];
     __syncthreads();
     if (tid < ncf) {
-      double s = v[i * ncf + tid];
-      for (int p = 0; p < nparts; ++p) s -= red[p * ncf + tid];
-      z[tid] = s;
+      double sacc = v[i * ncf + tid];
+      for (int p = 0; p < nparts; ++p) sacc -= red[p * ncf + tid];
+      z[tid] = sacc;
     }
     __syncthreads();
-    acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < VW; ++k) acc[k] = 0.0;
     if (active) {
       const T* Dinv = blk(I.diag[i]);
-      for (int c = c0; c < c1; ++c) acc = fma(static_cast<double>(Dinv[ncf * c]), z[c], acc);
+      for (int c = c0; c < c1; ++c) {
+        const vecT b = *reinterpret_cast<const vecT*>(Dinv + static_cast<long>(ncf) * c);
+        const double zc = z[c];
+#pragma unroll
+        for (int k = 0; k < VW; ++k) acc[k] = fma(static_cast<double>(b[k]), zc, acc[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < VW; ++k) red[part * ncf + r0 + k] = acc[k];
     }
-    red[tid] = acc;
     __syncthreads();
     if (tid < ncf) {
-      double s = 0.0;
-      for (int p = 0; p < nparts; ++p) s += red[p * ncf + tid];
-      v[i * ncf + tid] = s;
+      double sacc = 0.0;
+      for (int p = 0; p < nparts; ++p) sacc += red[p * ncf + tid];
+      v[i * ncf + tid] = sacc;
     }
     __syncthreads();
   }
   for (int idx = tid; idx < nfse * ncf; idx += 256) {
-    const int i = idx / ncf, rr = idx % ncf;
-    Xo[(n + static_cast<long>(I.nse) * i) * ncf + rr] = v[idx];
+    const int i = idx / ncf, q = idx % ncf;
+    Xo[(n + static_cast<long>(I.nse) * i) * ncf + q] = v[idx];
   }
 }
 
@@ -2045,12 +2074,18 @@ void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, do
                 double* out, hipStream_t s) {
   (void)Y;
   if (d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 256");
-  const size_t shm = (static_cast<size_t>(I.nfse) * d.ncf + 256 + d.ncf) * sizeof(double);
+  // two rows per thread (8-byte FP32 / 16-byte FP64 loads) when the block rows pair up
+  const int vw = (d.ncf % 2 == 0) ? 2 : 1;
+  const size_t shm = (static_cast<size_t>(I.nfse) * d.ncf + 256 * vw + d.ncf) * sizeof(double);
   if (shm > 160 * 1024) throw Error(5, "block-ILU(0) superelement vector larger than the LDS");
-  if (shm > 64 * 1024)
-    EXA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bilu_apply<T>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-  k_bilu_apply<T><<<I.nse, 256, shm, s>>>(d, I, BE, x, Xo);
+  auto launch = [&](auto kern) {
+    if (shm > 64 * 1024)
+      EXA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(shm)));
+    kern<<<I.nse, 256, shm, s>>>(d, I, BE, x, Xo);
+  };
+  if (vw == 2) launch(&k_bilu_apply<T, 2>);
+  else launch(&k_bilu_apply<T, 1>);
   k_bilu_scatter<<<nblocks(static_cast<long>(d.nf) * d.ncf, 256), 256, 0, s>>>(d, I, Xo, out);
   EXA_HIP(hipGetLastError());
 }

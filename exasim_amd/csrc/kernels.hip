@@ -324,6 +324,141 @@ __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols
   if (pivout && tid < nb) pivout[static_cast<long>(blockIdx.x) * n + k0 + tid] = piv[tid];
 }
 
+// The panel factorisation of k_lu_panel for panels of at most 256 rows, with the panel in
+// registers: thread t holds row t of the rows x nb panel.  Per column: wave argmax of |a[kk]|
+// (ties to the smaller row, as k_lu_panel) and a 4-entry combine, the pivot row and row kk
+// exchanged through a double-buffered LDS row, then each lower row scales its multiplier and
+// updates its own entries -- two barriers per column and no LDS traffic for the rank-1 update.
+// The operations on every entry are those of k_lu_panel, so the factors are the same.  The
+// columns right of the panel are left to k_lu_u12; the pivots go to piv_ws[blockIdx.x][nb]
+// (and to pivout as panel-relative rows).
+template <int BLU_NB>
+__global__ __launch_bounds__(256) void k_lu_panel_reg(double* Aall, int n, long stride, int k0, int* info,
+                                                      int* pivout, int* piv_ws) {
+  double* A = Aall + stride * blockIdx.x;
+  const int nb = min(BLU_NB, n - k0);
+  const int rows = n - k0;  // <= 256
+  __shared__ double wv[2][4];
+  __shared__ int wi[2][4];
+  __shared__ double prow[2][BLU_NB];
+  __shared__ double xrow[2][BLU_NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool mine = tid < rows;
+  double a[BLU_NB];
+#pragma unroll
+  for (int c = 0; c < BLU_NB; ++c)
+    a[c] = (mine && c < nb) ? A[k0 + tid + static_cast<long>(n) * (k0 + c)] : 0.0;
+  int singular = 0;
+#pragma clang loop unroll(full)
+  for (int kk = 0; kk < BLU_NB; ++kk) {
+    if (kk < nb) {
+      const int b = kk & 1;
+      double best = (mine && tid >= kk) ? fabs(a[kk]) : -1.0;
+      int bi = (mine && tid >= kk) ? tid : kk;
+      argmax_wave(best, bi);
+      if (lane == 0) {
+        wv[b][wave] = best;
+        wi[b][wave] = bi;
+      }
+      __syncthreads();
+      double bv = wv[b][0];
+      int p = wi[b][0];
+      for (int w = 1; w < 4; ++w)
+        if (wv[b][w] > bv || (wv[b][w] == bv && wi[b][w] < p)) { bv = wv[b][w]; p = wi[b][w]; }
+      if (tid == 0) {
+        piv_ws[static_cast<long>(blockIdx.x) * BLU_NB + kk] = p;
+        if (pivout) pivout[static_cast<long>(blockIdx.x) * n + k0 + kk] = p;
+      }
+      // the pivot row and row kk trade places
+      if (tid == p)
+#pragma unroll
+        for (int c = 0; c < BLU_NB; ++c) prow[b][c] = a[c];
+      if (tid == kk && p != kk)
+#pragma unroll
+        for (int c = 0; c < BLU_NB; ++c) xrow[b][c] = a[c];
+      __syncthreads();
+      if (tid == kk) {
+#pragma unroll
+        for (int c = 0; c < BLU_NB; ++c) a[c] = prow[b][c];
+      } else if (tid == p) {
+#pragma unroll
+        for (int c = 0; c < BLU_NB; ++c) a[c] = xrow[b][c];
+      }
+      const double pv = prow[b][kk];
+      if (pv == 0.0) singular = 1;
+      if (mine && tid > kk) {
+        const double l = a[kk] * (1.0 / pv);
+        a[kk] = l;
+#pragma unroll
+        for (int c = kk + 1; c < BLU_NB; ++c)
+          if (c < nb) a[c] = fma(-l, prow[b][c], a[c]);
+      }
+    }
+  }
+  if (mine)
+#pragma unroll
+    for (int c = 0; c < BLU_NB; ++c)
+      if (c < nb) A[k0 + tid + static_cast<long>(n) * (k0 + c)] = a[c];
+  if (tid == 0 && info) info[blockIdx.x] = (k0 == 0 ? 0 : info[blockIdx.x]) | singular;
+}
+
+// Columns right of a panel factored by k_lu_panel_reg: the panel's row exchanges, then
+// U12 = L11^{-1} A12 (one column per thread; blockIdx.y selects a chunk of 256 columns).
+template <int BLU_NB>
+__global__ __launch_bounds__(256) void k_lu_u12(double* Aall, int n, int ncols, long stride, int k0,
+                                                const int* piv_ws) {
+  double* A = Aall + stride * blockIdx.x;
+  const int nb = min(BLU_NB, n - k0);
+  // L11 with a run-time leading dimension: constant LDS addresses in the unrolled solve let the
+  // compiler hoist every load into registers (and spill)
+  __shared__ double L11[BLU_NB * BLU_NB];
+  __shared__ int piv[BLU_NB];
+  const int ldl = nb;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int r = idx % nb, c = idx / nb;
+    L11[r + ldl * c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
+  }
+  if (tid < nb) piv[tid] = piv_ws[static_cast<long>(blockIdx.x) * BLU_NB + tid];
+  __syncthreads();
+  const int j = k0 + nb + blockIdx.y * 256 + tid;
+  if (j >= ncols) return;
+  double* colj = A + static_cast<long>(n) * j + k0;
+  double x[BLU_NB];
+#pragma unroll
+  for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
+  // pivots in order; p < BLU_NB means both rows are in the register block
+#pragma unroll
+  for (int kk = 0; kk < BLU_NB; ++kk) {
+    if (kk < nb) {
+      const int p = piv[kk];
+      if (p != kk) {
+        if (p < BLU_NB) {
+#pragma unroll
+          for (int q = kk + 1; q < BLU_NB; ++q)
+            if (q == p) {
+              const double t = x[q];
+              x[q] = x[kk];
+              x[kk] = t;
+            }
+        } else {
+          const double t = colj[p];
+          colj[p] = x[kk];
+          x[kk] = t;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int kk = 1; kk < BLU_NB; ++kk)
+#pragma unroll
+    for (int t = 0; t < kk; ++t)
+      if (kk < nb) x[kk] = fma(-L11[kk + ldl * t], x[t], x[kk]);
+#pragma unroll
+  for (int kk = 0; kk < BLU_NB; ++kk)
+    if (kk < nb) colj[kk] = x[kk];
+}
+
 // X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
 template <int BLU_NB>
 __global__ __launch_bounds__(256) void k_lu_back(double* Aall, int n, int ncols, long stride, int k0) {
@@ -1805,11 +1940,19 @@ void gemm(const Gemm& g, hipStream_t s) {
 }
 
 template <int NB>
-void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, int* pivout, hipStream_t s) {
+void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, int* pivout, hipStream_t s,
+                int* piv_ws) {
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = std::min(NB, n - k0);
-    size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(double);
-    k_lu_panel<NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info, pivout);
+    if (piv_ws && n - k0 <= 256) {
+      k_lu_panel_reg<NB><<<batch, 256, 0, s>>>(A, n, stride, k0, info, pivout, piv_ws);
+      const int right = ncols - k0 - nb;
+      if (right > 0)
+        k_lu_u12<NB><<<dim3(batch, (right + 255) / 256), 256, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
+    } else {
+      size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(double);
+      k_lu_panel<NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info, pivout);
+    }
     EXA_HIP(hipGetLastError());
     if (k0 + nb < n) {  // A22 -= L21 * U12
       Gemm g{};
@@ -1839,14 +1982,17 @@ void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, 
   }
 }
 
-void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout) {
+void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
+              int* piv_ws) {
   if (batch <= 0) return;
   if ((n >= BLU_MIN_N || pivout) && ncols > n) {
-    // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide
+    // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide; with a pivot
+    // workspace, panels of up to 256 rows are factored in registers (64-wide register panels
+    // were measured: no gain at n = 250, slower at n = 100)
     if (lu_panel_width(n) == 32)
-      blocked_lu<32>(A, n, ncols, stride, batch, info, pivout, s);
+      blocked_lu<32>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
     else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000)
-      blocked_lu<16>(A, n, ncols, stride, batch, info, pivout, s);
+      blocked_lu<16>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
     else
       throw Error(5, "dense block larger than 500 rows");
     return;

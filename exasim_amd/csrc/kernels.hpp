@@ -41,7 +41,9 @@ void gemm(const Gemm& g, hipStream_t s);
 // pivout (optional, [n, batch]): the panel-relative row exchanges of the blocked
 // factorisation, for refine_rhs
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s,
-              int* pivout = nullptr);
+              int* pivout = nullptr, int* piv_ws = nullptr);
+// piv_ws (optional, device, >= 32*batch ints, owned by the caller): with it, panels of up to
+// 256 rows are factored with the panel in registers (k_lu_panel_reg + k_lu_u12)
 
 // ---- setup ----
 void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s);

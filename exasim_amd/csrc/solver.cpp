@@ -90,6 +90,16 @@ void Solver::setup() {
   init(sol);
 }
 
+// pivot workspace of the batched LU for `batch` matrices (grown with the stream idle)
+int* Solver::lu_ws(long batch) {
+  const size_t need = static_cast<size_t>(std::max(1L, batch)) * 32;
+  if (lupiv.n < need) {
+    EXA_HIP(hipStreamSynchronize(s));
+    lupiv.alloc(need);
+  }
+  return lupiv.p;
+}
+
 void Solver::set_dims() {
   if (master.nd != mesh.nd) throw Error(2, "master.bin and mesh.bin disagree on nd");
   d.nd = master.nd;
@@ -278,7 +288,7 @@ void Solver::init(SolData& sol) {
       for (int e = 0; e < eb; ++e)
         EXA_HIP(hipMemcpyAsync(Mass.p + static_cast<long>(e0 + e) * d.npe * d.npe, qaug.p + e * per,
                                sizeof(double) * d.npe * d.npe, hipMemcpyDeviceToDevice, s));
-      lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s);
+      lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s, nullptr, lu_ws(eb));
       qpack(d, e0, eb, qaug.p, MCs.p, MEs.p, Mass.p, s);
     }
     EXA_HIP(hipStreamSynchronize(s));
@@ -707,7 +717,7 @@ void Solver::assemble() {
     // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru]
     tic(T_LU, &t0);
     lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s,
-             precision ? piv32.p + static_cast<long>(e0) * n : nullptr);
+             precision ? piv32.p + static_cast<long>(e0) * n : nullptr, lu_ws(eb));
     toc(T_LU, t0);
     tic(T_ASSEMBLY, &t0);
     {  // H += K * DinvF (= -K X_F), Rh -= K * (D^{-1} Ru)
@@ -750,7 +760,7 @@ void Solver::assemble() {
     for (int e0 = 0; e0 < d.ne; e0 += eas_chunk) {
       const int eb = std::min(eas_chunk, d.ne - e0);
       eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
-      lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s);
+      lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s, nullptr, lu_ws(eb));
       EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
       EXA_HIP(hipStreamSynchronize(s));
       const int* hinfo = reinterpret_cast<const int*>(hpin);
@@ -761,7 +771,7 @@ void Solver::assemble() {
   } else if (precond == 0) {
     // hdgBlockJacobi (matvec.cpp:190-217)
     bj_assemble(d, f2e.p, elemcon.p, H.p, bjaug.p, s);
-    lu_solve(bjaug.p, d.ncf, 2 * d.ncf, 2L * d.ncf * d.ncf, d.nf, info.p, s);
+    lu_solve(bjaug.p, d.ncf, 2 * d.ncf, 2L * d.ncf * d.ncf, d.nf, info.p, s, nullptr, lu_ws(d.nf));
     bj_extract(d, bjaug.p, BE.p, s);
   } else {
     // hdgBlockILU0 (matvec.cpp:119-166)

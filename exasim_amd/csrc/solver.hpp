@@ -65,6 +65,7 @@ struct Solver {
   // mixed precision: FP32 copies of H, DinvF, the LU factors of D', K' and the preconditioner blocks
   DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32;
   DevBuf<int> piv32;  // row exchanges of the blocked LU of D' ([n, ne], panel-relative)
+  DevBuf<int> lupiv;  // this handle's pivots of the current LU panel (32 per matrix of a batched LU)
   // elemental additive Schwarz: inverted element blocks, inversion scratch
   DevBuf<double> BEeas, easaug;
   int eas_chunk = 0;
@@ -118,6 +119,7 @@ struct Solver {
   Solver(const std::vector<std::string>& datains, const std::string& model, int device);
   ~Solver();
   void set_dims();
+  int* lu_ws(long batch);
   void init(SolData& sol);
   // batch (segmented) Newton-GMRES
   void seg_setup();

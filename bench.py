@@ -11,9 +11,10 @@ residual (one Newton step, the problem is linear), with every input already
 resident in HBM.
 
 Two timed legs run on the same solver: the plain FP64 form and the mixed form
-(FP32 storage of the condensed operator, the local-solve factors and the
-preconditioner blocks; FP64 residuals, LU pivots and GMRES vectors; iterative
-refinement to 1e-10 of the first residual).  ``value`` is the mixed form's
+(FP32 storage of the condensed operator and the local-solve factors, BF16 storage
+of the block-ILU(0) factors -- ``--precond-fp32`` keeps them in FP32; FP64
+residuals, LU pivots and GMRES vectors; iterative refinement to 1e-10 of the
+first residual).  ``value`` is the mixed form's
 solves/s; the FP64 form, the ratio and the mixed solution's relative error
 against the FP64 one are reported beside it.
 
@@ -265,6 +266,8 @@ def main():
                     help="GMRESortho: 0 modified, 1 classical Gram-Schmidt (default; fewer passes over the basis)")
     ap.add_argument("--refine-first-tol", type=float, default=None)
     ap.add_argument("--refine-tol", type=float, default=None)
+    ap.add_argument("--precond-fp32", action="store_true",
+                    help="mixed leg: block-ILU(0) factors in FP32 (default: BF16, same iteration counts)")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
                     help="timed legs: FP64, mixed FP32/FP64, or both (value = the mixed leg)")
     ap.add_argument("--precond", default="bilu0", choices=["bilu0", "bjacobi"])
@@ -346,6 +349,8 @@ def main():
         S.set_option("refine_first_tol", args.refine_first_tol)
     if args.refine_tol is not None:
         S.set_option("refine_tol", args.refine_tol)
+    if not args.precond_fp32:
+        S.set_option("precond_bf16", 1)
     t_setup = time.perf_counter() - t0
     z = S.sizes
     print("# setup (read, connectivity, geometry, q-equation) %.2f s; sizes %s" % (t_setup, z),
@@ -448,7 +453,8 @@ def main():
                                   "CGS" if args.ortho else "MGS",
                                   "block-ILU(0) over %d^3-element superelements" % args.se
                                   if args.precond == "bilu0" else "block-Jacobi face preconditioner",
-                                  "mixed FP32 storage / FP64 refinement to 1e-10" if mixed else "FP64"),
+                                  ("mixed FP32 storage (%s block-ILU(0) factors) / FP64 refinement to 1e-10"
+                                   % ("FP32" if args.precond_fp32 else "BF16")) if mixed else "FP64"),
                    "ne": ne, "trace_dofs": N, "n": n, "m": m,
                    "precision": "mixed" if mixed else "fp64"},
         "p50_ms": float(np.percentile(lat_ms, 50)),

@@ -1566,10 +1566,29 @@ __global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const double* L
 // FP32 blocks move twice the bytes per load instruction of FP64 ones), and all 256 threads
 // issue loads in every step; the chunks are added in a fixed order.  Xo is written as
 // [slot = n + nse*i][ncf].
+// VW consecutive entries of a block column as doubles: FP64 / FP32 storage, or BF16 kept as the
+// upper 16 bits of an FP32 (unsigned short storage)
+template <typename T, int VW>
+__device__ __forceinline__ void load_rows(const T* p, double* out) {
+  typedef T vecT __attribute__((ext_vector_type(VW)));
+  const vecT v = *reinterpret_cast<const vecT*>(p);
+#pragma unroll
+  for (int k = 0; k < VW; ++k) out[k] = static_cast<double>(v[k]);
+}
+template <>
+__device__ __forceinline__ void load_rows<unsigned short, 2>(const unsigned short* p, double* out) {
+  const unsigned int w = *reinterpret_cast<const unsigned int*>(p);
+  out[0] = static_cast<double>(__uint_as_float(w << 16));
+  out[1] = static_cast<double>(__uint_as_float(w & 0xFFFF0000u));
+}
+template <>
+__device__ __forceinline__ void load_rows<unsigned short, 1>(const unsigned short* p, double* out) {
+  out[0] = static_cast<double>(__uint_as_float(static_cast<unsigned int>(*p) << 16));
+}
+
 template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* BE, const double* x,
                                                     double* Xo) {
-  typedef T vecT __attribute__((ext_vector_type(VW)));
   const int ncf = d.ncf, nfse = I.nfse;
   const long n = blockIdx.x;
   const long per = static_cast<long>(ncf) * ncf;
@@ -1600,10 +1619,11 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
         const T* B = blk(I.lidx[q]);
         const double* yj = v + I.lcol[q] * ncf;
         for (int c = c0; c < c1; ++c) {
-          const vecT b = *reinterpret_cast<const vecT*>(B + static_cast<long>(ncf) * c);
+          double b[VW];
+          load_rows<T, VW>(B + static_cast<long>(ncf) * c, b);
           const double yc = yj[c];
 #pragma unroll
-          for (int k = 0; k < VW; ++k) acc[k] = fma(static_cast<double>(b[k]), yc, acc[k]);
+          for (int k = 0; k < VW; ++k) acc[k] = fma(b[k], yc, acc[k]);
         }
       }
     if (active)
@@ -1625,10 +1645,11 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
         const T* B = blk(I.uidx[q]);
         const double* xj = v + I.ucol[q] * ncf;
         for (int c = c0; c < c1; ++c) {
-          const vecT b = *reinterpret_cast<const vecT*>(B + static_cast<long>(ncf) * c);
+          double b[VW];
+          load_rows<T, VW>(B + static_cast<long>(ncf) * c, b);
           const double xc = xj[c];
 #pragma unroll
-          for (int k = 0; k < VW; ++k) acc[k] = fma(static_cast<double>(b[k]), xc, acc[k]);
+          for (int k = 0; k < VW; ++k) acc[k] = fma(b[k], xc, acc[k]);
         }
       }
     if (active)
@@ -1646,10 +1667,11 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
     if (active) {
       const T* Dinv = blk(I.diag[i]);
       for (int c = c0; c < c1; ++c) {
-        const vecT b = *reinterpret_cast<const vecT*>(Dinv + static_cast<long>(ncf) * c);
+        double b[VW];
+        load_rows<T, VW>(Dinv + static_cast<long>(ncf) * c, b);
         const double zc = z[c];
 #pragma unroll
-        for (int k = 0; k < VW; ++k) acc[k] = fma(static_cast<double>(b[k]), zc, acc[k]);
+        for (int k = 0; k < VW; ++k) acc[k] = fma(b[k], zc, acc[k]);
       }
 #pragma unroll
       for (int k = 0; k < VW; ++k) red[part * ncf + r0 + k] = acc[k];
@@ -2115,6 +2137,21 @@ void to_float(float* dst, const double* src, int rows, int cols, long ld, long s
   EXA_HIP(hipGetLastError());
 }
 
+// BF16 (round to nearest even of the FP32 value), kept as unsigned short
+__global__ void k_to_bf16(unsigned short* dst, const double* src, long count) {
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= count) return;
+  unsigned int u = __float_as_uint(static_cast<float>(src[q]));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  dst[q] = static_cast<unsigned short>(u >> 16);
+}
+
+void to_bf16(unsigned short* dst, const double* src, long count, hipStream_t s) {
+  if (count <= 0) return;
+  k_to_bf16<<<nblocks(count, 256), 256, 0, s>>>(dst, src, count);
+  EXA_HIP(hipGetLastError());
+}
+
 void refine_rhs(const Dims& d, const float* LU, const int* piv, const float* K, const double* Ru, double* RuS,
                 double* Rh, hipStream_t s) {
   size_t shm = static_cast<size_t>(d.n) * sizeof(double);
@@ -2433,6 +2470,8 @@ template void bilu_apply<double>(const Dims&, const IluDev&, const double*, cons
                                  double*, hipStream_t);
 template void bilu_apply<float>(const Dims&, const IluDev&, const float*, const double*, double*, double*,
                                 double*, hipStream_t);
+template void bilu_apply<unsigned short>(const Dims&, const IluDev&, const unsigned short*, const double*, double*,
+                                         double*, double*, hipStream_t);
 
 }  // namespace exa
 

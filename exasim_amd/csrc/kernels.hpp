@@ -96,6 +96,8 @@ void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s);
 // dst_b[r + rows*c] = (float) src_b[r + ld*c]
+// BF16 copy (round to nearest even), stored as unsigned short (the upper half of an FP32)
+void to_bf16(unsigned short* dst, const double* src, long count, hipStream_t s);
 void to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
               hipStream_t s);
 // Condensed right-hand side of a new residual with stored factors (mixed precision):

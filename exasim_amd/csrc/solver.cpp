@@ -410,7 +410,10 @@ void Solver::alloc_mixed() {
   K32.alloc(m * n * ne);
   if (precond == 0) BE32.alloc(ncf2 * d.nf);
   if (precond == 1) BEeas32.alloc(m * m * ne);
-  if (precond == 2) BEilu32.alloc(ncf2 * ilu.nnz * ilu.nse);
+  if (precond == 2) {
+    if (precond_bf16) BEilu16.alloc(ncf2 * ilu.nnz * ilu.nse);
+    else BEilu32.alloc(ncf2 * ilu.nnz * ilu.nse);
+  }
 }
 
 void Solver::setup_eas() {
@@ -523,6 +526,14 @@ void Solver::set_option(const std::string& name, double v) {
   } else if (name == "refine_rtol") refine_rtol = v;
   else if (name == "refine_tol") refine_tol = v;
   else if (name == "refine_first_tol") refine_first_tol = v;
+  else if (name == "precond_bf16") {  // mixed form: block-ILU(0) factors stored in BF16
+    const int pv = v != 0.0;
+    if (pv != precond_bf16) {
+      precond_bf16 = pv;
+      if (precision) alloc_mixed();
+      assembled = false;
+    }
+  }
   else if (name == "refine_maxit") refine_maxit = static_cast<int>(v);
   else if (name == "timing") timing = v != 0.0;
   else if (name == "workspace_bytes") {
@@ -786,7 +797,10 @@ void Solver::assemble() {
       const long mm = static_cast<long>(d.m) * d.m;
       to_float(BEeas32.p, BEeas.p, d.m, d.m, d.m, mm, mm, d.ne, s);
     }
-    if (precond == 2) to_float(BEilu32.p, BEilu.p, d.ncf, d.ncf, d.ncf, c2, c2, ilu.nnz * ilu.nse, s);
+    if (precond == 2) {
+      if (precond_bf16) to_bf16(BEilu16.p, BEilu.p, c2 * ilu.nnz * ilu.nse, s);
+      else to_float(BEilu32.p, BEilu.p, d.ncf, d.ncf, d.ncf, c2, c2, ilu.nnz * ilu.nse, s);
+    }
   }
   if (precond != 1) {
     const int nchk = std::min(precond == 2 ? ilu.nse : d.nf, 8192);
@@ -836,7 +850,8 @@ void Solver::apply_precond(double* xv) {
   tic(T_PRECOND, &t0);
   copy(tmp.p, xv, d.N, s);
   if (precond == 2) {
-    if (precision) bilu_apply(d, ilud, BEilu32.p, tmp.p, Yilu.p, Xilu.p, xv, s);
+    if (precision && precond_bf16) bilu_apply(d, ilud, BEilu16.p, tmp.p, Yilu.p, Xilu.p, xv, s);
+    else if (precision) bilu_apply(d, ilud, BEilu32.p, tmp.p, Yilu.p, Xilu.p, xv, s);
     else bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
   } else if (precond == 1) {  // x = sum_e scatter(BE_e^{-1} gather_e(x)), as hdgMatVec with BE
     if (precision) matvec_elem(d, BEeas32.p, elemcon.p, tmp.p, we.p, s, einst.p, elem_active);

@@ -64,6 +64,9 @@ struct Solver {
   DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
   // mixed precision: FP32 copies of H, DinvF, the LU factors of D', K' and the preconditioner blocks
   DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32;
+  // option precond_bf16 (mixed form, block-ILU(0)): the factors in BF16 (upper half of FP32)
+  int precond_bf16 = 0;
+  DevBuf<unsigned short> BEilu16;
   DevBuf<int> piv32;  // row exchanges of the blocked LU of D' ([n, ne], panel-relative)
   DevBuf<int> lupiv;  // this handle's pivots of the current LU panel (32 per matrix of a batched LU)
   // elemental additive Schwarz: inverted element blocks, inversion scratch

@@ -75,7 +75,8 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * residuals, LU pivots and GMRES vectors stay FP64, and each Newton step is refined
  * with FP64 residuals until ||R|| <= "refine_rtol" (1e-10) times its first residual,
  * at most "refine_maxit" (5) corrections, each solved by GMRES to "refine_tol" (1e-4);
- * the first solve of each Newton step goes to max(gmres_tol, "refine_first_tol" (1e-7))). */
+ * the first solve of each Newton step goes to max(gmres_tol, "refine_first_tol" (1e-7)));
+ * "precond_bf16" (0/1): in the mixed form, keep the block-ILU(0) factors in BF16. */
 int exa_set_option(exa_handle* h, const char* name, double value);
 
 /* Full steady solve (NewtonSolver).  newton_iters/gmres_iters/final_norm may be NULL. */

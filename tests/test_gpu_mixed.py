@@ -91,3 +91,19 @@ def test_mixed_matvec_uses_fp32_operator(built):
     ref = P.matvec(H.astype(np.float32).astype(np.float64), v)
     assert rel(w, ref) < 1e-12
     assert rel(w, P.matvec(H, v)) > 1e-10          # it is not the FP64 operator
+
+
+def test_bf16_ilu_factors_solve_to_tolerance(built):
+    """precond_bf16: the mixed form with BF16 block-ILU(0) factors still reaches the FP64
+    oracle's solution to 1e-8 (the preconditioner only changes the iterations)."""
+    path, model = built("cd_ilu")
+    say("case cd_ilu: mixed form with BF16 block-ILU(0) factors")
+    from exasim_amd.hdg import HDGSolver
+    P = ora.HDGProblem(path, model)
+    ora.newton_solve(P)
+    with HDGSolver(path, model) as S:
+        S.set_option("precision", 1)
+        S.set_option("precond_bf16", 1)
+        S.solve()
+        uh = S.get("uh")
+    assert rel(uh, P.uh.reshape(-1)) < 1e-8

@@ -1195,6 +1195,111 @@ __global__ void k_face_jac(Dims d, AsmArgs a, int l, double* aug, double* Bfull,
   }
 }
 
+// The Jacobian parts of k_face_jac for all faces of an element, one workgroup per element.
+// The face matrices of the 2 ncu (nc + ncu) face-point fields f (the I and the B parts)
+//   M_f[aa, bb] = sum_g shapfgw[aa, g] shapfgt[g, bb] fpt_f[g]
+// are the product F^T Q^T, F[g, f] = fpt_f[g] and Q[p, g] = shapfgw[aa, g] shapfgt[g, bb]
+// (p = aa + npf bb), formed on the matrix cores: F^T (fields x points) from LDS as the A
+// operand, each lane's Q entry computed from the shape tables in LDS as the B operand.  In
+// the accumulators 16 lanes hold 16 consecutive pairs of one field, so their updates of
+// D/B/F (aug, Bfull) and K/G/H land in a few rows of one column, as in k_face_jac.  The
+// faces run in order l = 0 .. nfe-1 with a barrier between them: entries that two faces
+// share (D and B at the nodes of a common edge) are updated face by face, in that order.
+constexpr int FJ_THREADS = 512;
+
+__global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a, double* aug, double* Bfull,
+                                                              double* Kt, double* Gfull, double* H) {
+  extern __shared__ double fj_sm[];
+  const int npf = d.npf, ngf = d.ngf, ncu = d.ncu, nc = d.nc, npe = d.npe, nd = d.nd, nfe = d.nfe;
+  const int n = d.n, m = d.m;
+  const int nfi = ncu * (nc + ncu), nfl = 2 * nfi;
+  const int kp = (ngf + 3) & ~3, nfp = (nfl + 15) & ~15;
+  double* Ws = fj_sm;              // shapfgw, npf x ngf
+  double* Ts = Ws + npf * ngf;     // shapfgt, ngf x npf
+  double* Fs = Ts + ngf * npf;     // Fs[g * nfp + f], zero beyond ngf points / nfl fields
+  int* perm = reinterpret_cast<int*>(Fs + kp * nfp);
+  const long e = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  constexpr int NW = FJ_THREADS / 64;
+  for (int i = tid; i < npf * ngf; i += FJ_THREADS) {
+    Ws[i] = a.shapfgw[i];
+    Ts[i] = a.shapfgt[i];
+  }
+  for (int i = tid; i < npf * nfe; i += FJ_THREADS) perm[i] = a.perm[i];
+  const long P = static_cast<long>(a.eb) * nfe * ngf;
+  const int npair = npf * npf, ptiles = (npair + 15) / 16, ftiles = nfp / 16;
+  const long augs = a.augs;
+  double* augE = aug + e * augs;
+  double* BE = Bfull + e * static_cast<long>(n) * n * nd;
+  double* KE = Kt + e * static_cast<long>(m) * n;
+  double* GE = Gfull + e * static_cast<long>(m) * n * nd;
+  double* HE = H + (a.e0 + e) * static_cast<long>(m) * m;
+  for (int l = 0; l < nfe; ++l) {
+    const long pb = (e * nfe + l) * ngf;
+    __syncthreads();  // the previous face's reads of Fs and its updates are done
+    for (int i = tid; i < kp * nfp; i += FJ_THREADS) {
+      const int g = i / nfp, f = i % nfp;
+      // I fields are fpt fields ncu .. ncu+nfi-1, B fields 2 ncu + nfi .. 2 ncu + 2 nfi - 1
+      Fs[i] = (g < ngf && f < nfl) ? a.fpt[(f < nfi ? ncu + f : 2 * ncu + f) * P + pb + g] : 0.0;
+    }
+    __syncthreads();
+    for (int pt = wave; pt < ptiles; pt += NW) {
+      const int p = pt * 16 + li;
+      const bool pok = p < npair;
+      const int aa = pok ? p % npf : 0, bb = pok ? p / npf : 0;
+      const int pa = perm[aa + npf * l], pbn = perm[bb + npf * l];
+      for (int f0 = 0; f0 < ftiles; f0 += 4) {
+        dbl4 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = dbl4{0, 0, 0, 0};
+        for (int k0 = 0; k0 < kp; k0 += 4) {
+          const int g = k0 + lk;
+          const double q = (pok && g < ngf) ? Ws[aa + npf * g] * Ts[g + ngf * bb] : 0.0;
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (f0 + t < ftiles) acc[t] = mfma_step(Fs[g * nfp + (f0 + t) * 16 + li], q, acc[t]);
+        }
+        if (!pok) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          if (f0 + t >= ftiles) break;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = (f0 + t) * 16 + lk + 4 * i;
+            if (f >= nfl) continue;
+            const double v = acc[t][i];
+            const bool bpart = f >= nfi;
+            const int fl = bpart ? f - nfi : f;
+            const int k = fl % ncu, c = fl / ncu;
+            const long trow = k + ncu * (aa + npf * l);
+            if (c < ncu) {
+              if (bpart) KE[trow + static_cast<long>(m) * (pbn + npe * c)] += v;
+              else augE[pa + npe * k + static_cast<long>(n) * (pbn + npe * c)] += v;
+            } else if (c < nc) {
+              const int cq = c - ncu, cp = cq % ncu, dd = cq / ncu;
+              const long col = pbn + npe * dd + static_cast<long>(npe) * nd * cp;
+              if (bpart) GE[trow + m * col] += v;
+              else BE[pa + npe * k + n * col] += v;
+            } else {
+              const long col = (c - nc) + ncu * (bb + npf * l);
+              if (bpart) HE[trow + static_cast<long>(m) * col] = v;
+              else augE[pa + npe * k + static_cast<long>(n) * (n + col)] += v;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// LDS bytes of k_face_jac_elem
+size_t face_jac_elem_lds(const Dims& d) {
+  const int nfl = 2 * d.ncu * (d.nc + d.ncu);
+  const long kp = (d.ngf + 3) & ~3, nfp = (nfl + 15) & ~15;
+  return sizeof(double) * (2L * d.npf * d.ngf + kp * nfp) + sizeof(int) * static_cast<size_t>(d.npf) * d.nfe;
+}
+
 // After the solve: DinvF = -X_F, Ru = X_Ru (uequation.cpp:700-711).
 __global__ void k_post_schur(Dims d, int e0, int eb, const double* aug, long augs, double* DinvF,
                              double* RuS) {
@@ -2103,11 +2208,21 @@ void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long
   EXA_HIP(hipGetLastError());
 }
 
-void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull, double* Kt,
-              double* Gfull, double* H, hipStream_t s) {
+void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, double* Kt, double* Gfull, double* H,
+              hipStream_t s) {
+  if (a.eb <= 0) return;
+  const size_t lds = face_jac_elem_lds(d);
+  if (lds <= 64000) {
+    k_face_jac_elem<<<a.eb, FJ_THREADS, lds, s>>>(d, a, aug, Bfull, Kt, Gfull, H);
+    EXA_HIP(hipGetLastError());
+    return;
+  }
+  // face tables too large for LDS: one launch per local face (no two writes of a launch collide)
   long n1 = static_cast<long>(d.npf) * d.npf * d.ncu * (d.nc + d.ncu) * a.eb;
-  k_face_jac<<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, aug, Bfull, Kt, Gfull, H);
-  EXA_HIP(hipGetLastError());
+  for (int l = 0; l < d.nfe; ++l) {
+    k_face_jac<<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, aug, Bfull, Kt, Gfull, H);
+    EXA_HIP(hipGetLastError());
+  }
 }
 
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,

@@ -91,8 +91,9 @@ void vol_res(const Dims& d, const AsmArgs& a, double* out, long ldo, hipStream_t
 void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s);
 void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long ldru, double* Rh,
               hipStream_t s);
-void face_jac(const Dims& d, const AsmArgs& a, int l, double* aug, double* Bfull, double* Kt,
-              double* Gfull, double* H, hipStream_t s);
+// Jacobian parts of every face of the block's elements (face by face, in local order)
+void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, double* Kt, double* Gfull, double* H,
+              hipStream_t s);
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s);
 // dst_b[r + rows*c] = (float) src_b[r + ld*c]

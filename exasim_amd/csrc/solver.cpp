@@ -691,12 +691,10 @@ void Solver::assemble() {
       g.fp32 = precision;
       gemm(g, s);
     }
-    // face terms, one local face at a time (no two writes of a launch collide)
+    // face terms; residual parts one local face at a time (no two writes of a launch collide)
     face_points(d, a, true, s);
-    for (int l = 0; l < d.nfe; ++l) {
-      face_res(d, a, l, true, aug.p + n * (n + m), augs, Rh.p, s);
-      face_jac(d, a, l, aug.p, Bfull.p, Kt.p, Gfull.p, H.p, s);
-    }
+    for (int l = 0; l < d.nfe; ++l) face_res(d, a, l, true, aug.p + n * (n + m), augs, Rh.p, s);
+    face_jac(d, a, aug.p, Bfull.p, Kt.p, Gfull.p, H.p, s);
     // eliminate q: D += B*MinvC, F -= B*MinvE, K += G*MinvC, H -= G*MinvE (uequation.cpp:614-681)
     {
       Gemm g{};

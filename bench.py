@@ -264,8 +264,10 @@ def main():
     ap.add_argument("--maxit", type=int, default=20000)
     ap.add_argument("--ortho", type=int, default=1,
                     help="GMRESortho: 0 modified, 1 classical Gram-Schmidt (default; fewer passes over the basis)")
-    ap.add_argument("--refine-first-tol", type=float, default=None)
-    ap.add_argument("--refine-tol", type=float, default=None)
+    ap.add_argument("--refine-first-tol", type=float, default=1e-5,
+                    help="mixed form: relative tolerance of the first FP32 solve (measured best at 32^3 p=4)")
+    ap.add_argument("--refine-tol", type=float, default=1e-5,
+                    help="mixed form: relative tolerance of each refinement correction")
     ap.add_argument("--precond-fp32", action="store_true",
                     help="mixed leg: block-ILU(0) factors in FP32 (default: BF16, same iteration counts)")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
@@ -453,8 +455,10 @@ def main():
                                   "CGS" if args.ortho else "MGS",
                                   "block-ILU(0) over %d^3-element superelements" % args.se
                                   if args.precond == "bilu0" else "block-Jacobi face preconditioner",
-                                  ("mixed FP32 storage (%s block-ILU(0) factors) / FP64 refinement to 1e-10"
-                                   % ("FP32" if args.precond_fp32 else "BF16")) if mixed else "FP64"),
+                                  ("mixed FP32 storage (%s block-ILU(0) factors) / FP64 refinement to 1e-10 "
+                                   "(first solve %g, corrections %g)"
+                                   % ("FP32" if args.precond_fp32 else "BF16", args.refine_first_tol,
+                                      args.refine_tol)) if mixed else "FP64"),
                    "ne": ne, "trace_dofs": N, "n": n, "m": m,
                    "precision": "mixed" if mixed else "fp64"},
         "p50_ms": float(np.percentile(lat_ms, 50)),

@@ -1663,37 +1663,33 @@ __global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const double* L
   BE[(static_cast<long>(I.elim_ji[k0 + k]) * I.nse + n) * per + q % per] = Ltmp[q];
 }
 
+// Block entries as doubles: FP64 / FP32 storage, or BF16 kept as the upper 16 bits of an FP32
+// (unsigned short storage)
+template <typename T>
+__device__ __forceinline__ double blk_val(T x) { return static_cast<double>(x); }
+template <>
+__device__ __forceinline__ double blk_val<unsigned short>(unsigned short x) {
+  return static_cast<double>(__uint_as_float(static_cast<unsigned int>(x) << 16));
+}
+// loads a thread keeps in flight per step of a row's sweep (raw storage words in registers)
+template <typename T>
+constexpr int bilu_unroll() { return sizeof(T) == 2 ? 16 : sizeof(T) == 4 ? 12 : 8; }
+
 // One workgroup per superelement (ApplyBlockILU0, preconditioner.cpp:184-270):
 // forward sweep y_i = b_i - sum L_ij y_j, backward sweep x_i = D_i^{-1} (y_i - sum
 // U_il x_l).  The superelement's vector lives in LDS (y_i is overwritten by x_i).
 // Thread (rr, part) accumulates rows VW*rr .. VW*rr+VW-1 over the part-th chunk of the
-// columns of every block of the current row, with one VW-wide vector load per column (so
-// FP32 blocks move twice the bytes per load instruction of FP64 ones), and all 256 threads
-// issue loads in every step; the chunks are added in a fixed order.  Xo is written as
-// [slot = n + nse*i][ncf].
-// VW consecutive entries of a block column as doubles: FP64 / FP32 storage, or BF16 kept as the
-// upper 16 bits of an FP32 (unsigned short storage)
-template <typename T, int VW>
-__device__ __forceinline__ void load_rows(const T* p, double* out) {
-  typedef T vecT __attribute__((ext_vector_type(VW)));
-  const vecT v = *reinterpret_cast<const vecT*>(p);
-#pragma unroll
-  for (int k = 0; k < VW; ++k) out[k] = static_cast<double>(v[k]);
-}
-template <>
-__device__ __forceinline__ void load_rows<unsigned short, 2>(const unsigned short* p, double* out) {
-  const unsigned int w = *reinterpret_cast<const unsigned int*>(p);
-  out[0] = static_cast<double>(__uint_as_float(w << 16));
-  out[1] = static_cast<double>(__uint_as_float(w & 0xFFFF0000u));
-}
-template <>
-__device__ __forceinline__ void load_rows<unsigned short, 1>(const unsigned short* p, double* out) {
-  out[0] = static_cast<double>(__uint_as_float(static_cast<unsigned int>(*p) << 16));
-}
-
+// columns of every block of the current row, with one VW-wide vector load per column, and
+// all 256 threads issue loads in every step; the chunks are added in a fixed order.  The
+// (block, column) pairs of a row are walked as one list, U at a time: the U loads are issued
+// before their FMAs, so a thread keeps U loads in flight across block boundaries (the row
+// sweep is bound by HBM bytes in flight).  The sweep structure is staged in LDS.  Xo is
+// written as [slot = n + nse*i][ncf].
 template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* BE, const double* x,
                                                     double* Xo) {
+  typedef T vecT __attribute__((ext_vector_type(VW)));
+  constexpr int U = bilu_unroll<T>();
   const int ncf = d.ncf, nfse = I.nfse;
   const long n = blockIdx.x;
   const long per = static_cast<long>(ncf) * ncf;
@@ -1704,11 +1700,32 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
   const int r0 = VW * rr;
   const bool active = part < nparts;
   const int cw = (ncf + nparts - 1) / nparts;
-  const int c0 = part * cw, c1 = min(ncf, c0 + cw);
+  const int c0 = min(ncf, part * cw), c1 = min(ncf, c0 + cw);
   extern __shared__ double sm[];
   double* v = sm;                          // nfse * ncf
   double* red = sm + static_cast<long>(nfse) * ncf;   // nparts * ncf <= 256 * VW
   double* z = red + 256 * VW;              // ncf
+  // the sweep structure (the same for every superelement) in LDS
+  int* lptr = reinterpret_cast<int*>(z + ncf);
+  int* uuptr = lptr + nfse + 1;
+  int* diag = uuptr + nfse + 1;
+  int* lidx = diag + nfse;
+  int* lcol = lidx + I.nnz;
+  int* uidx = lcol + I.nnz;
+  int* ucol = uidx + I.nnz;
+  for (int idx = tid; idx <= nfse; idx += 256) {
+    lptr[idx] = I.lptr[idx];
+    uuptr[idx] = I.uuptr[idx];
+    if (idx < nfse) diag[idx] = I.diag[idx];
+  }
+  for (int idx = tid; idx < I.lptr[nfse]; idx += 256) {
+    lidx[idx] = I.lidx[idx];
+    lcol[idx] = I.lcol[idx];
+  }
+  for (int idx = tid; idx < I.uuptr[nfse]; idx += 256) {
+    uidx[idx] = I.uidx[idx];
+    ucol[idx] = I.ucol[idx];
+  }
   auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per + r0; };
   for (int idx = tid; idx < nfse * ncf; idx += 256) {
     const int i = idx / ncf, q = idx % ncf;
@@ -1716,21 +1733,41 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
   }
   __syncthreads();
   double acc[VW];
+  // acc += sum over the blocks q in [qb, qe) of block idx[q] (this thread's rows, columns
+  // c0 .. c1-1) times vec[col[q]], in the order (q, c); U loads are issued before their FMAs
+  auto row_pass = [&](int qb, int qe, const int* idx, const int* col, const double* vec) {
+    const int cwn = c1 - c0;
+    const int tot = (qe - qb) * cwn;
+    int qq = qb, cc = c0;
+    for (int t0 = 0; t0 < tot; t0 += U) {
+      vecT raw[U];
+      int yo[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (t0 + u < tot) {
+          raw[u] = *reinterpret_cast<const vecT*>(blk(idx[qq]) + static_cast<long>(ncf) * cc);
+          yo[u] = (col ? col[qq] * ncf : 0) + cc;
+          if (++cc == c1) {
+            cc = c0;
+            ++qq;
+          }
+        } else {
+          yo[u] = -1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (yo[u] >= 0) {
+          const double yc = vec[yo[u]];
+#pragma unroll
+          for (int k = 0; k < VW; ++k) acc[k] = fma(blk_val<T>(raw[u][k]), yc, acc[k]);
+        }
+    }
+  };
   for (int i = 0; i < nfse; ++i) {
 #pragma unroll
     for (int k = 0; k < VW; ++k) acc[k] = 0.0;
-    if (active)
-      for (int q = I.lptr[i]; q < I.lptr[i + 1]; ++q) {
-        const T* B = blk(I.lidx[q]);
-        const double* yj = v + I.lcol[q] * ncf;
-        for (int c = c0; c < c1; ++c) {
-          double b[VW];
-          load_rows<T, VW>(B + static_cast<long>(ncf) * c, b);
-          const double yc = yj[c];
-#pragma unroll
-          for (int k = 0; k < VW; ++k) acc[k] = fma(b[k], yc, acc[k]);
-        }
-      }
+    if (active) row_pass(lptr[i], lptr[i + 1], lidx, lcol, v);
     if (active)
 #pragma unroll
       for (int k = 0; k < VW; ++k) red[part * ncf + r0 + k] = acc[k];
@@ -1745,18 +1782,7 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
   for (int i = nfse - 1; i >= 0; --i) {
 #pragma unroll
     for (int k = 0; k < VW; ++k) acc[k] = 0.0;
-    if (active)
-      for (int q = I.uuptr[i]; q < I.uuptr[i + 1]; ++q) {
-        const T* B = blk(I.uidx[q]);
-        const double* xj = v + I.ucol[q] * ncf;
-        for (int c = c0; c < c1; ++c) {
-          double b[VW];
-          load_rows<T, VW>(B + static_cast<long>(ncf) * c, b);
-          const double xc = xj[c];
-#pragma unroll
-          for (int k = 0; k < VW; ++k) acc[k] = fma(b[k], xc, acc[k]);
-        }
-      }
+    if (active) row_pass(uuptr[i], uuptr[i + 1], uidx, ucol, v);
     if (active)
 #pragma unroll
       for (int k = 0; k < VW; ++k) red[part * ncf + r0 + k] = acc[k];
@@ -1770,14 +1796,7 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
 #pragma unroll
     for (int k = 0; k < VW; ++k) acc[k] = 0.0;
     if (active) {
-      const T* Dinv = blk(I.diag[i]);
-      for (int c = c0; c < c1; ++c) {
-        double b[VW];
-        load_rows<T, VW>(Dinv + static_cast<long>(ncf) * c, b);
-        const double zc = z[c];
-#pragma unroll
-        for (int k = 0; k < VW; ++k) acc[k] = fma(b[k], zc, acc[k]);
-      }
+      row_pass(0, 1, diag + i, nullptr, z);
 #pragma unroll
       for (int k = 0; k < VW; ++k) red[part * ncf + r0 + k] = acc[k];
     }
@@ -2374,7 +2393,8 @@ void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, do
   if (d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 256");
   // two rows per thread (8-byte FP32 / 16-byte FP64 loads) when the block rows pair up
   const int vw = (d.ncf % 2 == 0) ? 2 : 1;
-  const size_t shm = (static_cast<size_t>(I.nfse) * d.ncf + 256 * vw + d.ncf) * sizeof(double);
+  const size_t shm = (static_cast<size_t>(I.nfse) * d.ncf + 256 * vw + d.ncf) * sizeof(double) +
+                     (3 * static_cast<size_t>(I.nfse) + 2 + 4 * static_cast<size_t>(I.nnz)) * sizeof(int);
   if (shm > 160 * 1024) throw Error(5, "block-ILU(0) superelement vector larger than the LDS");
   auto launch = [&](auto kern) {
     if (shm > 64 * 1024)

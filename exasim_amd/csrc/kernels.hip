@@ -118,7 +118,13 @@ __device__ inline int mfma_row(float, int lk, int i) { return 4 * lk + i; }
 // that share an operand run together on one XCD and reuse it from its L2.
 constexpr int NXCD = 8;
 
-template <typename CT>
+// Operand storage: the double pointer, or the float one when the operand is stored in FP32.
+template <typename T> __device__ __forceinline__ const T* gsel(const double* d, const float* f);
+template <> __device__ __forceinline__ const double* gsel<double>(const double* d, const float*) { return d; }
+template <> __device__ __forceinline__ const float* gsel<float>(const double*, const float* f) { return f; }
+
+// CT: MFMA type; TA, TB, TC: storage of A, B, C
+template <typename CT, typename TA, typename TB, typename TC>
 __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tilesN, long ntiles) {
   typedef CT acc4 __attribute__((ext_vector_type(4)));
   const long nwg = static_cast<long>(gridDim.x);   // a multiple of NXCD
@@ -133,9 +139,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tiles
   const int b1 = static_cast<int>(t1 % g.nb1);
   const int b2 = static_cast<int>(t1 / g.nb1);
   const long a0 = g.iA ? g.iA[b0] : b0, bb0 = g.iB ? g.iB[b0] : b0, c0 = g.iC ? g.iC[b0] : b0;
-  const double* A = g.A + g.sA0 * a0 + g.sA1 * b1 + g.sA2 * b2;
-  const double* B = g.B + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
-  double* C = g.C + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
+  const TA* A = gsel<TA>(g.A, g.Af) + g.sA0 * a0 + g.sA1 * b1 + g.sA2 * b2;
+  const TB* B = gsel<TB>(g.B, g.Bf) + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
+  TC* C = const_cast<TC*>(gsel<TC>(g.C, g.Cf)) + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
   const int row0 = tile * MG_BM, col0 = tcol * MG_BN;
   __shared__ CT As[MG_BM * MG_BK];
   __shared__ CT Bs[MG_BN * MG_BK];
@@ -145,7 +151,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tiles
   const int ar = tid & 63, ak = tid >> 6;
   const int bk = tid & 31, bc = tid >> 5;
   constexpr int NL = MG_BM * MG_BK / 256;
-  double ra[NL], rb[NL];
+  TA ra[NL];
+  TB rb[NL];
   bool ma[NL], mb[NL];
   // out-of-range elements read a clamped (valid) address and are zeroed when stored to
   // LDS, so the loads need no branches and stay in flight through the MFMA loop
@@ -172,8 +179,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tiles
   for (int k0 = 0; k0 < g.K; k0 += MG_BK) {
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-      As[mg_off(ar, ak + 4 * j)] = static_cast<CT>(ma[j] ? ra[j] : 0.0);
-      Bs[mg_off(bc + 8 * j, bk)] = static_cast<CT>(mb[j] ? rb[j] : 0.0);
+      As[mg_off(ar, ak + 4 * j)] = ma[j] ? static_cast<CT>(ra[j]) : CT(0);
+      Bs[mg_off(bc + 8 * j, bk)] = mb[j] ? static_cast<CT>(rb[j]) : CT(0);
     }
     __syncthreads();
     load(k0 + MG_BK);  // unconditional: past the end it reads clamped, unused values
@@ -199,8 +206,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tiles
         const int c = col0 + 32 * wc + 16 * tj + li;
         if (r < g.M && c < g.N) {
           double v = g.alpha * static_cast<double>(acc[ti][tj][i]);
-          if (g.beta != 0.0) v = fma(g.beta, C[r + g.ldc * static_cast<long>(c)], v);
-          C[r + g.ldc * static_cast<long>(c)] = v;
+          if (g.beta != 0.0) v = fma(g.beta, static_cast<double>(C[r + g.ldc * static_cast<long>(c)]), v);
+          C[r + g.ldc * static_cast<long>(c)] = static_cast<TC>(v);
         }
       }
 }
@@ -223,13 +230,14 @@ __device__ inline void argmax_wave(double& v, int& i) {
   }
 }
 
-template <int BLU_NB>
-__global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols, long stride, int k0,
+template <typename T, int BLU_NB>
+__global__ __launch_bounds__(256) void k_lu_panel(T* Aall, int n, int ncols, long stride, int k0,
                                                   int* info, int* pivout) {
-  double* A = Aall + stride * blockIdx.x;
+  T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
   const int rows = n - k0;
-  extern __shared__ double P[];      // rows x nb, ld rows
+  extern __shared__ double lu_sm[];
+  T* P = reinterpret_cast<T*>(lu_sm);  // rows x nb, ld rows
   __shared__ double wv[4];
   __shared__ int wi[4];
   __shared__ int piv[BLU_NB];
@@ -260,14 +268,14 @@ __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols
     __syncthreads();
     const int p = piv[kk];
     if (p != kk && tid < nb) {
-      double t = P[kk + rows * tid];
+      T t = P[kk + rows * tid];
       P[kk + rows * tid] = P[p + rows * tid];
       P[p + rows * tid] = t;
     }
     __syncthreads();
-    const double pv = P[kk + rows * kk];
+    const T pv = P[kk + rows * kk];
     if (pv == 0.0) singular = 1;
-    const double inv = 1.0 / pv;
+    const T inv = T(1) / pv;
     for (int r = kk + 1 + tid; r < rows; r += 256) P[r + rows * kk] *= inv;
     __syncthreads();
     const int rr = rows - kk - 1, cc = nb - kk - 1;
@@ -283,8 +291,8 @@ __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols
   }
   // columns right of the panel: row swaps, then U12 = L11^{-1} A12
   for (int j = k0 + nb + tid; j < ncols; j += 256) {
-    double* colj = A + static_cast<long>(n) * j + k0;
-    double x[BLU_NB];
+    T* colj = A + static_cast<long>(n) * j + k0;
+    T x[BLU_NB];
 #pragma unroll
     for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
     // pivots are applied in order; p < BLU_NB means both rows are in the register
@@ -298,12 +306,12 @@ __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols
 #pragma unroll
             for (int q = kk + 1; q < BLU_NB; ++q)
               if (q == p) {
-                const double t = x[q];
+                const T t = x[q];
                 x[q] = x[kk];
                 x[kk] = t;
               }
           } else {
-            const double t = colj[p];
+            const T t = colj[p];
             colj[p] = x[kk];
             x[kk] = t;
           }
@@ -332,19 +340,19 @@ __global__ __launch_bounds__(256) void k_lu_panel(double* Aall, int n, int ncols
 // The operations on every entry are those of k_lu_panel, so the factors are the same.  The
 // columns right of the panel are left to k_lu_u12; the pivots go to piv_ws[blockIdx.x][nb]
 // (and to pivout as panel-relative rows).
-template <int BLU_NB>
-__global__ __launch_bounds__(256) void k_lu_panel_reg(double* Aall, int n, long stride, int k0, int* info,
+template <typename T, int BLU_NB>
+__global__ __launch_bounds__(256) void k_lu_panel_reg(T* Aall, int n, long stride, int k0, int* info,
                                                       int* pivout, int* piv_ws) {
-  double* A = Aall + stride * blockIdx.x;
+  T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
   const int rows = n - k0;  // <= 256
   __shared__ double wv[2][4];
   __shared__ int wi[2][4];
-  __shared__ double prow[2][BLU_NB];
-  __shared__ double xrow[2][BLU_NB];
+  __shared__ T prow[2][BLU_NB];
+  __shared__ T xrow[2][BLU_NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool mine = tid < rows;
-  double a[BLU_NB];
+  T a[BLU_NB];
 #pragma unroll
   for (int c = 0; c < BLU_NB; ++c)
     a[c] = (mine && c < nb) ? A[k0 + tid + static_cast<long>(n) * (k0 + c)] : 0.0;
@@ -384,10 +392,10 @@ __global__ __launch_bounds__(256) void k_lu_panel_reg(double* Aall, int n, long 
 #pragma unroll
         for (int c = 0; c < BLU_NB; ++c) a[c] = xrow[b][c];
       }
-      const double pv = prow[b][kk];
+      const T pv = prow[b][kk];
       if (pv == 0.0) singular = 1;
       if (mine && tid > kk) {
-        const double l = a[kk] * (1.0 / pv);
+        const T l = a[kk] * (T(1) / pv);
         a[kk] = l;
 #pragma unroll
         for (int c = kk + 1; c < BLU_NB; ++c)
@@ -404,14 +412,14 @@ __global__ __launch_bounds__(256) void k_lu_panel_reg(double* Aall, int n, long 
 
 // Columns right of a panel factored by k_lu_panel_reg: the panel's row exchanges, then
 // U12 = L11^{-1} A12 (one column per thread; blockIdx.y selects a chunk of 256 columns).
-template <int BLU_NB>
-__global__ __launch_bounds__(256) void k_lu_u12(double* Aall, int n, int ncols, long stride, int k0,
+template <typename T, int BLU_NB>
+__global__ __launch_bounds__(256) void k_lu_u12(T* Aall, int n, int ncols, long stride, int k0,
                                                 const int* piv_ws) {
-  double* A = Aall + stride * blockIdx.x;
+  T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
   // L11 with a run-time leading dimension: constant LDS addresses in the unrolled solve let the
   // compiler hoist every load into registers (and spill)
-  __shared__ double L11[BLU_NB * BLU_NB];
+  __shared__ T L11[BLU_NB * BLU_NB];
   __shared__ int piv[BLU_NB];
   const int ldl = nb;
   const int tid = threadIdx.x;
@@ -423,8 +431,8 @@ __global__ __launch_bounds__(256) void k_lu_u12(double* Aall, int n, int ncols, 
   __syncthreads();
   const int j = k0 + nb + blockIdx.y * 256 + tid;
   if (j >= ncols) return;
-  double* colj = A + static_cast<long>(n) * j + k0;
-  double x[BLU_NB];
+  T* colj = A + static_cast<long>(n) * j + k0;
+  T x[BLU_NB];
 #pragma unroll
   for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
   // pivots in order; p < BLU_NB means both rows are in the register block
@@ -437,12 +445,12 @@ __global__ __launch_bounds__(256) void k_lu_u12(double* Aall, int n, int ncols, 
 #pragma unroll
           for (int q = kk + 1; q < BLU_NB; ++q)
             if (q == p) {
-              const double t = x[q];
+              const T t = x[q];
               x[q] = x[kk];
               x[kk] = t;
             }
         } else {
-          const double t = colj[p];
+          const T t = colj[p];
           colj[p] = x[kk];
           x[kk] = t;
         }
@@ -460,25 +468,25 @@ __global__ __launch_bounds__(256) void k_lu_u12(double* Aall, int n, int ncols, 
 }
 
 // X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
-template <int BLU_NB>
-__global__ __launch_bounds__(256) void k_lu_back(double* Aall, int n, int ncols, long stride, int k0) {
-  double* A = Aall + stride * blockIdx.x;
+template <typename T, int BLU_NB>
+__global__ __launch_bounds__(256) void k_lu_back(T* Aall, int n, int ncols, long stride, int k0) {
+  T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
-  __shared__ double U[BLU_NB][BLU_NB + 1];
-  __shared__ double dinv[BLU_NB];
+  __shared__ T U[BLU_NB][BLU_NB + 1];
+  __shared__ T dinv[BLU_NB];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < nb * nb; idx += 256) {
     int r = idx % nb, c = idx / nb;
     U[r][c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
   }
   __syncthreads();
-  if (tid < nb) dinv[tid] = 1.0 / U[tid][tid];
+  if (tid < nb) dinv[tid] = T(1) / U[tid][tid];
   __syncthreads();
   // one right-hand-side column per thread (blockIdx.y selects a chunk of 256)
   const int j = n + blockIdx.y * 256 + tid;
   if (j >= ncols) return;
-  double* colj = A + static_cast<long>(n) * j + k0;
-  double x[BLU_NB];
+  T* colj = A + static_cast<long>(n) * j + k0;
+  T x[BLU_NB];
 #pragma unroll
   for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
 #pragma unroll
@@ -1301,15 +1309,15 @@ size_t face_jac_elem_lds(const Dims& d) {
 }
 
 // After the solve: DinvF = -X_F, Ru = X_Ru (uequation.cpp:700-711).
-__global__ void k_post_schur(Dims d, int e0, int eb, const double* aug, long augs, double* DinvF,
-                             double* RuS) {
+template <typename TA, typename TD>
+__global__ void k_post_schur(Dims d, int e0, int eb, const TA* aug, long augs, TD* DinvF, double* RuS) {
   const int n = d.n, m = d.m;
   const long per = static_cast<long>(n) * (m + 1);
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= per * eb) return;
   const long e = q / per;
   const long r = q % per;
-  const double v = aug[e * augs + static_cast<long>(n) * n + r];
+  const TA v = aug[e * augs + static_cast<long>(n) * n + r];
   const long E = e0 + e;
   if (r < static_cast<long>(n) * m) DinvF[E * n * static_cast<long>(m) + r] = -v;
   else RuS[E * n + (r - static_cast<long>(n) * m)] = v;
@@ -1849,7 +1857,8 @@ __global__ void k_u_insert(Dims d, const double* src, double* udg) {
 // ---------------------------------------------------------------------------
 // Mixed precision: stored FP32 factors and the condensed RHS of a new residual
 // ---------------------------------------------------------------------------
-__global__ void k_to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride,
+template <typename S>
+__global__ void k_to_float(float* dst, const S* src, int rows, int cols, long ld, long sstride,
                            long dstride, int batch) {
   const long per = static_cast<long>(rows) * cols;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -2070,7 +2079,8 @@ void gemm(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
   const long batch = static_cast<long>(g.nb0) * g.nb1 * g.nb2;
   if (batch <= 0) return;
-  if (g.M <= 32 && g.N <= 32) {
+  const int store = (g.Af ? 1 : 0) | (g.Bf ? 2 : 0) | (g.Cf ? 4 : 0);
+  if (g.M <= 32 && g.N <= 32 && store == 0) {
     int tm = (g.M + 31) / 32, tn = (g.N + 31) / 32;
     dim3 grid(static_cast<unsigned>(tm * batch), tn);
     k_gemm<32, 32, 16, 2, 2><<<grid, 256, 0, s>>>(g, tm);
@@ -2079,33 +2089,63 @@ void gemm(const Gemm& g, hipStream_t s) {
     const long ntiles = static_cast<long>(tm) * tn * batch;
     const long nwg = (ntiles + NXCD - 1) / NXCD * NXCD;
     if (nwg > 0x7fffffffL) throw Error(5, "GEMM batch too large for one launch");
-    if (g.fp32) k_gemm_mfma<float><<<static_cast<unsigned>(nwg), 256, 0, s>>>(g, tm, tn, ntiles);
-    else k_gemm_mfma<double><<<static_cast<unsigned>(nwg), 256, 0, s>>>(g, tm, tn, ntiles);
+    const unsigned grid = static_cast<unsigned>(nwg);
+    switch (store) {
+      case 0:
+        if (g.fp32) k_gemm_mfma<float, double, double, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
+        else k_gemm_mfma<double, double, double, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
+        break;
+      case 2:  // B in FP32
+        if (g.fp32) k_gemm_mfma<float, double, float, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
+        else k_gemm_mfma<double, double, float, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
+        break;
+      case 7:  // all in FP32
+        if (g.fp32) k_gemm_mfma<float, float, float, float><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
+        else k_gemm_mfma<double, float, float, float><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
+        break;
+      default:
+        throw Error(5, "GEMM operand storage combination not built");
+    }
   }
   EXA_HIP(hipGetLastError());
 }
 
-template <int NB>
-void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, int* pivout, hipStream_t s,
+// operands of a GEMM inside the LU: FP64 storage and products, or FP32 storage and products
+inline void lu_gemm_ptrs(Gemm& g, const double* A, const double* B, double* C) {
+  g.A = A;
+  g.B = B;
+  g.C = C;
+}
+inline void lu_gemm_ptrs(Gemm& g, const float* A, const float* B, float* C) {
+  g.Af = A;
+  g.Bf = B;
+  g.Cf = C;
+  g.fp32 = 1;
+}
+
+template <typename T, int NB>
+void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* pivout, hipStream_t s,
                 int* piv_ws) {
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = std::min(NB, n - k0);
     if (piv_ws && n - k0 <= 256) {
-      k_lu_panel_reg<NB><<<batch, 256, 0, s>>>(A, n, stride, k0, info, pivout, piv_ws);
+      k_lu_panel_reg<T, NB><<<batch, 256, 0, s>>>(A, n, stride, k0, info, pivout, piv_ws);
       const int right = ncols - k0 - nb;
       if (right > 0)
-        k_lu_u12<NB><<<dim3(batch, (right + 255) / 256), 256, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
+        k_lu_u12<T, NB><<<dim3(batch, (right + 255) / 256), 256, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
     } else {
-      size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(double);
-      k_lu_panel<NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info, pivout);
+      size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(T);
+      k_lu_panel<T, NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info, pivout);
     }
     EXA_HIP(hipGetLastError());
     if (k0 + nb < n) {  // A22 -= L21 * U12
       Gemm g{};
       g.M = n - k0 - nb; g.N = ncols - k0 - nb; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
-      g.A = A + (k0 + nb) + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
-      g.B = A + k0 + static_cast<long>(n) * (k0 + nb); g.ldb = n; g.sB0 = stride;
-      g.C = A + (k0 + nb) + static_cast<long>(n) * (k0 + nb); g.ldc = n; g.sC0 = stride;
+      lu_gemm_ptrs(g, A + (k0 + nb) + static_cast<long>(n) * k0, A + k0 + static_cast<long>(n) * (k0 + nb),
+                   A + (k0 + nb) + static_cast<long>(n) * (k0 + nb));
+      g.lda = n; g.sA0 = stride;
+      g.ldb = n; g.sB0 = stride;
+      g.ldc = n; g.sC0 = stride;
       g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
       gemm(g, s);
     }
@@ -2113,34 +2153,41 @@ void blocked_lu(double* A, int n, int ncols, long stride, int batch, int* info, 
   const int last = ((n - 1) / NB) * NB;
   for (int k0 = last; k0 >= 0; k0 -= NB) {
     dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 255) / 256));
-    k_lu_back<NB><<<grid, 256, 0, s>>>(A, n, ncols, stride, k0);
+    k_lu_back<T, NB><<<grid, 256, 0, s>>>(A, n, ncols, stride, k0);
     EXA_HIP(hipGetLastError());
     if (k0 > 0) {  // X[0:k0] -= U[0:k0, k0:k0+nb] * X[k0:k0+nb]
       const int nb = std::min(NB, n - k0);
       Gemm g{};
       g.M = k0; g.N = ncols - n; g.K = nb; g.alpha = -1.0; g.beta = 1.0;
-      g.A = A + static_cast<long>(n) * k0; g.lda = n; g.sA0 = stride;
-      g.B = A + k0 + static_cast<long>(n) * n; g.ldb = n; g.sB0 = stride;
-      g.C = A + static_cast<long>(n) * n; g.ldc = n; g.sC0 = stride;
+      lu_gemm_ptrs(g, A + static_cast<long>(n) * k0, A + k0 + static_cast<long>(n) * n, A + static_cast<long>(n) * n);
+      g.lda = n; g.sA0 = stride;
+      g.ldb = n; g.sB0 = stride;
+      g.ldc = n; g.sC0 = stride;
       g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
       gemm(g, s);
     }
   }
 }
 
+template <typename T>
+void lu_blocked_any(T* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
+                    int* piv_ws) {
+  // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide; with a pivot
+  // workspace, panels of up to 256 rows are factored in registers (64-wide register panels
+  // were measured: no gain at n = 250, slower at n = 100)
+  if (lu_panel_width(n) == 32)
+    blocked_lu<T, 32>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
+  else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000)
+    blocked_lu<T, 16>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
+  else
+    throw Error(5, "dense block larger than 500 rows");
+}
+
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
               int* piv_ws) {
   if (batch <= 0) return;
   if ((n >= BLU_MIN_N || pivout) && ncols > n) {
-    // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide; with a pivot
-    // workspace, panels of up to 256 rows are factored in registers (64-wide register panels
-    // were measured: no gain at n = 250, slower at n = 100)
-    if (lu_panel_width(n) == 32)
-      blocked_lu<32>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
-    else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000)
-      blocked_lu<16>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
-    else
-      throw Error(5, "dense block larger than 500 rows");
+    lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws);
     return;
   }
   if (pivout) throw Error(5, "pivot output needs right-hand-side columns");
@@ -2149,6 +2196,13 @@ void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hi
   size_t shm = static_cast<size_t>(n) * LU_NB * sizeof(double);
   k_lu_solve<<<batch, 256, shm, s>>>(A, n, ncols, stride, info);
   EXA_HIP(hipGetLastError());
+}
+
+void lu_solve(float* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
+              int* piv_ws) {
+  if (batch <= 0) return;
+  if (ncols <= n) throw Error(5, "FP32 LU needs right-hand-side columns");
+  lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws);
 }
 
 void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s) {
@@ -2247,7 +2301,14 @@ void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, doubl
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s) {
   long n1 = static_cast<long>(d.n) * (d.m + 1) * eb;
-  k_post_schur<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
+  k_post_schur<double, double><<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
+  EXA_HIP(hipGetLastError());
+}
+
+void post_schur(const Dims& d, int e0, int eb, const float* aug, long augs, float* DinvF, double* RuS,
+                hipStream_t s) {
+  long n1 = static_cast<long>(d.n) * (d.m + 1) * eb;
+  k_post_schur<float, float><<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2267,7 +2328,15 @@ void to_float(float* dst, const double* src, int rows, int cols, long ld, long s
               hipStream_t s) {
   long n1 = static_cast<long>(rows) * cols * batch;
   if (n1 == 0) return;
-  k_to_float<<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
+  k_to_float<double><<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
+  EXA_HIP(hipGetLastError());
+}
+
+void to_float(float* dst, const float* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
+              hipStream_t s) {
+  long n1 = static_cast<long>(rows) * cols * batch;
+  if (n1 == 0) return;
+  k_to_float<float><<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
   EXA_HIP(hipGetLastError());
 }
 

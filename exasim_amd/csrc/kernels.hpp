@@ -31,6 +31,10 @@ struct Gemm {
   const int* iB = nullptr;
   const int* iC = nullptr;
   int fp32 = 0;  // 1: operands rounded to FP32, FP32 MFMA products and accumulation
+  // operands stored in FP32 (set instead of A / B / C; combinations built: B only, all three)
+  const float* Af = nullptr;
+  const float* Bf = nullptr;
+  float* Cf = nullptr;
 };
 void gemm(const Gemm& g, hipStream_t s);
 
@@ -41,6 +45,9 @@ void gemm(const Gemm& g, hipStream_t s);
 // pivout (optional, [n, batch]): the panel-relative row exchanges of the blocked
 // factorisation, for refine_rhs
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s,
+              int* pivout = nullptr, int* piv_ws = nullptr);
+// The same in FP32 storage and arithmetic (blocked path only: ncols > n)
+void lu_solve(float* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s,
               int* pivout = nullptr, int* piv_ws = nullptr);
 // piv_ws (optional, device, >= 32*batch ints, owned by the caller): with it, panels of up to
 // 256 rows are factored with the panel in registers (k_lu_panel_reg + k_lu_u12)
@@ -96,10 +103,14 @@ void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, doubl
               hipStream_t s);
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s);
-// dst_b[r + rows*c] = (float) src_b[r + ld*c]
+void post_schur(const Dims& d, int e0, int eb, const float* aug, long augs, float* DinvF, double* RuS,
+                hipStream_t s);
 // BF16 copy (round to nearest even), stored as unsigned short (the upper half of an FP32)
 void to_bf16(unsigned short* dst, const double* src, long count, hipStream_t s);
+// dst_b[r + rows*c] = (float) src_b[r + ld*c]
 void to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
+              hipStream_t s);
+void to_float(float* dst, const float* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
               hipStream_t s);
 // Condensed right-hand side of a new residual with stored factors (mixed precision):
 // RuS_e = D'^{-1} Ru_e from the FP32 LU factors and pivots of D', Rh_e -= K'_e RuS_e.

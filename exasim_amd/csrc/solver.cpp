@@ -408,6 +408,7 @@ void Solver::alloc_mixed() {
   LU32.alloc(n * n * ne);
   piv32.alloc(n * ne);
   K32.alloc(m * n * ne);
+  aug32.alloc(static_cast<size_t>(neb) * n * augcols());
   if (precond == 0) BE32.alloc(ncf2 * d.nf);
   if (precond == 1) BEeas32.alloc(m * m * ne);
   if (precond == 2) {
@@ -657,6 +658,7 @@ void Solver::assemble() {
   const long augs = n * augcols();
   const long blk = static_cast<long>(npe) * d.nge;
   if (precision) alloc_mixed();
+  dinvf_fp32 = precision != 0;
   hipEvent_t t0{};
   for (int e0 = 0; e0 < d.ne; e0 += neb) {
     const int eb = std::min(neb, d.ne - e0);
@@ -723,36 +725,46 @@ void Solver::assemble() {
       gemm(k, s);
     }
     toc(T_ASSEMBLY, t0);
-    // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru]
+    // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru].  The mixed form, whose operator,
+    // factors and u-recovery matrices are all FP32, factors an FP32 copy of the block (same
+    // blocked algorithm and pivots, half the bytes per pass)
     tic(T_LU, &t0);
-    lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s,
-             precision ? piv32.p + static_cast<long>(e0) * n : nullptr, lu_ws(eb));
+    if (precision) {
+      to_float(aug32.p, aug.p, static_cast<int>(n), static_cast<int>(augcols()), n, augs, augs, eb, s);
+      lu_solve(aug32.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s,
+               piv32.p + static_cast<long>(e0) * n, lu_ws(eb));
+    } else {
+      lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s, nullptr, lu_ws(eb));
+    }
     toc(T_LU, t0);
     tic(T_ASSEMBLY, &t0);
     {  // H += K * DinvF (= -K X_F), Rh -= K * (D^{-1} Ru)
       Gemm g{};
       g.M = m; g.N = m; g.K = n; g.alpha = -1.0; g.beta = 1.0;
       g.A = Kt.p; g.lda = m; g.sA0 = m * n;
-      g.B = aug.p + n * n; g.ldb = n; g.sB0 = augs;
+      if (precision) g.Bf = aug32.p + n * n;
+      else g.B = aug.p + n * n;
+      g.ldb = n; g.sB0 = augs;
       g.C = H.p + e0 * m * m; g.ldc = m; g.sC0 = m * m;
       g.nb0 = eb; g.nb1 = 1; g.nb2 = 1;
       g.fp32 = precision;
       gemm(g, s);
       g.N = 1;
       g.fp32 = 0;
-      g.B = aug.p + n * (n + m);
+      if (precision) g.Bf = aug32.p + n * (n + m);
+      else g.B = aug.p + n * (n + m);
       g.C = Rh.p + e0 * m; g.sC0 = m;
       gemm(g, s);
     }
-    post_schur(d, e0, eb, aug.p, augs, DinvF.p, RuS.p, s);
-    if (precision) {  // FP32 copies of this block's H, DinvF, LU factors of D' and K'
+    if (precision) {  // DinvF straight into FP32; FP32 copies of this block's H, LU factors of D' and K'
       const long E0 = e0;
+      post_schur(d, e0, eb, aug32.p, augs, DinvF32.p, RuS.p, s);
       to_float(H32.p + E0 * m * m, H.p + E0 * m * m, static_cast<int>(m), static_cast<int>(m), m, m * m, m * m,
                eb, s);
-      to_float(DinvF32.p + E0 * n * m, DinvF.p + E0 * n * m, static_cast<int>(n), static_cast<int>(m), n, n * m,
-               n * m, eb, s);
-      to_float(LU32.p + E0 * n * n, aug.p, static_cast<int>(n), static_cast<int>(n), n, augs, n * n, eb, s);
+      to_float(LU32.p + E0 * n * n, aug32.p, static_cast<int>(n), static_cast<int>(n), n, augs, n * n, eb, s);
       to_float(K32.p + E0 * m * n, Kt.p, static_cast<int>(m), static_cast<int>(n), m, m * n, m * n, eb, s);
+    } else {
+      post_schur(d, e0, eb, aug.p, augs, DinvF.p, RuS.p, s);
     }
     toc(T_ASSEMBLY, t0);
     EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
@@ -1097,7 +1109,13 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
   if (name == "udg") { src = udg.p; size = static_cast<long>(udg.n); }
   else if (name == "uh") { src = uh.p; size = static_cast<long>(uh.n); }
   else if (name == "H") { src = H.p; size = static_cast<long>(H.n); }
-  else if (name == "DinvF") { src = DinvF.p; size = static_cast<long>(DinvF.n); }
+  else if (name == "DinvF" && dinvf_fp32) {  // the mixed form keeps D^{-1}F in FP32 only
+    std::vector<float> f(DinvF32.n);
+    EXA_HIP(hipMemcpyAsync(f.data(), DinvF32.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+    host.assign(f.begin(), f.end());
+    size = static_cast<long>(host.size());
+  } else if (name == "DinvF") { src = DinvF.p; size = static_cast<long>(DinvF.n); }
   else if (name == "Ru") { src = RuS.p; size = static_cast<long>(RuS.n); }
   else if (name == "Rh") { src = Rh.p; size = static_cast<long>(Rh.n); }
   else if (name == "b") { src = b.p; size = static_cast<long>(b.n); }

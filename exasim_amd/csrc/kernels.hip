@@ -973,7 +973,8 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
 }
 
 // Ru[i + npe*k] = sum_dp sum_g shapegw[i, g, dp] * rv  (Gauss2Node with shapegw)
-__global__ void k_vol_res(Dims d, AsmArgs a, double* out, long ldo) {
+template <typename T>
+__global__ void k_vol_res(Dims d, AsmArgs a, T* out, long ldo) {
   const int npe = d.npe, nge = d.nge, ncu = d.ncu, nd = d.nd;
   const long per = static_cast<long>(npe) * ncu;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -988,11 +989,12 @@ __global__ void k_vol_res(Dims d, AsmArgs a, double* out, long ldo) {
     const double* w = a.shapegw + static_cast<long>(npe) * nge * dp;
     for (int g = 0; g < nge; ++g) s += w[i + npe * g] * rv[g];
   }
-  out[e * ldo + r] = s;
+  out[e * ldo + r] = static_cast<T>(s);
 }
 
 // W[(e*NC + c)*NCU + k][i + npe*g] = sum_dp shapegw[i, g, dp] * rJ[dp, k, c]
-__global__ void k_vol_W(Dims d, AsmArgs a) {
+template <typename T>
+__global__ void k_vol_W(Dims d, AsmArgs a, T* W) {
   const int npe = d.npe, nge = d.nge, ncu = d.ncu, nc = d.nc, nd = d.nd;
   const long blk = static_cast<long>(npe) * nge;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1008,7 +1010,7 @@ __global__ void k_vol_W(Dims d, AsmArgs a) {
   double s = 0.0;
   for (int dp = 0; dp <= nd; ++dp)
     s += a.shapegw[i + npe * (g + nge * dp)] * a.rJ[((dp * ncu + k) * nc + c) * P + e * nge + g];
-  a.W[q] = s;
+  W[q] = static_cast<T>(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1138,7 +1140,8 @@ __global__ __launch_bounds__(64) void k_face_bou_points(Dims d, AsmArgs a) {
 
 // Residual parts of face l: Ru[perm[a,l] + npe*k] -= int phi_a fhI; Rh = -int phi_a fhB
 // (assembleRu, kokkosimpl.h:1650-1659; uequation.cpp:426-427).
-__global__ void k_face_res(Dims d, AsmArgs a, int l, int nf1, double* ru, long ldru, double* Rh) {
+template <typename T>
+__global__ void k_face_res(Dims d, AsmArgs a, int l, int nf1, T* ru, long ldru, double* Rh) {
   const int npf = d.npf, ngf = d.ngf, ncu = d.ncu;
   const long per = static_cast<long>(npf) * ncu;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1154,15 +1157,16 @@ __global__ void k_face_res(Dims d, AsmArgs a, int l, int nf1, double* ru, long l
     sI += w * a.fpt[k * P + pb + g];
     sB += w * a.fpt[(nf1 + k) * P + pb + g];
   }
-  ru[e * ldru + a.perm[aa + npf * l] + d.npe * k] -= sI;
+  T& dst = ru[e * ldru + a.perm[aa + npf * l] + d.npe * k];
+  dst = static_cast<T>(static_cast<double>(dst) - sI);
   Rh[(a.e0 + e) * d.m + k + ncu * (aa + npf * l)] = -sB;
 }
 
 // Jacobian parts of face l: D/B += int phi_a phi_b fhI_uq, F += int phi_a phi_b fhI_uh,
 // K/G = int phi_a phi_b fhB_uq, H = int phi_a phi_b fhB_uh
 // (assembleMatrixBD/F/GK/H, kokkosimpl.h:1661-1730).
-__global__ void k_face_jac(Dims d, AsmArgs a, int l, double* aug, double* Bfull, double* Kt,
-                           double* Gfull, double* H) {
+template <typename T>
+__global__ void k_face_jac(Dims d, AsmArgs a, int l, T* aug, T* Bfull, T* Kt, T* Gfull, double* H) {
   const int npf = d.npf, ngf = d.ngf, ncu = d.ncu, nc = d.nc, npe = d.npe, nd = d.nd;
   const int n = d.n, m = d.m;
   const int ncol = nc + ncu;
@@ -1189,16 +1193,16 @@ __global__ void k_face_jac(Dims d, AsmArgs a, int l, double* aug, double* Bfull,
   const long augs = a.augs;
   const long trow = k + ncu * (aa + npf * l);
   if (c < ncu) {
-    aug[e * augs + pa + npe * k + static_cast<long>(n) * (pbn + npe * c)] += sI;
-    Kt[e * m * static_cast<long>(n) + trow + static_cast<long>(m) * (pbn + npe * c)] += sB;
+    aug[e * augs + pa + npe * k + static_cast<long>(n) * (pbn + npe * c)] += static_cast<T>(sI);
+    Kt[e * m * static_cast<long>(n) + trow + static_cast<long>(m) * (pbn + npe * c)] += static_cast<T>(sB);
   } else if (c < nc) {
     const int cq = c - ncu, cp = cq % ncu, dd = cq / ncu;
     const long col = pbn + npe * dd + static_cast<long>(npe) * nd * cp;
-    Bfull[e * n * static_cast<long>(n) * nd + pa + npe * k + n * col] += sI;
-    Gfull[e * m * static_cast<long>(n) * nd + trow + m * col] += sB;
+    Bfull[e * n * static_cast<long>(n) * nd + pa + npe * k + n * col] += static_cast<T>(sI);
+    Gfull[e * m * static_cast<long>(n) * nd + trow + m * col] += static_cast<T>(sB);
   } else {
     const int c2 = c - nc;
-    aug[e * augs + pa + npe * k + static_cast<long>(n) * (n + c2 + ncu * (bb + npf * l))] += sI;
+    aug[e * augs + pa + npe * k + static_cast<long>(n) * (n + c2 + ncu * (bb + npf * l))] += static_cast<T>(sI);
     H[(a.e0 + e) * static_cast<long>(m) * m + trow + static_cast<long>(m) * (c2 + ncu * (bb + npf * l))] = sB;
   }
 }
@@ -1215,8 +1219,9 @@ __global__ void k_face_jac(Dims d, AsmArgs a, int l, double* aug, double* Bfull,
 // share (D and B at the nodes of a common edge) are updated face by face, in that order.
 constexpr int FJ_THREADS = 512;
 
-__global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a, double* aug, double* Bfull,
-                                                              double* Kt, double* Gfull, double* H) {
+template <typename T>
+__global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a, T* aug, T* Bfull, T* Kt, T* Gfull,
+                                                              double* H) {
   extern __shared__ double fj_sm[];
   const int npf = d.npf, ngf = d.ngf, ncu = d.ncu, nc = d.nc, npe = d.npe, nd = d.nd, nfe = d.nfe;
   const int n = d.n, m = d.m;
@@ -1238,10 +1243,10 @@ __global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a,
   const long P = static_cast<long>(a.eb) * nfe * ngf;
   const int npair = npf * npf, ptiles = (npair + 15) / 16, ftiles = nfp / 16;
   const long augs = a.augs;
-  double* augE = aug + e * augs;
-  double* BE = Bfull + e * static_cast<long>(n) * n * nd;
-  double* KE = Kt + e * static_cast<long>(m) * n;
-  double* GE = Gfull + e * static_cast<long>(m) * n * nd;
+  T* augE = aug + e * augs;
+  T* BE = Bfull + e * static_cast<long>(n) * n * nd;
+  T* KE = Kt + e * static_cast<long>(m) * n;
+  T* GE = Gfull + e * static_cast<long>(m) * n * nd;
   double* HE = H + (a.e0 + e) * static_cast<long>(m) * m;
   for (int l = 0; l < nfe; ++l) {
     const long pb = (e * nfe + l) * ngf;
@@ -1277,22 +1282,23 @@ __global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a,
             const int f = (f0 + t) * 16 + lk + 4 * i;
             if (f >= nfl) continue;
             const double v = acc[t][i];
+            const T vt = static_cast<T>(v);
             const bool bpart = f >= nfi;
             const int fl = bpart ? f - nfi : f;
             const int k = fl % ncu, c = fl / ncu;
             const long trow = k + ncu * (aa + npf * l);
             if (c < ncu) {
-              if (bpart) KE[trow + static_cast<long>(m) * (pbn + npe * c)] += v;
-              else augE[pa + npe * k + static_cast<long>(n) * (pbn + npe * c)] += v;
+              if (bpart) KE[trow + static_cast<long>(m) * (pbn + npe * c)] += vt;
+              else augE[pa + npe * k + static_cast<long>(n) * (pbn + npe * c)] += vt;
             } else if (c < nc) {
               const int cq = c - ncu, cp = cq % ncu, dd = cq / ncu;
               const long col = pbn + npe * dd + static_cast<long>(npe) * nd * cp;
-              if (bpart) GE[trow + m * col] += v;
-              else BE[pa + npe * k + n * col] += v;
+              if (bpart) GE[trow + m * col] += vt;
+              else BE[pa + npe * k + n * col] += vt;
             } else {
               const long col = (c - nc) + ncu * (bb + npf * l);
               if (bpart) HE[trow + static_cast<long>(m) * col] = v;
-              else augE[pa + npe * k + static_cast<long>(n) * (n + col)] += v;
+              else augE[pa + npe * k + static_cast<long>(n) * (n + col)] += vt;
             }
           }
         }
@@ -2091,20 +2097,22 @@ void gemm(const Gemm& g, hipStream_t s) {
     if (nwg > 0x7fffffffL) throw Error(5, "GEMM batch too large for one launch");
     const unsigned grid = static_cast<unsigned>(nwg);
     switch (store) {
-      case 0:
-        if (g.fp32) k_gemm_mfma<float, double, double, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
-        else k_gemm_mfma<double, double, double, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
-        break;
-      case 2:  // B in FP32
-        if (g.fp32) k_gemm_mfma<float, double, float, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
-        else k_gemm_mfma<double, double, float, double><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
-        break;
-      case 7:  // all in FP32
-        if (g.fp32) k_gemm_mfma<float, float, float, float><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
-        else k_gemm_mfma<double, float, float, float><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);
-        break;
+#define EXA_GEMM_CASE(CODE, TA, TB, TC)                                                                   \
+  case CODE:                                                                                              \
+    if (g.fp32) k_gemm_mfma<float, TA, TB, TC><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);                  \
+    else k_gemm_mfma<double, TA, TB, TC><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);                        \
+    break;
+      EXA_GEMM_CASE(0, double, double, double)
+      EXA_GEMM_CASE(1, float, double, double)
+      EXA_GEMM_CASE(2, double, float, double)
+      EXA_GEMM_CASE(3, float, float, double)
+      EXA_GEMM_CASE(4, double, double, float)
+      EXA_GEMM_CASE(5, float, double, float)
+      EXA_GEMM_CASE(6, double, float, float)
+      EXA_GEMM_CASE(7, float, float, float)
+#undef EXA_GEMM_CASE
       default:
-        throw Error(5, "GEMM operand storage combination not built");
+        throw Error(5, "GEMM operand storage combination");
     }
   }
   EXA_HIP(hipGetLastError());
@@ -2261,41 +2269,61 @@ void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemco
 void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s) { points(d, a, jac, true, s); }
 void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s) { points(d, a, jac, false, s); }
 
-void vol_res(const Dims& d, const AsmArgs& a, double* out, long ldo, hipStream_t s) {
+template <typename T>
+void vol_res_t(const Dims& d, const AsmArgs& a, T* out, long ldo, hipStream_t s) {
   long n1 = static_cast<long>(d.npe) * d.ncu * a.eb;
-  k_vol_res<<<nblocks(n1, 256), 256, 0, s>>>(d, a, out, ldo);
+  k_vol_res<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, out, ldo);
   EXA_HIP(hipGetLastError());
 }
+void vol_res(const Dims& d, const AsmArgs& a, double* out, long ldo, hipStream_t s) { vol_res_t(d, a, out, ldo, s); }
+void vol_res(const Dims& d, const AsmArgs& a, float* out, long ldo, hipStream_t s) { vol_res_t(d, a, out, ldo, s); }
 
-void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s) {
+template <typename T>
+void vol_W_t(const Dims& d, const AsmArgs& a, T* W, hipStream_t s) {
   long n1 = static_cast<long>(d.npe) * d.nge * d.ncu * d.nc * a.eb;
-  k_vol_W<<<nblocks(n1, 256), 256, 0, s>>>(d, a);
+  k_vol_W<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
   EXA_HIP(hipGetLastError());
 }
+void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s) { vol_W_t(d, a, a.W, s); }
+void vol_W(const Dims& d, const AsmArgs& a, float* W, hipStream_t s) { vol_W_t(d, a, W, s); }
 
-void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long ldru, double* Rh,
-              hipStream_t s) {
+template <typename T>
+void face_res_t(const Dims& d, const AsmArgs& a, int l, bool jac, T* ru, long ldru, double* Rh, hipStream_t s) {
   int nf1 = jac ? (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) : d.ncu;
   long n1 = static_cast<long>(d.npf) * d.ncu * a.eb;
-  k_face_res<<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, nf1, ru, ldru, Rh);
+  k_face_res<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, nf1, ru, ldru, Rh);
   EXA_HIP(hipGetLastError());
 }
+void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long ldru, double* Rh, hipStream_t s) {
+  face_res_t(d, a, l, jac, ru, ldru, Rh, s);
+}
+void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, float* ru, long ldru, double* Rh, hipStream_t s) {
+  face_res_t(d, a, l, jac, ru, ldru, Rh, s);
+}
 
-void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, double* Kt, double* Gfull, double* H,
-              hipStream_t s) {
+template <typename T>
+void face_jac_t(const Dims& d, const AsmArgs& a, T* aug, T* Bfull, T* Kt, T* Gfull, double* H, hipStream_t s) {
   if (a.eb <= 0) return;
   const size_t lds = face_jac_elem_lds(d);
   if (lds <= 64000) {
-    k_face_jac_elem<<<a.eb, FJ_THREADS, lds, s>>>(d, a, aug, Bfull, Kt, Gfull, H);
+    k_face_jac_elem<T><<<a.eb, FJ_THREADS, lds, s>>>(d, a, aug, Bfull, Kt, Gfull, H);
     EXA_HIP(hipGetLastError());
     return;
   }
   // face tables too large for LDS: one launch per local face (no two writes of a launch collide)
   long n1 = static_cast<long>(d.npf) * d.npf * d.ncu * (d.nc + d.ncu) * a.eb;
   for (int l = 0; l < d.nfe; ++l) {
-    k_face_jac<<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, aug, Bfull, Kt, Gfull, H);
+    k_face_jac<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, l, aug, Bfull, Kt, Gfull, H);
     EXA_HIP(hipGetLastError());
   }
+}
+void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, double* Kt, double* Gfull, double* H,
+              hipStream_t s) {
+  face_jac_t(d, a, aug, Bfull, Kt, Gfull, H, s);
+}
+void face_jac(const Dims& d, const AsmArgs& a, float* aug, float* Bfull, float* Kt, float* Gfull, double* H,
+              hipStream_t s) {
+  face_jac_t(d, a, aug, Bfull, Kt, Gfull, H, s);
 }
 
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,

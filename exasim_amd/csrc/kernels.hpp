@@ -31,7 +31,7 @@ struct Gemm {
   const int* iB = nullptr;
   const int* iC = nullptr;
   int fp32 = 0;  // 1: operands rounded to FP32, FP32 MFMA products and accumulation
-  // operands stored in FP32 (set instead of A / B / C; combinations built: B only, all three)
+  // operands stored in FP32 (set instead of A / B / C, in any combination)
   const float* Af = nullptr;
   const float* Bf = nullptr;
   float* Cf = nullptr;
@@ -95,11 +95,19 @@ void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
 void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
 // Ru[e*ldo + i + npe*k] = sum phiw * rv   (Ru block of element e0+e at out + e*ldo)
 void vol_res(const Dims& d, const AsmArgs& a, double* out, long ldo, hipStream_t s);
+void vol_res(const Dims& d, const AsmArgs& a, float* out, long ldo, hipStream_t s);
+// W into a.W, or into an FP32 buffer of the same layout
 void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s);
+void vol_W(const Dims& d, const AsmArgs& a, float* W, hipStream_t s);
 void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, double* ru, long ldru, double* Rh,
+              hipStream_t s);
+void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, float* ru, long ldru, double* Rh,
               hipStream_t s);
 // Jacobian parts of every face of the block's elements (face by face, in local order)
 void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, double* Kt, double* Gfull, double* H,
+              hipStream_t s);
+// the same into FP32 D/B/F, K and G blocks (H stays FP64)
+void face_jac(const Dims& d, const AsmArgs& a, float* aug, float* Bfull, float* Kt, float* Gfull, double* H,
               hipStream_t s);
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s);

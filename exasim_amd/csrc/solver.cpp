@@ -408,7 +408,11 @@ void Solver::alloc_mixed() {
   LU32.alloc(n * n * ne);
   piv32.alloc(n * ne);
   K32.alloc(m * n * ne);
+  // FP32 element-block workspace of the mixed-form assembly
   aug32.alloc(static_cast<size_t>(neb) * n * augcols());
+  Bfull32.alloc(static_cast<size_t>(neb) * n * n * d.nd);
+  Gfull32.alloc(static_cast<size_t>(neb) * m * n * d.nd);
+  W32.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
   if (precond == 0) BE32.alloc(ncf2 * d.nf);
   if (precond == 1) BEeas32.alloc(m * m * ne);
   if (precond == 2) {
@@ -660,24 +664,47 @@ void Solver::assemble() {
   if (precision) alloc_mixed();
   dinvf_fp32 = precision != 0;
   hipEvent_t t0{};
+  // The mixed form assembles the element blocks (W, [D | F | Ru], B, K, G) in FP32 storage:
+  // its operator, factors and u-recovery matrices are all FP32, and the Schur products and
+  // the LU move half the bytes.  H and Rh stay FP64 (H32 is rounded from H; the
+  // preconditioner is assembled from H).
+  const bool f32 = precision != 0;
+  // operand of a GEMM: element offset `off` into the FP64 or the FP32 buffer
+  auto opA = [&](Gemm& g, double* pd, float* pf, long off) { if (f32) g.Af = pf + off; else g.A = pd + off; };
+  auto opB = [&](Gemm& g, double* pd, float* pf, long off) { if (f32) g.Bf = pf + off; else g.B = pd + off; };
+  auto opC = [&](Gemm& g, double* pd, float* pf, long off) { if (f32) g.Cf = pf + off; else g.C = pd + off; };
   for (int e0 = 0; e0 < d.ne; e0 += neb) {
     const int eb = std::min(neb, d.ne - e0);
     AsmArgs a = asm_args(e0, eb);
+    float* K32b = f32 ? K32.p + static_cast<long>(e0) * m * n : nullptr;  // K of this block, in place
     tic(T_ASSEMBLY, &t0);
-    EXA_HIP(hipMemsetAsync(aug.p, 0, sizeof(double) * augs * eb, s));
-    EXA_HIP(hipMemsetAsync(Kt.p, 0, sizeof(double) * m * n * eb, s));
-    EXA_HIP(hipMemsetAsync(Gfull.p, 0, sizeof(double) * m * n * nd * eb, s));
+    if (f32) {
+      EXA_HIP(hipMemsetAsync(aug32.p, 0, sizeof(float) * augs * eb, s));
+      EXA_HIP(hipMemsetAsync(K32b, 0, sizeof(float) * m * n * eb, s));
+      EXA_HIP(hipMemsetAsync(Gfull32.p, 0, sizeof(float) * m * n * nd * eb, s));
+    } else {
+      EXA_HIP(hipMemsetAsync(aug.p, 0, sizeof(double) * augs * eb, s));
+      EXA_HIP(hipMemsetAsync(Kt.p, 0, sizeof(double) * m * n * eb, s));
+      EXA_HIP(hipMemsetAsync(Gfull.p, 0, sizeof(double) * m * n * nd * eb, s));
+    }
     EXA_HIP(hipMemsetAsync(H.p + e0 * m * m, 0, sizeof(double) * m * m * eb, s));
     // volume terms
     vol_points(d, a, true, s);
-    vol_res(d, a, aug.p + n * (n + m), augs, s);
-    vol_W(d, a, s);
+    if (f32) {
+      vol_res(d, a, aug32.p + n * (n + m), augs, s);
+      vol_W(d, a, W32.p, s);
+    } else {
+      vol_res(d, a, aug.p + n * (n + m), augs, s);
+      vol_W(d, a, s);
+    }
     {  // D block: A_kc = -W_kc * Phi for the u components
       Gemm g{};
       g.M = npe; g.N = npe; g.K = d.nge; g.alpha = -1.0; g.beta = 0.0;
-      g.A = W.p; g.lda = npe; g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
+      opA(g, W.p, W32.p, 0);
+      g.lda = npe; g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
       g.B = shapegt.p; g.ldb = d.nge; g.sB0 = g.sB1 = g.sB2 = 0;
-      g.C = aug.p; g.ldc = n; g.sC0 = npe; g.sC1 = n * npe; g.sC2 = augs;
+      opC(g, aug.p, aug32.p, 0);
+      g.ldc = n; g.sC0 = npe; g.sC1 = n * npe; g.sC2 = augs;
       g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
       g.fp32 = precision;
       gemm(g, s);
@@ -685,65 +712,73 @@ void Solver::assemble() {
     for (int dd = 0; dd < nd; ++dd) {  // B_d blocks for the q components
       Gemm g{};
       g.M = npe; g.N = npe; g.K = d.nge; g.alpha = -1.0; g.beta = 0.0;
-      g.A = W.p + (ncu + ncu * dd) * ncu * blk; g.lda = npe;
-      g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
+      opA(g, W.p, W32.p, (ncu + ncu * dd) * ncu * blk);
+      g.lda = npe; g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
       g.B = shapegt.p; g.ldb = d.nge;
-      g.C = Bfull.p + n * npe * dd; g.ldc = n; g.sC0 = npe; g.sC1 = n * npe * nd; g.sC2 = n * n * nd;
+      opC(g, Bfull.p, Bfull32.p, n * npe * dd);
+      g.ldc = n; g.sC0 = npe; g.sC1 = n * npe * nd; g.sC2 = n * n * nd;
       g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
       g.fp32 = precision;
       gemm(g, s);
     }
     // face terms; residual parts one local face at a time (no two writes of a launch collide)
     face_points(d, a, true, s);
-    for (int l = 0; l < d.nfe; ++l) face_res(d, a, l, true, aug.p + n * (n + m), augs, Rh.p, s);
-    face_jac(d, a, aug.p, Bfull.p, Kt.p, Gfull.p, H.p, s);
+    for (int l = 0; l < d.nfe; ++l) {
+      if (f32) face_res(d, a, l, true, aug32.p + n * (n + m), augs, Rh.p, s);
+      else face_res(d, a, l, true, aug.p + n * (n + m), augs, Rh.p, s);
+    }
+    if (f32) face_jac(d, a, aug32.p, Bfull32.p, K32b, Gfull32.p, H.p, s);
+    else face_jac(d, a, aug.p, Bfull.p, Kt.p, Gfull.p, H.p, s);
     // eliminate q: D += B*MinvC, F -= B*MinvE, K += G*MinvC, H -= G*MinvE (uequation.cpp:614-681)
     {
       Gemm g{};
       g.M = n; g.N = npe; g.K = npe * nd; g.alpha = 1.0; g.beta = 1.0;
-      g.A = Bfull.p; g.lda = n; g.sA0 = n * npe * nd; g.sA1 = n * n * nd;
+      opA(g, Bfull.p, Bfull32.p, 0);
+      g.lda = n; g.sA0 = n * npe * nd; g.sA1 = n * n * nd;
       g.B = MCs.p + e0 * npe * nd * npe; g.ldb = npe * nd; g.sB0 = 0; g.sB1 = npe * nd * npe;
-      g.C = aug.p; g.ldc = n; g.sC0 = n * npe; g.sC1 = augs;
+      opC(g, aug.p, aug32.p, 0);
+      g.ldc = n; g.sC0 = n * npe; g.sC1 = augs;
       g.nb0 = static_cast<int>(ncu); g.nb1 = eb; g.nb2 = 1;
       g.fp32 = precision;
       gemm(g, s);
       g.N = static_cast<int>(ndf); g.alpha = -1.0;
       g.B = MEs.p + e0 * npe * nd * ndf; g.sB1 = npe * nd * ndf;
-      g.C = aug.p + n * n; g.ldc = n * ncu; g.sC0 = n; g.sC1 = augs;
+      opC(g, aug.p, aug32.p, n * n);
+      g.ldc = n * ncu; g.sC0 = n; g.sC1 = augs;
       gemm(g, s);
       Gemm k{};
       k.M = m; k.N = npe; k.K = npe * nd; k.alpha = 1.0; k.beta = 1.0;
-      k.A = Gfull.p; k.lda = m; k.sA0 = m * npe * nd; k.sA1 = m * n * nd;
+      opA(k, Gfull.p, Gfull32.p, 0);
+      k.lda = m; k.sA0 = m * npe * nd; k.sA1 = m * n * nd;
       k.B = MCs.p + e0 * npe * nd * npe; k.ldb = npe * nd; k.sB0 = 0; k.sB1 = npe * nd * npe;
-      k.C = Kt.p; k.ldc = m; k.sC0 = m * npe; k.sC1 = m * n;
+      opC(k, Kt.p, K32b, 0);
+      k.ldc = m; k.sC0 = m * npe; k.sC1 = m * n;
       k.nb0 = static_cast<int>(ncu); k.nb1 = eb; k.nb2 = 1;
       k.fp32 = precision;
       gemm(k, s);
       k.N = static_cast<int>(ndf); k.alpha = -1.0;
       k.B = MEs.p + e0 * npe * nd * ndf; k.sB1 = npe * nd * ndf;
+      k.Cf = nullptr;
       k.C = H.p + e0 * m * m; k.ldc = m * ncu; k.sC0 = m; k.sC1 = m * m;
       gemm(k, s);
     }
     toc(T_ASSEMBLY, t0);
-    // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru].  The mixed form, whose operator,
-    // factors and u-recovery matrices are all FP32, factors an FP32 copy of the block (same
-    // blocked algorithm and pivots, half the bytes per pass)
+    // condense u: [D | F | Ru] -> [LU | D^{-1}F | D^{-1}Ru] (the mixed form in FP32, with the
+    // same blocked algorithm; its pivots are kept for the refinement)
     tic(T_LU, &t0);
-    if (precision) {
-      to_float(aug32.p, aug.p, static_cast<int>(n), static_cast<int>(augcols()), n, augs, augs, eb, s);
+    if (f32)
       lu_solve(aug32.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s,
                piv32.p + static_cast<long>(e0) * n, lu_ws(eb));
-    } else {
+    else
       lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s, nullptr, lu_ws(eb));
-    }
     toc(T_LU, t0);
     tic(T_ASSEMBLY, &t0);
     {  // H += K * DinvF (= -K X_F), Rh -= K * (D^{-1} Ru)
       Gemm g{};
       g.M = m; g.N = m; g.K = n; g.alpha = -1.0; g.beta = 1.0;
-      g.A = Kt.p; g.lda = m; g.sA0 = m * n;
-      if (precision) g.Bf = aug32.p + n * n;
-      else g.B = aug.p + n * n;
+      opA(g, Kt.p, K32b, 0);
+      g.lda = m; g.sA0 = m * n;
+      opB(g, aug.p, aug32.p, n * n);
       g.ldb = n; g.sB0 = augs;
       g.C = H.p + e0 * m * m; g.ldc = m; g.sC0 = m * m;
       g.nb0 = eb; g.nb1 = 1; g.nb2 = 1;
@@ -751,18 +786,16 @@ void Solver::assemble() {
       gemm(g, s);
       g.N = 1;
       g.fp32 = 0;
-      if (precision) g.Bf = aug32.p + n * (n + m);
-      else g.B = aug.p + n * (n + m);
+      opB(g, aug.p, aug32.p, n * (n + m));
       g.C = Rh.p + e0 * m; g.sC0 = m;
       gemm(g, s);
     }
-    if (precision) {  // DinvF straight into FP32; FP32 copies of this block's H, LU factors of D' and K'
+    if (f32) {  // DinvF straight into FP32; FP32 copies of this block's H and LU factors of D'
       const long E0 = e0;
       post_schur(d, e0, eb, aug32.p, augs, DinvF32.p, RuS.p, s);
       to_float(H32.p + E0 * m * m, H.p + E0 * m * m, static_cast<int>(m), static_cast<int>(m), m, m * m, m * m,
                eb, s);
       to_float(LU32.p + E0 * n * n, aug32.p, static_cast<int>(n), static_cast<int>(n), n, augs, n * n, eb, s);
-      to_float(K32.p + E0 * m * n, Kt.p, static_cast<int>(m), static_cast<int>(n), m, m * n, m * n, eb, s);
     } else {
       post_schur(d, e0, eb, aug.p, augs, DinvF.p, RuS.p, s);
     }

@@ -64,7 +64,7 @@ struct Solver {
   DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
   // mixed precision: FP32 copies of H, DinvF, the LU factors of D', K' and the preconditioner blocks
   DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32;
-  DevBuf<float> aug32;      // FP32 copy of an element block's [D | F | Ru] for the mixed-form LU
+  DevBuf<float> aug32, Bfull32, Gfull32, W32;  // FP32 element-block workspace (mixed form)
   bool dinvf_fp32 = false;  // the last assembly kept D^{-1}F in FP32 only (DinvF32)
   // option precond_bf16 (mixed form, block-ILU(0)): the factors in BF16 (upper half of FP32)
   int precond_bf16 = 0;

@@ -1556,8 +1556,8 @@ __global__ void k_update_udg(Dims d, const double* du, double alpha, double* udg
 // hdgBlockILU0 (matvec.cpp:119-166), ApplyBlockILU0 (preconditioner.cpp:184-270).
 // BE[((t*nse) + n)*ncf*ncf + r + ncf*c]: CRS entry t of superelement n.
 // ---------------------------------------------------------------------------
-__global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon, const double* H,
-                                double* BE) {
+template <typename T>
+__global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon, const double* H, T* BE) {
   const int ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf, m = d.m;
   const long per = static_cast<long>(ncf) * ncf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1595,28 +1595,29 @@ __global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* ele
     int n2 = an + ncu * (elemcon[bn + npf * k2 + ndf * e] - npf * fj);
     v = H[static_cast<long>(e) * m * m + (m1 + ncf * k1) + static_cast<long>(m) * (n2 + ncf * k2)];
   }
-  BE[q] = v;
+  BE[q] = static_cast<T>(v);
 }
 
 // Inverse of the diagonal block of face slot i of every superelement (one
 // workgroup each): Gauss-Jordan with partial pivoting on [D | I] in LDS
 // (row-major, 2ncf wide), the inverse written back in place.
-__global__ __launch_bounds__(256) void k_bilu_inv(Dims d, IluDev I, int i, double* BE, int* info) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_bilu_inv(Dims d, IluDev I, int i, T* BE, int* info) {
   const int ncf = d.ncf;
   const int n = blockIdx.x;
   const long per = static_cast<long>(ncf) * ncf;
   extern __shared__ double sm[];
-  double* A = sm;                     // ncf x 2ncf, row-major: A[r*2ncf + c]
-  double* fac = sm + 2 * per;         // ncf
+  T* A = reinterpret_cast<T*>(sm);    // ncf x 2ncf, row-major: A[r*2ncf + c]
+  T* fac = A + 2 * per;               // ncf
   __shared__ double wv[4];
   __shared__ int wi[4];
   __shared__ int pivs;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int w2 = 2 * ncf;
-  double* D = BE + (static_cast<long>(I.diag[i]) * I.nse + n) * per;
+  T* D = BE + (static_cast<long>(I.diag[i]) * I.nse + n) * per;
   for (int idx = tid; idx < ncf * w2; idx += 256) {
     int r = idx / w2, c = idx % w2;
-    A[idx] = (c < ncf) ? D[r + ncf * c] : ((c - ncf) == r ? 1.0 : 0.0);
+    A[idx] = (c < ncf) ? D[r + ncf * c] : ((c - ncf) == r ? T(1) : T(0));
   }
   __syncthreads();
   int singular = 0;
@@ -1641,16 +1642,18 @@ __global__ __launch_bounds__(256) void k_bilu_inv(Dims d, IluDev I, int i, doubl
     const int p = pivs;
     if (p != c)
       for (int k = tid; k < w2; k += 256) {
-        double tt = A[c * w2 + k];
+        T tt = A[c * w2 + k];
         A[c * w2 + k] = A[p * w2 + k];
         A[p * w2 + k] = tt;
       }
     __syncthreads();
-    const double pv = A[c * w2 + c];
+    const T pv = A[c * w2 + c];
     if (pv == 0.0) singular = 1;
-    const double inv = 1.0 / pv;
-    for (int k = tid; k < w2; k += 256) A[c * w2 + k] *= inv;
+    const T inv = T(1) / pv;
     for (int r = tid; r < ncf; r += 256) fac[r] = A[r * w2 + c];
+    // every wave has read the pivot before row c (the pivot itself included) is scaled
+    __syncthreads();
+    for (int k = tid; k < w2; k += 256) A[c * w2 + k] *= inv;
     __syncthreads();
     for (int idx = tid; idx < ncf * w2; idx += 256) {
       int r = idx / w2, k = idx % w2;
@@ -1666,7 +1669,8 @@ __global__ __launch_bounds__(256) void k_bilu_inv(Dims d, IluDev I, int i, doubl
 }
 
 // BE[elim_ji[k0 + k]][n] = Ltmp[k][n]
-__global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const double* Ltmp, double* BE) {
+template <typename T>
+__global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const T* Ltmp, T* BE) {
   const long per = static_cast<long>(d.ncf) * d.ncf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   const long tot = per * I.nse * ne_i;
@@ -2369,7 +2373,8 @@ void to_float(float* dst, const float* src, int rows, int cols, long ld, long ss
 }
 
 // BF16 (round to nearest even of the FP32 value), kept as unsigned short
-__global__ void k_to_bf16(unsigned short* dst, const double* src, long count) {
+template <typename S>
+__global__ void k_to_bf16(unsigned short* dst, const S* src, long count) {
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= count) return;
   unsigned int u = __float_as_uint(static_cast<float>(src[q]));
@@ -2377,11 +2382,14 @@ __global__ void k_to_bf16(unsigned short* dst, const double* src, long count) {
   dst[q] = static_cast<unsigned short>(u >> 16);
 }
 
-void to_bf16(unsigned short* dst, const double* src, long count, hipStream_t s) {
+template <typename S>
+void to_bf16_t(unsigned short* dst, const S* src, long count, hipStream_t s) {
   if (count <= 0) return;
-  k_to_bf16<<<nblocks(count, 256), 256, 0, s>>>(dst, src, count);
+  k_to_bf16<S><<<nblocks(count, 256), 256, 0, s>>>(dst, src, count);
   EXA_HIP(hipGetLastError());
 }
+void to_bf16(unsigned short* dst, const double* src, long count, hipStream_t s) { to_bf16_t(dst, src, count, s); }
+void to_bf16(unsigned short* dst, const float* src, long count, hipStream_t s) { to_bf16_t(dst, src, count, s); }
 
 void refine_rhs(const Dims& d, const float* LU, const int* piv, const float* K, const double* Ru, double* RuS,
                 double* Rh, hipStream_t s) {
@@ -2461,26 +2469,52 @@ void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipS
   EXA_HIP(hipGetLastError());
 }
 
-void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
-                   double* BE, hipStream_t s) {
+template <typename T>
+void bilu_assemble_t(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H, T* BE,
+                     hipStream_t s) {
   long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse * I.nnz;
-  k_bilu_assemble<<<nblocks(n1, 256), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
+  k_bilu_assemble<T><<<nblocks(n1, 256), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
   EXA_HIP(hipGetLastError());
 }
 
-void bilu_invert_diag(const Dims& d, const IluDev& I, int i, double* BE, int* info, hipStream_t s) {
+template <typename T>
+void bilu_invert_diag_t(const Dims& d, const IluDev& I, int i, T* BE, int* info, hipStream_t s) {
+  // the FP64 bound: FP32 blocks use the same limit, so both forms accept the same problems
   size_t shm = (2 * static_cast<size_t>(d.ncf) * d.ncf + d.ncf) * sizeof(double);
   if (shm > 60 * 1024) throw Error(5, "block-ILU(0) face blocks larger than 61 x 61");
-  k_bilu_inv<<<I.nse, 256, shm, s>>>(d, I, i, BE, info);
+  shm = (2 * static_cast<size_t>(d.ncf) * d.ncf + d.ncf) * sizeof(T);
+  k_bilu_inv<T><<<I.nse, 256, shm, s>>>(d, I, i, BE, info);
   EXA_HIP(hipGetLastError());
 }
 
-void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const double* Ltmp, double* BE,
-                hipStream_t s) {
+template <typename T>
+void bilu_put_l_t(const Dims& d, const IluDev& I, int k0, int ne_i, const T* Ltmp, T* BE, hipStream_t s) {
   long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse * ne_i;
   if (n1 == 0) return;
-  k_bilu_put_l<<<nblocks(n1, 256), 256, 0, s>>>(d, I, k0, ne_i, Ltmp, BE);
+  k_bilu_put_l<T><<<nblocks(n1, 256), 256, 0, s>>>(d, I, k0, ne_i, Ltmp, BE);
   EXA_HIP(hipGetLastError());
+}
+
+void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
+                   double* BE, hipStream_t s) {
+  bilu_assemble_t(d, I, f2e, elemcon, H, BE, s);
+}
+void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
+                   float* BE, hipStream_t s) {
+  bilu_assemble_t(d, I, f2e, elemcon, H, BE, s);
+}
+void bilu_invert_diag(const Dims& d, const IluDev& I, int i, double* BE, int* info, hipStream_t s) {
+  bilu_invert_diag_t(d, I, i, BE, info, s);
+}
+void bilu_invert_diag(const Dims& d, const IluDev& I, int i, float* BE, int* info, hipStream_t s) {
+  bilu_invert_diag_t(d, I, i, BE, info, s);
+}
+void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const double* Ltmp, double* BE,
+                hipStream_t s) {
+  bilu_put_l_t(d, I, k0, ne_i, Ltmp, BE, s);
+}
+void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const float* Ltmp, float* BE, hipStream_t s) {
+  bilu_put_l_t(d, I, k0, ne_i, Ltmp, BE, s);
 }
 
 template <typename T>

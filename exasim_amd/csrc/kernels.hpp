@@ -115,6 +115,7 @@ void post_schur(const Dims& d, int e0, int eb, const float* aug, long augs, floa
                 hipStream_t s);
 // BF16 copy (round to nearest even), stored as unsigned short (the upper half of an FP32)
 void to_bf16(unsigned short* dst, const double* src, long count, hipStream_t s);
+void to_bf16(unsigned short* dst, const float* src, long count, hipStream_t s);
 // dst_b[r + rows*c] = (float) src_b[r + ld*c]
 void to_float(float* dst, const double* src, int rows, int cols, long ld, long sstride, long dstride, int batch,
               hipStream_t s);
@@ -159,12 +160,16 @@ struct IluDev {
 };
 void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
                    double* BE, hipStream_t s);
+void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
+                   float* BE, hipStream_t s);
 // factorisation steps of face slot i, batched over the superelements: invert the
 // diagonal block (info[n] |= singular), and copy the ne_i blocks L = A_ji D^{-1}
 // computed into Ltmp[k][n] back into place; the products run on gemm()
 void bilu_invert_diag(const Dims& d, const IluDev& I, int i, double* BE, int* info, hipStream_t s);
+void bilu_invert_diag(const Dims& d, const IluDev& I, int i, float* BE, int* info, hipStream_t s);
 void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const double* Ltmp, double* BE,
                 hipStream_t s);
+void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const float* Ltmp, float* BE, hipStream_t s);
 template <typename T>
 void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
                 double* out, hipStream_t s);

@@ -372,34 +372,54 @@ void Solver::dirk_coefficients() {
 // slot i in order, invert the diagonal block, form L_ji = A_ji D_i^{-1} for the
 // blocks below it, and update A_jl -= L_ji A_il; the products are batched GEMMs
 // whose operands are gathered through the elimination index lists.
-void Solver::factor_bilu0() {
+namespace {
+// operands of the block-ILU(0) factorisation products: FP64, or FP32 storage and products
+void ilu_gemm_ptrs(Gemm& g, double* A, double* B, double* C) {
+  g.A = A;
+  g.B = B;
+  g.C = C;
+}
+void ilu_gemm_ptrs(Gemm& g, float* A, float* B, float* C) {
+  g.Af = A;
+  g.Bf = B;
+  g.Cf = C;
+  g.fp32 = 1;
+}
+}  // namespace
+
+template <typename T>
+void Solver::factor_bilu0_t(T* BE, T* L) {
   const int ncf = d.ncf;
   const long per = static_cast<long>(ncf) * ncf, slab = per * ilu.nse;
   for (int i = 0; i < ilu.nfse; ++i) {
-    bilu_invert_diag(d, ilud, i, BEilu.p, info.p, s);
+    bilu_invert_diag(d, ilud, i, BE, info.p, s);
     const int k0 = ilu.eptr[i], ne_i = ilu.eptr[i + 1] - k0;
     if (ne_i == 0) continue;
     Gemm g{};
     g.M = g.N = g.K = ncf;
     g.alpha = 1.0; g.beta = 0.0;
-    g.A = BEilu.p; g.iA = ilud.elim_ji + k0; g.lda = ncf; g.sA0 = slab; g.sA1 = per;
-    g.B = BEilu.p + ilu.diag[i] * slab; g.ldb = ncf; g.sB0 = 0; g.sB1 = per;
-    g.C = Lilu.p; g.ldc = ncf; g.sC0 = slab; g.sC1 = per;
+    ilu_gemm_ptrs(g, BE, BE + ilu.diag[i] * slab, L);
+    g.iA = ilud.elim_ji + k0; g.lda = ncf; g.sA0 = slab; g.sA1 = per;
+    g.ldb = ncf; g.sB0 = 0; g.sB1 = per;
+    g.ldc = ncf; g.sC0 = slab; g.sC1 = per;
     g.nb0 = ne_i; g.nb1 = ilu.nse; g.nb2 = 1;
     gemm(g, s);
-    bilu_put_l(d, ilud, k0, ne_i, Lilu.p, BEilu.p, s);
+    bilu_put_l(d, ilud, k0, ne_i, L, BE, s);
     const int u0 = ilu.uptr[k0], nu = ilu.uptr[k0 + ne_i] - u0;
     if (nu == 0) continue;
     Gemm u{};
     u.M = u.N = u.K = ncf;
     u.alpha = -1.0; u.beta = 1.0;
-    u.A = BEilu.p; u.iA = ilud.upd_ji + u0; u.lda = ncf; u.sA0 = slab; u.sA1 = per;
-    u.B = BEilu.p; u.iB = ilud.upd_il + u0; u.ldb = ncf; u.sB0 = slab; u.sB1 = per;
-    u.C = BEilu.p; u.iC = ilud.upd_jl + u0; u.ldc = ncf; u.sC0 = slab; u.sC1 = per;
+    ilu_gemm_ptrs(u, BE, BE, BE);
+    u.iA = ilud.upd_ji + u0; u.lda = ncf; u.sA0 = slab; u.sA1 = per;
+    u.iB = ilud.upd_il + u0; u.ldb = ncf; u.sB0 = slab; u.sB1 = per;
+    u.iC = ilud.upd_jl + u0; u.ldc = ncf; u.sC0 = slab; u.sC1 = per;
     u.nb0 = nu; u.nb1 = ilu.nse; u.nb2 = 1;
     gemm(u, s);
   }
 }
+
+void Solver::factor_bilu0() { factor_bilu0_t(BEilu.p, Lilu.p); }
 
 void Solver::alloc_mixed() {
   const size_t ne = d.ne, n = d.n, m = d.m, ncf2 = static_cast<size_t>(d.ncf) * d.ncf;
@@ -415,9 +435,10 @@ void Solver::alloc_mixed() {
   W32.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
   if (precond == 0) BE32.alloc(ncf2 * d.nf);
   if (precond == 1) BEeas32.alloc(m * m * ne);
-  if (precond == 2) {
+  if (precond == 2) {  // FP32 factorisation (and its factors), BF16 copy of the factors
+    BEilu32.alloc(ncf2 * ilu.nnz * ilu.nse);
+    Lilu32.alloc(Lilu.n);
     if (precond_bf16) BEilu16.alloc(ncf2 * ilu.nnz * ilu.nse);
-    else BEilu32.alloc(ncf2 * ilu.nnz * ilu.nse);
   }
 }
 
@@ -663,6 +684,7 @@ void Solver::assemble() {
   const long blk = static_cast<long>(npe) * d.nge;
   if (precision) alloc_mixed();
   dinvf_fp32 = precision != 0;
+  ilu_fp32 = precision != 0 && precond == 2;
   hipEvent_t t0{};
   // The mixed form assembles the element blocks (W, [D | F | Ru], B, K, G) in FP32 storage:
   // its operator, factors and u-recovery matrices are all FP32, and the Schur products and
@@ -829,8 +851,14 @@ void Solver::assemble() {
     bj_extract(d, bjaug.p, BE.p, s);
   } else {
     // hdgBlockILU0 (matvec.cpp:119-166)
-    bilu_assemble(d, ilud, f2e.p, elemcon.p, H.p, BEilu.p, s);
-    factor_bilu0();
+    // the mixed form assembles and factors in FP32 (its factors are applied in FP32 or BF16)
+    if (precision) {
+      bilu_assemble(d, ilud, f2e.p, elemcon.p, H.p, BEilu32.p, s);
+      factor_bilu0_t(BEilu32.p, Lilu32.p);
+    } else {
+      bilu_assemble(d, ilud, f2e.p, elemcon.p, H.p, BEilu.p, s);
+      factor_bilu0();
+    }
   }
   toc(T_LU, t0);
   if (precision) {
@@ -841,8 +869,7 @@ void Solver::assemble() {
       to_float(BEeas32.p, BEeas.p, d.m, d.m, d.m, mm, mm, d.ne, s);
     }
     if (precond == 2) {
-      if (precond_bf16) to_bf16(BEilu16.p, BEilu.p, c2 * ilu.nnz * ilu.nse, s);
-      else to_float(BEilu32.p, BEilu.p, d.ncf, d.ncf, d.ncf, c2, c2, ilu.nnz * ilu.nse, s);
+      if (precond_bf16) to_bf16(BEilu16.p, BEilu32.p, c2 * ilu.nnz * ilu.nse, s);
     }
   }
   if (precond != 1) {
@@ -1157,7 +1184,13 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
   else if (name == "MinvC") { src = MCs.p; size = static_cast<long>(MCs.n); }
   else if (name == "MinvE") { src = MEs.p; size = static_cast<long>(MEs.n); }
   else if (name == "res_b") { src = bres.p; size = static_cast<long>(bres.n); }
-  else if (name == "BEilu") { src = BEilu.p; size = static_cast<long>(BEilu.n); }
+  else if (name == "BEilu" && ilu_fp32) {  // the mixed form keeps its block-ILU(0) factors in FP32
+    std::vector<float> f(BEilu32.n);
+    EXA_HIP(hipMemcpyAsync(f.data(), BEilu32.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+    host.assign(f.begin(), f.end());
+    size = static_cast<long>(host.size());
+  } else if (name == "BEilu") { src = BEilu.p; size = static_cast<long>(BEilu.n); }
   else if (name == "BEeas") { src = BEeas.p; size = static_cast<long>(BEeas.n); }
   else if (name == "ilu_face") from_int(ilu.face);
   else if (name == "ilu_row_ptr") from_int(ilu.row_ptr);

@@ -66,6 +66,8 @@ struct Solver {
   DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32;
   DevBuf<float> aug32, Bfull32, Gfull32, W32;  // FP32 element-block workspace (mixed form)
   bool dinvf_fp32 = false;  // the last assembly kept D^{-1}F in FP32 only (DinvF32)
+  bool ilu_fp32 = false;    // the last assembly kept the block-ILU(0) factors in FP32 only (BEilu32)
+  DevBuf<float> Lilu32;
   // option precond_bf16 (mixed form, block-ILU(0)): the factors in BF16 (upper half of FP32)
   int precond_bf16 = 0;
   DevBuf<unsigned short> BEilu16;
@@ -164,6 +166,8 @@ struct Solver {
   void setup();
   void setup_bilu0();
   void factor_bilu0();
+  template <typename T>
+  void factor_bilu0_t(T* BE, T* L);
   void setup_eas();
   void alloc_workspace();
   AsmArgs asm_args(int e0, int eb);

@@ -107,3 +107,29 @@ def test_bf16_ilu_factors_solve_to_tolerance(built):
         S.solve()
         uh = S.get("uh")
     assert rel(uh, P.uh.reshape(-1)) < 1e-8
+
+
+def test_bilu_factorisation_reproducible(case_dir):
+    """Block-ILU(0) factorisation of 512 superelements with 50 x 50 face blocks (p = 4, two
+    components): two factorisations give the same factors bit for bit, in FP64 and in the
+    mixed form's FP32, and the FP32 factors agree with the FP64 ones block by block (a
+    regression test for the Gauss-Jordan inverse of the diagonal blocks, whose pivot row is
+    scaled only after every wave has read the pivot)."""
+    path, _ = case_dir("cd16_ilu", "convdiff3d", n=16, porder=4, ncu=2, restart=100, gmres_tol=1e-10,
+                       precond="bilu0", cartgridpart=[3, 16, 16, 16, 2, 2, 2, 0])
+    say("case cd16_ilu: block-ILU(0) factors of 512 superelements, FP64 and FP32, twice each")
+    from exasim_amd.hdg import HDGSolver
+    with HDGSolver(path, "convdiff") as S:
+        S.get_q()
+        fac = {}
+        for prec in (0, 1):
+            S.set_option("precision", prec)
+            S.assemble()
+            a = S.get("BEilu")
+            S.assemble()
+            b = S.get("BEilu")
+            assert np.isfinite(a).all()
+            assert np.array_equal(a, b), prec
+            fac[prec] = a.reshape(-1, 50 * 50)
+    d = np.abs(fac[1] - fac[0]).max(axis=1) / np.maximum(np.abs(fac[0]).max(axis=1), 1e-300)
+    assert d.max() < 1e-3, d.max()

@@ -1935,6 +1935,13 @@ __global__ __launch_bounds__(256) void k_refine_rhs(Dims d, int nbp, const float
 // 150 KB of LDS at n = 300) were measured slower: one workgroup per CU leaves the latency-bound
 // panel factorisation without company (C2 LU 0.36 s -> 0.55 s per assembly).
 int lu_panel_width(int n) { return (static_cast<size_t>(n) * 32 * sizeof(double) <= 64000) ? 32 : 16; }
+// the FP32 LU (the mixed form): 64-wide panels up to n = 250 (an n x 64 FP32 panel in 64 KB of LDS,
+// 64 registers per row in the register panel), which halves the passes of the bandwidth-bound
+// trailing update; 32 or 16 beyond
+int lu_panel_width_f32(int n) {
+  if (n <= 250) return 64;
+  return (static_cast<size_t>(n) * 32 * sizeof(float) <= 64000) ? 32 : 16;
+}
 
 // ---------------------------------------------------------------------------
 // Vector kernels (fixed-order reductions)
@@ -2187,9 +2194,16 @@ void lu_blocked_any(T* A, int n, int ncols, long stride, int batch, int* info, h
   // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide; with a pivot
   // workspace, panels of up to 256 rows are factored in registers (64-wide register panels
   // were measured: no gain at n = 250, slower at n = 100)
-  if (lu_panel_width(n) == 32)
+  const int w = sizeof(T) == 4 ? lu_panel_width_f32(n) : lu_panel_width(n);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 64) {
+      blocked_lu<T, 64>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
+      return;
+    }
+  }
+  if (w == 32)
     blocked_lu<T, 32>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
-  else if (static_cast<size_t>(n) * 16 * sizeof(double) <= 64000)
+  else if (static_cast<size_t>(n) * 16 * sizeof(T) <= 64000)
     blocked_lu<T, 16>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
   else
     throw Error(5, "dense block larger than 500 rows");
@@ -2394,7 +2408,7 @@ void to_bf16(unsigned short* dst, const float* src, long count, hipStream_t s) {
 void refine_rhs(const Dims& d, const float* LU, const int* piv, const float* K, const double* Ru, double* RuS,
                 double* Rh, hipStream_t s) {
   size_t shm = static_cast<size_t>(d.n) * sizeof(double);
-  k_refine_rhs<<<d.ne, 256, shm, s>>>(d, lu_panel_width(d.n), LU, piv, K, Ru, RuS, Rh);
+  k_refine_rhs<<<d.ne, 256, shm, s>>>(d, lu_panel_width_f32(d.n), LU, piv, K, Ru, RuS, Rh);
   EXA_HIP(hipGetLastError());
 }
 

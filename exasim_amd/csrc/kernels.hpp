@@ -49,8 +49,9 @@ void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hi
 // The same in FP32 storage and arithmetic (blocked path only: ncols > n)
 void lu_solve(float* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s,
               int* pivout = nullptr, int* piv_ws = nullptr);
-// piv_ws (optional, device, >= 32*batch ints, owned by the caller): with it, panels of up to
-// 256 rows are factored with the panel in registers (k_lu_panel_reg + k_lu_u12)
+// piv_ws (optional, device, >= LU_PIV_WS*batch ints, owned by the caller): with it, panels of up
+// to 256 rows are factored with the panel in registers (k_lu_panel_reg + k_lu_u12)
+constexpr int LU_PIV_WS = 64;
 
 // ---- setup ----
 void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s);

@@ -92,7 +92,7 @@ void Solver::setup() {
 
 // pivot workspace of the batched LU for `batch` matrices (grown with the stream idle)
 int* Solver::lu_ws(long batch) {
-  const size_t need = static_cast<size_t>(std::max(1L, batch)) * 32;
+  const size_t need = static_cast<size_t>(std::max(1L, batch)) * LU_PIV_WS;
   if (lupiv.n < need) {
     EXA_HIP(hipStreamSynchronize(s));
     lupiv.alloc(need);

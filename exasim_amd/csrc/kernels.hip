@@ -1446,12 +1446,16 @@ __global__ void k_bj_apply(Dims d, const T* BE, const double* x, double* y) {
 // quarter of the columns, lanes own VW consecutive rows (16-byte vector loads of
 // the column), accumulation is FP64, and the four partial sums are added in LDS
 // in a fixed order.
-template <typename T, int VW>
+// NRB > 0: exactly NRB row blocks of 64 VW rows (m > 64 VW (NRB - 1)), with CB columns' loads
+// issued before their FMAs (more bytes in flight per wave; a row's FMAs still run over the columns
+// in order); NRB = 0: any m <= 1024, one column at a time.
+template <typename T, int VW, int NRB>
 __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const int* elemcon,
                                                      const double* v, double* we, const int* einst,
                                                      const double* emask) {
   typedef T vecT __attribute__((ext_vector_type(VW)));
-  constexpr int MAXRB = (1024 + 64 * VW - 1) / (64 * VW);  // m <= 1024
+  constexpr int MAXRB = NRB > 0 ? NRB : (1024 + 64 * VW - 1) / (64 * VW);  // m <= 1024
+  constexpr int CB = NRB > 0 ? 4 : 1;
   const int m = d.m, ncu = d.ncu, ndf = d.ndf;
   const long e = blockIdx.x;
   if (emask && einst && emask[einst[e]] == 0.0) {  // block-uniform: the element's instance is inactive
@@ -1474,7 +1478,33 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const i
 #pragma unroll
     for (int q = 0; q < VW; ++q) acc[rb][q] = 0.0;
   const T* He = H + e * static_cast<long>(m) * m;
-  for (int c = c0; c < c1; ++c) {
+  int cfirst = c0;
+  if (CB > 1) {
+    for (; cfirst + CB <= c1; cfirst += CB) {
+      vecT h[CB][MAXRB];
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        const T* col = He + static_cast<long>(m) * (cfirst + j);
+#pragma unroll
+        for (int rb = 0; rb < MAXRB; ++rb) {
+          const int r = VW * (lane + 64 * rb);
+          if (r < m) h[j][rb] = *reinterpret_cast<const vecT*>(col + r);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        const double vc = ve[cfirst + j];
+#pragma unroll
+        for (int rb = 0; rb < MAXRB; ++rb) {
+          const int r = VW * (lane + 64 * rb);
+          if (r < m)
+#pragma unroll
+            for (int q = 0; q < VW; ++q) acc[rb][q] = fma(static_cast<double>(h[j][rb][q]), vc, acc[rb][q]);
+        }
+      }
+    }
+  }
+  for (int c = cfirst; c < c1; ++c) {
     const double vc = ve[c];
     const T* col = He + static_cast<long>(m) * c;
 #pragma unroll
@@ -2463,9 +2493,16 @@ void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v,
   }
   size_t shm = static_cast<size_t>(5) * d.m * sizeof(double);
   constexpr int VMAX = 16 / sizeof(T);  // 16-byte loads: 2 doubles or 4 floats per lane
-  if (d.m % VMAX == 0) k_matvec_elem<T, VMAX><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
-  else if (d.m % 2 == 0) k_matvec_elem<T, 2><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
-  else k_matvec_elem<T, 1><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+  if (d.m % VMAX == 0) {
+    const int nrb = (d.m + 64 * VMAX - 1) / (64 * VMAX);
+    if (nrb == 2) k_matvec_elem<T, VMAX, 2><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+    else if (nrb == 3) k_matvec_elem<T, VMAX, 3><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+    else k_matvec_elem<T, VMAX, 0><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+  } else if (d.m % 2 == 0) {
+    k_matvec_elem<T, 2, 0><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+  } else {
+    k_matvec_elem<T, 1, 0><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+  }
   EXA_HIP(hipGetLastError());
 }
 

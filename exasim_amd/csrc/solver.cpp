@@ -428,8 +428,18 @@ void Solver::alloc_mixed() {
   LU32.alloc(n * n * ne);
   piv32.alloc(n * ne);
   K32.alloc(m * n * ne);
-  // FP32 element-block workspace of the mixed-form assembly
+  // FP32 element-block workspace of the mixed-form assembly, FP32 copies of its FP64 operands
   aug32.alloc(static_cast<size_t>(neb) * n * augcols());
+  if (!q32_ready) {
+    MCs32.alloc(MCs.n);
+    MEs32.alloc(MEs.n);
+    shapegt32.alloc(shapegt.n);
+    const int rq = d.npe * d.nd;  // rows of M^-1 C and M^-1 E
+    to_float(MCs32.p, MCs.p, rq, d.npe * d.ne, rq, 0, 0, 1, s);
+    to_float(MEs32.p, MEs.p, rq, d.ndf * d.ne, rq, 0, 0, 1, s);
+    to_float(shapegt32.p, shapegt.p, static_cast<int>(shapegt.n), 1, static_cast<long>(shapegt.n), 0, 0, 1, s);
+    q32_ready = true;
+  }
   Bfull32.alloc(static_cast<size_t>(neb) * n * n * d.nd);
   Gfull32.alloc(static_cast<size_t>(neb) * m * n * d.nd);
   W32.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
@@ -724,7 +734,8 @@ void Solver::assemble() {
       g.M = npe; g.N = npe; g.K = d.nge; g.alpha = -1.0; g.beta = 0.0;
       opA(g, W.p, W32.p, 0);
       g.lda = npe; g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
-      g.B = shapegt.p; g.ldb = d.nge; g.sB0 = g.sB1 = g.sB2 = 0;
+      opB(g, shapegt.p, shapegt32.p, 0);
+      g.ldb = d.nge; g.sB0 = g.sB1 = g.sB2 = 0;
       opC(g, aug.p, aug32.p, 0);
       g.ldc = n; g.sC0 = npe; g.sC1 = n * npe; g.sC2 = augs;
       g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
@@ -736,7 +747,8 @@ void Solver::assemble() {
       g.M = npe; g.N = npe; g.K = d.nge; g.alpha = -1.0; g.beta = 0.0;
       opA(g, W.p, W32.p, (ncu + ncu * dd) * ncu * blk);
       g.lda = npe; g.sA0 = blk; g.sA1 = ncu * blk; g.sA2 = static_cast<long>(d.nc) * ncu * blk;
-      g.B = shapegt.p; g.ldb = d.nge;
+      opB(g, shapegt.p, shapegt32.p, 0);
+      g.ldb = d.nge;
       opC(g, Bfull.p, Bfull32.p, n * npe * dd);
       g.ldc = n; g.sC0 = npe; g.sC1 = n * npe * nd; g.sC2 = n * n * nd;
       g.nb0 = static_cast<int>(ncu); g.nb1 = static_cast<int>(ncu); g.nb2 = eb;
@@ -757,14 +769,16 @@ void Solver::assemble() {
       g.M = n; g.N = npe; g.K = npe * nd; g.alpha = 1.0; g.beta = 1.0;
       opA(g, Bfull.p, Bfull32.p, 0);
       g.lda = n; g.sA0 = n * npe * nd; g.sA1 = n * n * nd;
-      g.B = MCs.p + e0 * npe * nd * npe; g.ldb = npe * nd; g.sB0 = 0; g.sB1 = npe * nd * npe;
+      opB(g, MCs.p, MCs32.p, e0 * npe * nd * npe);
+      g.ldb = npe * nd; g.sB0 = 0; g.sB1 = npe * nd * npe;
       opC(g, aug.p, aug32.p, 0);
       g.ldc = n; g.sC0 = n * npe; g.sC1 = augs;
       g.nb0 = static_cast<int>(ncu); g.nb1 = eb; g.nb2 = 1;
       g.fp32 = precision;
       gemm(g, s);
       g.N = static_cast<int>(ndf); g.alpha = -1.0;
-      g.B = MEs.p + e0 * npe * nd * ndf; g.sB1 = npe * nd * ndf;
+      opB(g, MEs.p, MEs32.p, e0 * npe * nd * ndf);
+      g.sB1 = npe * nd * ndf;
       opC(g, aug.p, aug32.p, n * n);
       g.ldc = n * ncu; g.sC0 = n; g.sC1 = augs;
       gemm(g, s);
@@ -772,14 +786,16 @@ void Solver::assemble() {
       k.M = m; k.N = npe; k.K = npe * nd; k.alpha = 1.0; k.beta = 1.0;
       opA(k, Gfull.p, Gfull32.p, 0);
       k.lda = m; k.sA0 = m * npe * nd; k.sA1 = m * n * nd;
-      k.B = MCs.p + e0 * npe * nd * npe; k.ldb = npe * nd; k.sB0 = 0; k.sB1 = npe * nd * npe;
+      opB(k, MCs.p, MCs32.p, e0 * npe * nd * npe);
+      k.ldb = npe * nd; k.sB0 = 0; k.sB1 = npe * nd * npe;
       opC(k, Kt.p, K32b, 0);
       k.ldc = m; k.sC0 = m * npe; k.sC1 = m * n;
       k.nb0 = static_cast<int>(ncu); k.nb1 = eb; k.nb2 = 1;
       k.fp32 = precision;
       gemm(k, s);
       k.N = static_cast<int>(ndf); k.alpha = -1.0;
-      k.B = MEs.p + e0 * npe * nd * ndf; k.sB1 = npe * nd * ndf;
+      opB(k, MEs.p, MEs32.p, e0 * npe * nd * ndf);
+      k.sB1 = npe * nd * ndf;
       k.Cf = nullptr;
       k.C = H.p + e0 * m * m; k.ldc = m * ncu; k.sC0 = m; k.sC1 = m * m;
       gemm(k, s);

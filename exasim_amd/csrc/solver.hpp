@@ -65,6 +65,8 @@ struct Solver {
   // mixed precision: FP32 copies of H, DinvF, the LU factors of D', K' and the preconditioner blocks
   DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32;
   DevBuf<float> aug32, Bfull32, Gfull32, W32;  // FP32 element-block workspace (mixed form)
+  DevBuf<float> MCs32, MEs32, shapegt32;       // FP32 copies of M^-1 C, M^-1 E and the shape table
+  bool q32_ready = false;
   bool dinvf_fp32 = false;  // the last assembly kept D^{-1}F in FP32 only (DinvF32)
   bool ilu_fp32 = false;    // the last assembly kept the block-ILU(0) factors in FP32 only (BEilu32)
   DevBuf<float> Lilu32;

@@ -864,8 +864,43 @@ __global__ void k_qpack(Dims d, int e0, int eb, const double* aug, double* MCs, 
 }
 
 // hdgGetQ (qequation.cpp:372-487): q_d = MinvC_d u - MinvE_d uh, stored in udg.
-__global__ void k_getq(Dims d, const double* MCs, const double* MEs, const int* elemcon,
-                       const double* uh, double* udg) {
+// One thread per (row kd of M^-1 C / M^-1 E, element), all NCU components: the two matrices are
+// read once per element (the sums for each component run in the same order as per component).
+template <int NCU>
+__global__ void k_getq(Dims d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
+                       double* udg) {
+  const int npe = d.npe, nd = d.nd, nc = d.nc, ndf = d.ndf;
+  const int per = npe * nd;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(per) * d.ne) return;
+  const long e = q / per;
+  const int kd = static_cast<int>(q - e * per);
+  const int k = kd % npe, dd = kd / npe;
+  const double* mc = MCs + e * npe * nd * npe;
+  const double* me = MEs + e * npe * nd * ndf;
+  const double* u = udg + static_cast<long>(npe) * nc * e;
+  double s[NCU];
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) s[c] = 0.0;
+  for (int j = 0; j < npe; ++j) {
+    const double mv = mc[kd + static_cast<long>(npe) * nd * j];
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) s[c] += mv * u[j + npe * c];
+  }
+  const int* ec = elemcon + static_cast<long>(ndf) * e;
+  for (int t = 0; t < ndf; ++t) {
+    const double mv = me[kd + static_cast<long>(npe) * nd * t];
+    const double* uht = uh + NCU * static_cast<long>(ec[t]);
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) s[c] -= mv * uht[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) udg[k + npe * (NCU + c + NCU * dd) + static_cast<long>(npe) * nc * e] = s[c];
+}
+
+// the same for any ncu, one component per thread
+__global__ void k_getq_any(Dims d, const double* MCs, const double* MEs, const int* elemcon,
+                           const double* uh, double* udg) {
   const int npe = d.npe, nd = d.nd, ncu = d.ncu, nc = d.nc, ndf = d.ndf;
   const long per = static_cast<long>(npe) * nd * ncu;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -993,19 +1028,22 @@ __global__ void k_vol_res(Dims d, AsmArgs a, T* out, long ldo) {
 }
 
 // W[(e*NC + c)*NCU + k][i + npe*g] = sum_dp shapegw[i, g, dp] * rJ[dp, k, c]
-template <typename T>
+// IT: index type of the flattened thread index (32-bit when the launch fits: the divisions by
+// the run-time sizes cost more than the kernel's arithmetic in 64 bits)
+template <typename T, typename IT>
 __global__ void k_vol_W(Dims d, AsmArgs a, T* W) {
   const int npe = d.npe, nge = d.nge, ncu = d.ncu, nc = d.nc, nd = d.nd;
-  const long blk = static_cast<long>(npe) * nge;
-  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  const IT blk = static_cast<IT>(npe) * nge;
+  const IT q = blockIdx.x * static_cast<IT>(blockDim.x) + threadIdx.x;
   if (q >= blk * ncu * nc * a.eb) return;
-  const long bidx = q / blk;
-  const int ig = static_cast<int>(q % blk);
-  const int i = ig % npe, g = ig / npe;
-  const int k = static_cast<int>(bidx % ncu);
-  const long t = bidx / ncu;
-  const int c = static_cast<int>(t % nc);
-  const long e = t / nc;
+  const IT bidx = q / blk;
+  const int ig = static_cast<int>(q - bidx * blk);
+  const int g = ig / npe, i = ig - g * npe;
+  const IT t = bidx / ncu;
+  const int k = static_cast<int>(bidx - t * ncu);
+  const IT eidx = t / nc;
+  const int c = static_cast<int>(t - eidx * nc);
+  const long e = static_cast<long>(eidx);
   const long P = static_cast<long>(a.eb) * nge;
   double s = 0.0;
   for (int dp = 0; dp <= nd; ++dp)
@@ -1455,7 +1493,7 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const i
                                                      const double* emask) {
   typedef T vecT __attribute__((ext_vector_type(VW)));
   constexpr int MAXRB = NRB > 0 ? NRB : (1024 + 64 * VW - 1) / (64 * VW);  // m <= 1024
-  constexpr int CB = NRB > 0 ? 4 : 1;
+  constexpr int CB = NRB > 0 ? (sizeof(T) == 4 ? 4 : 2) : 1;
   const int m = d.m, ncu = d.ncu, ndf = d.ndf;
   const long e = blockIdx.x;
   if (emask && einst && emask[einst[e]] == 0.0) {  // block-uniform: the element's instance is inactive
@@ -2309,8 +2347,17 @@ void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double
 
 void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
           double* udg, hipStream_t s) {
-  long n1 = static_cast<long>(d.npe) * d.nd * d.ncu * d.ne;
-  k_getq<<<nblocks(n1, 256), 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg);
+  const long n1 = static_cast<long>(d.npe) * d.nd * d.ne;
+  const unsigned nb = nblocks(n1, 256);
+  switch (d.ncu) {
+    case 1: k_getq<1><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 2: k_getq<2><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 3: k_getq<3><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 4: k_getq<4><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 5: k_getq<5><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    default:
+      k_getq_any<<<nblocks(n1 * d.ncu, 256), 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg);
+  }
   EXA_HIP(hipGetLastError());
 }
 
@@ -2329,7 +2376,8 @@ void vol_res(const Dims& d, const AsmArgs& a, float* out, long ldo, hipStream_t 
 template <typename T>
 void vol_W_t(const Dims& d, const AsmArgs& a, T* W, hipStream_t s) {
   long n1 = static_cast<long>(d.npe) * d.nge * d.ncu * d.nc * a.eb;
-  k_vol_W<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
+  if (n1 < (1L << 31) - 256) k_vol_W<T, unsigned><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
+  else k_vol_W<T, long><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
   EXA_HIP(hipGetLastError());
 }
 void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s) { vol_W_t(d, a, a.W, s); }

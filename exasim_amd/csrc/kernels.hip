@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_lu_panel_reg(T* Aall, int n, long strid
       for (int w = 1; w < 4; ++w)
         if (wv[b][w] > bv || (wv[b][w] == bv && wi[b][w] < p)) { bv = wv[b][w]; p = wi[b][w]; }
       if (tid == 0) {
-        piv_ws[static_cast<long>(blockIdx.x) * BLU_NB + kk] = p;
+        piv_ws[static_cast<long>(blockIdx.x) * LU_PIV_WS + kk] = p;
         if (pivout) pivout[static_cast<long>(blockIdx.x) * n + k0 + kk] = p;
       }
       // the pivot row and row kk trade places
@@ -410,8 +410,68 @@ __global__ __launch_bounds__(256) void k_lu_panel_reg(T* Aall, int n, long strid
   if (tid == 0 && info) info[blockIdx.x] = (k0 == 0 ? 0 : info[blockIdx.x]) | singular;
 }
 
-// Columns right of a panel factored by k_lu_panel_reg: the panel's row exchanges, then
-// U12 = L11^{-1} A12 (one column per thread; blockIdx.y selects a chunk of 256 columns).
+// The row exchanges of a panel factored by k_lu_panel_reg, applied to the columns right of the
+// panel.  Thread 0 turns the panel's swaps (row kk <-> row piv[kk], in order) into their net
+// permutation of the rows they touch; every column (one per lane, 64 per workgroup) then reads
+// all the moved values into registers before writing any, so a column costs one round of
+// independent loads instead of a dependent load/store round trip per exchange.
+template <typename T, int BLU_NB>
+__global__ __launch_bounds__(64) void k_lu_swap(T* Aall, int n, int ncols, long stride, int k0, const int* piv_ws) {
+  T* A = Aall + stride * blockIdx.x;
+  const int nb = min(BLU_NB, n - k0);
+  __shared__ int rows[2 * BLU_NB];  // touched rows (panel-relative), then the moved ones
+  __shared__ int held[2 * BLU_NB];  // the original row whose value rows[i] holds
+  __shared__ int dst[2 * BLU_NB], srcr[2 * BLU_NB];
+  __shared__ int cnt;
+  if (threadIdx.x == 0) {
+    int nr = nb;
+    for (int kk = 0; kk < nb; ++kk) {
+      rows[kk] = kk;
+      held[kk] = kk;
+    }
+    for (int kk = 0; kk < nb; ++kk) {
+      const int p = piv_ws[static_cast<long>(blockIdx.x) * LU_PIV_WS + kk];
+      if (p == kk) continue;
+      int pos = p;
+      if (p >= nb) {
+        pos = -1;
+        for (int i = nb; i < nr; ++i)
+          if (rows[i] == p) pos = i;
+        if (pos < 0) {
+          rows[nr] = p;
+          held[nr] = p;
+          pos = nr++;
+        }
+      }
+      const int t = held[kk];
+      held[kk] = held[pos];
+      held[pos] = t;
+    }
+    int c = 0;
+    for (int i = 0; i < nr; ++i)
+      if (held[i] != rows[i]) {
+        dst[c] = rows[i];
+        srcr[c] = held[i];
+        ++c;
+      }
+    cnt = c;
+  }
+  __syncthreads();
+  const int j = k0 + nb + blockIdx.y * 64 + threadIdx.x;
+  if (j >= ncols) return;
+  T* colj = A + static_cast<long>(n) * j + k0;
+  const int nm = cnt;
+  T v[2 * BLU_NB];
+#pragma unroll
+  for (int i = 0; i < 2 * BLU_NB; ++i)
+    if (i < nm) v[i] = colj[srcr[i]];
+#pragma unroll
+  for (int i = 0; i < 2 * BLU_NB; ++i)
+    if (i < nm) colj[dst[i]] = v[i];
+}
+
+// Columns right of a panel factored by k_lu_panel_reg, after k_lu_swap: U12 = L11^{-1} A12 (one
+// column per thread; blockIdx.y selects a chunk of 256 columns).
 template <typename T, int BLU_NB>
 __global__ __launch_bounds__(256) void k_lu_u12(T* Aall, int n, int ncols, long stride, int k0,
                                                 const int* piv_ws) {
@@ -420,14 +480,12 @@ __global__ __launch_bounds__(256) void k_lu_u12(T* Aall, int n, int ncols, long 
   // L11 with a run-time leading dimension: constant LDS addresses in the unrolled solve let the
   // compiler hoist every load into registers (and spill)
   __shared__ T L11[BLU_NB * BLU_NB];
-  __shared__ int piv[BLU_NB];
   const int ldl = nb;
   const int tid = threadIdx.x;
   for (int idx = tid; idx < nb * nb; idx += 256) {
     const int r = idx % nb, c = idx / nb;
     L11[r + ldl * c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
   }
-  if (tid < nb) piv[tid] = piv_ws[static_cast<long>(blockIdx.x) * BLU_NB + tid];
   __syncthreads();
   const int j = k0 + nb + blockIdx.y * 256 + tid;
   if (j >= ncols) return;
@@ -435,28 +493,6 @@ __global__ __launch_bounds__(256) void k_lu_u12(T* Aall, int n, int ncols, long 
   T x[BLU_NB];
 #pragma unroll
   for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
-  // pivots in order; p < BLU_NB means both rows are in the register block
-#pragma unroll
-  for (int kk = 0; kk < BLU_NB; ++kk) {
-    if (kk < nb) {
-      const int p = piv[kk];
-      if (p != kk) {
-        if (p < BLU_NB) {
-#pragma unroll
-          for (int q = kk + 1; q < BLU_NB; ++q)
-            if (q == p) {
-              const T t = x[q];
-              x[q] = x[kk];
-              x[kk] = t;
-            }
-        } else {
-          const T t = colj[p];
-          colj[p] = x[kk];
-          x[kk] = t;
-        }
-      }
-    }
-  }
 #pragma unroll
   for (int kk = 1; kk < BLU_NB; ++kk)
 #pragma unroll
@@ -465,6 +501,59 @@ __global__ __launch_bounds__(256) void k_lu_u12(T* Aall, int n, int ncols, long 
 #pragma unroll
   for (int kk = 0; kk < BLU_NB; ++kk)
     if (kk < nb) colj[kk] = x[kk];
+}
+
+// Inverse of the unit lower triangle L11 (LOWER) or of the upper triangle U11 of the panel at
+// k0, one workgroup per matrix: thread c computes column c by substitution (static register
+// indices, predicated on c), written to W[blockIdx.x] (nb x nb, ld BLU_NB).  U12 = L11^{-1} A12
+// and X = U11^{-1} X then run as in-place GEMMs on the matrix cores: each output tile covers all
+// nb rows of its own columns, so it reads its B columns completely before writing them.
+template <typename T, int BLU_NB, bool LOWER>
+__global__ __launch_bounds__(64) void k_lu_tri_inv(const T* Aall, int n, long stride, int k0, T* W) {
+  const T* A = Aall + stride * blockIdx.x;
+  const int nb = min(BLU_NB, n - k0);
+  __shared__ T Ts[BLU_NB][BLU_NB + 1];
+  for (int idx = threadIdx.x; idx < nb * nb; idx += 64) {
+    const int r = idx % nb, c = idx / nb;
+    Ts[r][c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
+  }
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c >= nb) return;
+  T y[BLU_NB];
+  if (LOWER) {
+#pragma unroll
+    for (int r = 0; r < BLU_NB; ++r) {
+      if (r <= c || r >= nb) {
+        y[r] = (r == c) ? T(1) : T(0);
+      } else {
+        T s = T(0);
+#pragma unroll
+        for (int t = 0; t < r; ++t)
+          if (t >= c) s = fma(Ts[r][t], y[t], s);
+        y[r] = -s;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = BLU_NB - 1; r >= 0; --r) {
+      if (r > c || r >= nb) {
+        y[r] = T(0);
+      } else if (r == c) {
+        y[r] = T(1) / Ts[r][r];
+      } else {
+        T s = T(0);
+#pragma unroll
+        for (int t = r + 1; t < BLU_NB; ++t)
+          if (t <= c) s = fma(Ts[r][t], y[t], s);
+        y[r] = -s / Ts[r][r];
+      }
+    }
+  }
+  T* Wm = W + static_cast<long>(blockIdx.x) * BLU_NB * BLU_NB;
+#pragma unroll
+  for (int r = 0; r < BLU_NB; ++r)
+    if (r < nb) Wm[r + BLU_NB * c] = y[r];
 }
 
 // X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
@@ -2212,14 +2301,33 @@ inline void lu_gemm_ptrs(Gemm& g, const float* A, const float* B, float* C) {
 
 template <typename T, int NB>
 void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* pivout, hipStream_t s,
-                int* piv_ws) {
+                int* piv_ws, T* tri_ws) {
+  // X[r0 : r0+nb, cols] = Tinv X[r0 : r0+nb, cols] in place (Tinv in tri_ws, ld NB)
+  auto apply_tri = [&](int nb, int r0, long c0, int ncol) {
+    Gemm g{};
+    g.M = nb; g.N = ncol; g.K = nb; g.alpha = 1.0; g.beta = 0.0;
+    T* X = A + r0 + static_cast<long>(n) * c0;
+    lu_gemm_ptrs(g, tri_ws, X, X);
+    g.lda = NB; g.sA0 = static_cast<long>(NB) * NB;
+    g.ldb = n; g.sB0 = stride;
+    g.ldc = n; g.sC0 = stride;
+    g.nb0 = batch; g.nb1 = 1; g.nb2 = 1;
+    gemm(g, s);
+  };
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = std::min(NB, n - k0);
     if (piv_ws && n - k0 <= 256) {
       k_lu_panel_reg<T, NB><<<batch, 256, 0, s>>>(A, n, stride, k0, info, pivout, piv_ws);
       const int right = ncols - k0 - nb;
-      if (right > 0)
-        k_lu_u12<T, NB><<<dim3(batch, (right + 255) / 256), 256, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
+      if (right > 0) {
+        k_lu_swap<T, NB><<<dim3(batch, (right + 63) / 64), 64, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
+        if (tri_ws) {
+          k_lu_tri_inv<T, NB, true><<<batch, 64, 0, s>>>(A, n, stride, k0, tri_ws);
+          apply_tri(nb, k0, k0 + nb, right);
+        } else {
+          k_lu_u12<T, NB><<<dim3(batch, (right + 255) / 256), 256, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
+        }
+      }
     } else {
       size_t shm = static_cast<size_t>(n - k0) * nb * sizeof(T);
       k_lu_panel<T, NB><<<batch, 256, shm, s>>>(A, n, ncols, stride, k0, info, pivout);
@@ -2239,9 +2347,15 @@ void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* 
   }
   const int last = ((n - 1) / NB) * NB;
   for (int k0 = last; k0 >= 0; k0 -= NB) {
-    dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 255) / 256));
-    k_lu_back<T, NB><<<grid, 256, 0, s>>>(A, n, ncols, stride, k0);
-    EXA_HIP(hipGetLastError());
+    if (tri_ws) {
+      k_lu_tri_inv<T, NB, false><<<batch, 64, 0, s>>>(A, n, stride, k0, tri_ws);
+      EXA_HIP(hipGetLastError());
+      apply_tri(std::min(NB, n - k0), k0, n, ncols - n);
+    } else {
+      dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 255) / 256));
+      k_lu_back<T, NB><<<grid, 256, 0, s>>>(A, n, ncols, stride, k0);
+      EXA_HIP(hipGetLastError());
+    }
     if (k0 > 0) {  // X[0:k0] -= U[0:k0, k0:k0+nb] * X[k0:k0+nb]
       const int nb = std::min(NB, n - k0);
       Gemm g{};
@@ -2258,30 +2372,30 @@ void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* 
 
 template <typename T>
 void lu_blocked_any(T* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
-                    int* piv_ws) {
+                    int* piv_ws, T* tri_ws) {
   // 32-wide panels when an n x 32 panel fits the 64 KB of LDS, else 16-wide; with a pivot
   // workspace, panels of up to 256 rows are factored in registers (64-wide register panels
   // were measured: no gain at n = 250, slower at n = 100)
   const int w = sizeof(T) == 4 ? lu_panel_width_f32(n) : lu_panel_width(n);
   if constexpr (sizeof(T) == 4) {
     if (w == 64) {
-      blocked_lu<T, 64>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
+      blocked_lu<T, 64>(A, n, ncols, stride, batch, info, pivout, s, piv_ws, tri_ws);
       return;
     }
   }
   if (w == 32)
-    blocked_lu<T, 32>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
+    blocked_lu<T, 32>(A, n, ncols, stride, batch, info, pivout, s, piv_ws, tri_ws);
   else if (static_cast<size_t>(n) * 16 * sizeof(T) <= 64000)
-    blocked_lu<T, 16>(A, n, ncols, stride, batch, info, pivout, s, piv_ws);
+    blocked_lu<T, 16>(A, n, ncols, stride, batch, info, pivout, s, piv_ws, tri_ws);
   else
     throw Error(5, "dense block larger than 500 rows");
 }
 
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
-              int* piv_ws) {
+              int* piv_ws, void* tri_ws) {
   if (batch <= 0) return;
   if ((n >= BLU_MIN_N || pivout) && ncols > n) {
-    lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws);
+    lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws, static_cast<double*>(tri_ws));
     return;
   }
   if (pivout) throw Error(5, "pivot output needs right-hand-side columns");
@@ -2293,10 +2407,10 @@ void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hi
 }
 
 void lu_solve(float* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
-              int* piv_ws) {
+              int* piv_ws, void* tri_ws) {
   if (batch <= 0) return;
   if (ncols <= n) throw Error(5, "FP32 LU needs right-hand-side columns");
-  lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws);
+  lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws, static_cast<float*>(tri_ws));
 }
 
 void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s) {

@@ -45,13 +45,17 @@ void gemm(const Gemm& g, hipStream_t s);
 // pivout (optional, [n, batch]): the panel-relative row exchanges of the blocked
 // factorisation, for refine_rhs
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s,
-              int* pivout = nullptr, int* piv_ws = nullptr);
+              int* pivout = nullptr, int* piv_ws = nullptr, void* tri_ws = nullptr);
 // The same in FP32 storage and arithmetic (blocked path only: ncols > n)
 void lu_solve(float* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s,
-              int* pivout = nullptr, int* piv_ws = nullptr);
+              int* pivout = nullptr, int* piv_ws = nullptr, void* tri_ws = nullptr);
 // piv_ws (optional, device, >= LU_PIV_WS*batch ints, owned by the caller): with it, panels of up
 // to 256 rows are factored with the panel in registers (k_lu_panel_reg + k_lu_u12)
+// tri_ws (optional, device, >= LU_TRI_WS*batch doubles, owned by the caller): with it, the
+// triangular solves of each panel (U12 = L11^{-1} A12, X = U11^{-1} X) run as an explicit
+// triangle inverse and an in-place GEMM
 constexpr int LU_PIV_WS = 64;
+constexpr long LU_TRI_WS = 64L * 64L;
 
 // ---- setup ----
 void elem_geom(const Dims& d, const double* shapegt, const double* xdg, double* egeo, hipStream_t s);

@@ -91,6 +91,15 @@ void Solver::setup() {
 }
 
 // pivot workspace of the batched LU for `batch` matrices (grown with the stream idle)
+void* Solver::lu_tri(long batch) {
+  const size_t need = static_cast<size_t>(std::max(1L, batch)) * LU_TRI_WS;
+  if (lutri.n < need) {
+    EXA_HIP(hipStreamSynchronize(s));
+    lutri.alloc(need);
+  }
+  return lutri.p;
+}
+
 int* Solver::lu_ws(long batch) {
   const size_t need = static_cast<size_t>(std::max(1L, batch)) * LU_PIV_WS;
   if (lupiv.n < need) {
@@ -288,7 +297,7 @@ void Solver::init(SolData& sol) {
       for (int e = 0; e < eb; ++e)
         EXA_HIP(hipMemcpyAsync(Mass.p + static_cast<long>(e0 + e) * d.npe * d.npe, qaug.p + e * per,
                                sizeof(double) * d.npe * d.npe, hipMemcpyDeviceToDevice, s));
-      lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s, nullptr, lu_ws(eb));
+      lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
       qpack(d, e0, eb, qaug.p, MCs.p, MEs.p, Mass.p, s);
     }
     EXA_HIP(hipStreamSynchronize(s));
@@ -806,9 +815,10 @@ void Solver::assemble() {
     tic(T_LU, &t0);
     if (f32)
       lu_solve(aug32.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s,
-               piv32.p + static_cast<long>(e0) * n, lu_ws(eb));
+               piv32.p + static_cast<long>(e0) * n, lu_ws(eb), lu_tri(eb));
     else
-      lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s, nullptr, lu_ws(eb));
+      lu_solve(aug.p, static_cast<int>(n), static_cast<int>(augcols()), augs, eb, info.p, s, nullptr, lu_ws(eb),
+               lu_tri(eb));
     toc(T_LU, t0);
     tic(T_ASSEMBLY, &t0);
     {  // H += K * DinvF (= -K X_F), Rh -= K * (D^{-1} Ru)
@@ -852,7 +862,7 @@ void Solver::assemble() {
     for (int e0 = 0; e0 < d.ne; e0 += eas_chunk) {
       const int eb = std::min(eas_chunk, d.ne - e0);
       eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
-      lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s, nullptr, lu_ws(eb));
+      lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
       EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
       EXA_HIP(hipStreamSynchronize(s));
       const int* hinfo = reinterpret_cast<const int*>(hpin);

@@ -74,7 +74,8 @@ struct Solver {
   int precond_bf16 = 0;
   DevBuf<unsigned short> BEilu16;
   DevBuf<int> piv32;  // row exchanges of the blocked LU of D' ([n, ne], panel-relative)
-  DevBuf<int> lupiv;  // this handle's pivots of the current LU panel (32 per matrix of a batched LU)
+  DevBuf<int> lupiv;     // this handle's pivots of the current LU panel (LU_PIV_WS per matrix of a batched LU)
+  DevBuf<double> lutri;  // this handle's triangle inverses of the current LU panel (LU_TRI_WS per matrix)
   // elemental additive Schwarz: inverted element blocks, inversion scratch
   DevBuf<double> BEeas, easaug;
   int eas_chunk = 0;
@@ -129,6 +130,7 @@ struct Solver {
   ~Solver();
   void set_dims();
   int* lu_ws(long batch);
+  void* lu_tri(long batch);  // triangle-inverse workspace of the blocked LU (LU_TRI_WS doubles per matrix)
   void init(SolData& sol);
   // batch (segmented) Newton-GMRES
   void seg_setup();

@@ -1117,27 +1117,32 @@ __global__ void k_vol_res(Dims d, AsmArgs a, T* out, long ldo) {
 }
 
 // W[(e*NC + c)*NCU + k][i + npe*g] = sum_dp shapegw[i, g, dp] * rJ[dp, k, c]
-// IT: index type of the flattened thread index (32-bit when the launch fits: the divisions by
-// the run-time sizes cost more than the kernel's arithmetic in 64 bits)
-template <typename T, typename IT>
+// One thread per (i, g, element) for all ncu*nc (k, c) blocks: the shape weights shapegw[i, g, :]
+// are loaded once and reused (the sum over dp runs in the same order as per block).
+template <typename T>
 __global__ void k_vol_W(Dims d, AsmArgs a, T* W) {
+  constexpr int MAXD = 4;  // nd + 1 <= 4
   const int npe = d.npe, nge = d.nge, ncu = d.ncu, nc = d.nc, nd = d.nd;
-  const IT blk = static_cast<IT>(npe) * nge;
-  const IT q = blockIdx.x * static_cast<IT>(blockDim.x) + threadIdx.x;
-  if (q >= blk * ncu * nc * a.eb) return;
-  const IT bidx = q / blk;
-  const int ig = static_cast<int>(q - bidx * blk);
+  const int blk = npe * nge;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= static_cast<long>(blk) * a.eb) return;
+  const long e = q / blk;
+  const int ig = static_cast<int>(q - e * blk);
   const int g = ig / npe, i = ig - g * npe;
-  const IT t = bidx / ncu;
-  const int k = static_cast<int>(bidx - t * ncu);
-  const IT eidx = t / nc;
-  const int c = static_cast<int>(t - eidx * nc);
-  const long e = static_cast<long>(eidx);
   const long P = static_cast<long>(a.eb) * nge;
-  double s = 0.0;
-  for (int dp = 0; dp <= nd; ++dp)
-    s += a.shapegw[i + npe * (g + nge * dp)] * a.rJ[((dp * ncu + k) * nc + c) * P + e * nge + g];
-  W[q] = static_cast<T>(s);
+  double w[MAXD];
+#pragma unroll
+  for (int dp = 0; dp < MAXD; ++dp) w[dp] = dp <= nd ? a.shapegw[i + npe * (g + nge * dp)] : 0.0;
+  const double* rJ = a.rJ + e * nge + g;
+  T* We = W + e * static_cast<long>(nc) * ncu * blk + ig;
+  for (int kc = 0; kc < ncu * nc; ++kc) {
+    const int k = kc % ncu, c = kc / ncu;
+    double s = 0.0;
+#pragma unroll
+    for (int dp = 0; dp < MAXD; ++dp)
+      if (dp <= nd) s += w[dp] * rJ[((dp * ncu + k) * nc + c) * P];
+    We[static_cast<long>(c * ncu + k) * blk] = static_cast<T>(s);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2489,9 +2494,9 @@ void vol_res(const Dims& d, const AsmArgs& a, float* out, long ldo, hipStream_t 
 
 template <typename T>
 void vol_W_t(const Dims& d, const AsmArgs& a, T* W, hipStream_t s) {
-  long n1 = static_cast<long>(d.npe) * d.nge * d.ncu * d.nc * a.eb;
-  if (n1 < (1L << 31) - 256) k_vol_W<T, unsigned><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
-  else k_vol_W<T, long><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
+  if (d.nd > 3) throw Error(5, "volume Jacobian for nd > 3");
+  long n1 = static_cast<long>(d.npe) * d.nge * a.eb;
+  k_vol_W<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
   EXA_HIP(hipGetLastError());
 }
 void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s) { vol_W_t(d, a, a.W, s); }

@@ -1718,16 +1718,17 @@ __global__ void k_update_udg(Dims d, const double* du, double alpha, double* udg
 // hdgBlockILU0 (matvec.cpp:119-166), ApplyBlockILU0 (preconditioner.cpp:184-270).
 // BE[((t*nse) + n)*ncf*ncf + r + ncf*c]: CRS entry t of superelement n.
 // ---------------------------------------------------------------------------
+// grid: x over (entry rc, superelement n) of CRS entry t = blockIdx.y (32-bit index math)
 template <typename T>
 __global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon, const double* H, T* BE) {
   const int ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf, m = d.m;
-  const long per = static_cast<long>(ncf) * ncf;
-  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (q >= per * I.nse * I.nnz) return;
-  const int rc = static_cast<int>(q % per);
-  const long tn = q / per;
-  const int n = static_cast<int>(tn % I.nse);
-  const int t = static_cast<int>(tn / I.nse);
+  const int per = ncf * ncf;
+  const unsigned qn = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qn >= static_cast<unsigned>(per) * I.nse) return;
+  const int t = blockIdx.y;
+  const int n = static_cast<int>(qn / per);
+  const int rc = static_cast<int>(qn - static_cast<unsigned>(n) * per);
+  const long q = (static_cast<long>(t) * I.nse + n) * per + rc;
   const int rr = rc % ncf, cc = rc / ncf;
   const int am = rr % ncu, bm = rr / ncu, an = cc % ncu, bn = cc / ncu;
   const int i = I.row_of[t];
@@ -2690,8 +2691,9 @@ void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipS
 template <typename T>
 void bilu_assemble_t(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H, T* BE,
                      hipStream_t s) {
-  long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse * I.nnz;
-  k_bilu_assemble<T><<<nblocks(n1, 256), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
+  const long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse;
+  if (n1 >= (1L << 32) - 256 || I.nnz > 65535) throw Error(5, "block-ILU(0) assembly grid too large");
+  k_bilu_assemble<T><<<dim3(nblocks(n1, 256), I.nnz), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
   EXA_HIP(hipGetLastError());
 }
 

@@ -1298,7 +1298,7 @@ __global__ void k_face_res(Dims d, AsmArgs a, int l, int nf1, T* ru, long ldru, 
 // K/G = int phi_a phi_b fhB_uq, H = int phi_a phi_b fhB_uh
 // (assembleMatrixBD/F/GK/H, kokkosimpl.h:1661-1730).
 template <typename T>
-__global__ void k_face_jac(Dims d, AsmArgs a, int l, T* aug, T* Bfull, T* Kt, T* Gfull, double* H) {
+__global__ void k_face_jac(Dims d, AsmArgs a, int l, T* aug, T* Bfull, T* Kt, T* Gfull, T* H) {
   const int npf = d.npf, ngf = d.ngf, ncu = d.ncu, nc = d.nc, npe = d.npe, nd = d.nd;
   const int n = d.n, m = d.m;
   const int ncol = nc + ncu;
@@ -1335,7 +1335,8 @@ __global__ void k_face_jac(Dims d, AsmArgs a, int l, T* aug, T* Bfull, T* Kt, T*
   } else {
     const int c2 = c - nc;
     aug[e * augs + pa + npe * k + static_cast<long>(n) * (n + c2 + ncu * (bb + npf * l))] += static_cast<T>(sI);
-    H[(a.e0 + e) * static_cast<long>(m) * m + trow + static_cast<long>(m) * (c2 + ncu * (bb + npf * l))] = sB;
+    H[(a.e0 + e) * static_cast<long>(m) * m + trow + static_cast<long>(m) * (c2 + ncu * (bb + npf * l))] =
+        static_cast<T>(sB);
   }
 }
 
@@ -1353,7 +1354,7 @@ constexpr int FJ_THREADS = 512;
 
 template <typename T>
 __global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a, T* aug, T* Bfull, T* Kt, T* Gfull,
-                                                              double* H) {
+                                                              T* H) {
   extern __shared__ double fj_sm[];
   const int npf = d.npf, ngf = d.ngf, ncu = d.ncu, nc = d.nc, npe = d.npe, nd = d.nd, nfe = d.nfe;
   const int n = d.n, m = d.m;
@@ -1379,7 +1380,7 @@ __global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a,
   T* BE = Bfull + e * static_cast<long>(n) * n * nd;
   T* KE = Kt + e * static_cast<long>(m) * n;
   T* GE = Gfull + e * static_cast<long>(m) * n * nd;
-  double* HE = H + (a.e0 + e) * static_cast<long>(m) * m;
+  T* HE = H + (a.e0 + e) * static_cast<long>(m) * m;
   for (int l = 0; l < nfe; ++l) {
     const long pb = (e * nfe + l) * ngf;
     __syncthreads();  // the previous face's reads of Fs and its updates are done
@@ -1429,7 +1430,7 @@ __global__ __launch_bounds__(FJ_THREADS) void k_face_jac_elem(Dims d, AsmArgs a,
               else BE[pa + npe * k + n * col] += vt;
             } else {
               const long col = (c - nc) + ncu * (bb + npf * l);
-              if (bpart) HE[trow + static_cast<long>(m) * col] = v;
+              if (bpart) HE[trow + static_cast<long>(m) * col] = vt;
               else augE[pa + npe * k + static_cast<long>(n) * (n + col)] += vt;
             }
           }
@@ -1478,7 +1479,8 @@ __global__ void k_face_gather(long nnodes, int ncu, const long* src1, const long
 }
 
 // BlockJacobi (kokkosimpl.h:1067-1111) into an augmented [BE | I] for inversion.
-__global__ void k_bj_assemble(Dims d, const int* f2e, const int* elemcon, const double* H, double* aug) {
+template <typename TH>
+__global__ void k_bj_assemble(Dims d, const int* f2e, const int* elemcon, const TH* H, double* aug) {
   const int ncf = d.ncf, ncu = d.ncu, npf = d.npf, nfe = d.nfe, m = d.m;
   const long per = static_cast<long>(ncf) * 2 * ncf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1507,7 +1509,8 @@ __global__ void k_bj_assemble(Dims d, const int* f2e, const int* elemcon, const 
 // kokkosimpl.h:1113-1142): element e's m x m block of H with the diagonal block of
 // each interior face replaced by the sum of both sides' blocks, written as the
 // augmented [block | I] (m x 2m) for the batched LU inversion.
-__global__ void k_eas_assemble(Dims d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
+template <typename TH>
+__global__ void k_eas_assemble(Dims d, int e0, int eb, const TH* H, const int* f2e, const int* elemcon,
                                double* aug) {
   const int m = d.m, ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf;
   const long per = static_cast<long>(m) * 2 * m;
@@ -1719,8 +1722,8 @@ __global__ void k_update_udg(Dims d, const double* du, double alpha, double* udg
 // BE[((t*nse) + n)*ncf*ncf + r + ncf*c]: CRS entry t of superelement n.
 // ---------------------------------------------------------------------------
 // grid: x over (entry rc, superelement n) of CRS entry t = blockIdx.y (32-bit index math)
-template <typename T>
-__global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon, const double* H, T* BE) {
+template <typename T, typename TH>
+__global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon, const TH* H, T* BE) {
   const int ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf, m = d.m;
   const int per = ncf * ncf;
   const unsigned qn = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2518,7 +2521,7 @@ void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, float* ru, long 
 }
 
 template <typename T>
-void face_jac_t(const Dims& d, const AsmArgs& a, T* aug, T* Bfull, T* Kt, T* Gfull, double* H, hipStream_t s) {
+void face_jac_t(const Dims& d, const AsmArgs& a, T* aug, T* Bfull, T* Kt, T* Gfull, T* H, hipStream_t s) {
   if (a.eb <= 0) return;
   const size_t lds = face_jac_elem_lds(d);
   if (lds <= 64000) {
@@ -2537,7 +2540,7 @@ void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, doubl
               hipStream_t s) {
   face_jac_t(d, a, aug, Bfull, Kt, Gfull, H, s);
 }
-void face_jac(const Dims& d, const AsmArgs& a, float* aug, float* Bfull, float* Kt, float* Gfull, double* H,
+void face_jac(const Dims& d, const AsmArgs& a, float* aug, float* Bfull, float* Kt, float* Gfull, float* H,
               hipStream_t s) {
   face_jac_t(d, a, aug, Bfull, Kt, Gfull, H, s);
 }
@@ -2616,18 +2619,34 @@ void face_gather(long nnodes, int ncu, const long* src1, const long* src2, const
   EXA_HIP(hipGetLastError());
 }
 
-void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double* H, double* aug,
-                 hipStream_t s) {
+template <typename TH>
+void bj_assemble_t(const Dims& d, const int* f2e, const int* elemcon, const TH* H, double* aug, hipStream_t s) {
   long n1 = static_cast<long>(d.ncf) * 2 * d.ncf * d.nf;
-  k_bj_assemble<<<nblocks(n1, 256), 256, 0, s>>>(d, f2e, elemcon, H, aug);
+  k_bj_assemble<TH><<<nblocks(n1, 256), 256, 0, s>>>(d, f2e, elemcon, H, aug);
   EXA_HIP(hipGetLastError());
 }
+void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double* H, double* aug,
+                 hipStream_t s) {
+  bj_assemble_t(d, f2e, elemcon, H, aug, s);
+}
+void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const float* H, double* aug, hipStream_t s) {
+  bj_assemble_t(d, f2e, elemcon, H, aug, s);
+}
 
+template <typename TH>
+void eas_assemble_t(const Dims& d, int e0, int eb, const TH* H, const int* f2e, const int* elemcon, double* aug,
+                    hipStream_t s) {
+  long n1 = static_cast<long>(d.m) * 2 * d.m * eb;
+  k_eas_assemble<TH><<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, H, f2e, elemcon, aug);
+  EXA_HIP(hipGetLastError());
+}
 void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
                   double* aug, hipStream_t s) {
-  long n1 = static_cast<long>(d.m) * 2 * d.m * eb;
-  k_eas_assemble<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, H, f2e, elemcon, aug);
-  EXA_HIP(hipGetLastError());
+  eas_assemble_t(d, e0, eb, H, f2e, elemcon, aug, s);
+}
+void eas_assemble(const Dims& d, int e0, int eb, const float* H, const int* f2e, const int* elemcon,
+                  double* aug, hipStream_t s) {
+  eas_assemble_t(d, e0, eb, H, f2e, elemcon, aug, s);
 }
 
 void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s) {
@@ -2688,12 +2707,12 @@ void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipS
   EXA_HIP(hipGetLastError());
 }
 
-template <typename T>
-void bilu_assemble_t(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H, T* BE,
+template <typename T, typename TH>
+void bilu_assemble_t(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const TH* H, T* BE,
                      hipStream_t s) {
   const long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse;
   if (n1 >= (1L << 32) - 256 || I.nnz > 65535) throw Error(5, "block-ILU(0) assembly grid too large");
-  k_bilu_assemble<T><<<dim3(nblocks(n1, 256), I.nnz), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
+  k_bilu_assemble<T, TH><<<dim3(nblocks(n1, 256), I.nnz), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2719,7 +2738,7 @@ void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* el
                    double* BE, hipStream_t s) {
   bilu_assemble_t(d, I, f2e, elemcon, H, BE, s);
 }
-void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
+void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const float* H,
                    float* BE, hipStream_t s) {
   bilu_assemble_t(d, I, f2e, elemcon, H, BE, s);
 }

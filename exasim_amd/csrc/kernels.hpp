@@ -111,8 +111,8 @@ void face_res(const Dims& d, const AsmArgs& a, int l, bool jac, float* ru, long 
 // Jacobian parts of every face of the block's elements (face by face, in local order)
 void face_jac(const Dims& d, const AsmArgs& a, double* aug, double* Bfull, double* Kt, double* Gfull, double* H,
               hipStream_t s);
-// the same into FP32 D/B/F, K and G blocks (H stays FP64)
-void face_jac(const Dims& d, const AsmArgs& a, float* aug, float* Bfull, float* Kt, float* Gfull, double* H,
+// the same into FP32 D/B/F, K, G and H blocks
+void face_jac(const Dims& d, const AsmArgs& a, float* aug, float* Bfull, float* Kt, float* Gfull, float* H,
               hipStream_t s);
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s);
@@ -136,12 +136,15 @@ void face_gather(long nnodes, int ncu, const long* src1, const long* src2, const
                  hipStream_t s);
 void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double* H, double* aug,
                  hipStream_t s);
+void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const float* H, double* aug, hipStream_t s);
 void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s);
 template <typename T>
 void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_t s);
 // elemental additive Schwarz: augmented [block | I] for elements [e0, e0+eb), and the
 // inverse (columns m..2m-1 after lu_solve) into BEe (the block's first element)
 void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
+                  double* aug, hipStream_t s);
+void eas_assemble(const Dims& d, int e0, int eb, const float* H, const int* f2e, const int* elemcon,
                   double* aug, hipStream_t s);
 void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s);
 template <typename T>
@@ -165,7 +168,7 @@ struct IluDev {
 };
 void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
                    double* BE, hipStream_t s);
-void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
+void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const float* H,
                    float* BE, hipStream_t s);
 // factorisation steps of face slot i, batched over the superelements: invert the
 // diagonal block (info[n] |= singular), and copy the ne_i blocks L = A_ji D^{-1}

@@ -728,7 +728,8 @@ void Solver::assemble() {
       EXA_HIP(hipMemsetAsync(Kt.p, 0, sizeof(double) * m * n * eb, s));
       EXA_HIP(hipMemsetAsync(Gfull.p, 0, sizeof(double) * m * n * nd * eb, s));
     }
-    EXA_HIP(hipMemsetAsync(H.p + e0 * m * m, 0, sizeof(double) * m * m * eb, s));
+    if (f32) EXA_HIP(hipMemsetAsync(H32.p + e0 * m * m, 0, sizeof(float) * m * m * eb, s));
+    else EXA_HIP(hipMemsetAsync(H.p + e0 * m * m, 0, sizeof(double) * m * m * eb, s));
     // volume terms
     vol_points(d, a, true, s);
     if (f32) {
@@ -770,7 +771,7 @@ void Solver::assemble() {
       if (f32) face_res(d, a, l, true, aug32.p + n * (n + m), augs, Rh.p, s);
       else face_res(d, a, l, true, aug.p + n * (n + m), augs, Rh.p, s);
     }
-    if (f32) face_jac(d, a, aug32.p, Bfull32.p, K32b, Gfull32.p, H.p, s);
+    if (f32) face_jac(d, a, aug32.p, Bfull32.p, K32b, Gfull32.p, H32.p, s);
     else face_jac(d, a, aug.p, Bfull.p, Kt.p, Gfull.p, H.p, s);
     // eliminate q: D += B*MinvC, F -= B*MinvE, K += G*MinvC, H -= G*MinvE (uequation.cpp:614-681)
     {
@@ -805,8 +806,8 @@ void Solver::assemble() {
       k.N = static_cast<int>(ndf); k.alpha = -1.0;
       opB(k, MEs.p, MEs32.p, e0 * npe * nd * ndf);
       k.sB1 = npe * nd * ndf;
-      k.Cf = nullptr;
-      k.C = H.p + e0 * m * m; k.ldc = m * ncu; k.sC0 = m; k.sC1 = m * m;
+      opC(k, H.p, H32.p, e0 * m * m);
+      k.ldc = m * ncu; k.sC0 = m; k.sC1 = m * m;
       gemm(k, s);
     }
     toc(T_ASSEMBLY, t0);
@@ -828,21 +829,21 @@ void Solver::assemble() {
       g.lda = m; g.sA0 = m * n;
       opB(g, aug.p, aug32.p, n * n);
       g.ldb = n; g.sB0 = augs;
-      g.C = H.p + e0 * m * m; g.ldc = m; g.sC0 = m * m;
+      opC(g, H.p, H32.p, e0 * m * m);
+      g.ldc = m; g.sC0 = m * m;
       g.nb0 = eb; g.nb1 = 1; g.nb2 = 1;
       g.fp32 = precision;
       gemm(g, s);
       g.N = 1;
       g.fp32 = 0;
+      g.Cf = nullptr;
       opB(g, aug.p, aug32.p, n * (n + m));
       g.C = Rh.p + e0 * m; g.sC0 = m;
       gemm(g, s);
     }
-    if (f32) {  // DinvF straight into FP32; FP32 copies of this block's H and LU factors of D'
+    if (f32) {  // DinvF straight into FP32; FP32 copy of this block's LU factors of D'
       const long E0 = e0;
       post_schur(d, e0, eb, aug32.p, augs, DinvF32.p, RuS.p, s);
-      to_float(H32.p + E0 * m * m, H.p + E0 * m * m, static_cast<int>(m), static_cast<int>(m), m, m * m, m * m,
-               eb, s);
       to_float(LU32.p + E0 * n * n, aug32.p, static_cast<int>(n), static_cast<int>(n), n, augs, n * n, eb, s);
     } else {
       post_schur(d, e0, eb, aug.p, augs, DinvF.p, RuS.p, s);
@@ -861,7 +862,8 @@ void Solver::assemble() {
     // hdgElementalAdditiveSchwarz (matvec.cpp:168-188), inverted in element chunks
     for (int e0 = 0; e0 < d.ne; e0 += eas_chunk) {
       const int eb = std::min(eas_chunk, d.ne - e0);
-      eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
+      if (precision) eas_assemble(d, e0, eb, H32.p, f2e.p, elemcon.p, easaug.p, s);
+      else eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
       lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
       EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
       EXA_HIP(hipStreamSynchronize(s));
@@ -872,14 +874,15 @@ void Solver::assemble() {
     }
   } else if (precond == 0) {
     // hdgBlockJacobi (matvec.cpp:190-217)
-    bj_assemble(d, f2e.p, elemcon.p, H.p, bjaug.p, s);
+    if (precision) bj_assemble(d, f2e.p, elemcon.p, H32.p, bjaug.p, s);
+    else bj_assemble(d, f2e.p, elemcon.p, H.p, bjaug.p, s);
     lu_solve(bjaug.p, d.ncf, 2 * d.ncf, 2L * d.ncf * d.ncf, d.nf, info.p, s, nullptr, lu_ws(d.nf));
     bj_extract(d, bjaug.p, BE.p, s);
   } else {
     // hdgBlockILU0 (matvec.cpp:119-166)
     // the mixed form assembles and factors in FP32 (its factors are applied in FP32 or BF16)
     if (precision) {
-      bilu_assemble(d, ilud, f2e.p, elemcon.p, H.p, BEilu32.p, s);
+      bilu_assemble(d, ilud, f2e.p, elemcon.p, H32.p, BEilu32.p, s);
       factor_bilu0_t(BEilu32.p, Lilu32.p);
     } else {
       bilu_assemble(d, ilud, f2e.p, elemcon.p, H.p, BEilu.p, s);
@@ -1194,7 +1197,13 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
   };
   if (name == "udg") { src = udg.p; size = static_cast<long>(udg.n); }
   else if (name == "uh") { src = uh.p; size = static_cast<long>(uh.n); }
-  else if (name == "H") { src = H.p; size = static_cast<long>(H.n); }
+  else if (name == "H" && dinvf_fp32) {  // the mixed form assembles H in FP32 only
+    std::vector<float> f(H32.n);
+    EXA_HIP(hipMemcpyAsync(f.data(), H32.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+    host.assign(f.begin(), f.end());
+    size = static_cast<long>(host.size());
+  } else if (name == "H") { src = H.p; size = static_cast<long>(H.n); }
   else if (name == "DinvF" && dinvf_fp32) {  // the mixed form keeps D^{-1}F in FP32 only
     std::vector<float> f(DinvF32.n);
     EXA_HIP(hipMemcpyAsync(f.data(), DinvF32.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost, s));

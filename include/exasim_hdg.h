@@ -72,9 +72,9 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * "timing" (1: record per-kernel HIP events), "workspace_bytes" (element-block
  * workspace budget of the assembly), "precision" (0 FP64; 1 mixed: the element blocks
  * are assembled and condensed in FP32 (FP32 MFMA products, FP32 LU of D' with its pivots
- * kept), the condensed operator, the local-solve factors and the preconditioner blocks
- * are stored in FP32, H is accumulated and residuals and GMRES vectors stay in FP64, and
- * each Newton step is refined
+ * kept), the condensed operator H, the local-solve factors and the preconditioner blocks
+ * are stored in FP32, residuals and GMRES vectors stay in FP64, and each Newton step is
+ * refined
  * with FP64 residuals until ||R|| <= "refine_rtol" (1e-10) times its first residual,
  * at most "refine_maxit" (5) corrections, each solved by GMRES to "refine_tol" (1e-4);
  * the first solve of each Newton step goes to max(gmres_tol, "refine_first_tol" (1e-7)));
@@ -107,14 +107,14 @@ int exa_gmres(exa_handle* h, const double* b_dev, double* x_dev, int* iters);
  * "ilu_row_ptr" [nfse+1], "ilu_col_ind" [nnz] (as doubles), "Mass" [npe,npe,ne],
  * "MinvC" [npe,nd,npe,ne], "MinvE" [npe,nd,ndf,ne], "elemcon" [npf*nfe*ne] (as doubles),
  * "f2e" [4*nf] (as doubles), "res_b" [N], "res_Ru" [n,ne], "x", "w" [N], "egeo",
- * "fgeo" (geometry at quadrature points).  After a mixed-precision assembly "DinvF" and
- * "BEilu" are the FP32 arrays widened to FP64.  Returns the number of doubles of the array
+ * "fgeo" (geometry at quadrature points).  After a mixed-precision assembly "H", "DinvF"
+ * and "BEilu" are the FP32 arrays widened to FP64.  Returns the number of doubles of the array
  * (copies min(len, size)). */
 long exa_get_array(exa_handle* h, const char* name, double* dst, long len);
 /* Overwrite "udg", "uh", "x" or "w" from host memory (len must equal the array size). */
 int exa_set_array(exa_handle* h, const char* name, const double* src, long len);
 /* Device pointer of a named persistent device array (H, DinvF, uh, udg, b, x, w); the FP64
- * DinvF is the one of the last FP64 assembly. */
+ * H and DinvF are those of the last FP64 assembly. */
 int exa_device_ptr(exa_handle* h, const char* name, void** ptr, long* len);
 
 /* dataout/out{udg,uhat}_np0.bin with the reference's 3-double headers. */

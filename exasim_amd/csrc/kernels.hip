@@ -2403,8 +2403,11 @@ void lu_blocked_any(T* A, int n, int ncols, long stride, int batch, int* info, h
 void lu_solve(double* A, int n, int ncols, long stride, int batch, int* info, hipStream_t s, int* pivout,
               int* piv_ws, void* tri_ws) {
   if (batch <= 0) return;
+  // the FP64 form keeps the substitution kernels (k_lu_u12, k_lu_back): explicit triangle inverses
+  // amplify rounding by the triangle's condition number, which the FP64 form is not allowed to lose
+  (void)tri_ws;
   if ((n >= BLU_MIN_N || pivout) && ncols > n) {
-    lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws, static_cast<double*>(tri_ws));
+    lu_blocked_any(A, n, ncols, stride, batch, info, s, pivout, piv_ws, static_cast<double*>(nullptr));
     return;
   }
   if (pivout) throw Error(5, "pivot output needs right-hand-side columns");

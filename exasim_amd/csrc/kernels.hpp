@@ -51,9 +51,9 @@ void lu_solve(float* A, int n, int ncols, long stride, int batch, int* info, hip
               int* pivout = nullptr, int* piv_ws = nullptr, void* tri_ws = nullptr);
 // piv_ws (optional, device, >= LU_PIV_WS*batch ints, owned by the caller): with it, panels of up
 // to 256 rows are factored with the panel in registers (k_lu_panel_reg + k_lu_u12)
-// tri_ws (optional, device, >= LU_TRI_WS*batch doubles, owned by the caller): with it, the
-// triangular solves of each panel (U12 = L11^{-1} A12, X = U11^{-1} X) run as an explicit
-// triangle inverse and an in-place GEMM
+// tri_ws (optional, device, >= LU_TRI_WS*batch doubles, owned by the caller): with it, the FP32
+// LU runs the triangular solves of each panel (U12 = L11^{-1} A12, X = U11^{-1} X) as an explicit
+// triangle inverse and an in-place GEMM (the FP64 LU always uses substitution)
 constexpr int LU_PIV_WS = 64;
 constexpr long LU_TRI_WS = 64L * 64L;
 

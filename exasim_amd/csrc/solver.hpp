@@ -48,7 +48,7 @@ struct Solver {
   double dirk_c[3] = {0, 0, 0}, dirk_d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dirk_t[3] = {0, 0, 0};
   DevBuf<double> sdg, udgprev, utmp;
   bool timing = false;
-  double ws_budget_bytes = 4.0e9;
+  double ws_budget_bytes = 16.0e9;
 
   // device data
   DevBuf<double> shapegt, shapegw, shapfgt, shapfgw, param, tau;

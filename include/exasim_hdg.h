@@ -70,7 +70,7 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * additive Schwarz, 2 superelement block-ILU(0), which needs cartgridpart in mesh.bin),
  * "ortho" (0 MGS, 1 CGS), "verbose" (1: Newton and GMRES restart lines on stdout),
  * "timing" (1: record per-kernel HIP events), "workspace_bytes" (element-block
- * workspace budget of the assembly), "precision" (0 FP64; 1 mixed: the element blocks
+ * workspace budget of the assembly, default 16e9), "precision" (0 FP64; 1 mixed: the element blocks
  * are assembled and condensed in FP32 (FP32 MFMA products, FP32 LU of D' with its pivots
  * kept), the condensed operator H, the local-solve factors and the preconditioner blocks
  * are stored in FP32, residuals and GMRES vectors stay in FP64, and each Newton step is

@@ -125,9 +125,9 @@ int exa_sizes(const exa_handle* h, int* out, int len) {
   if (!h || !h->solver || !out) return 0;
   const exa::Solver& s = *h->solver;
   const exa::Dims& d = s.d;
-  int v[18] = {d.nd, d.ncu, d.nc, d.npe, d.npf, d.nfe, d.ne, d.nf, d.n, d.m, static_cast<int>(d.N),
-               d.nge, d.ngf, d.nco, s.restart, s.precond, s.precision, s.last_refinements};
-  int k = len < 18 ? len : 18;
+  int v[19] = {d.nd, d.ncu, d.nc, d.npe, d.npf, d.nfe, d.ne, d.nf, d.n, d.m, static_cast<int>(d.N),
+               d.nge, d.ngf, d.nco, s.restart, s.precond, s.precision, s.last_refinements, s.neb};
+  int k = len < 19 ? len : 19;
   for (int i = 0; i < k; ++i) out[i] = v[i];
   return k;
 }

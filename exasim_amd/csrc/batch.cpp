@@ -341,6 +341,7 @@ int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<
           continue;
         }
         if (std::fabs(svs[i + 1]) / nrmb[sg] < tol) {
+          iters[sg] = j;  // the reference's count (gmres.cpp:546)
           gmres_back_solve(y.data(), Hs(sg), svs, i, n1);
           for (int k = 0; k <= i; ++k) ycol[static_cast<size_t>(k) * ns + sg] = y[k];
           act[sg] = 0;

@@ -876,7 +876,7 @@ __global__ void k_interp_face(Dims d, int ncomp, const double* shapfgt, const in
 // C_d[r, i] = sum_g sum_m w dphi_r/dxi_m phi_i Xx[d, m], written into the
 // augmented matrix [Mass | C_1 .. C_nd | E_1 .. E_nd] (ld npe).
 __global__ void k_qsetup_mc(Dims d, int e0, int eb, const double* shapegt, const double* shapegw,
-                            const double* egeo, double* aug) {
+                            const double* egeo, double* aug, double* Mass) {
   const int npe = d.npe, nge = d.nge, nd = d.nd;
   const long ncq = static_cast<long>(npe) + npe * nd + static_cast<long>(d.ndf) * nd;
   const long per = static_cast<long>(npe) * (npe + npe * nd);
@@ -901,6 +901,8 @@ __global__ void k_qsetup_mc(Dims d, int e0, int eb, const double* shapegt, const
     }
   }
   aug[e * (npe * ncq) + r + static_cast<long>(npe) * col] = s;
+  // the mass matrix is kept (it is factored in place in aug)
+  if (col < npe) Mass[static_cast<long>(e0 + e) * npe * npe + r + static_cast<long>(npe) * col] = s;
 }
 
 // qEquationElemFaceBlock (qequation.cpp:258-340): E_d[perm[k,l], i + npf*l] = sum w phi_k phi_i n_d jac.
@@ -928,8 +930,7 @@ __global__ void k_qsetup_e(Dims d, int e0, int eb, const double* shapfgt, const 
 }
 
 // MinvC stacked MCs[(k + npe*d) + npe*nd*j], MinvE stacked MEs[(k + npe*d) + npe*nd*t].
-__global__ void k_qpack(Dims d, int e0, int eb, const double* aug, double* MCs, double* MEs,
-                        double* Mass) {
+__global__ void k_qpack(Dims d, int e0, int eb, const double* aug, double* MCs, double* MEs) {
   const int npe = d.npe, nd = d.nd, ndf = d.ndf;
   const long ncq = static_cast<long>(npe) + npe * nd + static_cast<long>(ndf) * nd;
   const long per = static_cast<long>(npe) * nd * (npe + ndf);
@@ -2181,6 +2182,15 @@ __global__ void k_axpby(double* y, const double* x, double a, double b, long n) 
   if (i < n) y[i] = (b == 0.0) ? a * x[i] : a * x[i] + b * y[i];
 }
 
+// lowest index i < n with info[i] != 0 into out[0] (INT_MAX when there is none)
+__global__ void k_first_init(int* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0x7fffffff;
+}
+__global__ void k_first_nonzero(const int* info, long n, int* out) {
+  const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (i < n && info[i] != 0) atomicMin(out, static_cast<int>(i));
+}
+
 __global__ void k_copy(double* y, const double* x, long n) {
   const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (i < n) y[i] = x[i];
@@ -2456,18 +2466,17 @@ void interp_face(const Dims& d, int ncomp, const double* shapfgt, const int* per
 
 void qsetup(const Dims& d, int e0, int eb, const double* shapegt, const double* shapegw,
             const double* shapfgt, const double* shapfgw, const int* perm, const double* egeo,
-            const double* fgeo, double* aug, hipStream_t s) {
+            const double* fgeo, double* aug, double* Mass, hipStream_t s) {
   long n1 = static_cast<long>(d.npe) * (d.npe + d.npe * d.nd) * eb;
-  k_qsetup_mc<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, shapegt, shapegw, egeo, aug);
+  k_qsetup_mc<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, shapegt, shapegw, egeo, aug, Mass);
   long n2 = static_cast<long>(d.nfe) * d.npf * d.npf * d.nd * eb;
   k_qsetup_e<<<nblocks(n2, 256), 256, 0, s>>>(d, e0, eb, shapfgt, shapfgw, perm, fgeo, aug);
   EXA_HIP(hipGetLastError());
 }
 
-void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double* MEs, double* Mass,
-           hipStream_t s) {
+void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double* MEs, hipStream_t s) {
   long n1 = static_cast<long>(d.npe) * d.nd * (d.npe + d.ndf) * eb;
-  k_qpack<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, MCs, MEs, Mass);
+  k_qpack<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, MCs, MEs);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2815,6 +2824,12 @@ void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s) {
 
 void axpby(double* y, const double* x, double a, double b, long n, hipStream_t s) {
   k_axpby<<<nblocks(n, 256), 256, 0, s>>>(y, x, a, b, n);
+  EXA_HIP(hipGetLastError());
+}
+
+void first_nonzero(const int* info, long n, int* out, hipStream_t s) {
+  k_first_init<<<1, 64, 0, s>>>(out);
+  if (n > 0) k_first_nonzero<<<nblocks(n, 256), 256, 0, s>>>(info, n, out);
   EXA_HIP(hipGetLastError());
 }
 

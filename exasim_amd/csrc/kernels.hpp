@@ -65,12 +65,12 @@ void interp_elem(const Dims& d, int ncomp, const double* shapegt, const double* 
                  hipStream_t s);
 void interp_face(const Dims& d, int ncomp, const double* shapfgt, const int* perm, const double* src,
                  double* dst, hipStream_t s);
-// Augmented [Mass | C_1..C_nd | E_1..E_nd] for elements [e0, e0+eb).
+// Augmented [Mass | C_1..C_nd | E_1..E_nd] for elements [e0, e0+eb); the mass matrices are
+// also written to Mass [npe, npe, ne] (elements e0..e0+eb-1).
 void qsetup(const Dims& d, int e0, int eb, const double* shapegt, const double* shapegw,
             const double* shapfgt, const double* shapfgw, const int* perm, const double* egeo,
-            const double* fgeo, double* aug, hipStream_t s);
-void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double* MEs, double* Mass,
-           hipStream_t s);
+            const double* fgeo, double* aug, double* Mass, hipStream_t s);
+void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double* MEs, hipStream_t s);
 void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
           double* udg, hipStream_t s);
 
@@ -194,6 +194,8 @@ void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s);
 // y = a x + b y; with b == 0 y is only written
 void axpby(double* y, const double* x, double a, double b, long n, hipStream_t s);
 void copy(double* y, const double* x, long n, hipStream_t s);
+// out[0] = lowest i < n with info[i] != 0, or INT_MAX (device int)
+void first_nonzero(const int* info, long n, int* out, hipStream_t s);
 void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
                hipStream_t s);
 void multidot(const double* V, long ldv, int k, const double* w, long n, double* partial, double* out,

@@ -128,6 +128,9 @@ void Solver::init(SolData& sol) {
   if (static_cast<long>(sol.xdg.size()) != static_cast<long>(d.npe) * d.nd * d.ne)
     throw Error(2, "xdg must have ncx = nd components");
   d.nco = sol.nco;
+  // the point kernels hold the auxiliary fields of a quadrature point in registers
+  if (d.nco > 8)
+    throw Error(5, "sol.bin has " + std::to_string(d.nco) + " auxiliary fields (vdg); at most 8 are supported");
   if (model >= 100) {  // a generated model is compiled for one (nd, ncu)
     int nd, ncu, nv;
     generated_model(model_name, &nd, &ncu, &nv);
@@ -292,13 +295,12 @@ void Solver::init(SolData& sol) {
     for (int e0 = 0; e0 < d.ne; e0 += qb) {
       int eb = std::min(qb, d.ne - e0);
       EXA_HIP(hipMemsetAsync(qaug.p, 0, static_cast<size_t>(eb) * per * sizeof(double), s));
-      qsetup(d, e0, eb, shapegt.p, shapegw.p, shapfgt.p, shapfgw.p, perm.p, egeo.p, fgeo.p, qaug.p, s);
-      // keep the mass matrix before it is factorised
-      for (int e = 0; e < eb; ++e)
-        EXA_HIP(hipMemcpyAsync(Mass.p + static_cast<long>(e0 + e) * d.npe * d.npe, qaug.p + e * per,
-                               sizeof(double) * d.npe * d.npe, hipMemcpyDeviceToDevice, s));
+      qsetup(d, e0, eb, shapegt.p, shapegw.p, shapfgt.p, shapfgw.p, perm.p, egeo.p, fgeo.p, qaug.p, Mass.p,
+             s);
       lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
-      qpack(d, e0, eb, qaug.p, MCs.p, MEs.p, Mass.p, s);
+      const int bad = first_nonzero(info.p, eb);
+      if (bad >= 0) throw Error(4, "singular mass matrix at element " + std::to_string(e0 + bad));
+      qpack(d, e0, eb, qaug.p, MCs.p, MEs.p, s);
     }
     EXA_HIP(hipStreamSynchronize(s));
   }
@@ -437,8 +439,8 @@ void Solver::alloc_mixed() {
   LU32.alloc(n * n * ne);
   piv32.alloc(n * ne);
   K32.alloc(m * n * ne);
-  // FP32 element-block workspace of the mixed-form assembly, FP32 copies of its FP64 operands
-  aug32.alloc(static_cast<size_t>(neb) * n * augcols());
+  // FP32 copies of the FP64 operands of the mixed-form assembly (its FP32 element-block
+  // workspace is sized by alloc_workspace)
   if (!q32_ready) {
     MCs32.alloc(MCs.n);
     MEs32.alloc(MEs.n);
@@ -449,9 +451,6 @@ void Solver::alloc_mixed() {
     to_float(shapegt32.p, shapegt.p, static_cast<int>(shapegt.n), 1, static_cast<long>(shapegt.n), 0, 0, 1, s);
     q32_ready = true;
   }
-  Bfull32.alloc(static_cast<size_t>(neb) * n * n * d.nd);
-  Gfull32.alloc(static_cast<size_t>(neb) * m * n * d.nd);
-  W32.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
   if (precond == 0) BE32.alloc(ncf2 * d.nf);
   if (precond == 1) BEeas32.alloc(m * m * ne);
   if (precond == 2) {  // FP32 factorisation (and its factors), BF16 copy of the factors
@@ -513,21 +512,41 @@ void Solver::setup_bilu0() {
   ilu_ready = true;
 }
 
+// Element-block workspace of the assembly, sized to ws_budget_bytes.  The point values (rv, rJ,
+// fpt) are FP64 in both forms; the block matrices W, [D | F | Ru], B and G are FP64 in the FP64
+// form and FP32 in the mixed form (which assembles K in place in its persistent K32), so the
+// mixed form fits about twice the elements per block in the same budget.
 void Solver::alloc_workspace() {
   const long n = d.n, m = d.m;
-  long per = n * augcols() + n * n * d.nd + m * n + m * n * d.nd +
-             static_cast<long>(d.nd + 1) * d.ncu * d.nge + static_cast<long>(d.nd + 1) * d.ncu * d.nc * d.nge +
-             static_cast<long>(d.npe) * d.nge * d.ncu * d.nc +
-             2L * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf;
-  long cap = static_cast<long>(ws_budget_bytes / 8.0 / per);
+  const long pts = static_cast<long>(d.nd + 1) * d.ncu * d.nge + static_cast<long>(d.nd + 1) * d.ncu * d.nc * d.nge +
+                   2L * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf;
+  const long blocks = n * augcols() + n * n * d.nd + m * n * d.nd + static_cast<long>(d.npe) * d.nge * d.ncu * d.nc;
+  const double per = precision ? 8.0 * pts + 4.0 * blocks : 8.0 * (pts + blocks + m * n);
+  long cap = static_cast<long>(ws_budget_bytes / per);
   neb = static_cast<int>(std::max(1L, std::min<long>(d.ne, cap)));
-  aug.alloc(static_cast<size_t>(neb) * n * augcols());
-  Bfull.alloc(static_cast<size_t>(neb) * n * n * d.nd);
-  Kt.alloc(static_cast<size_t>(neb) * m * n);
-  Gfull.alloc(static_cast<size_t>(neb) * m * n * d.nd);
+  if (precision) {
+    aug.release();
+    Bfull.release();
+    Kt.release();
+    Gfull.release();
+    W.release();
+    aug32.alloc(static_cast<size_t>(neb) * n * augcols());
+    Bfull32.alloc(static_cast<size_t>(neb) * n * n * d.nd);
+    Gfull32.alloc(static_cast<size_t>(neb) * m * n * d.nd);
+    W32.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
+  } else {
+    aug32.release();
+    Bfull32.release();
+    Gfull32.release();
+    W32.release();
+    aug.alloc(static_cast<size_t>(neb) * n * augcols());
+    Bfull.alloc(static_cast<size_t>(neb) * n * n * d.nd);
+    Kt.alloc(static_cast<size_t>(neb) * m * n);
+    Gfull.alloc(static_cast<size_t>(neb) * m * n * d.nd);
+    W.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
+  }
   rv.alloc(static_cast<size_t>(neb) * (d.nd + 1) * d.ncu * d.nge);
   rJ.alloc(static_cast<size_t>(neb) * (d.nd + 1) * d.ncu * d.nc * d.nge);
-  W.alloc(static_cast<size_t>(neb) * d.npe * d.nge * d.ncu * d.nc);
   fpt.alloc(static_cast<size_t>(neb) * 2 * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf);
   V.alloc(static_cast<size_t>(restart + 1) * d.N);
   hcol.alloc(restart + 2);
@@ -684,6 +703,15 @@ double Solver::norm(const double* v, long n) {
   return hpin[0];
 }
 
+int Solver::first_nonzero(const int* inf, long n) {
+  if (!ifirst.p) ifirst.alloc(1);
+  ::exa::first_nonzero(inf, n, ifirst.p, s);
+  EXA_HIP(hipMemcpyAsync(hpin, ifirst.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  const int k = *reinterpret_cast<const int*>(hpin);
+  return k == 0x7fffffff ? -1 : k;
+}
+
 void Solver::reset() {
   copy(udg.p, udg0.p, static_cast<long>(udg.n), s);
   copy(uh.p, uh0.p, static_cast<long>(uh.n), s);
@@ -707,8 +735,8 @@ void Solver::assemble() {
   hipEvent_t t0{};
   // The mixed form assembles the element blocks (W, [D | F | Ru], B, K, G) in FP32 storage:
   // its operator, factors and u-recovery matrices are all FP32, and the Schur products and
-  // the LU move half the bytes.  H and Rh stay FP64 (H32 is rounded from H; the
-  // preconditioner is assembled from H).
+  // the LU move half the bytes.  H is assembled in place in its FP32 copy H32 (the
+  // preconditioner is built from H32); Rh stays FP64.
   const bool f32 = precision != 0;
   // operand of a GEMM: element offset `off` into the FP64 or the FP32 buffer
   auto opA = [&](Gemm& g, double* pd, float* pf, long off) { if (f32) g.Af = pf + off; else g.A = pd + off; };
@@ -849,11 +877,8 @@ void Solver::assemble() {
       post_schur(d, e0, eb, aug.p, augs, DinvF.p, RuS.p, s);
     }
     toc(T_ASSEMBLY, t0);
-    EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
-    EXA_HIP(hipStreamSynchronize(s));
-    const int* hinfo = reinterpret_cast<const int*>(hpin);
-    for (int e = 0; e < std::min(eb, 8192); ++e)
-      if (hinfo[e]) throw Error(4, "singular element matrix D at element " + std::to_string(e0 + e));
+    const int bad = first_nonzero(info.p, eb);
+    if (bad >= 0) throw Error(4, "singular element matrix D at element " + std::to_string(e0 + bad));
   }
   // hdgAssembleRHS (matvec.cpp:103-117)
   face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
@@ -865,11 +890,8 @@ void Solver::assemble() {
       if (precision) eas_assemble(d, e0, eb, H32.p, f2e.p, elemcon.p, easaug.p, s);
       else eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
       lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
-      EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * std::min(eb, 8192), hipMemcpyDeviceToHost, s));
-      EXA_HIP(hipStreamSynchronize(s));
-      const int* hinfo = reinterpret_cast<const int*>(hpin);
-      for (int e = 0; e < std::min(eb, 8192); ++e)
-        if (hinfo[e]) throw Error(4, "singular Schwarz block at element " + std::to_string(e0 + e));
+      const int bad = first_nonzero(info.p, eb);
+      if (bad >= 0) throw Error(4, "singular Schwarz block at element " + std::to_string(e0 + bad));
       eas_extract(d, eb, easaug.p, BEeas.p + static_cast<long>(e0) * d.m * d.m, s);
     }
   } else if (precond == 0) {
@@ -902,12 +924,10 @@ void Solver::assemble() {
     }
   }
   if (precond != 1) {
-    const int nchk = std::min(precond == 2 ? ilu.nse : d.nf, 8192);
-    EXA_HIP(hipMemcpyAsync(hpin, info.p, sizeof(int) * nchk, hipMemcpyDeviceToHost, s));
-    EXA_HIP(hipStreamSynchronize(s));
-    const int* hinfo = reinterpret_cast<const int*>(hpin);
-    for (int k = 0; k < nchk; ++k)
-      if (hinfo[k]) throw Error(4, "singular preconditioner block");
+    const int bad = first_nonzero(info.p, precond == 2 ? ilu.nse : d.nf);
+    if (bad >= 0)
+      throw Error(4, std::string("singular preconditioner block at ") +
+                         (precond == 2 ? "superelement " : "face ") + std::to_string(bad));
   }
   assembled = true;
 }
@@ -1016,7 +1036,7 @@ int Solver::gmres(const double* bv, double* xv, double tol_rel) {
         EXA_HIP(hipMemcpyAsync(ydev.p, hpin, sizeof(double) * (i + 1), hipMemcpyHostToDevice, s));
         gemv_cols(xv, V.p, N, i + 1, ydev.p, 1.0, N, s);
         EXA_HIP(hipStreamSynchronize(s));
-        return j + 1;
+        return j;  // the reference's count: j before this step's increment (gmres.cpp:546)
       }
     }
     gmres_back_solve(y.data(), Hm.data(), sv.data(), i - 1, n1);

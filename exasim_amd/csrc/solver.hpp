@@ -52,7 +52,7 @@ struct Solver {
 
   // device data
   DevBuf<double> shapegt, shapegw, shapfgt, shapfgw, param, tau;
-  DevBuf<int> perm, elemcon, bf, f2e, info;
+  DevBuf<int> perm, elemcon, bf, f2e, info, ifirst;
   std::vector<int> bface_h;  // boundary faces E*nfe + l (bf > 0), ascending
   DevBuf<int> bface;
   DevBuf<long> src1, src2;
@@ -176,6 +176,8 @@ struct Solver {
   void alloc_workspace();
   AsmArgs asm_args(int e0, int eb);
   double norm(const double* x, long n);
+  // index of the first nonzero entry of the device array info[0..n), or -1 (synchronises)
+  int first_nonzero(const int* info, long n);
   void tic(int cls, hipEvent_t* a);
   void toc(int cls, hipEvent_t a);
   void collect_timings();

@@ -134,7 +134,7 @@ class HDGSolver:
         return out
 
     SIZE_KEYS = ["nd", "ncu", "nc", "npe", "npf", "nfe", "ne", "nf", "n", "m", "N", "nge", "ngf",
-                 "nco", "restart", "preconditioner", "precision", "refinements"]
+                 "nco", "restart", "preconditioner", "precision", "refinements", "neb"]
 
     def state(self):
         """exa_sizes: the sizes plus the current preconditioner, precision and the number of

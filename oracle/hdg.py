@@ -791,7 +791,7 @@ def gmres(matvec, precond, b, restart, maxit, tol, nonlinear_tol, ortho=0, x0=No
             if rel < tol:
                 y = back_solve(Hm, s, i)
                 x += V[:i + 1].T @ y
-                return x, j + 1
+                return x, j                             # the reference's count (gmres.cpp:546)
             i += 1
             j += 1
         y = back_solve(Hm, s, i - 1)

@@ -118,3 +118,36 @@ def test_batch_isolates_a_failed_instance(cd_batch, tmp_path):
     for k, p in ((0, cd_batch[0]), (2, cd_batch[1])):
         alone, _ = _solve_batch([p])
         assert np.array_equal(out[k][0], alone[0][0]), k
+
+
+@pytest.fixture(scope="module")
+def cd_batch_cgs(tmp_path_factory):
+    """The batch workload's solver settings (elemental Schwarz, classical Gram-Schmidt,
+    GMRES tolerance 1e-10, Newton 1e-8) with a restart of 5, so that restarts and the
+    segmented multi-vector dot of four and more vectors are exercised."""
+    from exasim_amd import cases
+    root = tmp_path_factory.mktemp("cdbatch_cgs")
+    paths = []
+    for i, kw in enumerate(INSTANCES):
+        p = str(root / ("i%d" % i))
+        cases.convdiff2d_param(p, porder=3, precond="eas", restart=5, gmres_tol=1e-10, newton_tol=1e-8,
+                               ortho=1, **kw)
+        paths.append(p)
+    return paths
+
+
+def test_batch_cgs_matches_oracle(cd_batch_cgs):
+    say("cd2d batch, CGS, Schwarz, GMRES(5): every instance vs the oracle (FP64 1e-10, mixed 1e-8)")
+    f64, stats = _solve_batch(cd_batch_cgs)
+    mix, _ = _solve_batch(cd_batch_cgs, precision=1)
+    for i, p in enumerate(cd_batch_cgs):
+        P = ora.HDGProblem(p, "cd2d")
+        assert P.ortho == 1 and P.restart == 5
+        hist = ora.newton_solve(P)
+        ref = P.uh.reshape(-1)
+        say("  instance %d: gmres gpu %d oracle %d" % (i, stats[1][i], sum(hist["gmres_iters"])))
+        assert stats[1][i] == sum(hist["gmres_iters"]), i
+        assert rel(f64[i][0], ref) < 1e-10, i
+        udg = f64[i][1].reshape(P.ne, P.nc, P.npe).transpose(0, 2, 1)
+        assert rel(udg, P.udg) < 1e-10, i
+        assert rel(mix[i][0], ref) < 1e-8, i

@@ -149,19 +149,23 @@ def test_matvec_precond_gmres(built, name):
     assert rel(xg, xo) < 1e-9
 
 
+@pytest.mark.parametrize("ortho", [0, 1])
 @pytest.mark.parametrize("name", ["c1", "c2r"])
-def test_solve_matches_oracle(built, name):
+def test_solve_matches_oracle(built, name, ortho):
     path, model = built(name)
-    say("case %s: full Newton solve" % name)
+    say("case %s: full Newton solve, %s" % (name, "CGS" if ortho else "MGS"))
     P = ora.HDGProblem(path, model)
-    ora.newton_solve(P)
+    P.ortho = ortho
+    hist = ora.newton_solve(P)
     with gpu_solver(path, model) as S:
+        S.set_option("ortho", ortho)
         it, gits, nrm = S.solve()
         z = S.sizes
         uh = S.get("uh")
         udg = S.get("udg").reshape(z["ne"], z["nc"], z["npe"]).transpose(0, 2, 1)
     assert rel(uh, P.uh.reshape(-1)) < RTOL
     assert rel(udg, P.udg) < RTOL
+    assert gits == sum(hist["gmres_iters"])
     if name == "c1":
         xe = P.xdg.reshape(-1, 2)
         assert np.abs(udg[:, :, 0].reshape(-1) - P.model.exact(xe)).max() < 1e-5
@@ -265,5 +269,5 @@ def test_elemental_schwarz_matches_oracle(built, name):
     xo, its_o = ora.gmres(lambda q: P.matvec(S_["H"], q), P.make_preconditioner(S_["H"]), b, P.restart,
                           P.gmres_maxit, P.gmres_tol, P.newton_tol, P.ortho)
     say("  gmres iterations: gpu %d oracle %d" % (its, its_o))
-    assert abs(its - its_o) <= 1
+    assert its == its_o
     assert rel(xg, xo) < 1e-9

@@ -1,7 +1,8 @@
 """The served form (exasim_amd/serve.py): a timed stream of Navier-Stokes and
 convection-diffusion requests of uneven size and order, admitted by HBM budget into
 concurrent batches.  Every request completes, at least one batch holds several requests,
-and each result is bitwise the result of the same request solved alone."""
+each result is bitwise the result of the same request solved alone, and each agrees with the FP64
+oracle (oracle/hdg.py, the same datain) to 1e-8 relative, the served form's stated tolerance."""
 import sys
 
 import numpy as np
@@ -26,3 +27,16 @@ def test_served_results_equal_alone(tmp_path):
             S.solve()
             uh = S.get_instance(0, "uh")
         assert np.array_equal(uh, srv.results[rid]), rid
+    from oracle import hdg as ora
+    worst = 0.0
+    for rid, path, model, t in reqs:
+        P = ora.HDGProblem(path, model)
+        if P.tdep:
+            ora.dirk_solve(P)
+        else:
+            ora.newton_solve(P)
+        ref = P.uh.reshape(-1)
+        err = np.abs(srv.results[rid] - ref).max() / np.abs(ref).max()
+        worst = max(worst, err)
+        assert err < 1e-8, (rid, model, err)
+    print("served results vs FP64 oracle: largest relative error %.2e" % worst, flush=True)

@@ -114,7 +114,7 @@ def batch_cpu_sample(budget_s=20.0, nmax=32):
 BATCH_KW = dict(precond="eas", restart=100, gmres_tol=1e-10, newton_tol=1e-8, ortho=1)
 
 
-def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True):
+def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurrent=True):
     """The batch workload: ``count`` parametrized 2D convection-diffusion instances (n x n quads,
     n in {8..64}, p in {2..5}, kappa log-uniform [1e-3, 1], velocity angle uniform, source
     sin(k pi x) cos(k pi y)), one batch handle per polynomial order, the four batches solved
@@ -167,11 +167,15 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True):
             def go(p, h):
                 h.solve()
                 done[p] = time.perf_counter() - t1
-            th = [threading.Thread(target=go, args=(p, h)) for p, h in hs.items()]
-            for t in th:
-                t.start()
-            for t in th:
-                t.join()
+            if concurrent:
+                th = [threading.Thread(target=go, args=(p, h)) for p, h in hs.items()]
+                for t in th:
+                    t.start()
+                for t in th:
+                    t.join()
+            else:  # one batch after the other on one host thread
+                for p, h in hs.items():
+                    go(p, h)
             wall = time.perf_counter() - t1
             if k >= warmup:
                 walls.append(wall)
@@ -277,6 +281,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-MiB device buffer 3 times with an elementwise kernel "
                          "(the scale for HBM byte counters)")
+    ap.add_argument("--batch-serial", action="store_true",
+                    help="solve the batch workload's per-order batches one after the other (default: concurrently)")
+    ap.add_argument("--gmres-graph", type=int, default=1, choices=[0, 1],
+                    help="replay the GMRES steps from captured HIP graphs (default) or launch them one by one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4)
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
@@ -299,6 +307,8 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path (the domain "
               "decomposition design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
         return 2
+    from exasim_amd import hdg as _hdg
+    _hdg.DEFAULT_OPTIONS["gmres_graph"] = args.gmres_graph
 
     from exasim_amd.hdg import HDGSolver
 
@@ -315,7 +325,8 @@ def main():
             "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"], "serve": sv, "roofline": None, "cpu_baseline": None}))
         return 0
     if args.workload == "batch":
-        b = run_batch(args.steps, args.warmup, cpu_baseline=not args.no_cpu_baseline)
+        b = run_batch(args.steps, args.warmup, cpu_baseline=not args.no_cpu_baseline,
+                      concurrent=not args.batch_serial)
         m = b["batched_mixed"]
         print(json.dumps({
             "metric": METRIC, "value": m["solves_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,

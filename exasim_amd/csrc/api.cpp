@@ -5,6 +5,7 @@
 #include "exasim_hdg.h"
 #include "solver.hpp"
 
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -205,6 +206,66 @@ int exa_reset_timings(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.r
 
 int exa_sync(exa_handle* h) {
   return guard(h, [&](exa::Solver& s) { EXA_HIP(hipStreamSynchronize(s.s)); });
+}
+
+// The reference library's entry point (backend/Main/libexasim.cpp: libmain -> premain,
+// backend/Main/exasim.hpp:143-360 without text2code): argv = {program, model, nummodels, filein,
+// fileout [, restart]}; filein is the datain prefix (a directory, "datain/"), fileout the output
+// prefix ("dataout/" -> dataout/udg_np0.bin, dataout/uhat_np0.bin).  One model (nummodels = 1) and
+// no restart file; the model is a built-in or generated model name, where the reference's
+// built-in "Poisson3D" (backend/Main/BuiltIn/Poisson3D: flux mu q, source 3 pi^2 sin sin sin,
+// Dirichlet zero) is the built-in "poisson".  Returns 0 or an EXA_ERR_* code.
+int libmain(int argc, char** argv) {
+  if (argc < 5 || !argv) {
+    std::fprintf(stderr, "Usage: %s model nummodels InputFile OutputFile [restart]\n",
+                 (argc > 0 && argv && argv[0]) ? argv[0] : "libmain");
+    return EXA_ERR_ARG;
+  }
+  std::string model = argv[1] ? argv[1] : "";
+  if (model == "Poisson3D" || model == "Poisson2D") model = "poisson";
+  int nummodels = 0, restart = 0;
+  try {
+    nummodels = std::stoi(argv[2]);
+  } catch (...) {
+    std::fprintf(stderr, "Error: invalid nummodels argument\n");
+    return EXA_ERR_ARG;
+  }
+  if (nummodels != 1) {
+    std::fprintf(stderr, "Error: %d models: coupled multi-model problems are not supported\n", nummodels);
+    return EXA_ERR_UNSUPPORTED;
+  }
+  // the optional restart step follows the in/out pair (premain's argv[2*nummodels + 2])
+  try {
+    if (argc > 5) restart = std::stoi(argv[5]);
+  } catch (...) {
+    std::fprintf(stderr, "Error: invalid restart argument\n");
+    return EXA_ERR_ARG;
+  }
+  if (restart != 0) {
+    std::fprintf(stderr, "Error: restarting from a saved time step is not supported\n");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  std::string filein = argv[3] ? argv[3] : "";
+  while (filein.size() > 1 && filein.back() == '/') filein.pop_back();
+  const std::string fileout = argv[4] ? argv[4] : "";
+  std::printf("Running model %s: datain %s, output prefix %s\n", model.c_str(), filein.c_str(), fileout.c_str());
+  exa_handle* h = nullptr;
+  int rc = exa_create(filein.c_str(), fileout.c_str(), model.c_str(), -1, &h);
+  if (rc != EXA_OK) {
+    std::fprintf(stderr, "Error: %s\n", exa_last_error(h));
+    exa_destroy(h);
+    return rc;
+  }
+  exa_set_option(h, "verbose", 1);
+  int newton = 0, gmres = 0;
+  double norm = 0.0;
+  rc = exa_solve(h, &newton, &gmres, &norm);
+  if (rc == EXA_OK) rc = exa_write_output(h);
+  if (rc != EXA_OK) std::fprintf(stderr, "Error: %s\n", exa_last_error(h));
+  else std::printf("Newton iterations %d, GMRES iterations %d, residual norm %g\n", newton, gmres, norm);
+  std::fflush(stdout);
+  exa_destroy(h);
+  return rc;
 }
 
 }  // extern "C"

@@ -173,8 +173,7 @@ void Solver::seg_setup() {
   }
   build_seg(segT, ot);
   build_seg(segU, ou);
-  if (!batch) return;
-  dus.alloc(static_cast<size_t>(d.n) * d.ne);
+  if (batch) dus.alloc(static_cast<size_t>(d.n) * d.ne);
   seg_alloc();
 }
 
@@ -193,6 +192,26 @@ void Solver::seg_alloc() {
   segy.alloc(static_cast<size_t>(n1) * ns);
   segcoef.alloc(static_cast<size_t>(4 * ns));
   segmask.alloc(static_cast<size_t>(2 * ns));
+  // device-resident GMRES state (the Krylov scalars of every instance)
+  gHm.alloc(static_cast<size_t>(ns) * n1 * n1);
+  gsv.alloc(static_cast<size_t>(ns) * (n1 + 1));
+  gcs.alloc(static_cast<size_t>(ns) * (n1 + 1));
+  gsn.alloc(static_cast<size_t>(ns) * (n1 + 1));
+  gnrmb.alloc(ns);
+  gjc.alloc(ns);
+  giters.alloc(ns);
+  gtol.alloc(1);
+  gjcount.alloc(1);
+  clear_graphs();  // captured launches hold the old buffers
+  if (!hstat) {  // per-step counts of iterating instances, written by the kernels into host memory
+    EXA_HIP(hipHostMalloc(reinterpret_cast<void**>(&hstat), sizeof(int) * GM_RING,
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    EXA_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&dstat), hstat, 0));
+    for (int k = 0; k < GM_RING; ++k) {
+      hstat[k] = -1;
+      EXA_HIP(hipEventCreateWithFlags(&gm_ev[k], hipEventDisableTiming));
+    }
+  }
   const long need = (n1 + 1) * ns + 2 * ns + n1 * ns + 2 * ns;
   if (need > hseg_len) {
     if (hseg) EXA_HIP(hipHostFree(hseg));
@@ -227,171 +246,6 @@ std::vector<double> Solver::residual_seg() {
   std::vector<double> out(ninst);
   for (int i = 0; i < ninst; ++i) out[i] = h[i] + h[ninst + i];
   return out;
-}
-
-// Restarted left-preconditioned GMRES, one Krylov recurrence per instance over shared
-// launches.  iters[i] = Arnoldi steps instance i took; returns the largest.
-int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<int>& iters) {
-  const long N = d.N;
-  const int ns = ninst;
-  const SegMap& m = segT.map;
-  SegMap ma = m;  // the same blocks, skipping instances that have left GMRES (segmask)
-  ma.act = segmask.p;
-  const int n1 = restart + 1;
-  iters.assign(ns, 0);
-  if (static_cast<long>(hcol.n) < static_cast<long>(n1 + 1) * ns) seg_alloc();
-  double* h = host_region(0);        // pulled columns and norms
-  double* hm = host_region(1);       // [0, ns) mask, [ns, 2ns) coefficients
-  double* hy = host_region(2);       // y
-  // every host-side write to a region below comes after a stream synchronisation (a pull)
-  // that followed the region's previous upload
-  auto pull = [&](const double* src, long cnt) {
-    EXA_HIP(hipMemcpyAsync(h, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
-    EXA_HIP(hipStreamSynchronize(s));
-  };
-  auto push_mask = [&](const std::vector<char>& act) {
-    for (int i = 0; i < ns; ++i) hm[i] = act[i] ? 1.0 : 0.0;
-    EXA_HIP(hipMemcpyAsync(segmask.p, hm, sizeof(double) * ns, hipMemcpyHostToDevice, s));
-  };
-
-  EXA_HIP(hipMemsetAsync(xv, 0, sizeof(double) * N, s));
-  seg_norms(segT, bv, segcoef.p);
-  pull(segcoef.p, ns);
-  std::vector<char> act(ns, 0);
-  int nact = 0;
-  for (int i = 0; i < ns; ++i) {
-    const double nb = h[i];
-    act[i] = std::isfinite(nb) && !(nb == 0.0 || (nb < newton_tol && tol_rel < 0.0));
-    nact += act[i];
-  }
-  if (!nact) return 0;
-  const double tol = std::min(0.1, tol_rel < 0.0 ? gmres_tol : tol_rel);
-  // the matvec and the Schwarz preconditioner skip the elements of inactive instances (their
-  // vectors are zero); segmask holds the activity from here on
-  struct ActiveScope {
-    const double*& p;
-    ActiveScope(const double*& q, const double* v) : p(q) { p = v; }
-    ~ActiveScope() { p = nullptr; }
-  } scope(elem_active, einst.p ? segmask.p : nullptr);
-  push_mask(act);  // before the first product that reads it
-  copy(r.p, bv, N, s);
-  apply_precond(r.p);
-  seg_scale(m, r.p, r.p, segmask.p, 0, s);
-  seg_norms(segT, r.p, segcoef.p);
-  pull(segcoef.p, ns);
-  std::vector<double> nrmb(h, h + ns), nrmr = nrmb;
-  bool drop = false;  // a non-finite preconditioned residual: the instance leaves GMRES (x = 0)
-  for (int i = 0; i < ns; ++i)
-    if (act[i] && !std::isfinite(nrmb[i])) {
-      act[i] = 0;
-      --nact;
-      drop = true;
-    }
-  if (drop) {
-    push_mask(act);
-    seg_scale(m, r.p, r.p, segmask.p, 0, s);
-    if (!nact) return 0;
-  }
-  std::vector<double> Hm(static_cast<size_t>(ns) * n1 * n1, 0.0), sv(static_cast<size_t>(ns) * (n1 + 1)),
-      cs(static_cast<size_t>(ns) * (n1 + 1)), sn(static_cast<size_t>(ns) * (n1 + 1)), y(n1 + 1), Hc(n1 + 1);
-  auto Hs = [&](int sg) { return Hm.data() + static_cast<size_t>(sg) * n1 * n1; };
-  int j = 0;
-  hipEvent_t t0{};
-  while (j < gmres_maxit && nact > 0) {
-    // V_0 = r / ||r|| per active instance, zero elsewhere
-    for (int i = 0; i < ns; ++i) hm[ns + i] = act[i] ? nrmr[i] : 0.0;
-    EXA_HIP(hipMemcpyAsync(segcoef.p + ns, hm + ns, sizeof(double) * ns, hipMemcpyHostToDevice, s));
-    seg_scale(m, V.p, r.p, segcoef.p + ns, 1, s);
-    std::fill(sv.begin(), sv.end(), 0.0);
-    for (int i = 0; i < ns; ++i) sv[static_cast<size_t>(i) * (n1 + 1)] = act[i] ? nrmr[i] : 0.0;
-    int i = 0;
-    for (; i < restart && j < gmres_maxit; ++i, ++j) {
-      double* vi = V.p + N * i;
-      double* vm = V.p + N * (i + 1);
-      matvec(vi, vm);
-      apply_precond(vm);
-      tic(T_ORTHO, &t0);
-      if (ortho == 0) {  // MGS, per instance
-        for (int k = 0; k <= i; ++k) {
-          seg_multidot(ma, V.p + N * k, N, 1, vm, segpart.p, hcol.p + static_cast<long>(k) * ns, 0, s);
-          seg_axpy(ma, vm, V.p + N * k, hcol.p + static_cast<long>(k) * ns, -1.0, s);
-        }
-      } else {  // CGS, per instance
-        seg_multidot(ma, V.p, N, i + 1, vm, segpart.p, hcol.p, 0, s);
-        seg_gemv_cols(ma, vm, V.p, N, i + 1, hcol.p, -1.0, s);
-      }
-      seg_multidot(ma, vm, N, 1, vm, segpart.p, hcol.p + static_cast<long>(i + 1) * ns, 1, s);
-      seg_scale(m, vm, vm, hcol.p + static_cast<long>(i + 1) * ns, 1, s);
-      toc(T_ORTHO, t0);
-      pull(hcol.p, static_cast<long>(i + 2) * ns);
-      bool any = false;
-      std::vector<double> ycol(static_cast<size_t>(i + 1) * ns, 0.0);
-      for (int sg = 0; sg < ns; ++sg) {
-        if (!act[sg]) continue;
-        for (int k = 0; k < i + 2; ++k) Hc[k] = h[static_cast<long>(k) * ns + sg];
-        double* svs = sv.data() + static_cast<size_t>(sg) * (n1 + 1);
-        gmres_givens(Hc.data(), svs, cs.data() + static_cast<size_t>(sg) * (n1 + 1),
-                   sn.data() + static_cast<size_t>(sg) * (n1 + 1), i);
-        for (int k = 0; k < n1; ++k) Hs(sg)[k + static_cast<size_t>(n1) * i] = (k < i + 2) ? Hc[k] : 0.0;
-        iters[sg] = j + 1;
-        if (!std::isfinite(svs[i + 1])) {  // breakdown to NaN/Inf: the instance leaves GMRES
-          act[sg] = 0;
-          --nact;
-          any = true;
-          continue;
-        }
-        if (std::fabs(svs[i + 1]) / nrmb[sg] < tol) {
-          iters[sg] = j;  // the reference's count (gmres.cpp:546)
-          gmres_back_solve(y.data(), Hs(sg), svs, i, n1);
-          for (int k = 0; k <= i; ++k) ycol[static_cast<size_t>(k) * ns + sg] = y[k];
-          act[sg] = 0;
-          --nact;
-          any = true;
-        }
-      }
-      if (any) {  // x += V y for the instances that converged at this step; their basis stops here
-        std::memcpy(hy, ycol.data(), sizeof(double) * ycol.size());
-        EXA_HIP(hipMemcpyAsync(segy.p, hy, sizeof(double) * ycol.size(), hipMemcpyHostToDevice, s));
-        seg_gemv_cols(ma, xv, V.p, N, i + 1, segy.p, 1.0, s);  // before the mask drops them
-        push_mask(act);
-        seg_scale(m, vm, vm, segmask.p, 0, s);
-      }
-      if (nact == 0) break;
-    }
-    if (nact == 0) break;
-    // restart: x += V y for the rest, then r = M^{-1}(b - A x)
-    std::vector<double> ycol(static_cast<size_t>(i) * ns, 0.0);
-    for (int sg = 0; sg < ns; ++sg) {
-      if (!act[sg]) continue;
-      gmres_back_solve(y.data(), Hs(sg), sv.data() + static_cast<size_t>(sg) * (n1 + 1), i - 1, n1);
-      for (int k = 0; k < i; ++k) ycol[static_cast<size_t>(k) * ns + sg] = y[k];
-    }
-    std::memcpy(hy, ycol.data(), sizeof(double) * ycol.size());
-    EXA_HIP(hipMemcpyAsync(segy.p, hy, sizeof(double) * ycol.size(), hipMemcpyHostToDevice, s));
-    seg_gemv_cols(ma, xv, V.p, N, i, segy.p, 1.0, s);
-    matvec(xv, r.p);
-    axpby(r.p, bv, 1.0, -1.0, N, s);
-    apply_precond(r.p);
-    push_mask(act);
-    seg_scale(m, r.p, r.p, segmask.p, 0, s);
-    seg_norms(segT, r.p, segcoef.p);
-    pull(segcoef.p, ns);
-    for (int sg = 0; sg < ns; ++sg) {
-      nrmr[sg] = h[sg];
-      if (act[sg] && nrmr[sg] / nrmb[sg] < tol) {
-        act[sg] = 0;
-        --nact;
-      }
-    }
-    if (verbose) {
-      std::printf("batch GMRES restart after %d iterations: %d of %d instances still iterating\n", j, nact, ns);
-      std::fflush(stdout);
-    }
-  }
-  if (nact > 0 && verbose)
-    std::printf("Warning: GMRES(%d) leaves %d instances above the tolerance %g after %d iterations\n", restart,
-                nact, tol, gmres_maxit);
-  return *std::max_element(iters.begin(), iters.end());
 }
 
 // Newton per instance over shared launches: an instance stops when its own residual norm is

@@ -905,6 +905,42 @@ __global__ void k_qsetup_mc(Dims d, int e0, int eb, const double* shapegt, const
   if (col < npe) Mass[static_cast<long>(e0 + e) * npe * npe + r + static_cast<long>(npe) * col] = s;
 }
 
+// Left factors of the volume q-equation matrices (qEquationElem, qequation.cpp:66-160), per
+// element e of the block: Aq_e[b] (npe x nge, b = 0..nd) with
+//   Aq_e[0][r, g]      = phiw_0[r, g] jac_g                      (Mass = Aq_e[0] Phi^T)
+//   Aq_e[1 + d][r, g]  = sum_m phiw_{m+1}[r, g] Xx[d, m]_g      (C_d  = Aq_e[1 + d] Phi^T)
+// so that the products are one batched GEMM against the shared shape table Phi^T.
+__global__ void k_qsetup_a(Dims d, int e0, int eb, const double* shapegw, const double* egeo, double* Aq) {
+  const int npe = d.npe, nge = d.nge, nd = d.nd;
+  const long per = static_cast<long>(npe) * nge;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const long e = q / per;
+  const long rg = q - e * per;
+  const int g = static_cast<int>(rg / npe);
+  const long P = static_cast<long>(d.ne) * nge;
+  const long pe = (e0 + e) * nge + g;
+  double* A = Aq + e * per * (nd + 1) + rg;
+  A[0] = shapegw[rg] * egeo[(nd + nd * nd) * P + pe];
+  for (int dd = 0; dd < nd; ++dd) {
+    double t = 0.0;
+    for (int m = 0; m < nd; ++m) t += shapegw[rg + per * (m + 1)] * egeo[(nd + dd + nd * m) * P + pe];
+    A[per * (1 + dd)] = t;
+  }
+}
+
+// Mass of elements e0..e0+eb-1 from the first npe columns of their augmented q-equation systems
+__global__ void k_qmass(Dims d, int e0, int eb, const double* aug, double* Mass) {
+  const int npe = d.npe;
+  const long per = static_cast<long>(npe) * npe;
+  const long ncq = static_cast<long>(npe) + npe * d.nd + static_cast<long>(d.ndf) * d.nd;
+  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (q >= per * eb) return;
+  const long e = q / per;
+  const long rc = q - e * per;
+  Mass[(e0 + e) * per + rc] = aug[e * npe * ncq + rc];
+}
+
 // qEquationElemFaceBlock (qequation.cpp:258-340): E_d[perm[k,l], i + npf*l] = sum w phi_k phi_i n_d jac.
 __global__ void k_qsetup_e(Dims d, int e0, int eb, const double* shapfgt, const double* shapfgw,
                            const int* perm, const double* fgeo, double* aug) {
@@ -1564,7 +1600,8 @@ __global__ void k_bj_extract(Dims d, const double* aug, double* BE) {
 
 // x_f <- BE_f x_f (preconditioner.cpp:282-292); one thread per row.
 template <typename T>
-__global__ void k_bj_apply(Dims d, const T* BE, const double* x, double* y) {
+__global__ void k_bj_apply(Dims d, const T* BE, const double* x, double* y, const double* act) {
+  if (act && act[0] == 0.0) return;
   const int ncf = d.ncf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= static_cast<long>(ncf) * d.nf) return;
@@ -1872,7 +1909,8 @@ constexpr int bilu_unroll() { return sizeof(T) == 2 ? 16 : sizeof(T) == 4 ? 12 :
 // written as [slot = n + nse*i][ncf].
 template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* BE, const double* x,
-                                                    double* Xo) {
+                                                    double* Xo, const double* act) {
+  if (act && act[0] == 0.0) return;  // a finished solve (grid-uniform)
   typedef T vecT __attribute__((ext_vector_type(VW)));
   constexpr int U = bilu_unroll<T>();
   const int ncf = d.ncf, nfse = I.nfse;
@@ -2000,7 +2038,8 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
 }
 
 // x[f] = Xo[slot1[f]] + Xo[slot2[f]] (faces shared by two superelements get both)
-__global__ void k_bilu_scatter(Dims d, IluDev I, const double* Xo, double* out) {
+__global__ void k_bilu_scatter(Dims d, IluDev I, const double* Xo, double* out, const double* act) {
+  if (act && act[0] == 0.0) return;
   const int ncf = d.ncf;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= static_cast<long>(d.nf) * ncf) return;
@@ -2117,7 +2156,8 @@ constexpr int DOT_BS = 256;
 constexpr int DOT_MAXB = 1024;
 
 __global__ __launch_bounds__(DOT_BS) void k_dot_partial(const double* x, const double* y, long n,
-                                                        double* partial) {
+                                                        double* partial, const double* act) {
+  if (act && act[0] == 0.0) return;  // a finished solve: nothing to do (grid-uniform)
   __shared__ double sh[DOT_BS];
   double s = 0.0;
   for (long i = blockIdx.x * static_cast<long>(DOT_BS) + threadIdx.x; i < n;
@@ -2134,7 +2174,8 @@ __global__ __launch_bounds__(DOT_BS) void k_dot_partial(const double* x, const d
 
 // partial is [nvec][nb]; out[v] = sum_b partial[v][b] (sqrt when sqrt_out)
 __global__ __launch_bounds__(DOT_BS) void k_reduce(const double* partial, int nb, double* out,
-                                                   int sqrt_out) {
+                                                   int sqrt_out, const double* act) {
+  if (act && act[0] == 0.0) return;
   __shared__ double sh[DOT_BS];
   const double* pv = partial + static_cast<long>(blockIdx.x) * nb;
   double s = 0.0;
@@ -2150,7 +2191,9 @@ __global__ __launch_bounds__(DOT_BS) void k_reduce(const double* partial, int nb
 
 // partial[v][b] for v < k: dot(V_v, w) -- blockIdx.y = v
 __global__ __launch_bounds__(DOT_BS) void k_multidot_partial(const double* V, long ldv,
-                                                             const double* w, long n, double* partial) {
+                                                             const double* w, long n, double* partial,
+                                                             const double* act) {
+  if (act && act[0] == 0.0) return;
   __shared__ double sh[DOT_BS];
   const double* x = V + ldv * blockIdx.y;
   double s = 0.0;
@@ -2166,12 +2209,15 @@ __global__ __launch_bounds__(DOT_BS) void k_multidot_partial(const double* V, lo
   if (threadIdx.x == 0) partial[static_cast<long>(blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
 }
 
-__global__ void k_axpy_dev(double* y, const double* x, const double* coef, double scale, long n) {
+__global__ void k_axpy_dev(double* y, const double* x, const double* coef, double scale, long n,
+                           const double* act) {
+  if (act && act[0] == 0.0) return;
   const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (i < n) y[i] = fma(scale * coef[0], x[i], y[i]);
 }
 
-__global__ void k_scale_inv_dev(double* x, const double* coef, long n) {
+__global__ void k_scale_inv_dev(double* x, const double* coef, long n, const double* act) {
+  if (act && act[0] == 0.0) return;
   const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (i < n) x[i] = x[i] * (1.0 / coef[0]);
 }
@@ -2198,7 +2244,8 @@ __global__ void k_copy(double* y, const double* x, long n) {
 
 // x += alpha * V[:, 0:k] y   (y on the device)
 __global__ void k_gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha,
-                            long n) {
+                            long n, const double* act) {
+  if (act && act[0] == 0.0) return;
   const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
@@ -2466,9 +2513,25 @@ void interp_face(const Dims& d, int ncomp, const double* shapfgt, const int* per
 
 void qsetup(const Dims& d, int e0, int eb, const double* shapegt, const double* shapegw,
             const double* shapfgt, const double* shapfgw, const int* perm, const double* egeo,
-            const double* fgeo, double* aug, double* Mass, hipStream_t s) {
-  long n1 = static_cast<long>(d.npe) * (d.npe + d.npe * d.nd) * eb;
-  k_qsetup_mc<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, shapegt, shapegw, egeo, aug, Mass);
+            const double* fgeo, double* aug, double* Mass, hipStream_t s, double* Aq) {
+  if (Aq) {  // [Mass | C_1 .. C_nd] = Aq_e Phi^T on MFMA (Aq: eb * (nd + 1) * npe * nge doubles)
+    const long per = static_cast<long>(d.npe) * d.nge;
+    k_qsetup_a<<<nblocks(per * eb, 256), 256, 0, s>>>(d, e0, eb, shapegw, egeo, Aq);
+    EXA_HIP(hipGetLastError());
+    const long ncq = static_cast<long>(d.npe) + d.npe * d.nd + static_cast<long>(d.ndf) * d.nd;
+    Gemm g{};
+    g.M = d.npe; g.N = d.npe; g.K = d.nge; g.alpha = 1.0; g.beta = 0.0;
+    g.A = Aq; g.lda = d.npe; g.sA0 = per; g.sA1 = per * (d.nd + 1);
+    g.B = shapegt; g.ldb = d.nge; g.sB0 = 0; g.sB1 = 0;
+    g.C = aug; g.ldc = d.npe; g.sC0 = static_cast<long>(d.npe) * d.npe; g.sC1 = d.npe * ncq;
+    g.nb0 = d.nd + 1; g.nb1 = eb; g.nb2 = 1;
+    gemm(g, s);
+    const long pm = static_cast<long>(d.npe) * d.npe;
+    k_qmass<<<nblocks(pm * eb, 256), 256, 0, s>>>(d, e0, eb, aug, Mass);
+  } else {
+    long n1 = static_cast<long>(d.npe) * (d.npe + d.npe * d.nd) * eb;
+    k_qsetup_mc<<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, shapegt, shapegw, egeo, aug, Mass);
+  }
   long n2 = static_cast<long>(d.nfe) * d.npf * d.npf * d.nd * eb;
   k_qsetup_e<<<nblocks(n2, 256), 256, 0, s>>>(d, e0, eb, shapfgt, shapfgw, perm, fgeo, aug);
   EXA_HIP(hipGetLastError());
@@ -2674,9 +2737,9 @@ void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s) {
 }
 
 template <typename T>
-void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_t s) {
+void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_t s, const double* act) {
   long n1 = static_cast<long>(d.ncf) * d.nf;
-  k_bj_apply<T><<<nblocks(n1, 256), 256, 0, s>>>(d, BE, x, y);
+  k_bj_apply<T><<<nblocks(n1, 256), 256, 0, s>>>(d, BE, x, y, act);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2770,7 +2833,7 @@ void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const float* L
 
 template <typename T>
 void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
-                double* out, hipStream_t s) {
+                double* out, hipStream_t s, const double* act) {
   (void)Y;
   if (d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 256");
   // two rows per thread (8-byte FP32 / 16-byte FP64 loads) when the block rows pair up
@@ -2782,11 +2845,11 @@ void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, do
     if (shm > 64 * 1024)
       EXA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(shm)));
-    kern<<<I.nse, 256, shm, s>>>(d, I, BE, x, Xo);
+    kern<<<I.nse, 256, shm, s>>>(d, I, BE, x, Xo, act);
   };
   if (vw == 2) launch(&k_bilu_apply<T, 2>);
   else launch(&k_bilu_apply<T, 1>);
-  k_bilu_scatter<<<nblocks(static_cast<long>(d.nf) * d.ncf, 256), 256, 0, s>>>(d, I, Xo, out);
+  k_bilu_scatter<<<nblocks(static_cast<long>(d.nf) * d.ncf, 256), 256, 0, s>>>(d, I, Xo, out, act);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2796,29 +2859,30 @@ int dot_partials_count(long n) {
 }
 
 void dot(const double* x, const double* y, long n, double* partial, double* out, int sqrt_out,
-         hipStream_t s) {
+         hipStream_t s, const double* act) {
   int nb = dot_partials_count(n);
-  k_dot_partial<<<nb, DOT_BS, 0, s>>>(x, y, n, partial);
-  k_reduce<<<1, DOT_BS, 0, s>>>(partial, nb, out, sqrt_out);
+  k_dot_partial<<<nb, DOT_BS, 0, s>>>(x, y, n, partial, act);
+  k_reduce<<<1, DOT_BS, 0, s>>>(partial, nb, out, sqrt_out, act);
   EXA_HIP(hipGetLastError());
 }
 
 void multidot(const double* V, long ldv, int k, const double* w, long n, double* partial, double* out,
-              hipStream_t s) {
+              hipStream_t s, const double* act) {
   int nb = dot_partials_count(n);
   dim3 grid(nb, k);
-  k_multidot_partial<<<grid, DOT_BS, 0, s>>>(V, ldv, w, n, partial);
-  k_reduce<<<k, DOT_BS, 0, s>>>(partial, nb, out, 0);
+  k_multidot_partial<<<grid, DOT_BS, 0, s>>>(V, ldv, w, n, partial, act);
+  k_reduce<<<k, DOT_BS, 0, s>>>(partial, nb, out, 0, act);
   EXA_HIP(hipGetLastError());
 }
 
-void axpy_dev(double* y, const double* x, const double* coef, double scale, long n, hipStream_t s) {
-  k_axpy_dev<<<nblocks(n, 256), 256, 0, s>>>(y, x, coef, scale, n);
+void axpy_dev(double* y, const double* x, const double* coef, double scale, long n, hipStream_t s,
+              const double* act) {
+  k_axpy_dev<<<nblocks(n, 256), 256, 0, s>>>(y, x, coef, scale, n, act);
   EXA_HIP(hipGetLastError());
 }
 
-void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s) {
-  k_scale_inv_dev<<<nblocks(n, 256), 256, 0, s>>>(x, coef, n);
+void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s, const double* act) {
+  k_scale_inv_dev<<<nblocks(n, 256), 256, 0, s>>>(x, coef, n, act);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2839,8 +2903,296 @@ void copy(double* y, const double* x, long n, hipStream_t s) {
 }
 
 void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
-               hipStream_t s) {
-  k_gemv_cols<<<nblocks(n, 256), 256, 0, s>>>(x, V, ldv, k, y, alpha, n);
+               hipStream_t s, const double* act) {
+  k_gemv_cols<<<nblocks(n, 256), 256, 0, s>>>(x, V, ldv, k, y, alpha, n, act);
+  EXA_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Restarted GMRES with the Krylov scalars on the device (gmres.cpp:361-597): the Givens
+// rotations, the convergence test and the back substitution of every instance run in one small
+// kernel per Arnoldi step, so the host never waits on a step.  One block; thread t handles
+// instances t, t + 256, ...; stat[0] receives the number of instances still iterating.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int GM_BS = 1024;  // one block: 16 waves
+
+// cpuApplyGivensRotation (backend/Common/cpuimpl.h:167-197) on the new Hessenberg column of one
+// instance: the rotations k < i are applied in order with the running entry (the column's entry k
+// after rotation k-1) kept in a register, so the chain waits on FMAs only, not on memory; the
+// column comes from hcol (entry k at hc[k*ns]) and the rotated column goes to H.
+__device__ void dev_givens(const double* __restrict__ hc, int ns, double* __restrict__ H, double* __restrict__ s,
+                           double* __restrict__ cs, double* __restrict__ sn, int i) {
+  double carry = hc[0];
+  int k = 0;
+  for (; k + 8 <= i; k += 8) {  // the chunk's loads are issued before its FMAs
+    double h1[8], c[8], sg[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      h1[u] = hc[static_cast<long>(k + u + 1) * ns];
+      c[u] = cs[k + u];
+      sg[u] = sn[k + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      H[k + u] = c[u] * carry + sg[u] * h1[u];
+      carry = -sg[u] * carry + c[u] * h1[u];
+    }
+  }
+  for (; k < i; ++k) {
+    const double h1 = hc[static_cast<long>(k + 1) * ns];
+    const double c = cs[k], sg = sn[k];
+    H[k] = c * carry + sg * h1;
+    carry = -sg * carry + c * h1;
+  }
+  const double a = carry, b = hc[static_cast<long>(i + 1) * ns];
+  double ci, si;
+  if (b == 0.0) {
+    ci = 1.0;
+    si = 0.0;
+  } else if (fabs(b) > fabs(a)) {
+    const double t = a / b;
+    si = 1.0 / sqrt(1.0 + t * t);
+    ci = t * si;
+  } else {
+    const double t = b / a;
+    ci = 1.0 / sqrt(1.0 + t * t);
+    si = t * ci;
+  }
+  cs[i] = ci;
+  sn[i] = si;
+  const double t = ci * s[i];
+  s[i + 1] = -si * s[i];
+  s[i] = t;
+  H[i] = ci * a + si * b;
+  H[i + 1] = 0.0;
+}
+
+// cpuBackSolve (cpuimpl.h:199-208) by one wave: y (stride ys) = R^{-1} s over rows 0..i, R the
+// leading (i+1) x (i+1) block of Hm (column-major, ld).  Column-oriented: lane l keeps the right-
+// hand sides of rows l, l + 64, ...; for k = i .. 0 the owner of row k divides, the value goes to
+// every lane, and each lane updates its rows above k; the columns' entries are loaded 8 columns
+// ahead.  Rows beyond 64 * GM_ROWS fall back to one lane.
+constexpr int GM_ROWS = 4;  // i + 1 <= 256 on the wave path
+__device__ void dev_back_solve_wave(double* __restrict__ y, int ys, const double* __restrict__ Hm,
+                                    const double* __restrict__ s, int i, int ld) {
+  const int lane = threadIdx.x & 63;
+  if (i + 1 > 64 * GM_ROWS) {
+    if (lane == 0) {
+      for (int j = i; j >= 0; --j) y[static_cast<long>(j) * ys] = s[j];
+      for (int j = i; j >= 0; --j) {
+        const double yj = y[static_cast<long>(j) * ys] / Hm[j + static_cast<long>(ld) * j];
+        y[static_cast<long>(j) * ys] = yj;
+        for (int k = j - 1; k >= 0; --k) y[static_cast<long>(k) * ys] -= Hm[k + static_cast<long>(ld) * j] * yj;
+      }
+    }
+    return;
+  }
+  double r[GM_ROWS];
+#pragma unroll
+  for (int q = 0; q < GM_ROWS; ++q) {
+    const int row = lane + 64 * q;
+    r[q] = row <= i ? s[row] : 0.0;
+  }
+  for (int k0 = i; k0 >= 0; k0 -= 8) {
+    double col[8][GM_ROWS], dg[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 - u;
+      const int kk = k >= 0 ? k : 0;
+      dg[u] = Hm[kk + static_cast<long>(ld) * kk];
+#pragma unroll
+      for (int q = 0; q < GM_ROWS; ++q) {
+        const int row = lane + 64 * q;
+        col[u][q] = row < kk ? Hm[row + static_cast<long>(ld) * kk] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 - u;
+      if (k < 0) break;
+      const int ql = k >> 6, owner = k & 63;
+      double rk = 0.0;
+#pragma unroll
+      for (int q = 0; q < GM_ROWS; ++q)
+        if (q == ql) rk = r[q];
+      rk = __shfl(rk, owner);
+      const double yk = rk / dg[u];
+      if (lane == owner) y[static_cast<long>(k) * ys] = yk;
+#pragma unroll
+      for (int q = 0; q < GM_ROWS; ++q) r[q] -= col[u][q] * yk;
+    }
+  }
+}
+
+__device__ void block_count(int mine, int* stat) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) stat[0] = cnt;
+}
+
+// start of a solve: an instance iterates unless ||b|| is zero, non-finite, or (Newton's own
+// solves) below the nonlinear tolerance (gmres.cpp:412-415)
+__global__ __launch_bounds__(GM_BS) void k_gmres_begin(GmresDev g, const double* nb, double tol, double ntol,
+                                                       int use_ntol) {
+  if (threadIdx.x == 0) {
+    g.tol[0] = tol;
+    g.jcount[0] = 0;
+  }
+  int mine = 0;
+  for (int sg = threadIdx.x; sg < g.ns; sg += GM_BS) {
+    const double b = nb[sg];
+    const bool a = isfinite(b) && !(b == 0.0 || (use_ntol && b < ntol));
+    g.act[sg] = a ? 1.0 : 0.0;
+    g.jc[sg] = 0.0;
+    g.iters[sg] = 0;
+    mine += a;
+  }
+  block_count(mine, g.stat);
+}
+
+// first cycle: nrmb = ||P r0|| (gmres.cpp:431-445), s = (nrmb, 0, ...), coef = 1/||r|| scale of v_0
+__global__ __launch_bounds__(GM_BS) void k_gmres_start(GmresDev g, const double* nr, double* coef) {
+  int mine = 0;
+  const int n1 = g.n1;
+  for (int sg = threadIdx.x; sg < g.ns; sg += GM_BS) {
+    const double r = nr[sg];
+    if (g.act[sg] != 0.0 && !isfinite(r)) g.act[sg] = 0.0;
+    const bool a = g.act[sg] != 0.0;
+    g.nrmb[sg] = r;
+    double* sv = g.sv + static_cast<long>(sg) * (n1 + 1);
+    sv[0] = a ? r : 0.0;
+    for (int k = 1; k <= n1; ++k) sv[k] = 0.0;
+    coef[sg] = a ? r : 0.0;
+    mine += a;
+  }
+  block_count(mine, g.stat);
+}
+
+// Arnoldi step j (column i of the cycle): rotate the new column hcol[k*ns + sg] (k <= i+1), test
+// |s_{i+1}| / nrmb < tol; a converged instance gets y = R^{-1} s in y[k*ns + sg] (k <= i), jc = 1,
+// act = 0 and the reference's iteration count j; a non-finite rotation stops the instance.
+__global__ __launch_bounds__(GM_BS) void k_gmres_givens(GmresDev g, int i, const double* hcol) {
+  __shared__ int conv[GM_BS];
+  __shared__ int nconv;
+  int mine = 0;
+  const int n1 = g.n1, ns = g.ns;
+  const int j = g.jcount[0];
+  const double tol = g.tol[0];
+  if (threadIdx.x == 0) nconv = 0;
+  __syncthreads();  // every thread has read j before it is advanced below
+  if (threadIdx.x == 0) g.jcount[0] = j + 1;
+  for (int s0 = 0; s0 < ns; s0 += GM_BS) {
+    // rotations and convergence test, one thread per instance
+    const int sg = s0 + static_cast<int>(threadIdx.x);
+    if (sg < ns) {
+      g.jc[sg] = 0.0;
+      if (g.act[sg] != 0.0) {
+        double* H = g.Hm + static_cast<long>(sg) * n1 * n1 + static_cast<long>(n1) * i;
+        double* sv = g.sv + static_cast<long>(sg) * (n1 + 1);
+        dev_givens(hcol + sg, ns, H, sv, g.cs + static_cast<long>(sg) * (n1 + 1),
+                   g.sn + static_cast<long>(sg) * (n1 + 1), i);
+        g.iters[sg] = j + 1;
+        const double si = sv[i + 1];
+        if (!isfinite(si)) {
+          g.act[sg] = 0.0;
+        } else if (fabs(si) / g.nrmb[sg] < tol) {
+          g.iters[sg] = j;
+          g.act[sg] = 0.0;
+          g.jc[sg] = 1.0;
+          conv[atomicAdd(&nconv, 1)] = sg;
+        } else {
+          ++mine;
+        }
+      }
+    }
+    __syncthreads();
+    // y = R^{-1} s for the instances that converged, one wave each
+    const int nc = nconv;
+    for (int c = static_cast<int>(threadIdx.x >> 6); c < nc; c += GM_BS / 64) {
+      const int sc = conv[c];
+      dev_back_solve_wave(g.y + sc, ns, g.Hm + static_cast<long>(sc) * n1 * n1,
+                          g.sv + static_cast<long>(sc) * (n1 + 1), i, n1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) nconv = 0;
+    __syncthreads();
+  }
+  block_count(mine, g.stat + j % g.ring);
+}
+
+// end of a cycle of i steps: y = R^{-1} s over rows 0..i-1 for the instances still iterating (0
+// for the others), for x += V y (gmres.cpp:552, UpdateSolution); one wave per instance
+__global__ __launch_bounds__(GM_BS) void k_gmres_cycle_y(GmresDev g, int i) {
+  const int n1 = g.n1, ns = g.ns;
+  const int lane = threadIdx.x & 63;
+  for (int sg = static_cast<int>(threadIdx.x >> 6); sg < ns; sg += GM_BS / 64) {
+    if (g.act[sg] != 0.0) {
+      dev_back_solve_wave(g.y + sg, ns, g.Hm + static_cast<long>(sg) * n1 * n1,
+                          g.sv + static_cast<long>(sg) * (n1 + 1), i - 1, n1);
+    } else {
+      for (int k = lane; k < i; k += 64) g.y[static_cast<long>(k) * ns + sg] = 0.0;
+    }
+  }
+}
+
+// restart test (gmres.cpp:561-566): ||P(b - A x)|| / nrmb < tol stops the instance; the others
+// restart with s = (||r||, 0, ...) and v_0 = r / ||r||
+__global__ __launch_bounds__(GM_BS) void k_gmres_restart(GmresDev g, const double* nr, double* coef) {
+  int mine = 0;
+  const int n1 = g.n1;
+  const double tol = g.tol[0];
+  const int j = g.jcount[0] - 1;  // the step this cycle ended with
+  for (int sg = threadIdx.x; sg < g.ns; sg += GM_BS) {
+    const double r = nr[sg];
+    if (g.act[sg] != 0.0 && r / g.nrmb[sg] < tol) g.act[sg] = 0.0;
+    const bool a = g.act[sg] != 0.0;
+    if (a) {
+      double* sv = g.sv + static_cast<long>(sg) * (n1 + 1);
+      sv[0] = r;
+      for (int k = 1; k <= n1; ++k) sv[k] = 0.0;
+    }
+    coef[sg] = a ? r : 0.0;
+    mine += a;
+  }
+  block_count(mine, g.stat + j % g.ring);
+}
+
+__global__ void k_fill(double* y, double v, long n) {
+  const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+}  // namespace
+
+void gmres_dev_begin(const GmresDev& g, const double* nb, double tol, double newton_tol, int use_newton_tol,
+                     hipStream_t s) {
+  k_gmres_begin<<<1, GM_BS, 0, s>>>(g, nb, tol, newton_tol, use_newton_tol);
+  EXA_HIP(hipGetLastError());
+}
+void gmres_dev_start(const GmresDev& g, const double* nr, double* coef, hipStream_t s) {
+  k_gmres_start<<<1, GM_BS, 0, s>>>(g, nr, coef);
+  EXA_HIP(hipGetLastError());
+}
+void gmres_dev_givens(const GmresDev& g, int i, const double* hcol, hipStream_t s) {
+  k_gmres_givens<<<1, GM_BS, 0, s>>>(g, i, hcol);
+  EXA_HIP(hipGetLastError());
+}
+void gmres_dev_cycle_y(const GmresDev& g, int i, hipStream_t s) {
+  k_gmres_cycle_y<<<1, GM_BS, 0, s>>>(g, i);
+  EXA_HIP(hipGetLastError());
+}
+void gmres_dev_restart(const GmresDev& g, const double* nr, double* coef, hipStream_t s) {
+  k_gmres_restart<<<1, GM_BS, 0, s>>>(g, nr, coef);
+  EXA_HIP(hipGetLastError());
+}
+void fill(double* y, double v, long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill<<<nblocks(n, 256), 256, 0, s>>>(y, v, n);
   EXA_HIP(hipGetLastError());
 }
 
@@ -2979,8 +3331,8 @@ void seg_gemv_cols(const SegMap& m, double* x, const double* V, long ldv, int k,
   EXA_HIP(hipGetLastError());
 }
 
-template void bj_apply<double>(const Dims&, const double*, const double*, double*, hipStream_t);
-template void bj_apply<float>(const Dims&, const float*, const double*, double*, hipStream_t);
+template void bj_apply<double>(const Dims&, const double*, const double*, double*, hipStream_t, const double*);
+template void bj_apply<float>(const Dims&, const float*, const double*, double*, hipStream_t, const double*);
 template void matvec_elem<double>(const Dims&, const double*, const int*, const double*, double*, hipStream_t,
                                   const int*, const double*);
 template void matvec_elem<float>(const Dims&, const float*, const int*, const double*, double*, hipStream_t,
@@ -2990,11 +3342,11 @@ template void dudg<double>(const Dims&, const double*, const double*, const int*
 template void dudg<float>(const Dims&, const double*, const float*, const int*, const double*, double*,
                           hipStream_t);
 template void bilu_apply<double>(const Dims&, const IluDev&, const double*, const double*, double*, double*,
-                                 double*, hipStream_t);
+                                 double*, hipStream_t, const double*);
 template void bilu_apply<float>(const Dims&, const IluDev&, const float*, const double*, double*, double*,
-                                double*, hipStream_t);
+                                double*, hipStream_t, const double*);
 template void bilu_apply<unsigned short>(const Dims&, const IluDev&, const unsigned short*, const double*, double*,
-                                         double*, double*, hipStream_t);
+                                         double*, double*, hipStream_t, const double*);
 
 }  // namespace exa
 

@@ -66,10 +66,12 @@ void interp_elem(const Dims& d, int ncomp, const double* shapegt, const double* 
 void interp_face(const Dims& d, int ncomp, const double* shapfgt, const int* perm, const double* src,
                  double* dst, hipStream_t s);
 // Augmented [Mass | C_1..C_nd | E_1..E_nd] for elements [e0, e0+eb); the mass matrices are
-// also written to Mass [npe, npe, ne] (elements e0..e0+eb-1).
+// also written to Mass [npe, npe, ne] (elements e0..e0+eb-1).  With Aq (device workspace of
+// eb*(nd+1)*npe*nge doubles) the volume blocks are one batched MFMA GEMM against the shape table;
+// without it, one thread per entry.
 void qsetup(const Dims& d, int e0, int eb, const double* shapegt, const double* shapegw,
             const double* shapfgt, const double* shapfgw, const int* perm, const double* egeo,
-            const double* fgeo, double* aug, double* Mass, hipStream_t s);
+            const double* fgeo, double* aug, double* Mass, hipStream_t s, double* Aq = nullptr);
 void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double* MEs, hipStream_t s);
 void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
           double* udg, hipStream_t s);
@@ -138,8 +140,10 @@ void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const double
                  hipStream_t s);
 void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const float* H, double* aug, hipStream_t s);
 void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s);
+// act (optional, device): the apply, dot and update kernels below do nothing when act[0] == 0 (a
+// solve that has finished while later steps were already queued)
 template <typename T>
-void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_t s);
+void bj_apply(const Dims& d, const T* BE, const double* x, double* y, hipStream_t s, const double* act = nullptr);
 // elemental additive Schwarz: augmented [block | I] for elements [e0, e0+eb), and the
 // inverse (columns m..2m-1 after lu_solve) into BEe (the block's first element)
 void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
@@ -180,7 +184,7 @@ void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const double* 
 void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const float* Ltmp, float* BE, hipStream_t s);
 template <typename T>
 void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
-                double* out, hipStream_t s);
+                double* out, hipStream_t s, const double* act = nullptr);
 
 // trace blocks up to this size use the one-thread-per-row matvec
 constexpr int MATVEC_SMALL_M = 96;
@@ -188,18 +192,44 @@ constexpr int MATVEC_SMALL_M = 96;
 // ---- vector kernels (results stay on the device) ----
 int dot_partials_count(long n);
 void dot(const double* x, const double* y, long n, double* partial, double* out, int sqrt_out,
-         hipStream_t s);
-void axpy_dev(double* y, const double* x, const double* coef, double scale, long n, hipStream_t s);
-void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s);
+         hipStream_t s, const double* act = nullptr);
+void axpy_dev(double* y, const double* x, const double* coef, double scale, long n, hipStream_t s,
+              const double* act = nullptr);
+void scale_inv_dev(double* x, const double* coef, long n, hipStream_t s, const double* act = nullptr);
 // y = a x + b y; with b == 0 y is only written
 void axpby(double* y, const double* x, double a, double b, long n, hipStream_t s);
 void copy(double* y, const double* x, long n, hipStream_t s);
 // out[0] = lowest i < n with info[i] != 0, or INT_MAX (device int)
 void first_nonzero(const int* info, long n, int* out, hipStream_t s);
 void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
-               hipStream_t s);
+               hipStream_t s, const double* act = nullptr);
 void multidot(const double* V, long ldv, int k, const double* w, long n, double* partial, double* out,
-              hipStream_t s);
+              hipStream_t s, const double* act = nullptr);
+
+// y[0..n) = v
+void fill(double* y, double v, long n, hipStream_t s);
+
+// ---- restarted GMRES with its scalars on the device (one or several independent instances) ----
+// Per instance sg: Hessenberg matrix Hm [ns][n1][n1] (column-major n1 x n1), rotations and
+// right-hand side sv, cs, sn [ns][n1 + 1], nrmb [ns] (preconditioned initial residual norm), the
+// activity mask act [ns] (1.0 iterating, 0.0 finished), jc [ns] (1.0: converged at the last
+// step), y [n1][ns] (y[k*ns + sg]) and the iteration count iters [ns] (the reference's count).
+// Each launcher writes the number of instances still iterating to stat[0].
+// The step kernels read the solve's tolerance from tol[0] and the number of steps taken so far
+// from jcount[0] (the Arnoldi step j), and write their count to stat[j % ring]: a step's launches
+// do not depend on j, so one captured graph per column i serves every cycle.
+struct GmresDev {
+  int ns, n1, ring;
+  double *Hm, *sv, *cs, *sn, *nrmb, *act, *jc, *y, *tol;
+  int *iters, *jcount, *stat;
+};
+// start of a solve: tol[0] = tol, jcount[0] = 0, the activity from ||b|| (nb); count to stat[0]
+void gmres_dev_begin(const GmresDev& g, const double* nb, double tol, double newton_tol, int use_newton_tol,
+                     hipStream_t s);
+void gmres_dev_start(const GmresDev& g, const double* nr, double* coef, hipStream_t s);
+void gmres_dev_givens(const GmresDev& g, int i, const double* hcol, hipStream_t s);
+void gmres_dev_cycle_y(const GmresDev& g, int i, hipStream_t s);
+void gmres_dev_restart(const GmresDev& g, const double* nr, double* coef, hipStream_t s);
 
 // ---- vectors split into per-instance segments (batched solves) ----
 // Segment s of a vector is entries [off[s], off[s+1]).  Blocks of at most SEG_CHUNK entries

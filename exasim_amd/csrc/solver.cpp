@@ -28,41 +28,6 @@ int resolve_model(const std::string& name) {
   throw Error(1, "unknown model '" + name + "'");
 }
 
-// cpuApplyGivensRotation (backend/Common/cpuimpl.h:167-197)
-void gmres_givens(double* Hc, double* s, double* cs, double* sn, int i) {
-  for (int k = 0; k < i; ++k) {
-    double t = cs[k] * Hc[k] + sn[k] * Hc[k + 1];
-    Hc[k + 1] = -sn[k] * Hc[k] + cs[k] * Hc[k + 1];
-    Hc[k] = t;
-  }
-  if (Hc[i + 1] == 0.0) {
-    cs[i] = 1.0;
-    sn[i] = 0.0;
-  } else if (std::fabs(Hc[i + 1]) > std::fabs(Hc[i])) {
-    double t = Hc[i] / Hc[i + 1];
-    sn[i] = 1.0 / std::sqrt(1.0 + t * t);
-    cs[i] = t * sn[i];
-  } else {
-    double t = Hc[i + 1] / Hc[i];
-    cs[i] = 1.0 / std::sqrt(1.0 + t * t);
-    sn[i] = t * cs[i];
-  }
-  double t = cs[i] * s[i];
-  s[i + 1] = -sn[i] * s[i];
-  s[i] = t;
-  Hc[i] = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
-  Hc[i + 1] = 0.0;
-}
-
-// cpuBackSolve (cpuimpl.h:199-208): Hm column-major with leading dimension ld.
-void gmres_back_solve(double* y, const double* Hm, const double* s, int i, int ld) {
-  for (int j = i; j >= 0; --j) y[j] = s[j];
-  for (int j = i; j >= 0; --j) {
-    y[j] /= Hm[j + ld * j];
-    for (int k = j - 1; k >= 0; --k) y[k] -= Hm[k + ld * j] * y[j];
-  }
-}
-
 Solver::Solver(const std::string& din, const std::string& dout, const std::string& mname, int dev)
     : datain(din), dataout(dout), model_name(mname), model(resolve_model(mname)), device(dev) {
   setup();
@@ -77,6 +42,13 @@ Solver::~Solver() {
   if (s) (void)hipStreamSynchronize(s);
   if (hpin) (void)hipHostFree(hpin);
   if (hseg) (void)hipHostFree(hseg);
+  for (auto& gx : gm_graphs)
+    if (gx) (void)hipGraphExecDestroy(gx);
+  if (hstat) {
+    (void)hipHostFree(hstat);
+    for (auto& e : gm_ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   if (s) (void)hipStreamDestroy(s);
 }
 
@@ -201,7 +173,9 @@ void Solver::init(SolData& sol) {
     inst_tau = app.tau;
     eoff = {0, d.ne};
     foff = {0, d.nf};
-  } else {  // every batch handle, a batch of one included, has the element -> instance map
+  }
+  {  // element -> instance map (all zero for one problem): the matvec and the Schwarz apply skip
+     // the elements of instances that have left GMRES
     std::vector<int> ei(d.ne);
     for (int i = 0; i < ninst; ++i)
       for (long e = eoff[i]; e < eoff[i + 1]; ++e) ei[e] = i;
@@ -289,14 +263,15 @@ void Solver::init(SolData& sol) {
     long ncq = static_cast<long>(d.npe) + d.npe * d.nd + static_cast<long>(d.ndf) * d.nd;
     long per = d.npe * ncq;
     int qb = static_cast<int>(std::max(1L, std::min<long>(d.ne, static_cast<long>(1.0e9 / 8 / per))));
-    DevBuf<double> qaug;
+    DevBuf<double> qaug, qA;
     qaug.alloc(static_cast<size_t>(qb) * per);
+    qA.alloc(static_cast<size_t>(qb) * (d.nd + 1) * d.npe * d.nge);
     info.alloc(std::max(d.ne, d.nf) + 1);
     for (int e0 = 0; e0 < d.ne; e0 += qb) {
       int eb = std::min(qb, d.ne - e0);
       EXA_HIP(hipMemsetAsync(qaug.p, 0, static_cast<size_t>(eb) * per * sizeof(double), s));
       qsetup(d, e0, eb, shapegt.p, shapegw.p, shapfgt.p, shapfgw.p, perm.p, egeo.p, fgeo.p, qaug.p, Mass.p,
-             s);
+             s, qA.p);
       lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
       const int bad = first_nonzero(info.p, eb);
       if (bad >= 0) throw Error(4, "singular mass matrix at element " + std::to_string(e0 + bad));
@@ -549,22 +524,19 @@ void Solver::alloc_workspace() {
   rJ.alloc(static_cast<size_t>(neb) * (d.nd + 1) * d.ncu * d.nc * d.nge);
   fpt.alloc(static_cast<size_t>(neb) * 2 * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf);
   V.alloc(static_cast<size_t>(restart + 1) * d.N);
-  hcol.alloc(restart + 2);
-  ydev.alloc(restart + 2);
   partial.alloc(static_cast<size_t>(dot_partials_count(d.N)) * (restart + 2));
-  if (batch) seg_alloc();
+  seg_alloc();
 }
 
 void Solver::set_option(const std::string& name, double v) {
+  clear_graphs();  // an option can change what a step launches
   if (name == "gmres_tol") gmres_tol = v;
   else if (name == "gmres_restart") {
     if (v < 1 || v > 2000) throw Error(1, "gmres_restart out of range");
     restart = static_cast<int>(v);
     V.alloc(static_cast<size_t>(restart + 1) * d.N);
-    hcol.alloc(restart + 2);
-    ydev.alloc(restart + 2);
     partial.alloc(static_cast<size_t>(dot_partials_count(d.N)) * (restart + 2));
-    if (batch) seg_alloc();
+    seg_alloc();
   } else if (name == "gmres_maxit") gmres_maxit = static_cast<int>(v);
   else if (name == "newton_tol") newton_tol = v;
   else if (name == "newton_maxit") newton_maxit = static_cast<int>(v);
@@ -578,6 +550,11 @@ void Solver::set_option(const std::string& name, double v) {
     precond = pv;
     assembled = false;
   } else if (name == "ortho") ortho = static_cast<int>(v);
+  else if (name == "gmres_graph") gm_graph = v != 0.0;
+  else if (name == "gmres_lag") {
+    if (v < 0 || v > GM_RING - 2) throw Error(1, "gmres_lag must be in [0, " + std::to_string(GM_RING - 2) + "]");
+    gm_lag = static_cast<int>(v);
+  }
   else if (name == "verbose") verbose = static_cast<int>(v);
   else if (name == "precision") {
     int pv = static_cast<int>(v);
@@ -969,93 +946,18 @@ void Solver::apply_precond(double* xv) {
   tic(T_PRECOND, &t0);
   copy(tmp.p, xv, d.N, s);
   if (precond == 2) {
-    if (precision && precond_bf16) bilu_apply(d, ilud, BEilu16.p, tmp.p, Yilu.p, Xilu.p, xv, s);
-    else if (precision) bilu_apply(d, ilud, BEilu32.p, tmp.p, Yilu.p, Xilu.p, xv, s);
-    else bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s);
+    if (precision && precond_bf16) bilu_apply(d, ilud, BEilu16.p, tmp.p, Yilu.p, Xilu.p, xv, s, gm_act);
+    else if (precision) bilu_apply(d, ilud, BEilu32.p, tmp.p, Yilu.p, Xilu.p, xv, s, gm_act);
+    else bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s, gm_act);
   } else if (precond == 1) {  // x = sum_e scatter(BE_e^{-1} gather_e(x)), as hdgMatVec with BE
     if (precision) matvec_elem(d, BEeas32.p, elemcon.p, tmp.p, we.p, s, einst.p, elem_active);
     else matvec_elem(d, BEeas.p, elemcon.p, tmp.p, we.p, s, einst.p, elem_active);
     face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, xv, s);
   } else {
-    if (precision) bj_apply(d, BE32.p, tmp.p, xv, s);
-    else bj_apply(d, BE.p, tmp.p, xv, s);
+    if (precision) bj_apply(d, BE32.p, tmp.p, xv, s, gm_act);
+    else bj_apply(d, BE.p, tmp.p, xv, s, gm_act);
   }
   toc(T_PRECOND, t0);
-}
-
-int Solver::gmres(const double* bv, double* xv, double tol_rel) {
-  const long N = d.N;
-  double nrmb = norm(bv, N);
-  EXA_HIP(hipMemsetAsync(xv, 0, sizeof(double) * N, s));
-  if (nrmb < newton_tol && tol_rel < 0.0) return 0;
-  if (nrmb == 0.0) return 0;
-  copy(r.p, bv, N, s);
-  double nrmr = nrmb;
-  double tol = std::min(0.1, (tol_rel < 0.0 ? gmres_tol : tol_rel) * nrmb / nrmr);
-  apply_precond(r.p);
-  nrmb = norm(r.p, N);
-  nrmr = nrmb;
-  const int n1 = restart + 1;
-  std::vector<double> Hm(static_cast<size_t>(n1) * n1, 0.0), sv(n1 + 1), cs(n1 + 1), sn(n1 + 1),
-      y(n1 + 1), Hc(n1 + 1);
-  int j = 0;
-  hipEvent_t t0{};
-  while (j < gmres_maxit) {
-    axpby(V.p, r.p, 1.0 / nrmr, 0.0, N, s);
-    std::fill(sv.begin(), sv.end(), 0.0);
-    sv[0] = nrmr;
-    int i = 0;
-    for (; i < restart && j < gmres_maxit; ++i, ++j) {
-      double* vi = V.p + N * i;
-      double* vm = V.p + N * (i + 1);
-      matvec(vi, vm);
-      apply_precond(vm);
-      tic(T_ORTHO, &t0);
-      if (ortho == 0) {  // MGS (getpoly.cpp:24-42)
-        for (int k = 0; k <= i; ++k) {
-          dot(vm, V.p + N * k, N, partial.p, hcol.p + k, 0, s);
-          axpy_dev(vm, V.p + N * k, hcol.p + k, -1.0, N, s);
-        }
-      } else {  // CGS (gmres.cpp:81-88)
-        multidot(V.p, N, i + 1, vm, N, partial.p, hcol.p, s);
-        gemv_cols(vm, V.p, N, i + 1, hcol.p, -1.0, N, s);
-      }
-      dot(vm, vm, N, partial.p, hcol.p + i + 1, 1, s);
-      scale_inv_dev(vm, hcol.p + i + 1, N, s);
-      toc(T_ORTHO, t0);
-      EXA_HIP(hipMemcpyAsync(hpin, hcol.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
-      EXA_HIP(hipStreamSynchronize(s));
-      for (int k = 0; k < i + 2; ++k) Hc[k] = hpin[k];
-      gmres_givens(Hc.data(), sv.data(), cs.data(), sn.data(), i);
-      for (int k = 0; k < n1; ++k) Hm[k + static_cast<size_t>(n1) * i] = (k < i + 2) ? Hc[k] : 0.0;
-      double rel = std::fabs(sv[i + 1]) / nrmb;
-      if (verbose > 1) std::printf("GMRES %d rel %g tol %g\n", j, rel, tol);
-      if (rel < tol) {
-        gmres_back_solve(y.data(), Hm.data(), sv.data(), i, n1);
-        std::memcpy(hpin, y.data(), sizeof(double) * (i + 1));
-        EXA_HIP(hipMemcpyAsync(ydev.p, hpin, sizeof(double) * (i + 1), hipMemcpyHostToDevice, s));
-        gemv_cols(xv, V.p, N, i + 1, ydev.p, 1.0, N, s);
-        EXA_HIP(hipStreamSynchronize(s));
-        return j;  // the reference's count: j before this step's increment (gmres.cpp:546)
-      }
-    }
-    gmres_back_solve(y.data(), Hm.data(), sv.data(), i - 1, n1);
-    std::memcpy(hpin, y.data(), sizeof(double) * i);
-    EXA_HIP(hipMemcpyAsync(ydev.p, hpin, sizeof(double) * i, hipMemcpyHostToDevice, s));
-    gemv_cols(xv, V.p, N, i, ydev.p, 1.0, N, s);
-    matvec(xv, r.p);
-    axpby(r.p, bv, 1.0, -1.0, N, s);  // r = b - A x
-    apply_precond(r.p);
-    nrmr = norm(r.p, N);
-    if (verbose) {
-      std::printf("GMRES restart after %d iterations: relative residual %g (tol %g)\n", j, nrmr / nrmb, tol);
-      std::fflush(stdout);
-    }
-    if (nrmr / nrmb < tol) return j;
-  }
-  if (verbose) std::printf("Warning: GMRES(%d) does not converge to the tolerance %g within %d iterations\n",
-                           restart, tol, gmres_maxit);
-  return j;
 }
 
 // Steady problems: one Newton solve.  Time-dependent problems: the t2c time loop

@@ -16,9 +16,9 @@ enum TimerClass { T_MATVEC = 0, T_PRECOND = 1, T_ORTHO = 2, T_ASSEMBLY = 3, T_LU
 
 // model id of a built-in or generated model name (Error for an unknown name)
 int resolve_model(const std::string& name);
-// cpuApplyGivensRotation / cpuBackSolve (backend/Common/cpuimpl.h:167-208) on one Hessenberg column
-void gmres_givens(double* Hc, double* s, double* cs, double* sn, int i);
-void gmres_back_solve(double* y, const double* Hm, const double* s, int i, int ld);
+
+// slots of the per-step status ring of the device-resident GMRES
+constexpr int GM_RING = 32;
 
 struct Solver {
   // problem description
@@ -86,8 +86,28 @@ struct Solver {
       ilptr, ilidx, ilcol, iuuptr, iuidx, iucol, islot1, islot2;
   DevBuf<double> BEilu, Yilu, Xilu, Lilu;
   IluDev ilud{};
-  // GMRES
-  DevBuf<double> V, x, r, tmp, wv, partial, hcol, ydev, scal;
+  // GMRES: Krylov basis V, vectors, dot partials; the Krylov scalars of every instance live on the
+  // device (GmresDev: gHm, gsv, gcs, gsn, gnrmb, the activity mask segmask[0, ns), gjc, segy, giters)
+  DevBuf<double> V, x, r, tmp, wv, partial, hcol, scal;
+  DevBuf<double> gHm, gsv, gcs, gsn, gnrmb, gjc, gtol;
+  DevBuf<int> giters, gjcount;
+  // option gmres_graph (default 1): the Arnoldi steps of the Newton loop's solves (b -> x) are
+  // replayed from HIP graphs captured on first use, one per column i (the cycle's last column
+  // with its restart work); gm_graphs[i] for the cycle length gm_graph_len
+  int gm_graph = 1;
+  std::vector<hipGraphExec_t> gm_graphs;
+  int gm_graph_len = 0;
+  void clear_graphs();
+  // host memory the status kernels write the number of iterating instances into (one slot per
+  // queued step), its device address, and an event per slot
+  int* hstat = nullptr;
+  int* dstat = nullptr;
+  hipEvent_t gm_ev[GM_RING] = {};
+  // option gmres_lag: steps queued ahead of the host's convergence check (1 .. GM_RING - 2)
+  int gm_lag = 3;
+  // during a one-problem GMRES: its activity flag, read by the preconditioner apply and the
+  // orthogonalisation kernels (steps queued after convergence do nothing); null otherwise
+  const double* gm_act = nullptr;
   double* hpin = nullptr;  // pinned host scratch for small device->host copies
   Stager st;               // pinned staging of bulk host <-> device copies
   bool assembled = false;
@@ -154,6 +174,8 @@ struct Solver {
   void matvec(const double* v, double* w);
   void apply_precond(double* x);
   int gmres(const double* bvec, double* xvec, double tol_rel = -1.0);
+  // every instance's GMRES at once; iters[i] = instance i's count; returns the largest
+  int gmres_run(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters);
   long augcols() const { return static_cast<long>(d.n) + d.m + 1; }
   void alloc_mixed();
   void solve(int* newton_iters, int* gmres_iters, double* final_norm);
@@ -178,6 +200,9 @@ struct Solver {
   double norm(const double* x, long n);
   // index of the first nonzero entry of the device array info[0..n), or -1 (synchronises)
   int first_nonzero(const int* info, long n);
+  GmresDev gmres_state();
+  void gmres_step(const GmresDev& g, int i, double* xv);
+  void gmres_cycle_end(const GmresDev& g, int i, const double* bv, double* xv);
   void tic(int cls, hipEvent_t* a);
   void toc(int cls, hipEvent_t a);
   void collect_timings();

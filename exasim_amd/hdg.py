@@ -47,6 +47,7 @@ SIGNATURES = [
     ("exa_batch_size", ctypes.c_int, [_P]),
     ("exa_batch_stats", ctypes.c_int, [_P, _I, _I, _D, ctypes.c_int]),
     ("exa_batch_get", ctypes.c_long, [_P, ctypes.c_int, ctypes.c_char_p, _D, ctypes.c_long]),
+    ("libmain", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
 ]
 TIMER_CLASSES = ["matvec", "precond", "ortho", "assembly", "lu", "other"]
 
@@ -78,6 +79,10 @@ def _dp(a):
     return a.ctypes.data_as(_D)
 
 
+# options applied to every handle this process creates (exa_set_option), before any other
+DEFAULT_OPTIONS = {}
+
+
 class HDGSolver:
     """One HDG problem resident on the GPU (``exa_handle``)."""
 
@@ -90,6 +95,8 @@ class HDGSolver:
         if rc != 0:
             raise ExaError(rc, lib.exa_last_error(None).decode())
         self._h = h
+        for k, v in DEFAULT_OPTIONS.items():
+            self.set_option(k, v)
         self.sizes = self.state()
 
     @classmethod
@@ -106,6 +113,8 @@ class HDGSolver:
         if rc != 0:
             raise ExaError(rc, lib.exa_last_error(None).decode())
         self._h = h
+        for k, v in DEFAULT_OPTIONS.items():
+            self.set_option(k, v)
         self.sizes = self.state()
         return self
 

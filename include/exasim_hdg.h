@@ -153,6 +153,17 @@ int exa_batch_stats(exa_handle* h, int* newton_iters, int* gmres_iters, double* 
  * and returns its size; with dst NULL returns the size only.  Negative EXA_ERR_* on failure. */
 long exa_batch_get(exa_handle* h, int inst, const char* name, double* dst, long len);
 
+/* ---- the reference library's entry point ----
+ * libmain replaces backend/Main/libexasim.h:5 (int libmain(int argc, char** argv), defined in
+ * backend/Main/libexasim.cpp, which hands argv without the model name to premain,
+ * backend/Main/exasim.hpp:143-360).  argv = {program, model, nummodels, InputFile, OutputFile
+ * [, restart]}, as in the reference's CI call "cpuEXASIMMAINLIB Poisson3D 1 datain/ dataout/"
+ * (.github/workflows/exasim-ci.yml:49-54).  InputFile is the datain directory; OutputFile the prefix of
+ * udg_np0.bin / uhat_np0.bin.  model: a model name of exa_create, or the reference's built-in
+ * "Poisson3D" (= "poisson").  Only nummodels = 1 and restart = 0.  Solves the problem on the current
+ * device, writes the output files and returns 0 or an EXA_ERR_* code (the reference exits on errors). */
+int libmain(int argc, char** argv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -160,6 +160,14 @@ int exa_matvec(exa_handle* h, const double* v_dev, double* w_dev) {
   });
 }
 
+int exa_matvec_noalloc(exa_handle* h, const double* v_dev, double* w_dev) {
+  if (!v_dev || !w_dev) return EXA_ERR_ARG;
+  return guard(h, [&](exa::Solver& s) {
+    if (!s.assembled) throw exa::Error(EXA_ERR_ARG, "exa_matvec_noalloc before exa_assemble");
+    exa::matvec_noalloc(s, v_dev, w_dev, s.noalloc_chunk);
+  });
+}
+
 int exa_precond(exa_handle* h, double* x_dev) {
   if (!x_dev) return EXA_ERR_ARG;
   return guard(h, [&](exa::Solver& s) {

@@ -125,6 +125,7 @@ void Solver::init(SolData& sol) {
   if (device >= 0) EXA_HIP(hipSetDevice(device));
   EXA_HIP(hipGetDevice(&device));
   EXA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  noalloc_init(device);
   EXA_HIP(hipHostMalloc(reinterpret_cast<void**>(&hpin), 4096 * sizeof(double), hipHostMallocDefault));
   st.init(size_t(32) << 20);
 
@@ -551,7 +552,10 @@ void Solver::set_option(const std::string& name, double v) {
     assembled = false;
   } else if (name == "ortho") ortho = static_cast<int>(v);
   else if (name == "gmres_graph") gm_graph = v != 0.0;
-  else if (name == "gmres_lag") {
+  else if (name == "noalloc_chunk") {
+    if (v < 0) throw Error(1, "noalloc_chunk must be >= 0");
+    noalloc_chunk = static_cast<int>(v);
+  } else if (name == "gmres_lag") {
     if (v < 0 || v > GM_RING - 2) throw Error(1, "gmres_lag must be in [0, " + std::to_string(GM_RING - 2) + "]");
     gm_lag = static_cast<int>(v);
   }

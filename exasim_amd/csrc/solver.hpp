@@ -20,6 +20,14 @@ int resolve_model(const std::string& name);
 // slots of the per-step status ring of the device-resident GMRES
 constexpr int GM_RING = 32;
 
+struct Solver;
+// no-allocation matvec (noalloc.hip): look up this device's scratch array (device current), and
+// w = A v with the element products in that array, in chunks of at most max_chunk elements (0: as
+// many as fit)
+void noalloc_init(int device);
+long noalloc_scratch_doubles();
+void matvec_noalloc(Solver& S, const double* v, double* w, int max_chunk);
+
 struct Solver {
   // problem description
   std::string datain, dataout, model_name;
@@ -105,6 +113,8 @@ struct Solver {
   hipEvent_t gm_ev[GM_RING] = {};
   // option gmres_lag: steps queued ahead of the host's convergence check (1 .. GM_RING - 2)
   int gm_lag = 3;
+  // option noalloc_chunk: largest element chunk of exa_matvec_noalloc (0: what the scratch holds)
+  int noalloc_chunk = 0;
   // during a one-problem GMRES: its activity flag, read by the preconditioner apply and the
   // orthogonalisation kernels (steps queued after convergence do nothing); null otherwise
   const double* gm_act = nullptr;

@@ -33,6 +33,7 @@ SIGNATURES = [
     ("exa_assemble", ctypes.c_int, [_P]),
     ("exa_residual", ctypes.c_int, [_P, _D, _D]),
     ("exa_matvec", ctypes.c_int, [_P, _P, _P]),
+    ("exa_matvec_noalloc", ctypes.c_int, [_P, _P, _P]),
     ("exa_precond", ctypes.c_int, [_P, _P]),
     ("exa_gmres", ctypes.c_int, [_P, _P, _P, _I]),
     ("exa_get_array", ctypes.c_long, [_P, ctypes.c_char_p, _D, ctypes.c_long]),

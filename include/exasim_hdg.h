@@ -94,6 +94,13 @@ int exa_assemble(exa_handle* h);              /* H, DinvF, Ru, Rh, b and the pre
 int exa_residual(exa_handle* h, double* norm_b, double* norm_ru);
 /* w = A v (device pointers, length N). */
 int exa_matvec(exa_handle* h, const double* v_dev, double* w_dev);
+/* exa_matvec without library-allocated scratch: no counterpart in the reference; for callers that
+ * cannot allocate (code under HIP graph capture, serving loops that forbid hipMalloc).  The element
+ * products go to a fixed 32 MiB device array per GPU (elements are processed in chunks that fit it;
+ * an element trace block larger than the array is rejected with EXA_ERR_UNSUPPORTED); w equals
+ * exa_matvec's bit for bit.  The call allocates nothing.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU race and must be serialised by the caller. */
+int exa_matvec_noalloc(exa_handle* h, const double* v_dev, double* w_dev);
 /* x = P x in place (device pointer, length N). */
 int exa_precond(exa_handle* h, double* x_dev);
 /* Solve A x = b with the handle's assembled system (device pointers); x starts at 0. */

@@ -283,8 +283,8 @@ def main():
                          "(the scale for HBM byte counters)")
     ap.add_argument("--batch-serial", action="store_true",
                     help="solve the batch workload's per-order batches one after the other (default: concurrently)")
-    ap.add_argument("--gmres-graph", type=int, default=1, choices=[0, 1],
-                    help="replay the GMRES steps from captured HIP graphs (default) or launch them one by one")
+    ap.add_argument("--gmres-graph", type=int, default=0, choices=[0, 1],
+                    help="1: replay the GMRES steps from captured HIP graphs; 0 (default): launch them one by one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=4)
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)

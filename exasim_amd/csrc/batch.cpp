@@ -201,7 +201,9 @@ void Solver::seg_alloc() {
   gjc.alloc(ns);
   giters.alloc(ns);
   gtol.alloc(1);
-  gjcount.alloc(1);
+  gjcount.alloc(2);  // step counter, completion ticket of the step kernel
+  gxmask.alloc(ns);
+  gbcount.alloc((ns + 15) / 16);
   clear_graphs();  // captured launches hold the old buffers
   if (!hstat) {  // per-step counts of iterating instances, written by the kernels into host memory
     EXA_HIP(hipHostMalloc(reinterpret_cast<void**>(&hstat), sizeof(int) * GM_RING,

@@ -37,9 +37,12 @@ GmresDev Solver::gmres_state() {
   g.jc = gjc.p;
   g.y = segy.p;
   g.tol = gtol.p;
+  g.xmask = gxmask.p;
   g.iters = giters.p;
   g.jcount = gjcount.p;
   g.stat = dstat;
+  g.bcount = gbcount.p;
+  g.ticket = gjcount.p + 1;
   return g;
 }
 
@@ -58,8 +61,8 @@ void Solver::gmres_step(const GmresDev& g, int i, double* xv) {
   const int ns = ninst;
   double* vi = V.p + N * i;
   double* vm = V.p + N * (i + 1);
-  matvec(vi, vm);
-  apply_precond(vm);
+  matvec(vi, tmp.p);
+  apply_precond_to(tmp.p, vm);
   hipEvent_t t0{};
   tic(T_ORTHO, &t0);
   if (!batch) {  // one problem: the full-length kernels, skipped once the solve has finished
@@ -92,13 +95,19 @@ void Solver::gmres_step(const GmresDev& g, int i, double* xv) {
   }
   toc(T_ORTHO, t0);
   gmres_dev_givens(g, i, hcol.p, s);
-  // x += V y for the instances that converged at this step (jc), before they are masked out
+  (void)xv;  // instances that converge here update x at the end of the cycle
+}
+
+// x += V y over the first k basis vectors for the instances in xmask (zero coefficients skipped)
+void Solver::gmres_update_x(const GmresDev& g, int k, double* xv) {
+  if (k <= 0) return;
+  const long N = d.N;
   if (!batch) {
-    gemv_cols(xv, V.p, N, i + 1, g.y, 1.0, N, s, g.jc);
+    gemv_cols(xv, V.p, N, k, g.y, 1.0, N, s, g.xmask, true);
   } else {
-    SegMap mj = segT.map;
-    mj.act = g.jc;
-    seg_gemv_cols(mj, xv, V.p, N, i + 1, g.y, 1.0, s);
+    SegMap mx = segT.map;
+    mx.act = g.xmask;
+    seg_gemv_cols(mx, xv, V.p, N, k, g.y, 1.0, s, true);
   }
 }
 
@@ -106,14 +115,11 @@ void Solver::gmres_step(const GmresDev& g, int i, double* xv) {
 void Solver::gmres_cycle_end(const GmresDev& g, int i, const double* bv, double* xv) {
   const long N = d.N;
   const int ns = ninst;
-  SegMap ma = segT.map;
-  ma.act = g.act;
-  gmres_dev_cycle_y(g, i, s);
-  if (!batch) gemv_cols(xv, V.p, N, i, g.y, 1.0, N, s, g.act);
-  else seg_gemv_cols(ma, xv, V.p, N, i, g.y, 1.0, s);
-  matvec(xv, r.p);
-  axpby(r.p, bv, 1.0, -1.0, N, s);  // r = b - A x
-  apply_precond(r.p);
+  gmres_dev_cycle_y(g, i, s);  // y of the instances still iterating; xmask: those and the converged
+  gmres_update_x(g, i, xv);
+  matvec(xv, tmp.p);
+  axpby(tmp.p, bv, 1.0, -1.0, N, s);  // tmp = b - A x
+  apply_precond_to(tmp.p, r.p);
   seg_scale(segT.map, r.p, r.p, g.act, 0, s);  // finished instances: r = 0
   seg_norms(segT, r.p, segcoef.p);
   gmres_dev_restart(g, segcoef.p, segcoef.p + ns, s);
@@ -157,10 +163,10 @@ int Solver::gmres_run(const double* bv, double* xv, double tol_rel, std::vector<
   seg_norms(segT, bv, segcoef.p);
   gmres_dev_begin(g, segcoef.p, tol, newton_tol, tol_rel < 0.0 ? 1 : 0, s);
   int nact = 0;
+  int icycle = 0;  // steps queued in the current cycle
   {
     Scope scope(*this, g.act);
-    copy(r.p, bv, N, s);
-    apply_precond(r.p);
+    apply_precond_to(bv, r.p);
     seg_scale(segT.map, r.p, r.p, g.act, 0, s);
     seg_norms(segT, r.p, segcoef.p);
     gmres_dev_start(g, segcoef.p, segcoef.p + ns, s);
@@ -200,8 +206,10 @@ int Solver::gmres_run(const double* bv, double* xv, double tol_rel, std::vector<
       }
       ++i;
       ++j;
+      icycle = i;
       if (last || j == gmres_maxit) {
         i = 0;
+        icycle = 0;
         if (verbose) {  // the restart line of the reference needs the count: wait for it
           EXA_HIP(hipStreamSynchronize(s));
           std::printf("GMRES restart after %d iterations: %d of %d instances still iterating\n", j,
@@ -216,6 +224,9 @@ int Solver::gmres_run(const double* bv, double* xv, double tol_rel, std::vector<
         nact = hstat[u % GM_RING];
       }
     }
+    // x += V y for the instances that converged in the last, unfinished cycle
+    gmres_dev_flush(g, s);
+    gmres_update_x(g, icycle, xv);
     EXA_HIP(hipStreamSynchronize(s));
   }
   iters.assign(ns, 0);

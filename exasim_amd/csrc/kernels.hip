@@ -2243,13 +2243,20 @@ __global__ void k_copy(double* y, const double* x, long n) {
 }
 
 // x += alpha * V[:, 0:k] y   (y on the device)
+// SKIPZ: columns whose coefficient is zero are not read (a basis column the solve has not
+// written in this cycle may hold anything)
+template <bool SKIPZ>
 __global__ void k_gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha,
                             long n, const double* act) {
   if (act && act[0] == 0.0) return;
   const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int j = 0; j < k; ++j) s = fma(V[i + ldv * j], y[j], s);
+  for (int j = 0; j < k; ++j) {
+    const double yj = y[j];
+    if (SKIPZ && yj == 0.0) continue;
+    s = fma(V[i + ldv * j], yj, s);
+  }
   x[i] = fma(alpha, s, x[i]);
 }
 
@@ -2903,8 +2910,9 @@ void copy(double* y, const double* x, long n, hipStream_t s) {
 }
 
 void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
-               hipStream_t s, const double* act) {
-  k_gemv_cols<<<nblocks(n, 256), 256, 0, s>>>(x, V, ldv, k, y, alpha, n, act);
+               hipStream_t s, const double* act, bool skip_zero) {
+  if (skip_zero) k_gemv_cols<true><<<nblocks(n, 256), 256, 0, s>>>(x, V, ldv, k, y, alpha, n, act);
+  else k_gemv_cols<false><<<nblocks(n, 256), 256, 0, s>>>(x, V, ldv, k, y, alpha, n, act);
   EXA_HIP(hipGetLastError());
 }
 
@@ -3042,6 +3050,7 @@ __global__ __launch_bounds__(GM_BS) void k_gmres_begin(GmresDev g, const double*
   if (threadIdx.x == 0) {
     g.tol[0] = tol;
     g.jcount[0] = 0;
+    g.ticket[0] = 0;
   }
   int mine = 0;
   for (int sg = threadIdx.x; sg < g.ns; sg += GM_BS) {
@@ -3076,57 +3085,125 @@ __global__ __launch_bounds__(GM_BS) void k_gmres_start(GmresDev g, const double*
 // Arnoldi step j (column i of the cycle): rotate the new column hcol[k*ns + sg] (k <= i+1), test
 // |s_{i+1}| / nrmb < tol; a converged instance gets y = R^{-1} s in y[k*ns + sg] (k <= i), jc = 1,
 // act = 0 and the reference's iteration count j; a non-finite rotation stops the instance.
+// Arnoldi step j: one wave per instance, 16 instances per block.  The wave stages the new column
+// (hcol[k*ns + sg], k <= i + 1) and the instance's rotations in LDS, lane 0 applies the rotations
+// in order and forms the new one (cpuApplyGivensRotation, cpuimpl.h:167-197, the same operations
+// as dev_givens), the lanes write the rotated column to H, and an instance with
+// |s_{i+1}| / nrmb < tol gets y = R^{-1} s (zero above row i), jc = 1 (x += V y at the end of the
+// cycle), act = 0 and the reference's iteration count j; a non-finite rotation stops the instance.
+// The last block to finish adds the blocks' counts in block order, writes them to stat[j % ring]
+// and advances jcount.
+constexpr int GM_WAVES = GM_BS / 64;
+constexpr int GM_LDS_N1 = 128;  // columns up to this length are staged in LDS
 __global__ __launch_bounds__(GM_BS) void k_gmres_givens(GmresDev g, int i, const double* hcol) {
-  __shared__ int conv[GM_BS];
-  __shared__ int nconv;
-  int mine = 0;
+  __shared__ double lh[GM_WAVES][GM_LDS_N1 + 1], lc[GM_WAVES][GM_LDS_N1], ls[GM_WAVES][GM_LDS_N1];
+  __shared__ int flag[GM_WAVES];
+  __shared__ int cnt;
+  __shared__ bool last;
   const int n1 = g.n1, ns = g.ns;
+  const int wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+  const int sg = static_cast<int>(blockIdx.x) * GM_WAVES + wave;
   const int j = g.jcount[0];
   const double tol = g.tol[0];
-  if (threadIdx.x == 0) nconv = 0;
-  __syncthreads();  // every thread has read j before it is advanced below
-  if (threadIdx.x == 0) g.jcount[0] = j + 1;
-  for (int s0 = 0; s0 < ns; s0 += GM_BS) {
-    // rotations and convergence test, one thread per instance
-    const int sg = s0 + static_cast<int>(threadIdx.x);
-    if (sg < ns) {
-      g.jc[sg] = 0.0;
-      if (g.act[sg] != 0.0) {
-        double* H = g.Hm + static_cast<long>(sg) * n1 * n1 + static_cast<long>(n1) * i;
-        double* sv = g.sv + static_cast<long>(sg) * (n1 + 1);
-        dev_givens(hcol + sg, ns, H, sv, g.cs + static_cast<long>(sg) * (n1 + 1),
-                   g.sn + static_cast<long>(sg) * (n1 + 1), i);
-        g.iters[sg] = j + 1;
-        const double si = sv[i + 1];
-        if (!isfinite(si)) {
-          g.act[sg] = 0.0;
-        } else if (fabs(si) / g.nrmb[sg] < tol) {
-          g.iters[sg] = j;
-          g.act[sg] = 0.0;
-          g.jc[sg] = 1.0;
-          conv[atomicAdd(&nconv, 1)] = sg;
-        } else {
-          ++mine;
-        }
+  const bool live = sg < ns && g.act[sg] != 0.0;
+  const bool staged = n1 <= GM_LDS_N1;
+  if (threadIdx.x == 0) cnt = 0;
+  double* H = nullptr;
+  double *sv = nullptr, *cs = nullptr, *sn = nullptr;
+  if (live) {
+    H = g.Hm + static_cast<long>(sg) * n1 * n1 + static_cast<long>(n1) * i;
+    sv = g.sv + static_cast<long>(sg) * (n1 + 1);
+    cs = g.cs + static_cast<long>(sg) * (n1 + 1);
+    sn = g.sn + static_cast<long>(sg) * (n1 + 1);
+    if (staged) {
+      for (int k = lane; k <= i + 1; k += 64) lh[wave][k] = hcol[static_cast<long>(k) * ns + sg];
+      for (int k = lane; k < i; k += 64) {
+        lc[wave][k] = cs[k];
+        ls[wave][k] = sn[k];
       }
     }
-    __syncthreads();
-    // y = R^{-1} s for the instances that converged, one wave each
-    const int nc = nconv;
-    for (int c = static_cast<int>(threadIdx.x >> 6); c < nc; c += GM_BS / 64) {
-      const int sc = conv[c];
-      dev_back_solve_wave(g.y + sc, ns, g.Hm + static_cast<long>(sc) * n1 * n1,
-                          g.sv + static_cast<long>(sc) * (n1 + 1), i, n1);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) nconv = 0;
-    __syncthreads();
   }
-  block_count(mine, g.stat + j % g.ring);
+  __syncthreads();
+  if (live && lane == 0) {
+    if (staged) {
+      double* h = lh[wave];
+      const double* c = lc[wave];
+      const double* sg_ = ls[wave];
+      double carry = h[0];
+      for (int k = 0; k < i; ++k) {
+        const double h1 = h[k + 1];
+        h[k] = c[k] * carry + sg_[k] * h1;
+        carry = -sg_[k] * carry + c[k] * h1;
+      }
+      const double a0 = carry, b0 = h[i + 1];
+      double ci, si;
+      if (b0 == 0.0) {
+        ci = 1.0;
+        si = 0.0;
+      } else if (fabs(b0) > fabs(a0)) {
+        const double t = a0 / b0;
+        si = 1.0 / sqrt(1.0 + t * t);
+        ci = t * si;
+      } else {
+        const double t = b0 / a0;
+        ci = 1.0 / sqrt(1.0 + t * t);
+        si = t * ci;
+      }
+      cs[i] = ci;
+      sn[i] = si;
+      const double t = ci * sv[i];
+      sv[i + 1] = -si * sv[i];
+      sv[i] = t;
+      h[i] = ci * a0 + si * b0;
+      h[i + 1] = 0.0;
+    } else {
+      dev_givens(hcol + sg, ns, H, sv, cs, sn, i);
+    }
+    g.iters[sg] = j + 1;
+    const double s1 = sv[i + 1];
+    int f = 1;  // 1 iterating, 2 converged, 0 stopped
+    if (!isfinite(s1)) f = 0;
+    else if (fabs(s1) / g.nrmb[sg] < tol) f = 2;
+    if (f != 1) {
+      g.act[sg] = 0.0;
+      if (f == 2) {
+        g.iters[sg] = j;
+        g.jc[sg] = 1.0;
+      }
+    }
+    flag[wave] = f;
+  }
+  __syncthreads();
+  if (live) {
+    if (staged)
+      for (int k = lane; k <= i + 1; k += 64) H[k] = lh[wave][k];
+    if (flag[wave] == 2) {  // y = R^{-1} s, zero above row i
+      __threadfence_block();
+      dev_back_solve_wave(g.y + sg, ns, g.Hm + static_cast<long>(sg) * n1 * n1, sv, i, n1);
+      for (int k = i + 1 + lane; k < n1; k += 64) g.y[static_cast<long>(k) * ns + sg] = 0.0;
+    }
+    if (lane == 0 && flag[wave] == 1) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    g.bcount[blockIdx.x] = cnt;
+    __threadfence();
+    last = atomicAdd(g.ticket, 1) == static_cast<int>(gridDim.x) - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    int tot = 0;
+    for (unsigned b = 0; b < gridDim.x; ++b) tot += atomicAdd(g.bcount + b, 0);
+    g.stat[j % g.ring] = tot;
+    g.jcount[0] = j + 1;
+    g.ticket[0] = 0;
+  }
 }
 
-// end of a cycle of i steps: y = R^{-1} s over rows 0..i-1 for the instances still iterating (0
-// for the others), for x += V y (gmres.cpp:552, UpdateSolution); one wave per instance
+// end of a cycle of i steps: y = R^{-1} s over rows 0..i-1 for the instances still iterating
+// (for x += V y, gmres.cpp:552, UpdateSolution); xmask marks them and the instances that converged
+// in this cycle (their y is set), one wave per instance
 __global__ __launch_bounds__(GM_BS) void k_gmres_cycle_y(GmresDev g, int i) {
   const int n1 = g.n1, ns = g.ns;
   const int lane = threadIdx.x & 63;
@@ -3134,9 +3211,17 @@ __global__ __launch_bounds__(GM_BS) void k_gmres_cycle_y(GmresDev g, int i) {
     if (g.act[sg] != 0.0) {
       dev_back_solve_wave(g.y + sg, ns, g.Hm + static_cast<long>(sg) * n1 * n1,
                           g.sv + static_cast<long>(sg) * (n1 + 1), i - 1, n1);
-    } else {
-      for (int k = lane; k < i; k += 64) g.y[static_cast<long>(k) * ns + sg] = 0.0;
+      for (int k = i + lane; k < n1; k += 64) g.y[static_cast<long>(k) * ns + sg] = 0.0;
     }
+    if (lane == 0) g.xmask[sg] = (g.act[sg] != 0.0 || g.jc[sg] != 0.0) ? 1.0 : 0.0;
+  }
+}
+
+// end of a solve: x += V y still due for the instances that converged in the last cycle
+__global__ __launch_bounds__(GM_BS) void k_gmres_flush(GmresDev g) {
+  for (int sg = threadIdx.x; sg < g.ns; sg += GM_BS) {
+    g.xmask[sg] = g.jc[sg];
+    g.jc[sg] = 0.0;
   }
 }
 
@@ -3148,6 +3233,7 @@ __global__ __launch_bounds__(GM_BS) void k_gmres_restart(GmresDev g, const doubl
   const double tol = g.tol[0];
   const int j = g.jcount[0] - 1;  // the step this cycle ended with
   for (int sg = threadIdx.x; sg < g.ns; sg += GM_BS) {
+    g.jc[sg] = 0.0;  // their x update was made with the cycle's
     const double r = nr[sg];
     if (g.act[sg] != 0.0 && r / g.nrmb[sg] < tol) g.act[sg] = 0.0;
     const bool a = g.act[sg] != 0.0;
@@ -3179,11 +3265,15 @@ void gmres_dev_start(const GmresDev& g, const double* nr, double* coef, hipStrea
   EXA_HIP(hipGetLastError());
 }
 void gmres_dev_givens(const GmresDev& g, int i, const double* hcol, hipStream_t s) {
-  k_gmres_givens<<<1, GM_BS, 0, s>>>(g, i, hcol);
+  k_gmres_givens<<<(g.ns + GM_WAVES - 1) / GM_WAVES, GM_BS, 0, s>>>(g, i, hcol);
   EXA_HIP(hipGetLastError());
 }
 void gmres_dev_cycle_y(const GmresDev& g, int i, hipStream_t s) {
   k_gmres_cycle_y<<<1, GM_BS, 0, s>>>(g, i);
+  EXA_HIP(hipGetLastError());
+}
+void gmres_dev_flush(const GmresDev& g, hipStream_t s) {
+  k_gmres_flush<<<1, GM_BS, 0, s>>>(g);
   EXA_HIP(hipGetLastError());
 }
 void gmres_dev_restart(const GmresDev& g, const double* nr, double* coef, hipStream_t s) {
@@ -3289,6 +3379,7 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_scale(SegMap m, double* y, const
 }
 
 // x += alpha * sum_j V_j * y[j*nseg + seg]
+template <bool SKIPZ>
 __global__ __launch_bounds__(SEG_BS) void k_seg_gemv_cols(SegMap m, double* x, const double* V, long ldv, int k,
                                                           const double* y, double alpha) {
   const int b = blockIdx.x;
@@ -3296,7 +3387,11 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_gemv_cols(SegMap m, double* x, c
   if (m.act && m.act[sg] == 0.0) return;
   for (long i = m.bbeg[b] + threadIdx.x; i < m.bend[b]; i += SEG_BS) {
     double acc = 0.0;
-    for (int j = 0; j < k; ++j) acc = fma(V[i + ldv * j], y[static_cast<long>(j) * m.nseg + sg], acc);
+    for (int j = 0; j < k; ++j) {
+      const double yj = y[static_cast<long>(j) * m.nseg + sg];
+      if (SKIPZ && yj == 0.0) continue;
+      acc = fma(V[i + ldv * j], yj, acc);
+    }
     x[i] = fma(alpha, acc, x[i]);
   }
 }
@@ -3325,9 +3420,10 @@ void seg_scale(const SegMap& m, double* y, const double* x, const double* coef, 
 }
 
 void seg_gemv_cols(const SegMap& m, double* x, const double* V, long ldv, int k, const double* y, double alpha,
-                   hipStream_t s) {
+                   hipStream_t s, bool skip_zero) {
   if (m.nblk == 0 || k == 0) return;
-  k_seg_gemv_cols<<<m.nblk, SEG_BS, 0, s>>>(m, x, V, ldv, k, y, alpha);
+  if (skip_zero) k_seg_gemv_cols<true><<<m.nblk, SEG_BS, 0, s>>>(m, x, V, ldv, k, y, alpha);
+  else k_seg_gemv_cols<false><<<m.nblk, SEG_BS, 0, s>>>(m, x, V, ldv, k, y, alpha);
   EXA_HIP(hipGetLastError());
 }
 

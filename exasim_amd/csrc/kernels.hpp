@@ -201,8 +201,9 @@ void axpby(double* y, const double* x, double a, double b, long n, hipStream_t s
 void copy(double* y, const double* x, long n, hipStream_t s);
 // out[0] = lowest i < n with info[i] != 0, or INT_MAX (device int)
 void first_nonzero(const int* info, long n, int* out, hipStream_t s);
+// skip_zero: columns with a zero coefficient are not read
 void gemv_cols(double* x, const double* V, long ldv, int k, const double* y, double alpha, long n,
-               hipStream_t s, const double* act = nullptr);
+               hipStream_t s, const double* act = nullptr, bool skip_zero = false);
 void multidot(const double* V, long ldv, int k, const double* w, long n, double* partial, double* out,
               hipStream_t s, const double* act = nullptr);
 
@@ -218,10 +219,13 @@ void fill(double* y, double v, long n, hipStream_t s);
 // The step kernels read the solve's tolerance from tol[0] and the number of steps taken so far
 // from jcount[0] (the Arnoldi step j), and write their count to stat[j % ring]: a step's launches
 // do not depend on j, so one captured graph per column i serves every cycle.
+// jc [ns]: 1.0 for an instance that converged in the current cycle (its x += V y is made with the
+// cycle's, or by gmres_dev_flush at the end); xmask [ns]: the instances of the next x update;
+// bcount [(ns + 15) / 16], ticket [1]: the step kernel's per-block counts and completion ticket.
 struct GmresDev {
   int ns, n1, ring;
-  double *Hm, *sv, *cs, *sn, *nrmb, *act, *jc, *y, *tol;
-  int *iters, *jcount, *stat;
+  double *Hm, *sv, *cs, *sn, *nrmb, *act, *jc, *y, *tol, *xmask;
+  int *iters, *jcount, *stat, *bcount, *ticket;
 };
 // start of a solve: tol[0] = tol, jcount[0] = 0, the activity from ||b|| (nb); count to stat[0]
 void gmres_dev_begin(const GmresDev& g, const double* nb, double tol, double newton_tol, int use_newton_tol,
@@ -229,6 +233,8 @@ void gmres_dev_begin(const GmresDev& g, const double* nb, double tol, double new
 void gmres_dev_start(const GmresDev& g, const double* nr, double* coef, hipStream_t s);
 void gmres_dev_givens(const GmresDev& g, int i, const double* hcol, hipStream_t s);
 void gmres_dev_cycle_y(const GmresDev& g, int i, hipStream_t s);
+// xmask = jc, jc = 0: the x update still due at the end of a solve
+void gmres_dev_flush(const GmresDev& g, hipStream_t s);
 void gmres_dev_restart(const GmresDev& g, const double* nr, double* coef, hipStream_t s);
 
 // ---- vectors split into per-instance segments (batched solves) ----
@@ -256,6 +262,6 @@ void seg_axpy(const SegMap& m, double* y, const double* x, const double* coef, d
 void seg_scale(const SegMap& m, double* y, const double* x, const double* coef, int mode, hipStream_t s);
 // x += alpha * sum_{j<k} V_j * y[j*nseg + s]
 void seg_gemv_cols(const SegMap& m, double* x, const double* V, long ldv, int k, const double* y, double alpha,
-                   hipStream_t s);
+                   hipStream_t s, bool skip_zero = false);
 
 }  // namespace exa

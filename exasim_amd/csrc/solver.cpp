@@ -946,20 +946,25 @@ void Solver::matvec(const double* v, double* w) {
 }
 
 void Solver::apply_precond(double* xv) {
+  copy(tmp.p, xv, d.N, s);
+  apply_precond_to(tmp.p, xv);
+}
+
+// out = P in (in and out distinct)
+void Solver::apply_precond_to(const double* in, double* out) {
   hipEvent_t t0{};
   tic(T_PRECOND, &t0);
-  copy(tmp.p, xv, d.N, s);
   if (precond == 2) {
-    if (precision && precond_bf16) bilu_apply(d, ilud, BEilu16.p, tmp.p, Yilu.p, Xilu.p, xv, s, gm_act);
-    else if (precision) bilu_apply(d, ilud, BEilu32.p, tmp.p, Yilu.p, Xilu.p, xv, s, gm_act);
-    else bilu_apply(d, ilud, BEilu.p, tmp.p, Yilu.p, Xilu.p, xv, s, gm_act);
+    if (precision && precond_bf16) bilu_apply(d, ilud, BEilu16.p, in, Yilu.p, Xilu.p, out, s, gm_act);
+    else if (precision) bilu_apply(d, ilud, BEilu32.p, in, Yilu.p, Xilu.p, out, s, gm_act);
+    else bilu_apply(d, ilud, BEilu.p, in, Yilu.p, Xilu.p, out, s, gm_act);
   } else if (precond == 1) {  // x = sum_e scatter(BE_e^{-1} gather_e(x)), as hdgMatVec with BE
-    if (precision) matvec_elem(d, BEeas32.p, elemcon.p, tmp.p, we.p, s, einst.p, elem_active);
-    else matvec_elem(d, BEeas.p, elemcon.p, tmp.p, we.p, s, einst.p, elem_active);
-    face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, xv, s);
+    if (precision) matvec_elem(d, BEeas32.p, elemcon.p, in, we.p, s, einst.p, elem_active);
+    else matvec_elem(d, BEeas.p, elemcon.p, in, we.p, s, einst.p, elem_active);
+    face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, out, s);
   } else {
-    if (precision) bj_apply(d, BE32.p, tmp.p, xv, s, gm_act);
-    else bj_apply(d, BE.p, tmp.p, xv, s, gm_act);
+    if (precision) bj_apply(d, BE32.p, in, out, s, gm_act);
+    else bj_apply(d, BE.p, in, out, s, gm_act);
   }
   toc(T_PRECOND, t0);
 }

@@ -97,12 +97,14 @@ struct Solver {
   // GMRES: Krylov basis V, vectors, dot partials; the Krylov scalars of every instance live on the
   // device (GmresDev: gHm, gsv, gcs, gsn, gnrmb, the activity mask segmask[0, ns), gjc, segy, giters)
   DevBuf<double> V, x, r, tmp, wv, partial, hcol, scal;
-  DevBuf<double> gHm, gsv, gcs, gsn, gnrmb, gjc, gtol;
-  DevBuf<int> giters, gjcount;
-  // option gmres_graph (default 1): the Arnoldi steps of the Newton loop's solves (b -> x) are
-  // replayed from HIP graphs captured on first use, one per column i (the cycle's last column
-  // with its restart work); gm_graphs[i] for the cycle length gm_graph_len
-  int gm_graph = 1;
+  DevBuf<double> gHm, gsv, gcs, gsn, gnrmb, gjc, gtol, gxmask;
+  DevBuf<int> giters, gjcount, gbcount;
+  // option gmres_graph (default 0): 1 replays the Arnoldi steps of the Newton loop's solves
+  // (b -> x) from HIP graphs captured on first use, one per column i (the cycle's last column
+  // with its restart work); gm_graphs[i] for the cycle length gm_graph_len.  Off by default: on
+  // the batch workload the replays were measured slower than the direct launches (concurrent
+  // batches 0.49 s against 0.43 s, one-instance solves 73 against 117 solves/s with the captures)
+  int gm_graph = 0;
   std::vector<hipGraphExec_t> gm_graphs;
   int gm_graph_len = 0;
   void clear_graphs();
@@ -183,6 +185,7 @@ struct Solver {
   void residual_fields();  // RuR, Rh and the assembled trace residual bres
   void matvec(const double* v, double* w);
   void apply_precond(double* x);
+  void apply_precond_to(const double* in, double* out);
   int gmres(const double* bvec, double* xvec, double tol_rel = -1.0);
   // every instance's GMRES at once; iters[i] = instance i's count; returns the largest
   int gmres_run(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters);
@@ -213,6 +216,7 @@ struct Solver {
   GmresDev gmres_state();
   void gmres_step(const GmresDev& g, int i, double* xv);
   void gmres_cycle_end(const GmresDev& g, int i, const double* bv, double* xv);
+  void gmres_update_x(const GmresDev& g, int k, double* xv);
   void tic(int cls, hipEvent_t* a);
   void toc(int cls, hipEvent_t a);
   void collect_timings();

@@ -21,8 +21,9 @@ against the FP64 one are reported beside it.
 The same line carries ``batch``: 256 parametrized 2D convection-diffusion instances of uneven
 size and order (see ``run_batch``) solved as four batches (one per polynomial order, shared
 launches, per-instance Newton/GMRES), in the mixed and FP64 forms, next to the plain FP64 form
-solving them one at a time; ``--workload batch`` prints that workload's line alone, and
-``--no-batch`` leaves it out.
+solving them one at a time; and ``batch3d``: the same comparison on 48 parametrized 3D
+convection-diffusion instances (n^3 hexes, n in {4..12}, p in {2, 3, 4}).  ``--workload batch``
+and ``--workload batch3d`` print one of them alone, and ``--no-batch`` leaves both out.
 
 Prints one JSON line (rank 0, N = 1).  ``roofline`` is the dominant kernel of
 the GMRES loop, the element matvec of the leg reported in ``value``:
@@ -114,7 +115,11 @@ def batch_cpu_sample(budget_s=20.0, nmax=32):
 BATCH_KW = dict(precond="eas", restart=100, gmres_tol=1e-10, newton_tol=1e-8, ortho=1)
 
 
-def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurrent=True):
+# mixed-form refinement tolerances of the batch legs (the C2 line's: first solve and corrections to 1e-5)
+BATCH_REFINE = dict(refine_first_tol=1e-5, refine_tol=1e-5)
+
+
+def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurrent=True, kind="2d"):
     """The batch workload: ``count`` parametrized 2D convection-diffusion instances (n x n quads,
     n in {8..64}, p in {2..5}, kappa log-uniform [1e-3, 1], velocity angle uniform, source
     sin(k pi x) cos(k pi y)), one batch handle per polynomial order, the four batches solved
@@ -127,13 +132,20 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
     from exasim_amd import cases
     from exasim_amd.hdg import HDGSolver
 
+    tag = "batch" if kind == "2d" else "batch3d"
+
     def log(msg):
         if verbose:
-            print("# batch: " + msg, file=sys.stderr, flush=True)
+            print("# %s: %s" % (tag, msg), file=sys.stderr, flush=True)
 
     work = tempfile.mkdtemp(prefix="exa_batch_")
     t0 = time.perf_counter()
-    inst = cases.convdiff2d_batch(work, count=count, **BATCH_KW)
+    if kind == "2d":
+        inst = cases.convdiff2d_batch(work, count=count, **BATCH_KW)
+        model = "cd2d"
+    else:
+        inst = cases.convdiff3d_batch(work, count=count)
+        model = "convdiff"
     log("%d instances written in %.1f s" % (count, time.perf_counter() - t0))
     groups = collections.defaultdict(list)
     for i, d in enumerate(inst):
@@ -141,7 +153,7 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
     # plain FP64 form: each instance alone, inputs resident (set-up outside the timed solve)
     plain_t, uh_ref = [], {}
     for i, d in enumerate(inst):
-        with HDGSolver(d["path"], "cd2d") as S:
+        with HDGSolver(d["path"], model) as S:
             S.sync()
             t1 = time.perf_counter()
             S.solve()
@@ -153,8 +165,11 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
     for prec in (0, 1):
         hs = {}
         for p, ids in sorted(groups.items()):
-            hs[p] = HDGSolver.batch([inst[i]["path"] for i in ids], "cd2d")
+            hs[p] = HDGSolver.batch([inst[i]["path"] for i in ids], model)
             hs[p].set_option("precision", prec)
+            if prec:
+                for k, v in BATCH_REFINE.items():
+                    hs[p].set_option(k, v)
         lat = collections.defaultdict(list)
         walls = []
         for k in range(warmup + steps):
@@ -200,7 +215,7 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
                           p50_ms=float(np.percentile(tts, 50) * 1e3), p99_ms=float(np.percentile(tts, 99) * 1e3),
                           group_s={int(p): float(np.mean(v)) for p, v in lat.items()})
     cpu = None
-    if cpu_baseline:
+    if cpu_baseline and kind == "2d":
         env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--batch-cpu-child"], env=env,
                            capture_output=True, text=True, timeout=900)
@@ -219,7 +234,9 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
     return dict(instances=count, groups={int(p): len(v) for p, v in groups.items()}, plain_fp64=plain,
                 batched_fp64=legs[0], batched_mixed=legs[1],
                 mixed_batched_over_plain_fp64=legs[1]["solves_s"] / plain["solves_s"],
-                fp64_batched_over_plain_fp64=legs[0]["solves_s"] / plain["solves_s"], cpu_baseline=cpu)
+                fp64_batched_over_plain_fp64=legs[0]["solves_s"] / plain["solves_s"],
+                mixed_batched_over_fp64_batched=legs[1]["solves_s"] / legs[0]["solves_s"], cpu_baseline=cpu,
+                mixed_refinement=BATCH_REFINE)
 
 
 def run_serve(seconds, rate, check_alone=4):
@@ -235,13 +252,13 @@ def run_serve(seconds, rate, check_alone=4):
     reqs = cases.serve_stream(work, seconds=seconds, rate=rate)
     print("# serve: %d requests written in %.1f s" % (len(reqs), time.perf_counter() - t0), file=sys.stderr,
           flush=True)
-    srv = serve.Server(hbm_budget_bytes=200e9, max_batch=64, workers=1, precision=1,
+    srv = serve.Server(hbm_budget_bytes=200e9, max_batch=64, max_concurrent=8, precision=1,
                        keep_results=check_alone > 0, window=0.02,
                        log=lambda msg: print("# serve: " + msg, file=sys.stderr, flush=True))
     summ = srv.run_stream(reqs)
     print("# serve: %s" % json.dumps({k: summ[k] for k in ("completed", "solves_per_s", "p50_ms", "p99_ms")}),
           file=sys.stderr, flush=True)
-    same = []
+    same, err64 = [], 0.0
     for rid, path, model, t in reqs[:check_alone]:
         if rid not in srv.results:
             continue
@@ -249,10 +266,16 @@ def run_serve(seconds, rate, check_alone=4):
             S.set_option("precision", srv.records[rid].get("precision", 1))
             S.solve()
             same.append(bool(np.array_equal(S.get_instance(0, "uh"), srv.results[rid])))
+        with HDGSolver.batch([path], model) as S:     # the same request in the plain FP64 form
+            S.set_option("precision", 0)
+            S.solve()
+            u64 = S.get_instance(0, "uh")
+        err64 = max(err64, float(np.abs(srv.results[rid] - u64).max() / np.abs(u64).max()))
     kinds = {}
     for rid, path, model, t in reqs:
         kinds[model] = kinds.get(model, 0) + 1
-    summ.update(stream_seconds=seconds, rate_per_s=rate, mix=kinds, alone_equal_sample=same)
+    summ.update(stream_seconds=seconds, rate_per_s=rate, mix=kinds, alone_equal_sample=same,
+                rel_err_vs_fp64_sample=err64 if same else None)
     return summ
 
 
@@ -289,11 +312,13 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=4)
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--batch-cpu-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--serve-seconds", type=float, default=2.0, help="length of the served request stream")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of the served request stream")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served stream in the c2 line")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
-    ap.add_argument("--workload", default="c2", choices=["c2", "batch", "serve"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "batch", "batch3d", "serve"],
                     help="c2: the steady 3D convection-diffusion solve (default, with the batch "
-                         "workload's results as a sub-object); batch: the batch workload alone")
+                         "workloads' results as sub-objects); batch: the 2D batch workload alone; "
+                         "batch3d: the 3D convection-diffusion batch alone")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch workload in the c2 line")
     args = ap.parse_args()
 
@@ -323,6 +348,20 @@ def main():
                                    "{2..5}, HBM budget 200 GB, mixed precision" % (args.serve_rate,
                                                                                    args.serve_seconds)},
             "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"], "serve": sv, "roofline": None, "cpu_baseline": None}))
+        return 0
+    if args.workload == "batch3d":
+        b = run_batch(args.steps, args.warmup, cpu_baseline=False, count=48, kind="3d",
+                      concurrent=not args.batch_serial)
+        m = b["batched_mixed"]
+        print(json.dumps({
+            "metric": METRIC, "value": m["solves_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": m["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "batch of 48 parametrized 3D convection-diffusion solves (n^3 hexes, n in "
+                                   "{4,6,8,12}, p in {2,3,4}, 2 components, kappa log-uniform [0.03, 0.3]), "
+                                   "one batch per p, mixed FP32/FP64 with refinement, block-Jacobi, GMRES(100) "
+                                   "CGS tol 1e-10"},
+            "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "batch3d": b, "roofline": None, "cpu_baseline": None}))
         return 0
     if args.workload == "batch":
         b = run_batch(args.steps, args.warmup, cpu_baseline=not args.no_cpu_baseline,
@@ -497,6 +536,15 @@ def main():
             out["batch"] = run_batch(2, 1, cpu_baseline=not args.no_cpu_baseline)
         except Exception as e:          # the c2 line is still printed
             out["batch"] = {"error": repr(e)}
+        try:
+            out["batch3d"] = run_batch(2, 1, cpu_baseline=False, count=48, kind="3d")
+        except Exception as e:
+            out["batch3d"] = {"error": repr(e)}
+    if not args.no_serve:
+        try:
+            out["serve"] = run_serve(args.serve_seconds, args.serve_rate)
+        except Exception as e:
+            out["serve"] = {"error": repr(e)}
     print(json.dumps(out))
     return 0
 

@@ -2,14 +2,15 @@
 
 Requests (a datain directory and a model name) arrive over time.  They are grouped by
 element shape -- (model, nd, p, quadrature, ncu), the condition for sharing launches --
-into lanes.  Each lane has ``workers`` threads; an idle worker takes every request of its
-lane that has arrived and fits the HBM budget (up to ``max_batch``), builds one batch handle
-(``exa_batch_create``: the instances' meshes, Jacobians and Krylov vectors resident on the
-device for the whole solve, one HIP stream), solves it (shared launches, per-instance
-Newton/GMRES, see exasim_amd/csrc/batch.cpp), hands each request its result and frees the
-memory.  Requests that arrive while batches are running wait only until a worker of their
-lane is free and memory allows; batches of different lanes (and of the same lane, with
-workers > 1) run concurrently on their own streams, so their work overlaps on the device.
+into lanes.  A dispatcher starts a batch as soon as requests of a lane wait (after the
+batching window) and the HBM budget admits them, while fewer than ``max_concurrent`` batches
+run: it takes every waiting request of the lane that fits (up to ``max_batch``), builds one
+batch handle (``exa_batch_create``: the instances' meshes, Jacobians and Krylov vectors
+resident on the device for the whole solve, one HIP stream), solves it on a thread of its own
+(shared launches, per-instance Newton/GMRES, see exasim_amd/csrc/batch.cpp), hands each request
+its result and frees the memory.  A request that arrives while batches of its lane are running
+does not wait for them: it is admitted into a new batch as soon as memory and a slot free up,
+and the batches run concurrently on their own streams, so their work overlaps on the device.
 An instance's result does not depend on the batch it was solved in (the batch keeps every
 Krylov/Newton scalar per instance), so a request gets the same answer alone and in the mix.
 
@@ -54,11 +55,14 @@ Request = collections.namedtuple("Request", "rid datain model arrival")
 class Server:
     """Continuous-batching request loop over batch handles."""
 
-    def __init__(self, hbm_budget_bytes=200e9, max_batch=64, workers=1, precision=1, options=None,
+    def __init__(self, hbm_budget_bytes=200e9, max_batch=64, max_concurrent=8, precision=1, options=None,
                  keep_results=False, log=None, window=0.0):
         self.budget = float(hbm_budget_bytes)
         self.max_batch = max_batch
-        self.workers = workers
+        self.max_concurrent = int(max_concurrent)  # batches solving at once (a host thread and a HIP stream each)
+        self.running = 0
+        self.threads = []
+        self.dispatcher = None
         self.precision = precision
         self.options = dict(options or {})
         # each batch handle's assembly workspace, counted against the budget per batch
@@ -70,7 +74,6 @@ class Server:
         self.window = float(window)
         self.lock = threading.Condition()
         self.queues = collections.defaultdict(collections.deque)   # lane key -> waiting requests
-        self.lanes = {}                                             # lane key -> worker threads
         self.in_flight = 0.0
         self.peak_bytes = 0.0
         self.closed = False
@@ -91,25 +94,24 @@ class Server:
             t = time.perf_counter() - self.t0 if arrival is None else arrival
             self.queues[key].append(Request(rid, datain, model, t))
             self.records[rid] = dict(arrival=t)
-            if key not in self.lanes:
-                ths = [threading.Thread(target=self._worker, args=(key,), daemon=True) for _ in range(self.workers)]
-                self.lanes[key] = ths
-                for th in ths:
-                    th.start()
+            if self.dispatcher is None:
+                self.dispatcher = threading.Thread(target=self._dispatch, daemon=True)
+                self.dispatcher.start()
             self.lock.notify_all()
 
     def close(self):
-        """No more requests: workers exit when their queues are empty."""
+        """No more requests: the dispatcher returns once every waiting request has been solved."""
         with self.lock:
             self.closed = True
             self.lock.notify_all()
 
     def join(self):
-        for ths in list(self.lanes.values()):
-            for th in ths:
-                th.join()
+        if self.dispatcher is not None:
+            self.dispatcher.join()
+        for th in list(self.threads):
+            th.join()
 
-    # -- workers -----------------------------------------------------------
+    # -- dispatch ----------------------------------------------------------
     def _take(self, key):
         """Requests of the lane that fit the budget now (at least one when nothing is in flight)."""
         q = self.queues[key]
@@ -131,51 +133,68 @@ class Server:
                                                    self.precision)
         return self._est[req.datain]
 
-    def _worker(self, key):
-        while True:
-            with self.lock:
-                while True:
-                    q = self.queues[key]
-                    if q:
-                        age = time.perf_counter() - self.t0 - q[0].arrival
-                        if age < self.window and len(q) < self.max_batch and not self.closed:
-                            self.lock.wait(min(0.05, self.window - age))
-                            continue
-                        take, need = self._take(key)
-                        if take:
-                            break
-                    elif self.closed:
-                        return
-                    self.lock.wait(0.05)
-                self.in_flight += need
-                self.peak_bytes = max(self.peak_bytes, self.in_flight)
-                t_admit = time.perf_counter() - self.t0
-                for r in take:
-                    self.records[r.rid]["admit"] = t_admit
+    def _ready(self, now):
+        """The lane whose batch should start now (oldest waiting request first), or None."""
+        best = None
+        for key, q in self.queues.items():
+            if not q:
+                continue
+            age = now - q[0].arrival
+            if age < self.window and len(q) < self.max_batch and not self.closed:
+                continue
+            if best is None or q[0].arrival < self.queues[best][0].arrival:
+                best = key
+        return best
+
+    def _dispatch(self):
+        """Start batches while requests wait, slots are free and the budget admits them."""
+        with self.lock:
+            while True:
+                now = time.perf_counter() - self.t0
+                key = self._ready(now) if self.running < self.max_concurrent else None
+                take = []
+                if key is not None:
+                    take, need = self._take(key)
+                if take:
+                    self.in_flight += need
+                    self.peak_bytes = max(self.peak_bytes, self.in_flight)
+                    self.running += 1
+                    for r in take:
+                        self.records[r.rid]["admit"] = now
+                    th = threading.Thread(target=self._run_batch, args=(key, take, need), daemon=True)
+                    self.threads.append(th)
+                    th.start()
+                    continue
+                if self.closed and self.running == 0 and not any(self.queues.values()):
+                    return
+                self.lock.wait(0.005)
+
+    def _run_batch(self, key, take, need):
+        try:
             try:
-                try:
-                    failed = self._solve(key, take, self.precision)
-                except Exception as e:
-                    # a numerical failure of the whole batch (EXA_ERR_NUMERIC) in the mixed form:
-                    # solve it again in FP64
-                    if self.precision == 0 or getattr(e, "code", None) != 4:
-                        raise
-                    failed = take
-                if failed and self.precision != 0:
-                    # instances whose Newton iteration broke down with the FP32 factors: again, in FP64
-                    with self.lock:
-                        self.fallbacks.append((key, len(failed)))
-                    failed = self._solve(key, failed, 0)
+                failed = self._solve(key, take, self.precision)
+            except Exception as e:
+                # a numerical failure of the whole batch (EXA_ERR_NUMERIC) in the mixed form:
+                # solve it again in FP64
+                if self.precision == 0 or getattr(e, "code", None) != 4:
+                    raise
+                failed = take
+            if failed and self.precision != 0:
+                # instances whose Newton iteration broke down with the FP32 factors: again, in FP64
                 with self.lock:
-                    for r in failed:
-                        self.records[r.rid]["failed"] = True
-            except Exception as e:  # the requests fail, the server goes on
-                with self.lock:
-                    self.errors.append((key, [r.rid for r in take], repr(e)))
-            finally:
-                with self.lock:
-                    self.in_flight -= need
-                    self.lock.notify_all()
+                    self.fallbacks.append((key, len(failed)))
+                failed = self._solve(key, failed, 0)
+            with self.lock:
+                for r in failed:
+                    self.records[r.rid]["failed"] = True
+        except Exception as e:  # the requests fail, the server goes on
+            with self.lock:
+                self.errors.append((key, [r.rid for r in take], repr(e)))
+        finally:
+            with self.lock:
+                self.in_flight -= need
+                self.running -= 1
+                self.lock.notify_all()
 
     def _solve(self, key, take, precision):
         t1 = time.perf_counter()

@@ -17,7 +17,8 @@ def test_served_results_equal_alone(tmp_path):
     reqs = cases.serve_stream(str(tmp_path), seconds=0.4, rate=40.0, sizes=(4, 6), orders=(2, 3), ns_steps=1)
     print("served stream of %d requests" % len(reqs), flush=True)
     sys.stdout.flush()
-    srv = serve.Server(hbm_budget_bytes=8e9, max_batch=4, workers=2, precision=1, keep_results=True, window=0.2)
+    srv = serve.Server(hbm_budget_bytes=8e9, max_batch=4, max_concurrent=2, precision=1, keep_results=True,
+                       window=0.2)
     summ = srv.run_stream(reqs)
     assert summ["completed"] == len(reqs) and summ["failed"] == 0, summ
     assert max(b[1] for b in srv.batches) > 1

@@ -358,9 +358,9 @@ def main():
             "warmup": args.warmup, "ms_per_step": m["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
             "config": {"workload": "batch of 48 parametrized 3D convection-diffusion solves (n^3 hexes, n in "
-                                   "{4,6,8,12}, p in {2,3,4}, 2 components, kappa log-uniform [0.03, 0.3]), "
-                                   "one batch per p, mixed FP32/FP64 with refinement, block-Jacobi, GMRES(100) "
-                                   "CGS tol 1e-10"},
+                                   "{4,6,8,12}, p in {2,3,4}, 2 components, kappa log-uniform [0.01, 0.1]), "
+                                   "one batch per p, mixed FP32/FP64 with refinement, elemental Schwarz, "
+                                   "GMRES(100) CGS tol 1e-10"},
             "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "batch3d": b, "roofline": None, "cpu_baseline": None}))
         return 0
     if args.workload == "batch":

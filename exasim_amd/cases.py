@@ -287,19 +287,20 @@ def convdiff2d_batch(root, count=256, sizes=(8, 16, 24, 32, 48, 64), orders=(2, 
 
 
 def convdiff3d_batch(root, count=48, sizes=(4, 6, 8, 12), orders=(2, 3, 4), seed=20261202, ncu=2,
-                     precond="bjacobi", restart=100, gmres_tol=1e-10, newton_tol=1e-8, ortho=1):
+                     precond="eas", restart=100, gmres_tol=1e-10, newton_tol=1e-8, ortho=1):
     """``count`` parametrized 3D convection-diffusion instances (``convdiff3d``: n^3 hexes of the
     unit cube, ncu components) under root/inst####: n uniform over ``sizes``, p uniform over
-    ``orders``, kappa log-uniform in [3e-2, 3e-1] (block-Jacobi GMRES(100) converges in about 100
-    iterations there; at kappa = 1e-2 it takes 250-600), velocity a uniform random direction of magnitude
+    ``orders``, kappa log-uniform in [1e-2, 1e-1], velocity a uniform random direction of magnitude
     1, a per-instance seeded uniform[-1, 1] nodal source, Dirichlet u = 0 on x = 0 and zero
-    diffusive flux elsewhere.  Returns the instance descriptions (with their datain paths)."""
+    diffusive flux elsewhere; elemental additive Schwarz (block-Jacobi needs 300-400 GMRES(100)
+    iterations at kappa = 1e-2 on 6^3 and 8^3 meshes, the Schwarz blocks 75-80).  Returns the
+    instance descriptions (with their datain paths)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
         n = int(rng.choice(sizes))
         porder = int(rng.choice(orders))
-        kappa = float(10.0 ** rng.uniform(np.log10(3e-2), np.log10(3e-1)))
+        kappa = float(10.0 ** rng.uniform(-2.0, -1.0))
         v = rng.normal(size=3)
         v /= np.linalg.norm(v)
         path = os.path.join(root, "inst%04d" % i)

@@ -61,14 +61,23 @@ def poisson2d(path, n=16, porder=3, pgauss=14, model="poisson", kappa=1.0, tau=1
 
 def convdiff3d(path, n=32, porder=4, pgauss=None, ncu=2, kappa=1e-2, vel=(1.0, 0.5, 0.25),
                tau=1.0, seed=20261202, precond="bjacobi", restart=100, gmres_tol=1e-10,
-               gmres_maxit=2000, newton_iter=1, newton_tol=1e-8, ortho=0, cartgridpart=None):
-    """Steady convection-diffusion on an n^3 (or n[0] x n[1] x n[2]) hex mesh."""
+               gmres_maxit=2000, newton_iter=1, newton_tol=1e-8, ortho=0, cartgridpart=None,
+               inflow_dirichlet=False):
+    """Steady convection-diffusion on an n^3 (or n[0] x n[1] x n[2]) hex mesh of the unit cube:
+    u = 0 on x = 0 and zero diffusive flux elsewhere, or, with ``inflow_dirichlet``, u = 0 on
+    every inflow face (c . n < 0) and zero diffusive flux on the outflow faces.  A zero-flux
+    inflow face leaves the convective term without control of u there (its boundary term
+    c . n u^2 / 2 has the wrong sign), which makes GMRES stagnate at small kappa for some
+    velocity directions."""
     os.makedirs(path, exist_ok=True)
     nn = [n] * 3 if np.isscalar(n) else list(n)
     pgauss = 2 * porder if pgauss is None else pgauss
     m = dw.make_master(3, porder, pgauss)
     p, t = dw.structured_grid(nn)
-    exprs = [lambda v: np.all(np.abs(v[:, 0]) < 1e-8), lambda v: True]
+    planes = [(0, 0.0)]
+    if inflow_dirichlet:
+        planes = [(i, 0.0 if vel[i] > 0 else 1.0) for i in range(3) if abs(vel[i]) > 1e-12] or planes
+    exprs = [lambda v: any(np.all(np.abs(v[:, i] - x) < 1e-8) for i, x in planes), lambda v: True]
     bnd = dw.boundary_face_tags(p, t, 3, exprs)
     bcs = [1, 2]
     bf = np.where(bnd >= 0, np.array(bcs)[np.maximum(bnd, 0)], 0)
@@ -89,9 +98,13 @@ def convdiff3d(path, n=32, porder=4, pgauss=None, ncu=2, kappa=1e-2, vel=(1.0, 0
 
 def convdiff_unsteady(path, nd=2, n=4, porder=3, pgauss=None, ncu=1, kappa=1.0, vel=None, tau=1.0,
                       dt=0.01, nsteps=5, torder=3, nstage=3, precond="bjacobi", restart=50,
-                      gmres_tol=1e-12, gmres_maxit=1000, newton_iter=2, newton_tol=1e-10):
+                      gmres_tol=1e-12, gmres_maxit=1000, newton_iter=2, newton_tol=1e-10, scheme=0,
+                      capacity=None):
     """u_t + div(c u - kappa grad u) = 0 with u = 0 on the boundary and
-    u(x, 0) = prod_d sin(pi x_d) (exact for c = 0: exp(-nd pi^2 kappa t) u(x, 0))."""
+    u(x, 0) = prod_d sin(pi x_d) (exact for c = 0: exp(-nd pi^2 kappa t) u(x, 0)).
+    scheme: 0 DIRK (torder, nstage), 1 BDF (order torder).  capacity = (c0, c1): the model
+    heat2d_tdfunc, (c0 + c1 x) u_t = kappa lap u, with Tdfunc switched on (flag 10); the exact
+    decay for c1 = 0 is exp(-2 pi^2 kappa t / c0)."""
     os.makedirs(path, exist_ok=True)
     vel = [0.0] * nd if vel is None else list(vel)
     pgauss = 2 * porder if pgauss is None else pgauss
@@ -99,19 +112,26 @@ def convdiff_unsteady(path, nd=2, n=4, porder=3, pgauss=None, ncu=1, kappa=1.0, 
     p, t = dw.structured_grid([n] * nd)
     bnd = dw.boundary_face_tags(p, t, nd, [lambda v: True])
     bf = np.where(bnd >= 0, 1, 0)
-    dw.write_app(os.path.join(path, "app.bin"), nd=nd, ncu=ncu, nco=ncu,
-                 flag={0: 1, 6: PRECOND[precond] if isinstance(precond, str) else precond},
-                 problem={2: 0, 3: torder, 4: nstage, 13: newton_iter, 14: gmres_maxit, 15: restart},
-                 physicsparam=[kappa] + vel, tau=[tau], dt=[dt] * nsteps,
+    flags = {0: 1, 6: PRECOND[precond] if isinstance(precond, str) else precond}
+    param = [kappa] + vel
+    if capacity is not None:
+        flags[10] = 1
+        param = [kappa] + list(capacity)
+    dw.write_app(os.path.join(path, "app.bin"), nd=nd, ncu=ncu, nco=0 if capacity is not None else ncu,
+                 flag=flags,
+                 problem={2: scheme, 3: torder, 4: 1 if scheme == 1 else nstage, 13: newton_iter,
+                          14: gmres_maxit, 15: restart},
+                 physicsparam=param, tau=[tau], dt=[dt] * nsteps,
                  solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
     dw.write_master(os.path.join(path, "master.bin"), m)
     dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, [1])
     xdg = dw.compute_dgnodes(m, p, t)
     u0 = np.prod(np.sin(np.pi * xdg), axis=2)
     udg = np.repeat(u0[:, :, None], ncu, axis=2)
-    odg = np.zeros((t.shape[0], m["npe"], ncu))
+    odg = None if capacity is not None else np.zeros((t.shape[0], m["npe"], ncu))
     dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=ncu, nc=ncu * (nd + 1), udg=udg, odg=odg)
-    return dict(model="convdiff", nd=nd, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=ncu)
+    return dict(model="heat2d_tdfunc" if capacity is not None else "convdiff", nd=nd, ne=t.shape[0],
+                npe=m["npe"], npf=m["npf"], ncu=ncu)
 
 
 def ns_freestream(gam=1.4, mach=0.5):
@@ -291,10 +311,9 @@ def convdiff3d_batch(root, count=48, sizes=(4, 6, 8, 12), orders=(2, 3, 4), seed
     """``count`` parametrized 3D convection-diffusion instances (``convdiff3d``: n^3 hexes of the
     unit cube, ncu components) under root/inst####: n uniform over ``sizes``, p uniform over
     ``orders``, kappa log-uniform in [1e-2, 1e-1], velocity a uniform random direction of magnitude
-    1, a per-instance seeded uniform[-1, 1] nodal source, Dirichlet u = 0 on x = 0 and zero
-    diffusive flux elsewhere; elemental additive Schwarz (block-Jacobi needs 300-400 GMRES(100)
-    iterations at kappa = 1e-2 on 6^3 and 8^3 meshes, the Schwarz blocks 75-80).  Returns the
-    instance descriptions (with their datain paths)."""
+    1, a per-instance seeded uniform[-1, 1] nodal source, Dirichlet u = 0 on the inflow faces and
+    zero diffusive flux on the outflow faces; elemental additive Schwarz.  Returns the instance
+    descriptions (with their datain paths)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
@@ -306,7 +325,7 @@ def convdiff3d_batch(root, count=48, sizes=(4, 6, 8, 12), orders=(2, 3, 4), seed
         path = os.path.join(root, "inst%04d" % i)
         info = convdiff3d(path, n=n, porder=porder, ncu=ncu, kappa=kappa, vel=tuple(float(x) for x in v),
                           seed=seed + 1 + i, precond=precond, restart=restart, gmres_tol=gmres_tol,
-                          newton_tol=newton_tol, ortho=ortho)
+                          newton_tol=newton_tol, ortho=ortho, inflow_dirichlet=True)
         info.update(path=path, n=n, porder=porder, kappa=kappa)
         out.append(info)
     return out

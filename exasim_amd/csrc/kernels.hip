@@ -17,6 +17,7 @@
 #include "generated/registry.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace exa {
 
@@ -470,37 +471,48 @@ __global__ __launch_bounds__(64) void k_lu_swap(T* Aall, int n, int ncols, long 
     if (i < nm) colj[dst[i]] = v[i];
 }
 
-// Columns right of a panel factored by k_lu_panel_reg, after k_lu_swap: U12 = L11^{-1} A12 (one
-// column per thread; blockIdx.y selects a chunk of 256 columns).
+// Columns right of a panel factored by k_lu_panel_reg, after k_lu_swap: U12 = L11^{-1} A12.  A
+// workgroup (one wave) takes 64 columns: their nb panel rows are staged in LDS with coalesced loads
+// (consecutive lanes read consecutive rows of a column), each lane solves one column from LDS in
+// registers (forward substitution in order), and the tile goes back with coalesced stores.
 template <typename T, int BLU_NB>
-__global__ __launch_bounds__(256) void k_lu_u12(T* Aall, int n, int ncols, long stride, int k0,
-                                                const int* piv_ws) {
+__global__ __launch_bounds__(64) void k_lu_u12(T* Aall, int n, int ncols, long stride, int k0,
+                                               const int* piv_ws) {
   T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
-  // L11 with a run-time leading dimension: constant LDS addresses in the unrolled solve let the
-  // compiler hoist every load into registers (and spill)
   __shared__ T L11[BLU_NB * BLU_NB];
-  const int ldl = nb;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nb; idx += 256) {
+  __shared__ T X[BLU_NB][65];
+  const int lane = threadIdx.x;
+  for (int idx = lane; idx < nb * nb; idx += 64) {
     const int r = idx % nb, c = idx / nb;
-    L11[r + ldl * c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
+    L11[r + nb * c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
+  }
+  const int c0 = k0 + nb + blockIdx.y * 64;
+  const int ncol = min(64, ncols - c0);
+  for (int idx = lane; idx < nb * 64; idx += 64) {
+    const int r = idx % nb, c = idx / nb;
+    if (c < ncol) X[r][c] = A[k0 + r + static_cast<long>(n) * (c0 + c)];
   }
   __syncthreads();
-  const int j = k0 + nb + blockIdx.y * 256 + tid;
-  if (j >= ncols) return;
-  T* colj = A + static_cast<long>(n) * j + k0;
-  T x[BLU_NB];
+  if (lane < ncol) {
+    T x[BLU_NB];
 #pragma unroll
-  for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
+    for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? X[kk][lane] : T(0);
 #pragma unroll
-  for (int kk = 1; kk < BLU_NB; ++kk)
+    for (int kk = 1; kk < BLU_NB; ++kk)
 #pragma unroll
-    for (int t = 0; t < kk; ++t)
-      if (kk < nb) x[kk] = fma(-L11[kk + ldl * t], x[t], x[kk]);
+      for (int t = 0; t < kk; ++t)
+        if (kk < nb) x[kk] = fma(-L11[kk + nb * t], x[t], x[kk]);
 #pragma unroll
-  for (int kk = 0; kk < BLU_NB; ++kk)
-    if (kk < nb) colj[kk] = x[kk];
+    for (int kk = 0; kk < BLU_NB; ++kk)
+      if (kk < nb) X[kk][lane] = x[kk];
+  }
+  __syncthreads();
+  for (int idx = lane; idx < nb * 64; idx += 64) {
+    const int r = idx % nb, c = idx / nb;
+    if (c < ncol) A[k0 + r + static_cast<long>(n) * (c0 + c)] = X[r][c];
+  }
+  (void)piv_ws;
 }
 
 // Inverse of the unit lower triangle L11 (LOWER) or of the upper triangle U11 of the panel at
@@ -556,39 +568,51 @@ __global__ __launch_bounds__(64) void k_lu_tri_inv(const T* Aall, int n, long st
     if (r < nb) Wm[r + BLU_NB * c] = y[r];
 }
 
-// X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
+// Back substitution of one diagonal block of U on the right-hand-side columns n..ncols-1,
+// X[k0 : k0+nb] = U11^{-1} X[k0 : k0+nb]: tiles of 64 columns staged in LDS with coalesced loads
+// and stores as in k_lu_u12, one column per lane.
 template <typename T, int BLU_NB>
-__global__ __launch_bounds__(256) void k_lu_back(T* Aall, int n, int ncols, long stride, int k0) {
+__global__ __launch_bounds__(64) void k_lu_back(T* Aall, int n, int ncols, long stride, int k0) {
   T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
   __shared__ T U[BLU_NB][BLU_NB + 1];
   __shared__ T dinv[BLU_NB];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    int r = idx % nb, c = idx / nb;
+  __shared__ T X[BLU_NB][65];
+  const int lane = threadIdx.x;
+  for (int idx = lane; idx < nb * nb; idx += 64) {
+    const int r = idx % nb, c = idx / nb;
     U[r][c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
   }
-  __syncthreads();
-  if (tid < nb) dinv[tid] = T(1) / U[tid][tid];
-  __syncthreads();
-  // one right-hand-side column per thread (blockIdx.y selects a chunk of 256)
-  const int j = n + blockIdx.y * 256 + tid;
-  if (j >= ncols) return;
-  T* colj = A + static_cast<long>(n) * j + k0;
-  T x[BLU_NB];
-#pragma unroll
-  for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
-#pragma unroll
-  for (int kk = BLU_NB - 1; kk >= 0; --kk) {
-    if (kk < nb) {
-      x[kk] *= dinv[kk];
-#pragma unroll
-      for (int t = 0; t < kk; ++t) x[t] = fma(-U[t][kk], x[kk], x[t]);
-    }
+  const int c0 = n + blockIdx.y * 64;
+  const int ncol = min(64, ncols - c0);
+  for (int idx = lane; idx < nb * 64; idx += 64) {
+    const int r = idx % nb, c = idx / nb;
+    if (c < ncol) X[r][c] = A[k0 + r + static_cast<long>(n) * (c0 + c)];
   }
+  __syncthreads();
+  if (lane < nb) dinv[lane] = T(1) / U[lane][lane];
+  __syncthreads();
+  if (lane < ncol) {
+    T x[BLU_NB];
 #pragma unroll
-  for (int kk = 0; kk < BLU_NB; ++kk)
-    if (kk < nb) colj[kk] = x[kk];
+    for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? X[kk][lane] : T(0);
+#pragma unroll
+    for (int kk = BLU_NB - 1; kk >= 0; --kk) {
+      if (kk < nb) {
+        x[kk] *= dinv[kk];
+#pragma unroll
+        for (int t = 0; t < kk; ++t) x[t] = fma(-U[t][kk], x[kk], x[t]);
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < BLU_NB; ++kk)
+      if (kk < nb) X[kk][lane] = x[kk];
+  }
+  __syncthreads();
+  for (int idx = lane; idx < nb * 64; idx += 64) {
+    const int r = idx % nb, c = idx / nb;
+    if (c < ncol) A[k0 + r + static_cast<long>(n) * (c0 + c)] = X[r][c];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1047,6 +1071,12 @@ __global__ void k_getq_any(Dims d, const double* MCs, const double* MEs, const i
 }
 
 // ---------------------------------------------------------------------------
+// models with a time-derivative coefficient (Tdfunc) define tdfunc(x, uq, v, p, t, T)
+template <class M, class = void>
+struct has_tdfunc : std::false_type {};
+template <class M>
+struct has_tdfunc<M, std::void_t<decltype(&M::tdfunc)>> : std::true_type {};
+
 // physicsparam and tau of the instance element E belongs to (one instance: a.einst is null)
 __device__ __forceinline__ const double* inst_param(const AsmArgs& a, long E) {
   return a.einst ? a.param + static_cast<long>(a.einst[E]) * a.nparam : a.param;
@@ -1090,7 +1120,12 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
   const double* prm = inst_param(a, E);
   M::template flux<JAC>(x, uq, v, prm, a.time, f, f_uq);
   M::template source<JAC>(x, uq, v, prm, a.time, s, s_uq);
-  if (a.tdep) {  // uequation.cpp:126-151, Tdfunc = 1 for the built-in models: s += sdg - dtfactor*u
+  if (a.tdep) {  // uequation.cpp:126-151: s += (sdg - dtfactor*u) T, s_u -= dtfactor T
+    double T[NCU];
+#pragma unroll
+    for (int k = 0; k < NCU; ++k) T[k] = 1.0;
+    if constexpr (has_tdfunc<M>::value)
+      if (a.tdfunc) M::tdfunc(x, uq, v, prm, a.time, T);
     double sg[NCU];
 #pragma unroll
     for (int k = 0; k < NCU; ++k) sg[k] = 0.0;
@@ -1101,8 +1136,8 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < NCU; ++k) {
-      s[k] += sg[k] - a.dtfactor * uq[k];
-      if (JAC) s_uq[k + NCU * k] -= a.dtfactor;
+      s[k] += (sg[k] - a.dtfactor * uq[k]) * T[k];
+      if (JAC) s_uq[k + NCU * k] -= a.dtfactor * T[k];
     }
   }
 #pragma unroll
@@ -2398,7 +2433,7 @@ void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* 
           k_lu_tri_inv<T, NB, true><<<batch, 64, 0, s>>>(A, n, stride, k0, tri_ws);
           apply_tri(nb, k0, k0 + nb, right);
         } else {
-          k_lu_u12<T, NB><<<dim3(batch, (right + 255) / 256), 256, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
+          k_lu_u12<T, NB><<<dim3(batch, (right + 63) / 64), 64, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
         }
       }
     } else {
@@ -2425,8 +2460,8 @@ void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* 
       EXA_HIP(hipGetLastError());
       apply_tri(std::min(NB, n - k0), k0, n, ncols - n);
     } else {
-      dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 255) / 256));
-      k_lu_back<T, NB><<<grid, 256, 0, s>>>(A, n, ncols, stride, k0);
+      dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 63) / 64));
+      k_lu_back<T, NB><<<grid, 64, 0, s>>>(A, n, ncols, stride, k0);
       EXA_HIP(hipGetLastError());
     }
     if (k0 > 0) {  // X[0:k0] -= U[0:k0, k0:k0+nb] * X[k0:k0+nb]

@@ -93,8 +93,9 @@ struct AsmArgs {
   const double *udg, *uh, *egeo, *fgeo, *og, *ogf;
   double *rv, *rJ, *W, *fpt;
   long augs;  // per-element stride of the augmented system [D | F | Ru (| I)]
-  // time-dependent problems: source += sdg - dtfactor*u (sdg nodal [npe, ncu, ne])
-  int tdep;
+  // time-dependent problems: source += (sdg - dtfactor*u) T (sdg nodal [npe, ncu, ne]), T the
+  // model's Tdfunc when tdfunc is 1 (and the model has one), else 1
+  int tdep, tdfunc;
   double dtfactor;
   const double* sdg;
 };

@@ -134,15 +134,18 @@ void Solver::init(SolData& sol) {
   // time dependence (setstructs.cpp:126-149, 194): flag[0] tdep, flag[1] wave, flag[10] tdfunc,
   // problem[2] temporal scheme (0 DIRK), problem[3] order, problem[4] stages
   tdep = static_cast<int>(fl[0]);
+  tdfunc = static_cast<int>(fl[10]);  // 1: the model's Tdfunc multiplies the time derivative
   if (tdep) {
     if (static_cast<int>(fl[1]) != 0) throw Error(5, "wave problems are not supported");
-    // Tdfunc (flag[10]) is 1 for the built-in models, so the mass term is the identity
-    if (static_cast<int>(pr[2]) != 0) throw Error(5, "only DIRK time stepping is supported");
+    tscheme = static_cast<int>(pr[2]);
+    if (tscheme != 0 && tscheme != 1) throw Error(5, "temporal scheme " + std::to_string(tscheme) +
+                                                         ": 0 (DIRK) and 1 (BDF) are supported");
     torder = static_cast<int>(pr[3]);
-    tstages = static_cast<int>(pr[4]);
+    tstages = tscheme == 1 ? 1 : static_cast<int>(pr[4]);
     dts = app.dt;
     time0 = app.factor.empty() ? 0.0 : app.factor[0];
-    dirk_coefficients();
+    if (tscheme == 0) dirk_coefficients();
+    else if (torder < 1 || torder > 3) throw Error(5, "BDF" + std::to_string(torder) + " is not implemented");
   }
   tnow = time0;
   precond = static_cast<int>(fl[6]);
@@ -307,6 +310,7 @@ void Solver::init(SolData& sol) {
     udgprev.alloc(nu);
     utmp.alloc(nu);
     sdg.zero(s);
+    if (tscheme == 1) bdfhist.alloc(nu * torder);
   }
 }
 
@@ -591,6 +595,7 @@ AsmArgs Solver::asm_args(int e0, int eb) {
   AsmArgs a{};
   a.augs = static_cast<long>(d.n) * augcols();
   a.tdep = tdep;
+  a.tdfunc = tdfunc;
   a.dtfactor = dtfactor;
   a.sdg = sdg.p;
   a.model = model;
@@ -985,6 +990,10 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
     else newton(newton_iters, gmres_total, final_norm);
     return;
   }
+  if (tscheme == 1) {
+    solve_bdf(newton_iters, gmres_total, final_norm);
+    return;
+  }
   const int S = tstages;
   double csum = 0.0;
   for (int j = 0; j < S; ++j) csum += dirk_c[j];
@@ -1020,6 +1029,47 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
                     tnow, ni, gi, nrm);
         std::fflush(stdout);
       }
+    }
+    t += dt;
+  }
+  tnow = t;
+  EXA_HIP(hipStreamSynchronize(s));
+  if (newton_iters) *newton_iters = nit;
+  if (gmres_total) *gmres_total = gtot;
+  if (final_norm) *final_norm = nrm;
+}
+
+// BDF1-3 (PreviousSolutions, previoussolutions.cpp:62-100; UpdateSourceBDF, updatesource.cpp:104-140;
+// BDFcoeff, timestepcoeff.cpp:106-130): per step the history takes u^n, dtfactor = c0/dt and
+// sdg = -sum_k (c_k/dt) u^{n+1-k}, and one Newton solve at t + dt gives u^{n+1}.  The first steps
+// use the highest order the stored history allows (the reference reads history it has not
+// written in its first torder - 1 steps).
+void Solver::solve_bdf(int* newton_iters, int* gmres_total, double* final_norm) {
+  static const double C[3][4] = {{1.0, -1.0, 0, 0}, {1.5, -2.0, 0.5, 0}, {11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0}};
+  const long nu = static_cast<long>(d.npe) * d.ncu * d.ne;
+  int nit = 0, gtot = 0, nhist = 0;
+  double nrm = 0.0, t = tnow;
+  for (size_t istep = 0; istep < dts.size(); ++istep) {
+    const double dt = dts[istep];
+    // history: slot k holds u^{n-k}; shift by one and store u^n in slot 0
+    for (int k = std::min(nhist, torder - 1); k >= 1; --k) copy(bdfhist.p + k * nu, bdfhist.p + (k - 1) * nu, nu, s);
+    u_axpby(d, udg.p, 1.0, 0.0, bdfhist.p, s);
+    nhist = std::min(nhist + 1, torder);
+    const int q = std::min(torder, nhist);
+    const double* c = C[q - 1];
+    dtfactor = c[0] / dt;
+    axpby(sdg.p, bdfhist.p, -c[1] / dt, 0.0, nu, s);
+    for (int k = 1; k < q; ++k) axpby(sdg.p, bdfhist.p + k * nu, -c[k + 1] / dt, 1.0, nu, s);
+    tnow = t + dt;
+    int ni = 0, gi = 0;
+    if (batch) newton_seg(&ni, &gi, &nrm);
+    else newton(&ni, &gi, &nrm);
+    nit += ni;
+    gtot += gi;
+    if (verbose) {
+      std::printf("Timestep %zu (BDF%d), time %g: Newton %d, GMRES %d, residual %g\n", istep + 1, q, tnow, ni, gi,
+                  nrm);
+      std::fflush(stdout);
     }
     t += dt;
   }

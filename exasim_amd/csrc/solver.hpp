@@ -50,11 +50,11 @@ struct Solver {
   double refine_rtol = 1e-10, refine_tol = 1e-4, refine_first_tol = 1e-7;
   int last_refinements = 0;
   // time dependence (DIRK): steps app.dt, stage tables c, d (= A^{-1}, column-major), t
-  int tdep = 0, torder = 1, tstages = 1;
+  int tdep = 0, torder = 1, tstages = 1, tscheme = 0, tdfunc = 0;
   std::vector<double> dts;
   double time0 = 0.0, tnow = 0.0, dtfactor = 0.0;
   double dirk_c[3] = {0, 0, 0}, dirk_d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dirk_t[3] = {0, 0, 0};
-  DevBuf<double> sdg, udgprev, utmp;
+  DevBuf<double> sdg, udgprev, utmp, bdfhist;  // bdfhist: BDF history u^{n-k}, k < torder
   bool timing = false;
   double ws_budget_bytes = 16.0e9;
 
@@ -193,6 +193,7 @@ struct Solver {
   void alloc_mixed();
   void solve(int* newton_iters, int* gmres_iters, double* final_norm);
   void newton(int* newton_iters, int* gmres_iters, double* final_norm);
+  void solve_bdf(int* newton_iters, int* gmres_iters, double* final_norm);
   void dirk_coefficients();
   void write_output();
   long get_array(const std::string& name, double* dst, long len);

@@ -411,11 +411,9 @@ def emit_cpp(text, name, source="a pdemodel.txt"):
     nd, ncu, nc, nco = model_sizes(header)
     if header["vectors"].get("w", 0):
         raise ModelSyntaxError("models with a w equation (ncw > 0) are not supported")
-    for fname in ("Tdfunc",):
-        if fname in funcs:
-            body = [st for st in funcs[fname].body if st[0] != "size"]
-            if not (len(body) == 1 and body[0][0] == "fill" and body[0][2] == 1.0):
-                raise ModelSyntaxError("Tdfunc must be ones(...): time-derivative functions are not supported")
+    has_td = "Tdfunc" in funcs
+    if has_td and funcs["Tdfunc"].output[1] != ncu:
+        raise ModelSyntaxError("Tdfunc must have ncu outputs")
     nbc = funcs["FbouHdg"].output[1] // ncu
     if funcs["FbouHdg"].output[1] != nbc * ncu:
         raise ModelSyntaxError("FbouHdg output size must be a multiple of ncu")
@@ -424,7 +422,7 @@ def emit_cpp(text, name, source="a pdemodel.txt"):
     ident = re.sub(r"\W", "_", name)
     parts = ["// Generated by exasim_amd/modelgen.py from %s -- do not edit." % source,
              "#pragma once", "#include \"dual.hpp\"", "", "namespace exa {", "namespace gen_%s {" % ident, ""]
-    for fname in _call_order(funcs, ["Flux", "Source", "FbouHdg"]):
+    for fname in _call_order(funcs, ["Flux", "Source", "FbouHdg"] + (["Tdfunc"] if has_td else [])):
         parts.append(_Emitter(funcs, fname).emit(header))
         parts.append("")
     vec = header["vectors"]
@@ -487,6 +485,17 @@ def emit_cpp(text, name, source="a pdemodel.txt"):
     parts.append("      if (JAC) for (int k = 0; k < NC; ++k) s_uq[i + NCU * k] = F[i].deriv(k);")
     parts.append("    }")
     parts.append("  }")
+    if has_td:  # the time-derivative coefficient (values only: its derivative is not used)
+        parts.append("  __device__ static void tdfunc(const double* x, const double* uq, const double* v, const double* p,")
+        parts.append("                                double t, double* tcoef) {")
+        parts.append("    using S = Dual<0>;")
+        parts.append(prologue)
+        parts.append("    S U[NC];")
+        parts.append("    for (int k = 0; k < NC; ++k) U[k] = S(uq[k]);")
+        parts.append("    S F[NCU];")
+        parts.append("    gen_%s::fn_Tdfunc<S>(%s, F);" % (ident, argv("Tdfunc", "tdfunc")))
+        parts.append("    for (int i = 0; i < NCU; ++i) tcoef[i] = F[i].v;")
+        parts.append("  }")
     # fbou: the Jacobian in passes of KB derivative directions (of NC + NCU), so the
     # dual numbers of a large boundary function stay small
     parts.append("  static constexpr int NT = NC + NCU, KB = NT < 4 ? NT : 4;")

@@ -272,6 +272,15 @@ class DSLModel:
         g = np.stack([S[c].g for c in range(ncu)], 0)          # [ncu, nc, P]
         return s, g.transpose(2, 0, 1)
 
+    def tdfunc(self, x, uq, v, param, time):
+        """Tdfunc values [N, ncu] (the time-derivative coefficient; ones when the file has no
+        Tdfunc).  Only the values enter the equations (uequation.cpp:129-145)."""
+        P, ncu, nc = x.shape[0], self.ncu, self.nc
+        if "Tdfunc" not in self.funcs:
+            return np.ones((P, ncu))
+        T = _Interp(self.funcs, self.header, nc, P).run("Tdfunc", self._args("Tdfunc", P, nc, x, uq, v, param, time))
+        return np.stack([np.broadcast_to(T[c].v, (P,)) for c in range(ncu)], 1)
+
     def fbou(self, ib, x, uq, v, uh, nl, tau, param, time):
         P, ncu, nc = x.shape[0], self.ncu, self.nc
         nv = nc + ncu

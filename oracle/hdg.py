@@ -351,6 +351,7 @@ class HDGProblem:
         self.tdep = int(flag[0])
         self.tdfunc = int(flag[10])
         self.torder, self.tstages = int(prob[3]), int(prob[4])
+        self.tscheme = int(prob[2])                 # 0 DIRK, 1 BDF
         self.dt = np.asarray(app["dt"], dtype=float)
         self.time = float(app["factor"][0]) if len(app["factor"]) else 0.0
         self.dtfactor = 0.0
@@ -406,13 +407,17 @@ class HDGProblem:
         v = self.og.reshape(N, -1)
         s, s_uq = self.model.source(x, uq, v, self.param, self.time)
         if self.tdep:
-            # uequation.cpp:126-151; Tdfunc is 1 for the models here (so tdfunc = 0 or 1
-            # give the same term): s += sdg - dtfactor*u, s_u -= dtfactor
+            # uequation.cpp:126-151: s += (sdg - dtfactor*u) T, s_u -= dtfactor T, with T the
+            # model's Tdfunc values when flag tdfunc is 1 and ones otherwise (the derivative of T
+            # does not enter the Jacobian)
             sdgg = np.einsum("gi,eic->egc", phi[0], self.sdg).reshape(N, ncu)
-            s = s + sdgg - self.dtfactor * uq[:, :ncu]
+            T = np.ones((N, ncu))
+            if self.tdfunc == 1 and hasattr(self.model, "tdfunc"):
+                T = self.model.tdfunc(x, uq, v, self.param, self.time)
+            s = s + (sdgg - self.dtfactor * uq[:, :ncu]) * T
             s_uq = s_uq.copy()
             for c in range(ncu):
-                s_uq[:, c, c] -= self.dtfactor
+                s_uq[:, c, c] -= self.dtfactor * T[:, c]
         f, f_uq = self.model.flux(x, uq, v, self.param, self.time)
         ng = phi.shape[1]
         s = s.reshape(ne, ng, ncu)
@@ -912,6 +917,42 @@ def update_source_dirk(P, stage, dt, d, udgprev):
         P.sdg = sc * P.sdg - (d[S + 2] / dt) * Rq + (pc / dt) * udgprev
     else:
         raise NotImplementedError("DIRK with more than 3 stages")
+
+
+def bdf_coeffs(order):
+    """BDFcoeff (timestepcoeff.cpp:106-130): u_t ~ (c0 u^{n+1} + c1 u^n + ... ) / dt."""
+    return {1: [1.0, -1.0], 2: [1.5, -2.0, 0.5], 3: [11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0]}[order]
+
+
+def bdf_solve(P, nsteps=None):
+    """Time loop with BDF1-3 (PreviousSolutions, previoussolutions.cpp:62-100; UpdateSourceBDF,
+    updatesource.cpp:104-140): dtfactor = c0/dt, sdg = -sum_k (c_k/dt) u^{n+1-k}, one Newton solve
+    per step at time t + dt.  The first steps use the highest order the stored history allows
+    (BDF1, then BDF2, ...): the reference reads history it has not written in its first torder - 1
+    steps."""
+    nsteps = len(P.dt) if nsteps is None else nsteps
+    time = P.time
+    hist = []                                       # hist[k] = u^{n-k}
+    gmres_total = 0
+    for istep in range(nsteps):
+        dt = P.dt[istep]
+        hist.insert(0, P.udg[:, :, :P.ncu].copy())
+        del hist[P.torder:]
+        order = min(P.torder, len(hist))
+        c = bdf_coeffs(order)
+        P.dtfactor = c[0] / dt
+        P.sdg = sum(-(c[k + 1] / dt) * hist[k] for k in range(order))
+        P.time = time + dt
+        h = newton_solve(P)
+        gmres_total += sum(h["gmres_iters"])
+        time += dt
+    P.time = time
+    return gmres_total
+
+
+def time_solve(P, nsteps=None):
+    """The time loop of the problem's scheme (DIRK or BDF)."""
+    return bdf_solve(P, nsteps) if P.tscheme == 1 else dirk_solve(P, nsteps)
 
 
 def dirk_solve(P, nsteps=None):

@@ -155,7 +155,8 @@ def test_batch_cgs_matches_oracle(cd_batch_cgs):
 
 def test_batch3d_convdiff_matches_oracle(tmp_path):
     """A batch of 3D convection-diffusion instances (the 3D batch workload's settings: Schwarz blocks,
-    classical Gram-Schmidt, two components, per-instance velocity, diffusivity and source) against
+    classical Gram-Schmidt, two components, per-instance velocity, diffusivity and source, Dirichlet
+    inflow faces) against
     the oracle: FP64 1e-10, mixed form with the benchmark's refinement tolerances 1e-8."""
     say("convdiff3d batch of 3 (n = 2, 3, 2; p = 2) vs oracle")
     from exasim_amd import cases
@@ -165,7 +166,7 @@ def test_batch3d_convdiff_matches_oracle(tmp_path):
                                          (2, 0.1, (0.48, 0.6, 0.64))]):
         p = str(tmp_path / ("i%d" % i))
         cases.convdiff3d(p, n=n, porder=2, ncu=2, kappa=kappa, vel=vel, seed=7 + i, precond="eas",
-                         restart=20, gmres_tol=1e-10, newton_tol=1e-8, ortho=1)
+                         restart=100, gmres_tol=1e-10, newton_tol=1e-8, ortho=1, inflow_dirichlet=True)
         paths.append(p)
     f64, stats = _solve_batch(paths, "convdiff")
     with HDGSolver.batch(paths, "convdiff") as S:
@@ -178,6 +179,7 @@ def test_batch3d_convdiff_matches_oracle(tmp_path):
         P = ora.HDGProblem(p, "convdiff")
         hist = ora.newton_solve(P)
         ref = P.uh.reshape(-1)
+        assert max(hist["gmres_iters"]) < 2000, "the oracle did not converge (instance %d)" % i
         assert stats[1][i] == sum(hist["gmres_iters"]), i
         assert rel(f64[i][0], ref) < 1e-10, i
         assert rel(mix[i], ref) < 1e-8, i

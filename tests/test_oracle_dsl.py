@@ -190,8 +190,9 @@ def test_generated_headers_are_current():
      "function FbouHdg(x, uq, uhat)\n output_size(fb) = 1;\n fb[0] = uhat[0];\nend\n", "unknown name"),
     ("vectors x(2), uq(3), uhat(1)\nfunction Flux(x, uq)\n output_size(f) = 2;\n f[0] = uq[1];\n"
      " f[1] = uq[2];\nend\nfunction Source(x, uq)\n output_size(s) = 1;\n zeros(s);\nend\n"
-     "function Tdfunc(x, uq)\n output_size(m) = 1;\n m[0] = uq[0];\nend\n"
-     "function FbouHdg(x, uq, uhat)\n output_size(fb) = 1;\n fb[0] = uhat[0];\nend\n", "Tdfunc"),
+     "function Tdfunc(x, uq)\n output_size(m) = 2;\n m[0] = uq[0];\n m[1] = 1.0;\nend\n"
+     "function FbouHdg(x, uq, uhat)\n output_size(fb) = 1;\n fb[0] = uhat[0];\nend\n",
+     "Tdfunc must have ncu outputs"),
 ])
 def test_modelgen_rejects_unsupported(text, msg):
     with pytest.raises(modelgen.ModelSyntaxError, match=msg):

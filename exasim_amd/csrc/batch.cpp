@@ -305,7 +305,8 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
     assemble();
     push(mask_d, mask_of(act));
     seg_scale(segT.map, b.p, b.p, mask_d, 0, s);
-    gtot += gmres_seg(b.p, x.p, precision ? std::max(gmres_tol, refine_first_tol) : -1.0, its);
+    gtot += gmres_seg(b.p, x.p, precision ? std::max(gmres_tol, refine_first_tol) : -1.0, its,
+                      precision ? refine_maxit_gmres() : -1);
     for (int i = 0; i < ns; ++i)
       if (act[i]) {
         inst_gmres[i] += its[i];
@@ -361,7 +362,7 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
         face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
         push(mask_d, mask_of(ref));
         seg_scale(segT.map, b.p, b.p, mask_d, 0, s);
-        gtot += gmres_seg(b.p, x.p, refine_tol, its);
+        gtot += gmres_seg(b.p, x.p, refine_tol, its, refine_maxit_gmres());
         for (int i = 0; i < ns; ++i)
           if (ref[i]) inst_gmres[i] += its[i];
         axpby(u.p, x.p, 1.0, 1.0, N, s);

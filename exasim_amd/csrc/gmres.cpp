@@ -126,13 +126,14 @@ void Solver::gmres_cycle_end(const GmresDev& g, int i, const double* bv, double*
   seg_scale(segT.map, V.p, r.p, segcoef.p + ns, 1, s);  // v_0 = r / ||r|| (0 when finished)
 }
 
-int Solver::gmres(const double* bv, double* xv, double tol_rel) {
+int Solver::gmres(const double* bv, double* xv, double tol_rel, int maxit) {
   std::vector<int> its;
-  return gmres_run(bv, xv, tol_rel, its);
+  return gmres_run(bv, xv, tol_rel, its, maxit);
 }
 
-int Solver::gmres_run(const double* bv, double* xv, double tol_rel, std::vector<int>& iters) {
+int Solver::gmres_run(const double* bv, double* xv, double tol_rel, std::vector<int>& iters, int maxit_in) {
   const long N = d.N;
+  const int gmres_maxit = maxit_in < 0 ? this->gmres_maxit : maxit_in;
   const int ns = ninst;
   if (static_cast<long>(gHm.n) < static_cast<long>(ns) * (restart + 1) * (restart + 1)) seg_alloc();
   const GmresDev g = gmres_state();
@@ -243,8 +244,8 @@ int Solver::gmres_run(const double* bv, double* xv, double tol_rel, std::vector<
   return *std::max_element(iters.begin(), iters.end());
 }
 
-int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<int>& iters) {
-  return gmres_run(bv, xv, tol_rel, iters);
+int Solver::gmres_seg(const double* bv, double* xv, double tol_rel, std::vector<int>& iters, int maxit) {
+  return gmres_run(bv, xv, tol_rel, iters, maxit);
 }
 
 }  // namespace exa

@@ -471,48 +471,41 @@ __global__ __launch_bounds__(64) void k_lu_swap(T* Aall, int n, int ncols, long 
     if (i < nm) colj[dst[i]] = v[i];
 }
 
-// Columns right of a panel factored by k_lu_panel_reg, after k_lu_swap: U12 = L11^{-1} A12.  A
-// workgroup (one wave) takes 64 columns: their nb panel rows are staged in LDS with coalesced loads
-// (consecutive lanes read consecutive rows of a column), each lane solves one column from LDS in
-// registers (forward substitution in order), and the tile goes back with coalesced stores.
+// Columns right of a panel factored by k_lu_panel_reg, after k_lu_swap: U12 = L11^{-1} A12 (one
+// column per thread; blockIdx.y selects a chunk of 256 columns).  Each thread reads its column's
+// nb contiguous entries; a variant with one wave per 64 columns staged through LDS with coalesced
+// loads was measured slower (0.47 against 0.33 ms per launch, averaged over the FP64 LUs of the
+// 32^3, p = 4 mesh's set-up and assembly; k_lu_back 0.31 against 0.26 ms): the LDS tile cut
+// occupancy.
 template <typename T, int BLU_NB>
-__global__ __launch_bounds__(64) void k_lu_u12(T* Aall, int n, int ncols, long stride, int k0,
-                                               const int* piv_ws) {
+__global__ __launch_bounds__(256) void k_lu_u12(T* Aall, int n, int ncols, long stride, int k0,
+                                                const int* piv_ws) {
   T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
+  // L11 with a run-time leading dimension: constant LDS addresses in the unrolled solve let the
+  // compiler hoist every load into registers (and spill)
   __shared__ T L11[BLU_NB * BLU_NB];
-  __shared__ T X[BLU_NB][65];
-  const int lane = threadIdx.x;
-  for (int idx = lane; idx < nb * nb; idx += 64) {
+  const int ldl = nb;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
     const int r = idx % nb, c = idx / nb;
-    L11[r + nb * c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
-  }
-  const int c0 = k0 + nb + blockIdx.y * 64;
-  const int ncol = min(64, ncols - c0);
-  for (int idx = lane; idx < nb * 64; idx += 64) {
-    const int r = idx % nb, c = idx / nb;
-    if (c < ncol) X[r][c] = A[k0 + r + static_cast<long>(n) * (c0 + c)];
+    L11[r + ldl * c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
   }
   __syncthreads();
-  if (lane < ncol) {
-    T x[BLU_NB];
+  const int j = k0 + nb + blockIdx.y * 256 + tid;
+  if (j >= ncols) return;
+  T* colj = A + static_cast<long>(n) * j + k0;
+  T x[BLU_NB];
 #pragma unroll
-    for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? X[kk][lane] : T(0);
+  for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
 #pragma unroll
-    for (int kk = 1; kk < BLU_NB; ++kk)
+  for (int kk = 1; kk < BLU_NB; ++kk)
 #pragma unroll
-      for (int t = 0; t < kk; ++t)
-        if (kk < nb) x[kk] = fma(-L11[kk + nb * t], x[t], x[kk]);
+    for (int t = 0; t < kk; ++t)
+      if (kk < nb) x[kk] = fma(-L11[kk + ldl * t], x[t], x[kk]);
 #pragma unroll
-    for (int kk = 0; kk < BLU_NB; ++kk)
-      if (kk < nb) X[kk][lane] = x[kk];
-  }
-  __syncthreads();
-  for (int idx = lane; idx < nb * 64; idx += 64) {
-    const int r = idx % nb, c = idx / nb;
-    if (c < ncol) A[k0 + r + static_cast<long>(n) * (c0 + c)] = X[r][c];
-  }
-  (void)piv_ws;
+  for (int kk = 0; kk < BLU_NB; ++kk)
+    if (kk < nb) colj[kk] = x[kk];
 }
 
 // Inverse of the unit lower triangle L11 (LOWER) or of the upper triangle U11 of the panel at
@@ -568,51 +561,39 @@ __global__ __launch_bounds__(64) void k_lu_tri_inv(const T* Aall, int n, long st
     if (r < nb) Wm[r + BLU_NB * c] = y[r];
 }
 
-// Back substitution of one diagonal block of U on the right-hand-side columns n..ncols-1,
-// X[k0 : k0+nb] = U11^{-1} X[k0 : k0+nb]: tiles of 64 columns staged in LDS with coalesced loads
-// and stores as in k_lu_u12, one column per lane.
+// X[k0:k0+nb, c] = U11^{-1} X[k0:k0+nb, c] for the right-hand-side columns c >= n
 template <typename T, int BLU_NB>
-__global__ __launch_bounds__(64) void k_lu_back(T* Aall, int n, int ncols, long stride, int k0) {
+__global__ __launch_bounds__(256) void k_lu_back(T* Aall, int n, int ncols, long stride, int k0) {
   T* A = Aall + stride * blockIdx.x;
   const int nb = min(BLU_NB, n - k0);
   __shared__ T U[BLU_NB][BLU_NB + 1];
   __shared__ T dinv[BLU_NB];
-  __shared__ T X[BLU_NB][65];
-  const int lane = threadIdx.x;
-  for (int idx = lane; idx < nb * nb; idx += 64) {
-    const int r = idx % nb, c = idx / nb;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    int r = idx % nb, c = idx / nb;
     U[r][c] = A[k0 + r + static_cast<long>(n) * (k0 + c)];
   }
-  const int c0 = n + blockIdx.y * 64;
-  const int ncol = min(64, ncols - c0);
-  for (int idx = lane; idx < nb * 64; idx += 64) {
-    const int r = idx % nb, c = idx / nb;
-    if (c < ncol) X[r][c] = A[k0 + r + static_cast<long>(n) * (c0 + c)];
-  }
   __syncthreads();
-  if (lane < nb) dinv[lane] = T(1) / U[lane][lane];
+  if (tid < nb) dinv[tid] = T(1) / U[tid][tid];
   __syncthreads();
-  if (lane < ncol) {
-    T x[BLU_NB];
+  // one right-hand-side column per thread (blockIdx.y selects a chunk of 256)
+  const int j = n + blockIdx.y * 256 + tid;
+  if (j >= ncols) return;
+  T* colj = A + static_cast<long>(n) * j + k0;
+  T x[BLU_NB];
 #pragma unroll
-    for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? X[kk][lane] : T(0);
+  for (int kk = 0; kk < BLU_NB; ++kk) x[kk] = (kk < nb) ? colj[kk] : 0.0;
 #pragma unroll
-    for (int kk = BLU_NB - 1; kk >= 0; --kk) {
-      if (kk < nb) {
-        x[kk] *= dinv[kk];
+  for (int kk = BLU_NB - 1; kk >= 0; --kk) {
+    if (kk < nb) {
+      x[kk] *= dinv[kk];
 #pragma unroll
-        for (int t = 0; t < kk; ++t) x[t] = fma(-U[t][kk], x[kk], x[t]);
-      }
+      for (int t = 0; t < kk; ++t) x[t] = fma(-U[t][kk], x[kk], x[t]);
     }
+  }
 #pragma unroll
-    for (int kk = 0; kk < BLU_NB; ++kk)
-      if (kk < nb) X[kk][lane] = x[kk];
-  }
-  __syncthreads();
-  for (int idx = lane; idx < nb * 64; idx += 64) {
-    const int r = idx % nb, c = idx / nb;
-    if (c < ncol) A[k0 + r + static_cast<long>(n) * (c0 + c)] = X[r][c];
-  }
+  for (int kk = 0; kk < BLU_NB; ++kk)
+    if (kk < nb) colj[kk] = x[kk];
 }
 
 // ---------------------------------------------------------------------------
@@ -1580,10 +1561,10 @@ __global__ void k_bj_assemble(Dims d, const int* f2e, const int* elemcon, const 
 // Elemental additive Schwarz (hdgElementalAdditiveSchwarz, matvec.cpp:168-188;
 // kokkosimpl.h:1113-1142): element e's m x m block of H with the diagonal block of
 // each interior face replaced by the sum of both sides' blocks, written as the
-// augmented [block | I] (m x 2m) for the batched LU inversion.
-template <typename TH>
+// augmented [block | I] (m x 2m) for the batched LU inversion (FP64, or FP32 in the mixed form).
+template <typename TH, typename TA>
 __global__ void k_eas_assemble(Dims d, int e0, int eb, const TH* H, const int* f2e, const int* elemcon,
-                               double* aug) {
+                               TA* aug) {
   const int m = d.m, ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf;
   const long per = static_cast<long>(m) * 2 * m;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1614,10 +1595,11 @@ __global__ void k_eas_assemble(Dims d, int e0, int eb, const TH* H, const int* f
       }
     }
   }
-  aug[static_cast<long>(le) * per + rs] = v;
+  aug[static_cast<long>(le) * per + rs] = static_cast<TA>(v);
 }
 
-__global__ void k_eas_extract(Dims d, int eb, const double* aug, double* BEe) {
+template <typename T>
+__global__ void k_eas_extract(Dims d, int eb, const T* aug, T* BEe) {
   const long per = static_cast<long>(d.m) * d.m;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (q >= per * eb) return;
@@ -1649,6 +1631,15 @@ __global__ void k_bj_apply(Dims d, const T* BE, const double* x, double* y, cons
   y[q] = s;
 }
 
+// Block entries as doubles: FP64 / FP32 storage, or BF16 kept as the upper 16 bits of an FP32
+// (unsigned short storage)
+template <typename T>
+__device__ __forceinline__ double blk_val(T x) { return static_cast<double>(x); }
+template <>
+__device__ __forceinline__ double blk_val<unsigned short>(unsigned short x) {
+  return static_cast<double>(__uint_as_float(static_cast<unsigned int>(x) << 16));
+}
+
 // hdgMatVec element part (matvec.cpp:377-380): we_e = H_e * v_e with v_e gathered
 // through elemcon.  One workgroup per element; each of the 4 waves owns a
 // quarter of the columns, lanes own VW consecutive rows (16-byte vector loads of
@@ -1663,7 +1654,7 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const i
                                                      const double* emask) {
   typedef T vecT __attribute__((ext_vector_type(VW)));
   constexpr int MAXRB = NRB > 0 ? NRB : (1024 + 64 * VW - 1) / (64 * VW);  // m <= 1024
-  constexpr int CB = NRB > 0 ? (sizeof(T) == 4 ? 4 : 2) : 1;
+  constexpr int CB = NRB > 0 ? (sizeof(T) <= 4 ? 4 : 2) : 1;
   const int m = d.m, ncu = d.ncu, ndf = d.ndf;
   const long e = blockIdx.x;
   if (emask && einst && emask[einst[e]] == 0.0) {  // block-uniform: the element's instance is inactive
@@ -1707,7 +1698,7 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const i
           const int r = VW * (lane + 64 * rb);
           if (r < m)
 #pragma unroll
-            for (int q = 0; q < VW; ++q) acc[rb][q] = fma(static_cast<double>(h[j][rb][q]), vc, acc[rb][q]);
+            for (int q = 0; q < VW; ++q) acc[rb][q] = fma(blk_val(h[j][rb][q]), vc, acc[rb][q]);
         }
       }
     }
@@ -1721,7 +1712,7 @@ __global__ __launch_bounds__(256) void k_matvec_elem(Dims d, const T* H, const i
       if (rb < nrb && r < m) {
         const vecT h = *reinterpret_cast<const vecT*>(col + r);
 #pragma unroll
-        for (int q = 0; q < VW; ++q) acc[rb][q] = fma(static_cast<double>(h[q]), vc, acc[rb][q]);
+        for (int q = 0; q < VW; ++q) acc[rb][q] = fma(blk_val(h[q]), vc, acc[rb][q]);
       }
     }
   }
@@ -1757,7 +1748,7 @@ __global__ __launch_bounds__(256) void k_matvec_small(Dims d, const T* H, const 
   const int* ec = elemcon + ndf * e;
   double acc = 0.0;
   for (int c = 0; c < m; ++c)
-    acc = fma(static_cast<double>(He[static_cast<long>(m) * c]), v[(c % ncu) + ncu * static_cast<long>(ec[c / ncu])],
+    acc = fma(blk_val(He[static_cast<long>(m) * c]), v[(c % ncu) + ncu * static_cast<long>(ec[c / ncu])],
               acc);
   we[q] = acc;
 }
@@ -1920,14 +1911,6 @@ __global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const T* Ltmp, 
   BE[(static_cast<long>(I.elim_ji[k0 + k]) * I.nse + n) * per + q % per] = Ltmp[q];
 }
 
-// Block entries as doubles: FP64 / FP32 storage, or BF16 kept as the upper 16 bits of an FP32
-// (unsigned short storage)
-template <typename T>
-__device__ __forceinline__ double blk_val(T x) { return static_cast<double>(x); }
-template <>
-__device__ __forceinline__ double blk_val<unsigned short>(unsigned short x) {
-  return static_cast<double>(__uint_as_float(static_cast<unsigned int>(x) << 16));
-}
 // loads a thread keeps in flight per step of a row's sweep (raw storage words in registers)
 template <typename T>
 constexpr int bilu_unroll() { return sizeof(T) == 2 ? 16 : sizeof(T) == 4 ? 12 : 8; }
@@ -2433,7 +2416,7 @@ void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* 
           k_lu_tri_inv<T, NB, true><<<batch, 64, 0, s>>>(A, n, stride, k0, tri_ws);
           apply_tri(nb, k0, k0 + nb, right);
         } else {
-          k_lu_u12<T, NB><<<dim3(batch, (right + 63) / 64), 64, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
+          k_lu_u12<T, NB><<<dim3(batch, (right + 255) / 256), 256, 0, s>>>(A, n, ncols, stride, k0, piv_ws);
         }
       }
     } else {
@@ -2460,8 +2443,8 @@ void blocked_lu(T* A, int n, int ncols, long stride, int batch, int* info, int* 
       EXA_HIP(hipGetLastError());
       apply_tri(std::min(NB, n - k0), k0, n, ncols - n);
     } else {
-      dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 63) / 64));
-      k_lu_back<T, NB><<<grid, 64, 0, s>>>(A, n, ncols, stride, k0);
+      dim3 grid(static_cast<unsigned>(batch), static_cast<unsigned>((ncols - n + 255) / 256));
+      k_lu_back<T, NB><<<grid, 256, 0, s>>>(A, n, ncols, stride, k0);
       EXA_HIP(hipGetLastError());
     }
     if (k0 > 0) {  // X[0:k0] -= U[0:k0, k0:k0+nb] * X[k0:k0+nb]
@@ -2750,11 +2733,11 @@ void bj_assemble(const Dims& d, const int* f2e, const int* elemcon, const float*
   bj_assemble_t(d, f2e, elemcon, H, aug, s);
 }
 
-template <typename TH>
-void eas_assemble_t(const Dims& d, int e0, int eb, const TH* H, const int* f2e, const int* elemcon, double* aug,
+template <typename TH, typename TA>
+void eas_assemble_t(const Dims& d, int e0, int eb, const TH* H, const int* f2e, const int* elemcon, TA* aug,
                     hipStream_t s) {
   long n1 = static_cast<long>(d.m) * 2 * d.m * eb;
-  k_eas_assemble<TH><<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, H, f2e, elemcon, aug);
+  k_eas_assemble<TH, TA><<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, H, f2e, elemcon, aug);
   EXA_HIP(hipGetLastError());
 }
 void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e, const int* elemcon,
@@ -2765,11 +2748,22 @@ void eas_assemble(const Dims& d, int e0, int eb, const float* H, const int* f2e,
                   double* aug, hipStream_t s) {
   eas_assemble_t(d, e0, eb, H, f2e, elemcon, aug, s);
 }
+void eas_assemble(const Dims& d, int e0, int eb, const float* H, const int* f2e, const int* elemcon,
+                  float* aug, hipStream_t s) {
+  eas_assemble_t(d, e0, eb, H, f2e, elemcon, aug, s);
+}
 
-void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s) {
+template <typename T>
+void eas_extract_t(const Dims& d, int eb, const T* aug, T* BEe, hipStream_t s) {
   long n1 = static_cast<long>(d.m) * d.m * eb;
-  k_eas_extract<<<nblocks(n1, 256), 256, 0, s>>>(d, eb, aug, BEe);
+  k_eas_extract<T><<<nblocks(n1, 256), 256, 0, s>>>(d, eb, aug, BEe);
   EXA_HIP(hipGetLastError());
+}
+void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s) {
+  eas_extract_t(d, eb, aug, BEe, s);
+}
+void eas_extract(const Dims& d, int eb, const float* aug, float* BEe, hipStream_t s) {
+  eas_extract_t(d, eb, aug, BEe, s);
 }
 
 void bj_extract(const Dims& d, const double* aug, double* BE, hipStream_t s) {
@@ -2796,12 +2790,14 @@ void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v,
     return;
   }
   size_t shm = static_cast<size_t>(5) * d.m * sizeof(double);
-  constexpr int VMAX = 16 / sizeof(T);  // 16-byte loads: 2 doubles or 4 floats per lane
+  constexpr int VMAX = 16 / sizeof(T);  // 16-byte loads: 2 doubles, 4 floats or 8 BF16 per lane
   if (d.m % VMAX == 0) {
     const int nrb = (d.m + 64 * VMAX - 1) / (64 * VMAX);
     if (nrb == 2) k_matvec_elem<T, VMAX, 2><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
     else if (nrb == 3) k_matvec_elem<T, VMAX, 3><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
     else k_matvec_elem<T, VMAX, 0><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
+  } else if (sizeof(T) == 2 && d.m % 4 == 0) {  // BF16 blocks: 8-byte loads
+    k_matvec_elem<T, 4, 0><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
   } else if (d.m % 2 == 0) {
     k_matvec_elem<T, 2, 0><<<d.ne, 256, shm, s>>>(d, H, elemcon, v, we, einst, emask);
   } else {
@@ -3468,6 +3464,8 @@ template void matvec_elem<double>(const Dims&, const double*, const int*, const 
                                   const int*, const double*);
 template void matvec_elem<float>(const Dims&, const float*, const int*, const double*, double*, hipStream_t,
                                  const int*, const double*);
+template void matvec_elem<unsigned short>(const Dims&, const unsigned short*, const int*, const double*, double*,
+                                          hipStream_t, const int*, const double*);
 template void dudg<double>(const Dims&, const double*, const double*, const int*, const double*, double*,
                            hipStream_t);
 template void dudg<float>(const Dims&, const double*, const float*, const int*, const double*, double*,

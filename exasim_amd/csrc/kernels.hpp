@@ -151,7 +151,10 @@ void eas_assemble(const Dims& d, int e0, int eb, const double* H, const int* f2e
                   double* aug, hipStream_t s);
 void eas_assemble(const Dims& d, int e0, int eb, const float* H, const int* f2e, const int* elemcon,
                   double* aug, hipStream_t s);
+void eas_assemble(const Dims& d, int e0, int eb, const float* H, const int* f2e, const int* elemcon,
+                  float* aug, hipStream_t s);
 void eas_extract(const Dims& d, int eb, const double* aug, double* BEe, hipStream_t s);
+void eas_extract(const Dims& d, int eb, const float* aug, float* BEe, hipStream_t s);
 template <typename T>
 void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v, double* we, hipStream_t s,
                  const int* einst = nullptr, const double* emask = nullptr);

@@ -432,7 +432,11 @@ void Solver::alloc_mixed() {
     q32_ready = true;
   }
   if (precond == 0) BE32.alloc(ncf2 * d.nf);
-  if (precond == 1) BEeas32.alloc(m * m * ne);
+  if (precond == 1) {  // the mixed form inverts the Schwarz blocks in FP32 (applied in FP32 or BF16)
+    BEeas32.alloc(m * m * ne);
+    easaug32.alloc(static_cast<size_t>(std::max(eas_chunk, 1)) * 2 * m * m);
+    if (precond_bf16) BEeas16.alloc(m * m * ne);
+  }
   if (precond == 2) {  // FP32 factorisation (and its factors), BF16 copy of the factors
     BEilu32.alloc(ncf2 * ilu.nnz * ilu.nse);
     Lilu32.alloc(Lilu.n);
@@ -575,7 +579,7 @@ void Solver::set_option(const std::string& name, double v) {
   } else if (name == "refine_rtol") refine_rtol = v;
   else if (name == "refine_tol") refine_tol = v;
   else if (name == "refine_first_tol") refine_first_tol = v;
-  else if (name == "precond_bf16") {  // mixed form: block-ILU(0) factors stored in BF16
+  else if (name == "precond_bf16") {  // mixed form: block-ILU(0) factors or Schwarz blocks stored in BF16
     const int pv = v != 0.0;
     if (pv != precond_bf16) {
       precond_bf16 = pv;
@@ -584,6 +588,10 @@ void Solver::set_option(const std::string& name, double v) {
     }
   }
   else if (name == "refine_maxit") refine_maxit = static_cast<int>(v);
+  else if (name == "refine_cycles") {
+    if (v < 0) throw Error(1, "refine_cycles must be >= 0");
+    refine_cycles = static_cast<int>(v);
+  }
   else if (name == "timing") timing = v != 0.0;
   else if (name == "workspace_bytes") {
     ws_budget_bytes = v;
@@ -870,16 +878,24 @@ void Solver::assemble() {
   face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
   tic(T_LU, &t0);
   if (precond == 1) {
-    // hdgElementalAdditiveSchwarz (matvec.cpp:168-188), inverted in element chunks
+    // hdgElementalAdditiveSchwarz (matvec.cpp:168-188), inverted in element chunks; the mixed
+    // form assembles, inverts and stores the blocks in FP32 (its apply reads FP32 blocks)
+    const long mm = static_cast<long>(d.m) * d.m;
     for (int e0 = 0; e0 < d.ne; e0 += eas_chunk) {
       const int eb = std::min(eas_chunk, d.ne - e0);
-      if (precision) eas_assemble(d, e0, eb, H32.p, f2e.p, elemcon.p, easaug.p, s);
-      else eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
-      lu_solve(easaug.p, d.m, 2 * d.m, 2L * d.m * d.m, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
+      if (precision) {
+        eas_assemble(d, e0, eb, H32.p, f2e.p, elemcon.p, easaug32.p, s);
+        lu_solve(easaug32.p, d.m, 2 * d.m, 2 * mm, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
+      } else {
+        eas_assemble(d, e0, eb, H.p, f2e.p, elemcon.p, easaug.p, s);
+        lu_solve(easaug.p, d.m, 2 * d.m, 2 * mm, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
+      }
       const int bad = first_nonzero(info.p, eb);
       if (bad >= 0) throw Error(4, "singular Schwarz block at element " + std::to_string(e0 + bad));
-      eas_extract(d, eb, easaug.p, BEeas.p + static_cast<long>(e0) * d.m * d.m, s);
+      if (precision) eas_extract(d, eb, easaug32.p, BEeas32.p + e0 * mm, s);
+      else eas_extract(d, eb, easaug.p, BEeas.p + e0 * mm, s);
     }
+    if (precision && precond_bf16) to_bf16(BEeas16.p, BEeas32.p, mm * d.ne, s);
   } else if (precond == 0) {
     // hdgBlockJacobi (matvec.cpp:190-217)
     if (precision) bj_assemble(d, f2e.p, elemcon.p, H32.p, bjaug.p, s);
@@ -901,10 +917,6 @@ void Solver::assemble() {
   if (precision) {
     const long c2 = static_cast<long>(d.ncf) * d.ncf;
     if (precond == 0) to_float(BE32.p, BE.p, d.ncf, d.ncf, d.ncf, c2, c2, d.nf, s);
-    if (precond == 1) {
-      const long mm = static_cast<long>(d.m) * d.m;
-      to_float(BEeas32.p, BEeas.p, d.m, d.m, d.m, mm, mm, d.ne, s);
-    }
     if (precond == 2) {
       if (precond_bf16) to_bf16(BEilu16.p, BEilu32.p, c2 * ilu.nnz * ilu.nse, s);
     }
@@ -964,7 +976,8 @@ void Solver::apply_precond_to(const double* in, double* out) {
     else if (precision) bilu_apply(d, ilud, BEilu32.p, in, Yilu.p, Xilu.p, out, s, gm_act);
     else bilu_apply(d, ilud, BEilu.p, in, Yilu.p, Xilu.p, out, s, gm_act);
   } else if (precond == 1) {  // x = sum_e scatter(BE_e^{-1} gather_e(x)), as hdgMatVec with BE
-    if (precision) matvec_elem(d, BEeas32.p, elemcon.p, in, we.p, s, einst.p, elem_active);
+    if (precision && precond_bf16) matvec_elem(d, BEeas16.p, elemcon.p, in, we.p, s, einst.p, elem_active);
+    else if (precision) matvec_elem(d, BEeas32.p, elemcon.p, in, we.p, s, einst.p, elem_active);
     else matvec_elem(d, BEeas.p, elemcon.p, in, we.p, s, einst.p, elem_active);
     face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, we.p, out, s);
   } else {
@@ -1095,7 +1108,8 @@ void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
     assemble();
     // mixed precision: the FP32 operator limits the first solve to about 1e-7, so it is
     // solved to refine_first_tol and the refinement loop below supplies the rest
-    int gi = precision ? gmres(b.p, x.p, std::max(gmres_tol, refine_first_tol)) : gmres(b.p, x.p);
+    int gi = precision ? gmres(b.p, x.p, std::max(gmres_tol, refine_first_tol), refine_maxit_gmres())
+                       : gmres(b.p, x.p);
     gtot += gi;
     double alpha = 1.0;
     axpby(u.p, x.p, 1.0, 1.0, N, s);  // u += x
@@ -1131,7 +1145,7 @@ void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
     while (precision && nrmr > refine_rtol * nrm0 && last_refinements < refine_maxit) {
       refine_rhs(d, LU32.p, piv32.p, K32.p, RuR.p, RuS.p, Rh.p, s);
       face_gather(static_cast<long>(d.npf) * d.nf, d.ncu, src1.p, src2.p, Rh.p, b.p, s);
-      gi = gmres(b.p, x.p, refine_tol);
+      gi = gmres(b.p, x.p, refine_tol, refine_maxit_gmres());
       gtot += gi;
       axpby(u.p, x.p, 1.0, 1.0, N, s);
       copy(uh.p, u.p, N, s);
@@ -1207,7 +1221,13 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
     host.assign(f.begin(), f.end());
     size = static_cast<long>(host.size());
   } else if (name == "BEilu") { src = BEilu.p; size = static_cast<long>(BEilu.n); }
-  else if (name == "BEeas") { src = BEeas.p; size = static_cast<long>(BEeas.n); }
+  else if (name == "BEeas" && dinvf_fp32) {  // the mixed form inverts its Schwarz blocks in FP32
+    std::vector<float> f(BEeas32.n);
+    EXA_HIP(hipMemcpyAsync(f.data(), BEeas32.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+    host.assign(f.begin(), f.end());
+    size = static_cast<long>(host.size());
+  } else if (name == "BEeas") { src = BEeas.p; size = static_cast<long>(BEeas.n); }
   else if (name == "ilu_face") from_int(ilu.face);
   else if (name == "ilu_row_ptr") from_int(ilu.row_ptr);
   else if (name == "ilu_col_ind") from_int(ilu.col_ind);

@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <array>
 #include <string>
 #include <vector>
@@ -48,6 +49,10 @@ struct Solver {
   // loop to refine_rtol * (first residual norm), corrections solved to refine_tol
   int precision = 0, refine_maxit = 5;
   double refine_rtol = 1e-10, refine_tol = 1e-4, refine_first_tol = 1e-7;
+  // option refine_cycles (mixed form): each GMRES solve of the Newton step and of its refinement
+  // corrections stops after at most this many restart cycles (0: no cap), so that a correction
+  // starts from the FP64 residual where plain restarted GMRES would restart
+  int refine_cycles = 0;
   int last_refinements = 0;
   // time dependence (DIRK): steps app.dt, stage tables c, d (= A^{-1}, column-major), t
   int tdep = 0, torder = 1, tstages = 1, tscheme = 0, tdfunc = 0;
@@ -71,16 +76,17 @@ struct Solver {
   int neb = 0;
   DevBuf<double> aug, Bfull, Kt, Gfull, rv, rJ, W, fpt;
   // mixed precision: FP32 copies of H, DinvF, the LU factors of D', K' and the preconditioner blocks
-  DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32;
+  DevBuf<float> H32, DinvF32, LU32, K32, BE32, BEilu32, BEeas32, easaug32;
   DevBuf<float> aug32, Bfull32, Gfull32, W32;  // FP32 element-block workspace (mixed form)
   DevBuf<float> MCs32, MEs32, shapegt32;       // FP32 copies of M^-1 C, M^-1 E and the shape table
   bool q32_ready = false;
   bool dinvf_fp32 = false;  // the last assembly kept D^{-1}F in FP32 only (DinvF32)
   bool ilu_fp32 = false;    // the last assembly kept the block-ILU(0) factors in FP32 only (BEilu32)
   DevBuf<float> Lilu32;
-  // option precond_bf16 (mixed form, block-ILU(0)): the factors in BF16 (upper half of FP32)
+  // option precond_bf16 (mixed form, block-ILU(0) or Schwarz blocks): the preconditioner's blocks
+  // in BF16 (upper half of FP32)
   int precond_bf16 = 0;
-  DevBuf<unsigned short> BEilu16;
+  DevBuf<unsigned short> BEilu16, BEeas16;
   DevBuf<int> piv32;  // row exchanges of the blocked LU of D' ([n, ne], panel-relative)
   DevBuf<int> lupiv;     // this handle's pivots of the current LU panel (LU_PIV_WS per matrix of a batched LU)
   DevBuf<double> lutri;  // this handle's triangle inverses of the current LU panel (LU_TRI_WS per matrix)
@@ -171,7 +177,7 @@ struct Solver {
   double* host_region(int which);
   void seg_norms(const SegDev& sd, const double* v, double* out_dev);
   std::vector<double> residual_seg();
-  int gmres_seg(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters);
+  int gmres_seg(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters, int maxit = -1);
   void newton_seg(int* newton_iters, int* gmres_iters, double* final_norm);
   long get_inst_array(int inst, const std::string& name, double* dst, long len);
   void restore_instance(int inst, double* newton_iterate);
@@ -186,9 +192,14 @@ struct Solver {
   void matvec(const double* v, double* w);
   void apply_precond(double* x);
   void apply_precond_to(const double* in, double* out);
-  int gmres(const double* bvec, double* xvec, double tol_rel = -1.0);
+  int gmres(const double* bvec, double* xvec, double tol_rel = -1.0, int maxit = -1);
   // every instance's GMRES at once; iters[i] = instance i's count; returns the largest
-  int gmres_run(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters);
+  // maxit < 0: gmres_maxit
+  int gmres_run(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters, int maxit = -1);
+  // iteration cap of the mixed form's solves (refine_cycles restart cycles, or gmres_maxit)
+  int refine_maxit_gmres() const {
+    return refine_cycles > 0 ? std::min(gmres_maxit, refine_cycles * restart) : gmres_maxit;
+  }
   long augcols() const { return static_cast<long>(d.n) + d.m + 1; }
   void alloc_mixed();
   void solve(int* newton_iters, int* gmres_iters, double* final_norm);

@@ -78,7 +78,10 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * with FP64 residuals until ||R|| <= "refine_rtol" (1e-10) times its first residual,
  * at most "refine_maxit" (5) corrections, each solved by GMRES to "refine_tol" (1e-4);
  * the first solve of each Newton step goes to max(gmres_tol, "refine_first_tol" (1e-7)));
- * "precond_bf16" (0/1): in the mixed form, keep the block-ILU(0) factors in BF16;
+ * "refine_cycles" (default 0 = no cap): in the mixed form, each of those GMRES solves stops after
+ * at most this many restart cycles, so the next correction restarts from the FP64 residual;
+ * "precond_bf16" (0/1): in the mixed form, keep the block-ILU(0) factors or the inverted Schwarz
+ * blocks in BF16;
  * "gmres_lag" (0..30, default 3): GMRES steps queued ahead of the host's convergence check (the
  * Krylov scalars stay on the device; steps queued after convergence do no work);
  * "gmres_graph" (0/1, default 0): replay the Newton loop's GMRES steps from captured HIP graphs;

@@ -153,6 +153,26 @@ def test_batch_cgs_matches_oracle(cd_batch_cgs):
         assert rel(mix[i][0], ref) < 1e-8, i
 
 
+def test_batch_mixed_refine_cycles_matches_oracle(cd_batch_cgs):
+    """Mixed form with every GMRES solve capped at one restart cycle (refine_cycles = 1, GMRES(5)):
+    the refinement corrections carry the convergence, per instance, to the FP64 oracle's solution
+    within 1e-8."""
+    say("cd2d batch, mixed, refine_cycles 1 with GMRES(5): every instance vs the oracle (1e-8)")
+    from exasim_amd.hdg import HDGSolver
+    with HDGSolver.batch(cd_batch_cgs, "cd2d") as S:
+        for k, v in dict(precision=1, refine_cycles=1, refine_maxit=200, refine_first_tol=1e-7,
+                         refine_tol=1e-7).items():
+            S.set_option(k, v)
+        S.solve()
+        a, g, nrm = S.instance_stats()
+        mix = [S.get_instance(i, "uh") for i in range(len(cd_batch_cgs))]
+    for i, p in enumerate(cd_batch_cgs):
+        P = ora.HDGProblem(p, "cd2d")
+        ora.newton_solve(P)
+        say("  instance %d: gmres %d, residual %.2e" % (i, g[i], nrm[i]))
+        assert rel(mix[i], P.uh.reshape(-1)) < 1e-8, i
+
+
 def test_batch3d_convdiff_matches_oracle(tmp_path):
     """A batch of 3D convection-diffusion instances (the 3D batch workload's settings: Schwarz blocks,
     classical Gram-Schmidt, two components, per-instance velocity, diffusivity and source, Dirichlet

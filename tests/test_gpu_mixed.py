@@ -20,6 +20,17 @@ CASES = {
     "cd_ilu": ("convdiff3d", dict(n=4, porder=2, ncu=2, restart=40, gmres_tol=1e-12, precond="bilu0",
                                   cartgridpart=[3, 4, 4, 4, 2, 2, 2, 0]), "convdiff"),
     "p2_eas": ("poisson2d", dict(n=8, porder=3, pgauss=8, precond=1), "poisson"),
+    # Schwarz blocks inverted in FP32 and applied in BF16 (option precond_bf16): small blocks
+    # (one thread per row), m = 192 (16-byte loads) and m = 300 (8-byte loads)
+    "p2_eas_bf16": ("poisson2d", dict(n=8, porder=3, pgauss=8, precond=1), "poisson", {"precond_bf16": 1}),
+    "cd3_eas_bf16": ("convdiff3d", dict(n=3, porder=3, ncu=2, restart=100, gmres_tol=1e-12, precond="eas",
+                                        inflow_dirichlet=True), "convdiff", {"precond_bf16": 1}),
+    "cd4_eas_bf16": ("convdiff3d", dict(n=2, porder=4, ncu=2, restart=100, gmres_tol=1e-12, precond="eas"),
+                     "convdiff", {"precond_bf16": 1}),
+    # each solve capped at one restart cycle of 10: the refinement loop carries the convergence
+    "cd_ilu_cycles": ("convdiff3d", dict(n=4, porder=2, ncu=2, restart=10, gmres_tol=1e-12, precond="bilu0",
+                                         cartgridpart=[3, 4, 4, 4, 2, 2, 2, 0]), "convdiff",
+                      {"refine_cycles": 1, "refine_maxit": 60, "refine_first_tol": 1e-7, "refine_tol": 1e-7}),
 }
 
 
@@ -38,7 +49,7 @@ def built(case_dir):
 
     def get(name):
         if name not in out:
-            builder, kw, model = CASES[name]
+            builder, kw, model = CASES[name][:3]
             path, info = case_dir("mixed_" + name, builder, **kw)
             out[name] = (path, model)
         return out[name]
@@ -55,6 +66,8 @@ def test_mixed_solve_matches_fp64_oracle(built, name):
     ora.newton_solve(P)
     with HDGSolver(path, model) as S:
         S.set_option("precision", 1)
+        for k, v in (CASES[name][3] if len(CASES[name]) > 3 else {}).items():
+            S.set_option(k, v)
         S.get_q()
         nb, nru = S.residual()
         r0 = nb + nru

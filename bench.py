@@ -245,7 +245,7 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
                 mixed_refinement=BATCH_REFINE[kind])
 
 
-def run_serve(seconds, rate, check_alone=4):
+def run_serve(seconds, rate, check_alone=6):
     """The served workload: a Poisson stream of Navier-Stokes (70 %) and convection-diffusion
     solve requests (n x n quads, n in {32, 64, 128}, p in {2..5}; see cases.serve_stream)
     replayed in real time into the continuous-batching server (HBM budget 200 GB, mixed
@@ -264,7 +264,7 @@ def run_serve(seconds, rate, check_alone=4):
     summ = srv.run_stream(reqs)
     print("# serve: %s" % json.dumps({k: summ[k] for k in ("completed", "solves_per_s", "p50_ms", "p99_ms")}),
           file=sys.stderr, flush=True)
-    same, err64 = [], 0.0
+    same, err64, by_model = [], 0.0, {}
     for rid, path, model, t in reqs[:check_alone]:
         if rid not in srv.results:
             continue
@@ -276,12 +276,14 @@ def run_serve(seconds, rate, check_alone=4):
             S.set_option("precision", 0)
             S.solve()
             u64 = S.get_instance(0, "uh")
-        err64 = max(err64, float(np.abs(srv.results[rid] - u64).max() / np.abs(u64).max()))
+        e = float(np.abs(srv.results[rid] - u64).max() / np.abs(u64).max())
+        err64 = max(err64, e)
+        by_model[model] = max(by_model.get(model, 0.0), e)
     kinds = {}
     for rid, path, model, t in reqs:
         kinds[model] = kinds.get(model, 0) + 1
     summ.update(stream_seconds=seconds, rate_per_s=rate, mix=kinds, alone_equal_sample=same,
-                rel_err_vs_fp64_sample=err64 if same else None)
+                rel_err_vs_fp64_sample=err64 if same else None, rel_err_vs_fp64_by_model=by_model)
     return summ
 
 

@@ -58,7 +58,7 @@ def build(verbose=False, jobs=None):
     if failed:
         raise RuntimeError("hipcc failed for " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-ldl"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -72,7 +72,35 @@ def build(verbose=False, jobs=None):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, EXE)
+    build_plugins(verbose)
     return LIB
+
+
+PLUGINS = os.path.join(HERE, "plugins")
+
+
+def build_plugins(verbose=False):
+    """Build each sample model library under plugins/<name>/ (model functions in text2code's GPU
+    form, compiled against include/CudaHip.hpp) into plugins/lib<name>.so."""
+    out = []
+    for name in sorted(os.listdir(PLUGINS)):
+        src = os.path.join(PLUGINS, name)
+        if not os.path.isdir(src):
+            continue
+        cpps = sorted(os.path.join(src, f) for f in os.listdir(src) if f.endswith(".cpp"))
+        so = os.path.join(PLUGINS, "lib%s.so" % name)
+        cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-I" + os.path.join(ROOT, "include"), "-I" + src] + cpps + ["-o", so + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            print(r.stdout.decode(errors="replace"), file=sys.stderr)
+            raise RuntimeError("hipcc failed for the model library " + name)
+        os.replace(so + ".tmp", so)
+        out.append(so)
+    return out
 
 
 if __name__ == "__main__":

@@ -338,7 +338,8 @@ def serve_stream(root, seconds=60.0, rate=50.0, frac_ns=0.7, sizes=(32, 64, 128)
     Navier-Stokes solve (model ``ns2d``: n x n quads of the unit square, wall at y = 0 and far
     field elsewhere, free-stream start, Mach uniform in [0.2, 0.6], Reynolds number
     log-uniform in [100, 5000], ``ns_steps`` DIRK(2,2) steps of ``ns_dt``, stabilization tau =
-    1 + 1/M, the free-stream wave speed |u| + c), and otherwise a
+    1 + 1/M, the free-stream wave speed |u| + c; GMRES and Newton tolerances 1e-10, which keep
+    the mixed form within 1e-8 of the FP64 form), and otherwise a
     steady convection-diffusion solve (model ``cd2d``, kappa log-uniform in [1e-3, 1], velocity
     angle uniform, source sin(k pi x) cos(k pi y), k uniform in 1..4); n uniform over ``sizes``,
     p uniform over ``orders``.  Mesh, master element and node coordinates are written once per
@@ -364,7 +365,7 @@ def serve_stream(root, seconds=60.0, rate=50.0, frac_ns=0.7, sizes=(32, 64, 128)
             if ns:
                 navierstokes2d(tdir, nx=n, ny=n, porder=porder, length=(1.0, 1.0), stretch=1.0, nbumps=0,
                                nsteps=ns_steps, torder=2, nstage=2, dt=ns_dt, precond="eas", restart=50,
-                               gmres_tol=1e-8, newton_tol=1e-8, ortho=1, cartgridpart=False)
+                               gmres_tol=1e-10, newton_tol=1e-10, ortho=1, cartgridpart=False)
             else:
                 convdiff2d_param(tdir, n=n, porder=porder, precond="eas", restart=100, gmres_tol=1e-10,
                                  newton_tol=1e-8, ortho=1)
@@ -382,7 +383,7 @@ def serve_stream(root, seconds=60.0, rate=50.0, frac_ns=0.7, sizes=(32, 64, 128)
                          flag={0: 1, 5: 1, 6: PRECOND["eas"]},
                          problem={2: 0, 3: 2, 4: 2, 13: 8, 14: 600, 15: 50},
                          physicsparam=[gam, re, pr, mach] + list(uinf) + [1.0], tau=[1.0 + 1.0 / mach],
-                         dt=[ns_dt] * ns_steps, solversparam=[1e-8, 1e-8, 1e-3, 0.0])
+                         dt=[ns_dt] * ns_steps, solversparam=[1e-10, 1e-10, 1e-3, 0.0])
             udg = np.broadcast_to(uinf, xdg.shape[:2] + (4,))
             dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=4, nc=12, udg=np.ascontiguousarray(udg))
         else:

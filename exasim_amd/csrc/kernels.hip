@@ -1058,6 +1058,29 @@ struct has_tdfunc : std::false_type {};
 template <class M>
 struct has_tdfunc<M, std::void_t<decltype(&M::tdfunc)>> : std::true_type {};
 
+// Model-library models (MODEL_PLUGIN): the library's functions have run over every point of the
+// block before the point kernels, into a.pbuf (plugin_points); the point kernels read their values
+// and Jacobians from there.  Staging buffer layout, P points of the block:
+//   volume: inputs x [ND], uq [NC], v [nco]; outputs f [NCU ND], f_uq [NCU ND NC], s [NCU], s_uq [NCU NC]
+//   faces:  inputs x [ND], uq [NC], uq with u -> uhat [NC], uhat [NCU], n [ND], v [nco];
+//           outputs f(uhat, q) [NCU ND], its f_uq [NCU ND NC], then per boundary condition
+//           ib = 1..nbc: fb [NCU], fb_uq [NCU NC], fb_uh [NCU NCU]
+// each entry k of a group at (group offset + k) * P + p.
+template <int ND_, int NCU_>
+struct PluginModel {
+  static constexpr int ND = ND_, NCU = NCU_, NC = NCU_ * (ND_ + 1);
+  static constexpr bool SOA = true;
+  __host__ __device__ static long vol_in(int nco) { return ND + NC + nco; }
+  __host__ __device__ static long vol_out() { return NCU * ND + NCU * ND * NC + NCU + NCU * NC; }
+  __host__ __device__ static long face_in(int nco) { return 2 * ND + 2 * NC + NCU + nco; }
+  __host__ __device__ static long face_bc() { return NCU + NCU * NC + NCU * NCU; }
+  __host__ __device__ static long face_out(int nbc) { return NCU * ND + NCU * ND * NC + nbc * face_bc(); }
+};
+template <class M, class = void>
+struct is_soa : std::false_type {};
+template <class M>
+struct is_soa<M, std::void_t<decltype(M::SOA)>> : std::true_type {};
+
 // physicsparam and tau of the instance element E belongs to (one instance: a.einst is null)
 __device__ __forceinline__ const double* inst_param(const AsmArgs& a, long E) {
   return a.einst ? a.param + static_cast<long>(a.einst[E]) * a.nparam : a.param;
@@ -1099,8 +1122,25 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
   double f[NCU * ND], s[NCU];
   double f_uq[JAC ? NCU * ND * NC : 1], s_uq[JAC ? NCU * NC : 1];
   const double* prm = inst_param(a, E);
-  M::template flux<JAC>(x, uq, v, prm, a.time, f, f_uq);
-  M::template source<JAC>(x, uq, v, prm, a.time, s, s_uq);
+  if constexpr (is_soa<M>::value) {  // the model library's values at this point
+    const double* o = a.pbuf + M::vol_in(d.nco) * P + p;
+#pragma unroll
+    for (int i = 0; i < NCU * ND; ++i) f[i] = o[i * P];
+    o += NCU * ND * P;
+    if (JAC)
+#pragma unroll
+      for (int i = 0; i < NCU * ND * NC; ++i) f_uq[i] = o[i * P];
+    o += NCU * ND * NC * P;
+#pragma unroll
+    for (int i = 0; i < NCU; ++i) s[i] = o[i * P];
+    o += NCU * P;
+    if (JAC)
+#pragma unroll
+      for (int i = 0; i < NCU * NC; ++i) s_uq[i] = o[i * P];
+  } else {
+    M::template flux<JAC>(x, uq, v, prm, a.time, f, f_uq);
+    M::template source<JAC>(x, uq, v, prm, a.time, s, s_uq);
+  }
   if (a.tdep) {  // uequation.cpp:126-151: s += (sdg - dtfactor*u) T, s_u -= dtfactor T
     double T[NCU];
 #pragma unroll
@@ -1256,7 +1296,16 @@ __global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
     uq[c] = uh[c];
   }
   double f[NCU * ND], f_uq[JAC ? NCU * ND * NC : 1];
-  M::template flux<JAC>(x, uq, v, inst_param(a, E), a.time, f, f_uq);
+  if constexpr (is_soa<M>::value) {  // the model library's f(uhat, q) at this point
+    const double* o = a.pbuf + M::face_in(d.nco) * P + p;
+#pragma unroll
+    for (int i = 0; i < NCU * ND; ++i) f[i] = o[i * P];
+    if (JAC)
+#pragma unroll
+      for (int i = 0; i < NCU * ND * NC; ++i) f_uq[i] = o[(NCU * ND + i) * P];
+  } else {
+    M::template flux<JAC>(x, uq, v, inst_param(a, E), a.time, f, f_uq);
+  }
 #pragma unroll
   for (int c = 0; c < NCU; ++c) uq[c] = usave[c];
 #pragma unroll
@@ -1312,7 +1361,21 @@ __global__ __launch_bounds__(64) void k_face_bou_points(Dims d, AsmArgs a) {
   face_state<M>(d, a, E, l, g, uh, uq, x, nl, v, jac);
   const int ib = a.bf[l + d.nfe * E];
   double fb[NCU], fb_uq[JAC ? NCU * NC : 1], fb_uh[JAC ? NCU * NCU : 1];
-  M::template fbou<JAC>(ib, x, uq, v, uh, nl, inst_tau(a, E), inst_param(a, E), a.time, fb, fb_uq, fb_uh);
+  if constexpr (is_soa<M>::value) {  // the model library's condition ib at this point
+    const int nbc = a.plugin_nbc;
+    const int b = (ib >= 1 && ib <= nbc) ? ib - 1 : 0;
+    const double* o = a.pbuf + (M::face_in(d.nco) + NCU * ND + NCU * ND * NC + b * M::face_bc()) * P + p;
+#pragma unroll
+    for (int i = 0; i < NCU; ++i) fb[i] = o[i * P];
+    if (JAC) {
+#pragma unroll
+      for (int i = 0; i < NCU * NC; ++i) fb_uq[i] = o[(NCU + i) * P];
+#pragma unroll
+      for (int i = 0; i < NCU * NCU; ++i) fb_uh[i] = o[(NCU + NCU * NC + i) * P];
+    }
+  } else {
+    M::template fbou<JAC>(ib, x, uq, v, uh, nl, inst_tau(a, E), inst_param(a, E), a.time, fb, fb_uq, fb_uh);
+  }
 #pragma unroll
   for (int k = 0; k < NCU; ++k) a.fpt[(NF1 + k) * P + p] = fb[k] * jac;
   if (JAC) {
@@ -2307,9 +2370,134 @@ void dispatch_points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStr
   throw Error(5, "model instantiated for nd in {2,3}, ncu in {1,2} only");
 }
 
+// inputs of a model library at the volume points of the block (the interpolation of k_vol_points)
+template <class M>
+__global__ __launch_bounds__(256) void k_plug_vol_in(Dims d, AsmArgs a) {
+  constexpr int ND = M::ND, NC = M::NC;
+  const long P = static_cast<long>(a.eb) * d.nge;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const long E = a.e0 + p / d.nge;
+  const int g = static_cast<int>(p % d.nge);
+  const long PG = static_cast<long>(d.ne) * d.nge;
+  const long pg = E * d.nge + g;
+  double uq[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) uq[c] = 0.0;
+  for (int i = 0; i < d.npe; ++i) {
+    const double phi = a.shapegt[g + d.nge * i];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) uq[c] += phi * a.udg[i + d.npe * (c + NC * E)];
+  }
+  double* in = a.pbuf;
+#pragma unroll
+  for (int b = 0; b < ND; ++b) in[b * P + p] = a.egeo[b * PG + pg];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) in[(ND + c) * P + p] = uq[c];
+  for (int c = 0; c < d.nco; ++c) in[(ND + NC + c) * P + p] = a.og[c * PG + pg];
+}
+
+// inputs of a model library at the face points of the block (face_state): x, uq, uq with u
+// replaced by uhat (the interior flux is f(uhat, q)), uhat, n, v
+template <class M>
+__global__ __launch_bounds__(256) void k_plug_face_in(Dims d, AsmArgs a) {
+  constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
+  const long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const int g = static_cast<int>(p % d.ngf);
+  const long el = p / d.ngf;
+  const int l = static_cast<int>(el % d.nfe);
+  const long E = a.e0 + el / d.nfe;
+  double uh[NCU], uq[NC], x[ND], nl[ND], v[8], jac;
+  face_state<M>(d, a, E, l, g, uh, uq, x, nl, v, jac);
+  double* in = a.pbuf;
+#pragma unroll
+  for (int b = 0; b < ND; ++b) in[b * P + p] = x[b];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    in[(ND + c) * P + p] = uq[c];
+    in[(ND + NC + c) * P + p] = c < NCU ? uh[c] : uq[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) in[(ND + 2 * NC + c) * P + p] = uh[c];
+#pragma unroll
+  for (int b = 0; b < ND; ++b) in[(ND + 2 * NC + NCU + b) * P + p] = nl[b];
+  for (int c = 0; c < d.nco && c < 8; ++c) in[(2 * ND + 2 * NC + NCU + c) * P + p] = v[c];
+}
+
+// The model library's functions over the block's points (one instance: its physicsparam and tau),
+// on the solver's stream, then the point kernels reading their values.  The boundary function runs
+// once per condition over all face points of the block (the point kernel picks each boundary
+// point's condition), and only for blocks with boundary faces.
+template <class M>
+void plugin_launch(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) {
+  constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
+  const PluginFns& pf = *a.plugin;
+  const int nco = d.nco;
+  if (pf.set_stream) pf.set_stream(s);
+  if (vol) {
+    const long P = static_cast<long>(a.eb) * d.nge;
+    k_plug_vol_in<M><<<nblocks(P, 256), 256, 0, s>>>(d, a);
+    EXA_HIP(hipGetLastError());
+    const double* x = a.pbuf;
+    const double* uq = x + ND * P;
+    const double* v = uq + NC * P;
+    double* f = a.pbuf + M::vol_in(nco) * P;
+    double* f_uq = f + NCU * ND * P;
+    double* sv = f_uq + NCU * ND * NC * P;
+    double* s_uq = sv + NCU * P;
+    const int n = static_cast<int>(P);
+    pf.flux(f, f_uq, nullptr, x, uq, v, nullptr, pf.uinf, a.param, a.time, pf.modelnumber, n, NC, NCU, ND, ND, nco,
+            0);
+    pf.source(sv, s_uq, nullptr, x, uq, v, nullptr, pf.uinf, a.param, a.time, pf.modelnumber, n, NC, NCU, ND, ND,
+              nco, 0);
+  } else {
+    const long P = static_cast<long>(a.eb) * d.nfe * d.ngf;
+    k_plug_face_in<M><<<nblocks(P, 256), 256, 0, s>>>(d, a);
+    EXA_HIP(hipGetLastError());
+    const double* x = a.pbuf;
+    const double* uq = x + ND * P;
+    const double* uqh = uq + NC * P;
+    const double* uh = uqh + NC * P;
+    const double* nl = uh + NCU * P;
+    const double* v = nl + ND * P;
+    double* f = a.pbuf + M::face_in(nco) * P;
+    const int n = static_cast<int>(P);
+    pf.flux(f, f + NCU * ND * P, nullptr, x, uqh, v, nullptr, pf.uinf, a.param, a.time, pf.modelnumber, n, NC, NCU,
+            ND, ND, nco, 0);
+    if (a.nbf > 0)
+      for (int ib = 1; ib <= pf.nbc; ++ib) {
+        double* fb = f + (NCU * ND + NCU * ND * NC + (ib - 1) * M::face_bc()) * P;
+        pf.fbou(fb, fb + NCU * P, nullptr, fb + (NCU + NCU * NC) * P, x, uq, v, nullptr, uh, nl, a.tau, pf.uinf,
+                a.param, a.time, pf.modelnumber, ib, n, NC, NCU, ND, ND, nco, 0);
+      }
+  }
+  EXA_HIP(hipGetLastError());
+  launch_points<M>(d, a, jac, vol, s);
+}
+
+void plugin_points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) {
+  if (!a.plugin || !a.pbuf) throw Error(1, "model library not loaded");
+  switch (d.nd * 10 + d.ncu) {
+    case 21: return plugin_launch<PluginModel<2, 1>>(d, a, jac, vol, s);
+    case 22: return plugin_launch<PluginModel<2, 2>>(d, a, jac, vol, s);
+    case 23: return plugin_launch<PluginModel<2, 3>>(d, a, jac, vol, s);
+    case 24: return plugin_launch<PluginModel<2, 4>>(d, a, jac, vol, s);
+    case 25: return plugin_launch<PluginModel<2, 5>>(d, a, jac, vol, s);
+    case 31: return plugin_launch<PluginModel<3, 1>>(d, a, jac, vol, s);
+    case 32: return plugin_launch<PluginModel<3, 2>>(d, a, jac, vol, s);
+    case 33: return plugin_launch<PluginModel<3, 3>>(d, a, jac, vol, s);
+    case 34: return plugin_launch<PluginModel<3, 4>>(d, a, jac, vol, s);
+    case 35: return plugin_launch<PluginModel<3, 5>>(d, a, jac, vol, s);
+    default: throw Error(5, "model libraries: nd in {2, 3} and ncu in 1..5 only");
+  }
+}
+
 void points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) {
   if (d.nco > 8) throw Error(5, "at most 8 auxiliary components");
   switch (a.model) {
+    case MODEL_PLUGIN: return plugin_points(d, a, jac, vol, s);
     case MODEL_POISSON: return dispatch_points<Poisson>(d, a, jac, vol, s);
     case MODEL_POISSON_POLY: return dispatch_points<PoissonPoly>(d, a, jac, vol, s);
     case MODEL_CONVDIFF: return dispatch_points<ConvDiff>(d, a, jac, vol, s);
@@ -2327,6 +2515,14 @@ void points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) 
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+long plugin_buffer_doubles(const Dims& d, int nbc, int eb) {
+  const long nd = d.nd, ncu = d.ncu, nc = d.nc, nco = d.nco;
+  const long pv = static_cast<long>(eb) * d.nge, pf = static_cast<long>(eb) * d.nfe * d.ngf;
+  const long vol = (nd + nc + nco + ncu * nd + ncu * nd * nc + ncu + ncu * nc) * pv;
+  const long face = (2 * nd + 2 * nc + ncu + nco + ncu * nd + ncu * nd * nc + nbc * (ncu + ncu * nc + ncu * ncu)) * pf;
+  return std::max(vol, face);
+}
+
 int generated_model(const std::string& name, int* nd, int* ncu, int* nv) {
 #define EXA_GEN_FIND(ID, NAME, G) \
   if (name == NAME) {             \

@@ -77,6 +77,29 @@ void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemco
           double* udg, hipStream_t s);
 
 // ---- assembly (model-templated inside) ----
+// Model functions loaded from a model library (model name "plugin:<path>"): text2code's GPU form
+// with the signatures of backend/Model/libpdemodel.hpp:1-41 (HdgFlux, HdgSource, HdgFbou), each a
+// host function that launches its own kernel over ng points, arrays [component * ng + point].
+typedef void (*PlugFluxFn)(double* f, double* f_udg, double* f_wdg, const double* xdg, const double* udg,
+                           const double* odg, const double* wdg, const double* uinf, const double* param,
+                           double time, int modelnumber, int ng, int nc, int ncu, int nd, int ncx, int nco,
+                           int ncw);
+typedef void (*PlugFbouFn)(double* f, double* f_udg, double* f_wdg, double* f_uhg, const double* xdg,
+                           const double* udg, const double* odg, const double* wdg, const double* uhg,
+                           const double* nlg, const double* tau, const double* uinf, const double* param,
+                           double time, int modelnumber, int ib, int ng, int nc, int ncu, int nd, int ncx,
+                           int nco, int ncw);
+typedef void (*PlugStreamFn)(hipStream_t);
+struct PluginFns {
+  PlugFluxFn flux = nullptr, source = nullptr;
+  PlugFbouFn fbou = nullptr;
+  PlugStreamFn set_stream = nullptr;  // optional
+  int nbc = 0, modelnumber = 0;
+  const double* uinf = nullptr;       // device
+};
+// doubles of the staging buffer a model library needs per block of eb elements
+long plugin_buffer_doubles(const Dims& d, int nbc, int eb);
+
 struct AsmArgs {
   int model;
   double time;
@@ -98,6 +121,11 @@ struct AsmArgs {
   int tdep, tdfunc;
   double dtfactor;
   const double* sdg;
+  // model library (MODEL_PLUGIN): its functions (host) and the staging buffer of the point
+  // inputs and outputs (device, plugin_buffer_doubles)
+  const PluginFns* plugin;
+  double* pbuf;
+  int plugin_nbc;
 };
 void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
 void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);

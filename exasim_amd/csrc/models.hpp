@@ -15,7 +15,7 @@
 
 namespace exa {
 
-enum ModelId { MODEL_POISSON = 0, MODEL_POISSON_POLY = 1, MODEL_CONVDIFF = 2 };
+enum ModelId { MODEL_POISSON = 0, MODEL_POISSON_POLY = 1, MODEL_CONVDIFF = 2, MODEL_PLUGIN = 50 };
 
 // examples/Poisson/poisson{2d,3d}/pdemodel.txt: Flux = kappa*q, Source =
 // nd*pi^2*prod sin(pi x_d), FbouHdg = tau*(0 - uhat).

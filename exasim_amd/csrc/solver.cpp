@@ -12,6 +12,8 @@
 #include "models.hpp"
 
 #include <algorithm>
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -19,6 +21,7 @@
 namespace exa {
 
 int resolve_model(const std::string& name) {
+  if (name.rfind("plugin:", 0) == 0) return MODEL_PLUGIN;
   if (name == "poisson") return MODEL_POISSON;
   if (name == "poisson_poly") return MODEL_POISSON_POLY;
   if (name == "convdiff") return MODEL_CONVDIFF;
@@ -26,6 +29,35 @@ int resolve_model(const std::string& name) {
   const int id = generated_model(name, &nd, &ncu, &nv);
   if (id >= 0) return id;
   throw Error(1, "unknown model '" + name + "'");
+}
+
+// A model library (model name "plugin:<path>"): text2code's GPU form of the model functions,
+// built with include/CudaHip.hpp; HdgFlux, HdgSource and HdgFbou with the signatures of
+// backend/Model/libpdemodel.hpp:1-41 (required), exa_plugin_set_stream (optional).  The library
+// stays loaded until the handle is destroyed.  One instance per handle (the functions take one
+// physicsparam array); the boundary conditions are 1 .. the largest one mesh.bin's bf uses.
+void Solver::load_plugin() {
+  if (batch && ninst > 1)
+    throw Error(5, "a model library serves single problems (batches of one instance)");
+  const std::string path = model_name.substr(7);
+  plugin_handle = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+  if (!plugin_handle) {
+    const char* e = dlerror();
+    throw Error(1, "cannot load model library '" + path + "': " + (e ? e : "unknown error"));
+  }
+  auto sym = [&](const char* name, bool need) -> void* {
+    void* f = dlsym(plugin_handle, name);
+    if (!f && need) throw Error(1, "model library '" + path + "' has no " + name);
+    return f;
+  };
+  plugin_fns.flux = reinterpret_cast<PlugFluxFn>(sym("HdgFlux", true));
+  plugin_fns.source = reinterpret_cast<PlugFluxFn>(sym("HdgSource", true));
+  plugin_fns.fbou = reinterpret_cast<PlugFbouFn>(sym("HdgFbou", true));
+  plugin_fns.set_stream = reinterpret_cast<PlugStreamFn>(sym("exa_plugin_set_stream", false));
+  int nbc = 0;
+  for (int b : mesh.bf) nbc = std::max(nbc, b);
+  plugin_fns.nbc = nbc;
+  plugin_fns.modelnumber = app.flag.size() > 12 ? static_cast<int>(app.flag[12]) : 0;
 }
 
 Solver::Solver(const std::string& din, const std::string& dout, const std::string& mname, int dev)
@@ -50,6 +82,7 @@ Solver::~Solver() {
       if (e) (void)hipEventDestroy(e);
   }
   if (s) (void)hipStreamDestroy(s);
+  if (plugin_handle) dlclose(plugin_handle);
 }
 
 void Solver::setup() {
@@ -103,6 +136,7 @@ void Solver::init(SolData& sol) {
   // the point kernels hold the auxiliary fields of a quadrature point in registers
   if (d.nco > 8)
     throw Error(5, "sol.bin has " + std::to_string(d.nco) + " auxiliary fields (vdg); at most 8 are supported");
+  if (model == MODEL_PLUGIN) load_plugin();
   if (model >= 100) {  // a generated model is compiled for one (nd, ncu)
     int nd, ncu, nv;
     generated_model(model_name, &nd, &ncu, &nv);
@@ -187,6 +221,12 @@ void Solver::init(SolData& sol) {
   }
   param.upload(inst_param, st, s);
   tau.upload(inst_tau, st, s);
+  if (model == MODEL_PLUGIN) {  // the model library's uinf argument (app.bin's externalparam)
+    std::vector<double> ui = app.uinf;
+    if (ui.empty()) ui.push_back(0.0);
+    uinf_d.upload(ui, st, s);
+    plugin_fns.uinf = uinf_d.p;
+  }
   seg_setup();
   perm.upload(mesh.perm, st, s);
   elemcon.upload(conn.elemcon, st, s);
@@ -532,6 +572,7 @@ void Solver::alloc_workspace() {
   rv.alloc(static_cast<size_t>(neb) * (d.nd + 1) * d.ncu * d.nge);
   rJ.alloc(static_cast<size_t>(neb) * (d.nd + 1) * d.ncu * d.nc * d.nge);
   fpt.alloc(static_cast<size_t>(neb) * 2 * (d.ncu + d.ncu * d.nc + d.ncu * d.ncu) * d.nfe * d.ngf);
+  if (model == MODEL_PLUGIN) pbuf.alloc(static_cast<size_t>(plugin_buffer_doubles(d, plugin_fns.nbc, neb)));
   V.alloc(static_cast<size_t>(restart + 1) * d.N);
   partial.alloc(static_cast<size_t>(dot_partials_count(d.N)) * (restart + 2));
   seg_alloc();
@@ -631,6 +672,9 @@ AsmArgs Solver::asm_args(int e0, int eb) {
     a.bf0 = lo;
     a.nbf = hi - lo;
   }
+  a.plugin = model == MODEL_PLUGIN ? &plugin_fns : nullptr;
+  a.pbuf = pbuf.p;
+  a.plugin_nbc = plugin_fns.nbc;
   a.udg = udg.p;
   a.uh = uh.p;
   a.egeo = egeo.p;

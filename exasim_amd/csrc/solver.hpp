@@ -65,6 +65,11 @@ struct Solver {
 
   // device data
   DevBuf<double> shapegt, shapegw, shapfgt, shapfgw, param, tau;
+  // model library (model name "plugin:<path>"): its functions, uinf and the point staging buffer
+  void* plugin_handle = nullptr;
+  PluginFns plugin_fns;
+  DevBuf<double> uinf_d, pbuf;
+  void load_plugin();
   DevBuf<int> perm, elemcon, bf, f2e, info, ifirst;
   std::vector<int> bface_h;  // boundary faces E*nfe + l (bf > 0), ascending
   DevBuf<int> bface;

@@ -1,0 +1,21 @@
+// Model functions of exasim_amd/models/poisson2d.txt (2D Poisson, one component)
+// written in the form text2code gives with framework "hip": host functions whose bodies are
+// parallel_for(..., GPU_LAMBDA(const size_t i) { ... }) over ng points, arrays [component * ng + i],
+// with the signatures of backend/Model/libpdemodel.hpp.  A sample of the model-library interface
+// that libexahip.so loads with the model name "plugin:<path to the built .so>".
+#pragma once
+
+#include "CudaHip.hpp"
+
+extern "C" {
+void HdgFlux(dstype* f, dstype* f_udg, dstype* f_wdg, const dstype* xdg, const dstype* udg, const dstype* odg,
+             const dstype* wdg, const dstype* uinf, const dstype* param, const dstype time, const int modelnumber,
+             const int ng, const int nc, const int ncu, const int nd, const int ncx, const int nco, const int ncw);
+void HdgSource(dstype* f, dstype* f_udg, dstype* f_wdg, const dstype* xdg, const dstype* udg, const dstype* odg,
+               const dstype* wdg, const dstype* uinf, const dstype* param, const dstype time, const int modelnumber,
+               const int ng, const int nc, const int ncu, const int nd, const int ncx, const int nco, const int ncw);
+void HdgFbou(dstype* f, dstype* f_udg, dstype* f_wdg, dstype* f_uhg, const dstype* xdg, const dstype* udg,
+             const dstype* odg, const dstype* wdg, const dstype* uhg, const dstype* nlg, const dstype* tau,
+             const dstype* uinf, const dstype* param, const dstype time, const int modelnumber, const int ib,
+             const int ng, const int nc, const int ncu, const int nd, const int ncx, const int nco, const int ncw);
+}

@@ -79,3 +79,24 @@ def test_batch_create_checks_instances(tmp_path):
     with pytest.raises(hdg.ExaError) as e:
         hdg.HDGSolver.batch([c, c], "cd2d", device=-2)
     assert e.value.code == 5
+
+
+def test_model_library_errors(tmp_path):
+    """A model library ("plugin:<path>") that cannot be loaded, that lacks HdgFlux, or that is
+    asked to serve a batch of several instances is refused before any device work."""
+    if not os.path.exists(hdg.LIB_PATH):
+        pytest.skip("libexahip.so not built")
+    from exasim_amd import cases
+    a = str(tmp_path / "a")
+    cases.poisson2d(a, n=2, porder=2, pgauss=4)
+    with pytest.raises(hdg.ExaError) as e:
+        hdg.HDGSolver(a, "plugin:" + str(tmp_path / "missing.so"), device=-2)
+    assert e.value.code == 1 and "cannot load model library" in str(e.value)
+    with pytest.raises(hdg.ExaError) as e:
+        hdg.HDGSolver(a, "plugin:" + hdg.LIB_PATH, device=-2)
+    assert e.value.code == 1 and "has no HdgFlux" in str(e.value)
+    lib = os.path.join(ROOT, "exasim_amd", "plugins", "libpoisson2d_t2c.so")
+    if os.path.exists(lib):
+        with pytest.raises(hdg.ExaError) as e:
+            hdg.HDGSolver.batch([a, a], "plugin:" + lib, device=-2)
+        assert e.value.code == 5

@@ -352,10 +352,12 @@ def boundary_face_tags(p, t, nd, exprs):
 
 
 def write_app(path, *, nd, ncu, ncw=0, nco=0, flag=None, problem=None, physicsparam=(),
-              tau=(1.0,), solversparam=(1e-6, 1e-8, 1e-3, 0.0), dt=(), factor=(0, 0, 0, 0, 0),
+              tau=(1.0,), solversparam=(1e-6, 1e-8, 1e-3, 0.0), dt=(), factor=(0, 1, 0, 0, 0),
               externalparam=(0.0, 0.0)):
     """readpdeapp.cpp:830-915 (``writepde``); ``flag``/``problem`` are dicts of
-    overrides of the defaults below (index meanings: setstructs.cpp:123-181)."""
+    overrides of the defaults below (index meanings: setstructs.cpp:123-181; the defaults are the
+    reference's PDE defaults, readpdeapp.cpp:360-440, where the path does not set them: nch = ncu,
+    AVsmoothingIter 2, frozenAVflag 1, dae_alpha 1)."""
     nc = ncu * (nd + 1)
     ndims = np.zeros(40)
     ndims[0] = 1                    # mpiprocs
@@ -364,6 +366,7 @@ def write_app(path, *, nd, ncu, ncw=0, nco=0, flag=None, problem=None, physicspa
     ndims[6] = ncu
     ndims[7] = nc - ncu
     ndims[9] = nco
+    ndims[10] = ncu                 # nch
     ndims[11] = nd                  # ncx
     ndims[13] = ncw
     fl = np.zeros(17)
@@ -381,6 +384,8 @@ def write_app(path, *, nd, ncu, ncw=0, nco=0, flag=None, problem=None, physicspa
     pr[15] = 50        # GMRESrestart
     pr[17] = 1
     pr[18] = 1         # saveSolOpt
+    pr[25] = 2         # AVsmoothingIter
+    pr[26] = 1         # frozenAVflag
     pr[27] = 1         # ppdegree
     for k, v in (problem or {}).items():
         pr[k] = v

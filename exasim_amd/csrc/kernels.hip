@@ -1976,7 +1976,7 @@ __global__ void k_bilu_put_l(Dims d, IluDev I, int k0, int ne_i, const T* Ltmp, 
 
 // loads a thread keeps in flight per step of a row's sweep (raw storage words in registers)
 template <typename T>
-constexpr int bilu_unroll() { return sizeof(T) == 2 ? 16 : sizeof(T) == 4 ? 12 : 8; }
+constexpr int bilu_unroll() { return sizeof(T) == 2 ? 12 : sizeof(T) == 4 ? 12 : 8; }
 
 // One workgroup per superelement (ApplyBlockILU0, preconditioner.cpp:184-270):
 // forward sweep y_i = b_i - sum L_ij y_j, backward sweep x_i = D_i^{-1} (y_i - sum
@@ -2005,10 +2005,14 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
   const bool active = part < nparts;
   const int cw = (ncf + nparts - 1) / nparts;
   const int c0 = min(ncf, part * cw), c1 = min(ncf, c0 + cw);
+  // the superelement's vector in LDS: FP64 with FP64 factors, FP32 with FP32 / BF16 factors (it
+  // carries values the factors' own rounding already limits; the smaller LDS footprint lets 8
+  // superelements share a CU instead of 6 at C2's 36 faces of 50 unknowns)
+  using VT = typename std::conditional<sizeof(T) == 8, double, float>::type;
   extern __shared__ double sm[];
-  double* v = sm;                          // nfse * ncf
-  double* red = sm + static_cast<long>(nfse) * ncf;   // nparts * ncf <= 256 * VW
-  double* z = red + 256 * VW;              // ncf
+  double* red = sm;                        // nparts * ncf <= 256 * VW
+  VT* v = reinterpret_cast<VT*>(red + 256 * VW);   // nfse * ncf
+  VT* z = v + static_cast<long>(nfse) * ncf;       // ncf
   // the sweep structure (the same for every superelement) in LDS
   int* lptr = reinterpret_cast<int*>(z + ncf);
   int* uuptr = lptr + nfse + 1;
@@ -2033,13 +2037,13 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
   auto blk = [&](int t) { return BE + (static_cast<long>(t) * I.nse + n) * per + r0; };
   for (int idx = tid; idx < nfse * ncf; idx += 256) {
     const int i = idx / ncf, q = idx % ncf;
-    v[idx] = x[static_cast<long>(I.face[n + static_cast<long>(I.nse) * i]) * ncf + q];
+    v[idx] = static_cast<VT>(x[static_cast<long>(I.face[n + static_cast<long>(I.nse) * i]) * ncf + q]);
   }
   __syncthreads();
   double acc[VW];
   // acc += sum over the blocks q in [qb, qe) of block idx[q] (this thread's rows, columns
   // c0 .. c1-1) times vec[col[q]], in the order (q, c); U loads are issued before their FMAs
-  auto row_pass = [&](int qb, int qe, const int* idx, const int* col, const double* vec) {
+  auto row_pass = [&](int qb, int qe, const int* idx, const int* col, const VT* vec) {
     const int cwn = c1 - c0;
     const int tot = (qe - qb) * cwn;
     int qq = qb, cc = c0;
@@ -2062,7 +2066,7 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (yo[u] >= 0) {
-          const double yc = vec[yo[u]];
+          const double yc = static_cast<double>(vec[yo[u]]);
 #pragma unroll
           for (int k = 0; k < VW; ++k) acc[k] = fma(blk_val<T>(raw[u][k]), yc, acc[k]);
         }
@@ -3072,7 +3076,8 @@ void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, do
   if (d.ncf > 256) throw Error(5, "block-ILU(0) face blocks larger than 256");
   // two rows per thread (8-byte FP32 / 16-byte FP64 loads) when the block rows pair up
   const int vw = (d.ncf % 2 == 0) ? 2 : 1;
-  const size_t shm = (static_cast<size_t>(I.nfse) * d.ncf + 256 * vw + d.ncf) * sizeof(double) +
+  const size_t vt = sizeof(T) == 8 ? sizeof(double) : sizeof(float);  // the kernel's LDS vector type
+  const size_t shm = 256 * vw * sizeof(double) + (static_cast<size_t>(I.nfse) * d.ncf + d.ncf) * vt +
                      (3 * static_cast<size_t>(I.nfse) + 2 + 4 * static_cast<size_t>(I.nnz)) * sizeof(int);
   if (shm > 160 * 1024) throw Error(5, "block-ILU(0) superelement vector larger than the LDS");
   auto launch = [&](auto kern) {

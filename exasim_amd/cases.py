@@ -33,11 +33,12 @@ def _app_flags(precond, ortho):
 
 def poisson2d(path, n=16, porder=3, pgauss=14, model="poisson", kappa=1.0, tau=1.0,
               precond="bjacobi", restart=50, gmres_tol=1e-12, gmres_maxit=1000,
-              newton_iter=1, newton_tol=1e-6, ortho=0, nd=2, cartgridpart=None):
-    """Poisson on the unit square (nd=2) or cube (nd=3), all boundaries Dirichlet."""
+              newton_iter=1, newton_tol=1e-6, ortho=0, nd=2, cartgridpart=None, elemtype=1):
+    """Poisson on the unit square (nd=2) or cube (nd=3), all boundaries Dirichlet; quadrilaterals /
+    hexahedra (elemtype 1) or triangles / tetrahedra (elemtype 0, datain.simplex_grid)."""
     os.makedirs(path, exist_ok=True)
-    m = dw.make_master(nd, porder, pgauss)
-    p, t = dw.structured_grid([n] * nd)
+    m = dw.make_master(nd, porder, pgauss, elemtype=elemtype)
+    p, t = dw.structured_grid([n] * nd) if elemtype == 1 else dw.simplex_grid([n] * nd)
     if nd == 2:
         exprs = [lambda v: np.all(np.abs(v[:, 1]) < 1e-8), lambda v: np.all(np.abs(v[:, 0] - 1) < 1e-8),
                  lambda v: np.all(np.abs(v[:, 1] - 1) < 1e-8), lambda v: np.all(np.abs(v[:, 0]) < 1e-8)]

@@ -1,7 +1,8 @@
 """Writers and readers for the Exasim ``datain/`` and ``dataout/`` binary layout.
 
 This mirrors what the reference's text2code writes for a ``pdeapp.txt`` with
-``mpiprocs = 1`` on tensor-product elements (quadrilaterals and hexahedra):
+``mpiprocs = 1`` on tensor-product elements (quadrilaterals and hexahedra) and on simplices
+(triangles and tetrahedra; their quadrature rules are this module's own, see make_master):
 
 * ``app.bin``    -- text2code/text2code/readpdeapp.cpp:830-915 (``writepde``)
 * ``master.bin`` -- text2code/text2code/makemaster.cpp:1019-1225 (``initializeMaster``,
@@ -78,6 +79,12 @@ HEX_VERTS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
 # (``getelemface``); face node orientation follows from this ordering.
 QUAD_FACES = [[0, 1], [1, 2], [2, 3], [3, 0]]
 HEX_FACES = [[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [2, 3, 7, 6], [1, 2, 6, 5], [3, 0, 4, 7]]
+# Triangle and tetrahedron (elemtype 0): vertices of the reference simplex and the local faces
+# of connectivity.cpp:1010-1106 (face l is opposite vertex l)
+TRI_VERTS = np.array([[0, 0], [1, 0], [0, 1]], dtype=np.float64)
+TET_VERTS = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
+TRI_FACES = [[1, 2], [2, 0], [0, 1]]
+TET_FACES = [[1, 2, 3], [0, 3, 2], [0, 1, 3], [0, 2, 1]]
 
 
 def _lagrange_1d(nodes, x):
@@ -153,6 +160,102 @@ def _tensor_shape(nodes1, pts, nd):
     return shap
 
 
+def _simplex_points(nd, p):
+    """Equispaced nodes of the reference simplex of degree p, the first coordinate fastest
+    (the order of text2code's simplex node tables: masternodes.bin, elemtype 0)."""
+    if nd == 1:
+        return (np.arange(p + 1, dtype=np.float64) / p)[:, None]
+    pts = []
+    if nd == 2:
+        for j in range(p + 1):
+            for i in range(p + 1 - j):
+                pts.append((i / p, j / p))
+    else:
+        for k in range(p + 1):
+            for j in range(p + 1 - k):
+                for i in range(p + 1 - k - j):
+                    pts.append((i / p, j / p, k / p))
+    return np.array(pts, dtype=np.float64)
+
+
+def _simplex_quadrature(nd, degree):
+    """Points [n, nd] and weights of a rule exact for total degree ``degree`` on the reference
+    simplex: conical (collapsed-coordinate) products of Gauss-Legendre and Gauss-Jacobi rules
+    (Stroud), the Jacobi weights (1 - b), (1 - c)^2 absorbing the collapse.  Weights sum to the
+    simplex's measure (1/2, 1/6)."""
+    from scipy.special import roots_jacobi, roots_legendre
+    n = max(1, (degree + 2) // 2)
+    a, wa = roots_legendre(n)
+    b, wb = roots_jacobi(n, 1.0, 0.0)
+    pts, w = [], []
+    if nd == 2:
+        for j in range(n):
+            for i in range(n):
+                pts.append(((1 + a[i]) * (1 - b[j]) / 4.0, (1 + b[j]) / 2.0))
+                w.append(wa[i] * wb[j] / 8.0)
+    else:
+        c, wc = roots_jacobi(n, 2.0, 0.0)
+        for k in range(n):
+            for j in range(n):
+                for i in range(n):
+                    pts.append(((1 + a[i]) * (1 - b[j]) * (1 - c[k]) / 8.0, (1 + b[j]) * (1 - c[k]) / 4.0,
+                                (1 + c[k]) / 2.0))
+                    w.append(wa[i] * wb[j] * wc[k] / 64.0)
+    return np.array(pts, dtype=np.float64), np.array(w, dtype=np.float64)
+
+
+def _simplex_shape(nodes, pts, nd, p):
+    """shap[d, g, j] of the Lagrange basis on the simplex ``nodes`` (degree p) at ``pts``: d = 0
+    values, d >= 1 derivatives.  The basis is the total-degree space spanned by products of
+    Legendre polynomials L_i(2x-1) L_j(2y-1) (L_k(2z-1)), i + j (+ k) <= p, made nodal by
+    inverting its Vandermonde matrix at the nodes (the construction of makemaster.cpp:774-846,
+    with another modal basis of the same space, so the nodal basis is the same)."""
+    from numpy.polynomial import legendre as L
+    if nd == 2:
+        exps = [(i, j) for j in range(p + 1) for i in range(p + 1 - j)]
+    else:
+        exps = [(i, j, k) for k in range(p + 1) for j in range(p + 1 - k) for i in range(p + 1 - k - j)]
+
+    def modal(x):
+        x = np.asarray(x, dtype=np.float64)
+        val = np.ones((nd + 1, x.shape[0], len(exps)))
+        for m, e in enumerate(exps):
+            for d in range(nd):
+                c = np.zeros(e[d] + 1)
+                c[-1] = 1.0
+                v = L.legval(2.0 * x[:, d] - 1.0, c)
+                dv = 2.0 * L.legval(2.0 * x[:, d] - 1.0, L.legder(c)) if e[d] > 0 else np.zeros(x.shape[0])
+                val[0, :, m] *= v
+                for a in range(nd):
+                    val[1 + a, :, m] *= dv if a == d else v
+        return val
+
+    V = modal(nodes)[0]
+    if V.shape[0] != V.shape[1]:
+        raise ValueError("simplex nodes do not match the degree")
+    Vinv = np.linalg.inv(V)
+    return np.einsum("dgm,mj->dgj", modal(pts), Vinv)
+
+
+def _face_perm_simplex(xpe, xpf, nd):
+    """perm[l, j] for the triangle / tetrahedron: face node j of local face l at
+    V0 + s (V1 - V0) (+ t (V2 - V0)) for the face's vertices (V0, V1 (, V2))."""
+    verts, faces = (TRI_VERTS, TRI_FACES) if nd == 2 else (TET_VERTS, TET_FACES)
+    perm = np.zeros((len(faces), xpf.shape[0]), dtype=np.int64)
+    for l, fv in enumerate(faces):
+        V = verts[fv]
+        for j in range(xpf.shape[0]):
+            pos = V[0] + xpf[j, 0] * (V[1] - V[0])
+            if nd == 3:
+                pos = pos + xpf[j, 1] * (V[2] - V[0])
+            d = np.abs(xpe - pos[None, :]).max(axis=1)
+            k = int(np.argmin(d))
+            if d[k] > 1e-8:
+                raise RuntimeError("face node not found on the master element")
+            perm[l, j] = k
+    return perm
+
+
 def _face_perm(xpe, xpf, nd):
     """perm[l, j]: element node that sits at face node j of local face l."""
     if nd == 2:
@@ -180,14 +283,17 @@ def _face_perm(xpe, xpf, nd):
     return perm
 
 
-def make_master(nd, porder, pgauss):
-    """Master element for quadrilaterals (nd=2) and hexahedra (nd=3).
+def make_master(nd, porder, pgauss, elemtype=1):
+    """Master element for quadrilaterals / hexahedra (elemtype 1) or triangles / tetrahedra
+    (elemtype 0), nd = 2, 3.
 
     Follows text2code/text2code/makemaster.cpp:1019-1146 (``initializeMaster``).
     Returns a dict of numpy arrays in natural index order plus ``ndims``.
     """
     if nd not in (2, 3):
-        raise ValueError("tensor master elements are built for nd = 2, 3")
+        raise ValueError("master elements are built for nd = 2, 3")
+    if elemtype == 0:
+        return _make_master_simplex(nd, porder, pgauss)
     if porder not in NODES_1D:
         raise ValueError("porder must be in 1..8")
     ng1 = pgauss // 2 + 1
@@ -209,6 +315,41 @@ def make_master(nd, porder, pgauss):
     return dict(nd=nd, elemtype=1, nodetype=1, porder=porder, pgauss=pgauss,
                 npe=xpe.shape[0], npf=xpf.shape[0], nge=gpe.shape[0], ngf=gpf.shape[0],
                 np1d=x1.size, ng1d=g1.size, nfe=2 * nd, nve=2 ** nd,
+                xpe=xpe, xpf=xpf, gpe=gpe, gwe=gwe, gpf=gpf, gwf=gwf,
+                shapeg=shapeg, shapen=shapen, shapfg=shapfg, shapfn=shapfn,
+                shap1g=shap1g, shap1n=shap1n, xp1d=x1, gp1d=g1, gw1d=w1, perm=perm)
+
+
+def _make_master_simplex(nd, porder, pgauss):
+    """Triangle / tetrahedron master element: equispaced nodes, conical-product quadrature of
+    degree pgauss on the element and its faces (1D faces: the Gauss-Legendre rule of the tensor
+    elements), the nodal basis of _simplex_shape."""
+    if porder < 1 or porder > 8:
+        raise ValueError("porder must be in 1..8")
+    ng1 = pgauss // 2 + 1
+    if ng1 not in GAUSS_1D:
+        raise ValueError("pgauss must be in 1..15")
+    xpe = _simplex_points(nd, porder)
+    xpf = _simplex_points(nd - 1, porder)
+    x1 = _simplex_points(1, porder)[:, 0]
+    g1, w1 = (np.array(a) for a in GAUSS_1D[ng1])
+    gpe, gwe = _simplex_quadrature(nd, pgauss)
+    if nd == 2:
+        gpf, gwf = g1[:, None].copy(), w1.copy()
+        shapfg = np.stack(_lagrange_1d(x1, g1), 0)
+        shapfn = np.stack(_lagrange_1d(x1, x1), 0)
+    else:
+        gpf, gwf = _simplex_quadrature(2, pgauss)
+        shapfg = _simplex_shape(xpf, gpf, 2, porder)
+        shapfn = _simplex_shape(xpf, xpf, 2, porder)
+    shapeg = _simplex_shape(xpe, gpe, nd, porder)
+    shapen = _simplex_shape(xpe, xpe, nd, porder)
+    shap1g = np.stack(_lagrange_1d(x1, g1), 0)
+    shap1n = np.stack(_lagrange_1d(x1, x1), 0)
+    perm = _face_perm_simplex(xpe, xpf, nd)
+    return dict(nd=nd, elemtype=0, nodetype=1, porder=porder, pgauss=pgauss,
+                npe=xpe.shape[0], npf=xpf.shape[0], nge=gpe.shape[0], ngf=gpf.shape[0],
+                np1d=x1.size, ng1d=g1.size, nfe=nd + 1, nve=nd + 1,
                 xpe=xpe, xpf=xpf, gpe=gpe, gwe=gwe, gpf=gpf, gwf=gwf,
                 shapeg=shapeg, shapen=shapen, shapfg=shapfg, shapfn=shapfn,
                 shap1g=shap1g, shap1n=shap1n, xp1d=x1, gp1d=g1, gw1d=w1, perm=perm)
@@ -301,14 +442,40 @@ def structured_grid(n, lo=None, hi=None, stretch=None):
     return p, np.array(t, dtype=np.int64)
 
 
+def simplex_grid(n, lo=None, hi=None, stretch=None):
+    """Vertices and triangles (nd = 2) or tetrahedra (nd = 3) of the structured grid of
+    ``structured_grid``: each quadrilateral split into 2 triangles along its (i, j)-(i+1, j+1)
+    diagonal, each hexahedron into the 6 tetrahedra around its (0,0,0)-(1,1,1) diagonal (Kuhn),
+    which match across shared faces; every element positively oriented.  Elements of a cell
+    are numbered together, cells in the order of ``structured_grid``."""
+    p, q = structured_grid(n, lo, hi, stretch)
+    nd = p.shape[1]
+    t = []
+    if nd == 2:
+        for a, b, c, d in q:
+            t += [[a, b, c], [a, c, d]]
+    else:
+        for v in q:
+            for path in ((1, 2), (1, 5), (4, 5), (4, 7), (3, 7), (3, 2)):
+                tet = [v[0], v[path[0]], v[path[1]], v[6]]
+                x = p[tet]
+                if np.linalg.det(np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]])) < 0:
+                    tet[2], tet[3] = tet[3], tet[2]
+                t.append(tet)
+    return p, np.array(t, dtype=np.int64)
+
+
 def compute_dgnodes(m, p, t):
     """xdg[ne, npe, nd] = vertex basis at the master nodes times vertex coordinates.
 
     text2code/text2code/makemesh.cpp:809-827 (``compute_dgnodes``) with the
-    bilinear/trilinear ``phielem`` of makemaster.cpp:848-938.
+    bilinear/trilinear ``phielem`` of makemaster.cpp:848-938 (barycentric for simplices).
     """
     xpe = m["xpe"]
     nd = m["nd"]
+    if m.get("elemtype", 1) == 0:
+        phi = np.concatenate([1.0 - xpe.sum(axis=1, keepdims=True), xpe], axis=1)
+        return np.einsum("iv,evd->eid", phi, p[t])
     if nd == 2:
         xi, eta = xpe[:, 0], xpe[:, 1]
         phi = np.stack([(1 - xi) * (1 - eta), xi * (1 - eta), xi * eta, (1 - xi) * eta], 1)
@@ -328,7 +495,10 @@ def boundary_face_tags(p, t, nd, exprs):
     ``exprs`` are callables taking the face-vertex coordinates [nvf, nd] and
     returning True when the face lies on that boundary.
     """
-    faces = QUAD_FACES if nd == 2 else HEX_FACES
+    if t.shape[1] == nd + 1:
+        faces = TRI_FACES if nd == 2 else TET_FACES
+    else:
+        faces = QUAD_FACES if nd == 2 else HEX_FACES
     ne = t.shape[0]
     nfe = len(faces)
     keys = {}

@@ -97,6 +97,45 @@ def convdiff3d(path, n=32, porder=4, pgauss=None, ncu=2, kappa=1e-2, vel=(1.0, 0
     return dict(model="convdiff", nd=3, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=ncu)
 
 
+def wmodel2d(path, model="wnl2d", n=4, porder=3, pgauss=None, mu=(1.0, 1.0, 1.0), alpha=0.0, beta=0.0,
+             tau=1.0, neumann=True, dt=None, torder=1, nstage=1, scheme=0, w0=1.0, u0=0.0,
+             precond="bjacobi", restart=50, gmres_tol=1e-12, gmres_maxit=1000, newton_iter=20,
+             newton_tol=1e-10):
+    """A model with a w equation (ncw = 1; models wnl2d, wsub2d, or wsub2d_ref without w) on the
+    unit square, n x n quads: condition 1 (u = 0) on y = 0 and x = 0, condition 2 (zero total
+    diffusive flux) on the other sides when ``neumann``, else condition 1 everywhere; physicsparam
+    ``mu``; app.bin factor[1:3] = (dae_alpha, dae_beta) = (alpha, beta); the initial state u = u0
+    and w = w0 at the nodes.  ``dt`` (a list of steps) makes the problem time dependent with DIRK
+    (scheme 0; torder, nstage) or BDF (scheme 1; torder)."""
+    os.makedirs(path, exist_ok=True)
+    pgauss = 2 * porder if pgauss is None else pgauss
+    m = dw.make_master(2, porder, pgauss)
+    p, t = dw.structured_grid([n, n])
+    if neumann:
+        exprs = [lambda v: np.all(np.abs(v[:, 1]) < 1e-8) or np.all(np.abs(v[:, 0]) < 1e-8), lambda v: True]
+    else:
+        exprs = [lambda v: True]
+    bnd = dw.boundary_face_tags(p, t, 2, exprs)
+    bcs = [1, 2] if neumann else [1]
+    bf = np.where(bnd >= 0, np.array(bcs)[np.maximum(bnd, 0)], 0)
+    ncw = 0 if model.endswith("_ref") else 1
+    tdep = dt is not None
+    dw.write_app(os.path.join(path, "app.bin"), nd=2, ncu=1, ncw=ncw,
+                 flag={0: 1 if tdep else 0, 6: PRECOND[precond] if isinstance(precond, str) else precond},
+                 problem={2: scheme, 3: torder, 4: 1 if scheme == 1 else nstage, 13: newton_iter,
+                          14: gmres_maxit, 15: restart},
+                 physicsparam=list(mu), tau=[tau], dt=list(dt) if tdep else [0.0],
+                 factor=(0.0, alpha, beta, 0.0, 0.0), solversparam=[newton_tol, gmres_tol, 1e-3, 0.0])
+    dw.write_master(os.path.join(path, "master.bin"), m)
+    dw.write_mesh(os.path.join(path, "mesh.bin"), m, t, bf, bcs)
+    xdg = dw.compute_dgnodes(m, p, t)
+    udg = np.zeros(xdg.shape[:2] + (3,))
+    udg[:, :, 0] = u0
+    wdg = np.full(xdg.shape[:2] + (1,), float(w0)) if ncw else None
+    dw.write_sol(os.path.join(path, "sol.bin"), m, xdg, ncu=1, nc=3, udg=udg, wdg=wdg)
+    return dict(model=model, nd=2, ne=t.shape[0], npe=m["npe"], npf=m["npf"], ncu=1, ncw=ncw)
+
+
 def convdiff_unsteady(path, nd=2, n=4, porder=3, pgauss=None, ncu=1, kappa=1.0, vel=None, tau=1.0,
                       dt=0.01, nsteps=5, torder=3, nstage=3, precond="bjacobi", restart=50,
                       gmres_tol=1e-12, gmres_maxit=1000, newton_iter=2, newton_tol=1e-10, scheme=0,

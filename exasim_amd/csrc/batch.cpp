@@ -92,6 +92,11 @@ Solver::Solver(const std::vector<std::string>& dins, const std::string& mname, i
     }
     append(sol.udg, u);
     append(sol.odg, so.odg);
+    append(sol.wdg, so.wdg);
+    if (i > 0 && (a.ndims.size() > 13 ? a.ndims[13] : 0.0) != (app.ndims.size() > 13 ? app.ndims[13] : 0.0))
+      throw Error(1, din + ": a batch needs one w-equation width (ncw)");
+    if (i > 0 && a.factor != app.factor)
+      throw Error(1, din + ": a batch needs one initial time and one w-equation alpha, beta (app factor)");
     uhs.push_back(so.uh);
     append(mesh.bf, me.bf);
     // connectivity with element, face and face-node offsets
@@ -319,6 +324,7 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
     seg_scale(segU.map, du.p, du.p, mask_d, 0, s);
     update_udg(d, du.p, 1.0, udg.p, s);
     get_q();
+    get_w();
     std::vector<double> nrm0 = nr;
     nr = residual_seg();
     std::vector<double> alpha(ns, 1.0);
@@ -341,6 +347,7 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
       seg_scale(segU.map, dus.p, du.p, coef_d, 0, s);
       update_udg(d, dus.p, 1.0, udg.p, s);
       get_q();
+      get_w();
       std::vector<double> nn = residual_seg();
       for (int i = 0; i < ns; ++i)
         if (coef[i] != 0.0) nr[i] = nn[i];
@@ -371,6 +378,7 @@ void Solver::newton_seg(int* newton_iters, int* gmres_total, double* final_norm)
         seg_scale(segU.map, du.p, du.p, mask_d, 0, s);
         update_udg(d, du.p, 1.0, udg.p, s);
         get_q();
+        get_w();
         std::vector<double> nn = residual_seg();
         for (int i = 0; i < ns; ++i)
           if (ref[i]) nr[i] = nn[i];

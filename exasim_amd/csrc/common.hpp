@@ -115,7 +115,7 @@ struct MeshData {
 };
 
 struct SolData {
-  std::vector<double> xdg, udg, odg, uh;
+  std::vector<double> xdg, udg, odg, wdg, uh;
   int udg_ncomp = 0, nco = 0;
 };
 

@@ -174,7 +174,8 @@ SolData read_sol(const std::string& path, int npe, int nc, int ncu, int ne) {
     if (sol.odg.size() % npene) throw Error(2, "Input files are incorrect (vdg size)");
     sol.nco = static_cast<int>(sol.odg.size() / npene);
   }
-  s.take(ns[4]);
+  sol.wdg = s.take(ns[4]);  // w (the w equation's field), [npe, ncw, ne]
+  if (sol.wdg.size() % npene) throw Error(2, "Input files are incorrect (wdg size)");
   sol.uh = s.take(ns[5]);
   return sol;
 }

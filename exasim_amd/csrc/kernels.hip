@@ -1081,6 +1081,103 @@ struct is_soa : std::false_type {};
 template <class M>
 struct is_soa<M, std::void_t<decltype(M::SOA)>> : std::true_type {};
 
+// Models with a w equation (NCW > 0: generated models with a Sourcew function) define flux_w,
+// source_w, fbou_w and sourcew.
+template <class M, class = void>
+struct model_ncw : std::integral_constant<int, 0> {};
+template <class M>
+struct model_ncw<M, std::void_t<decltype(M::NCW)>> : std::integral_constant<int, M::NCW> {};
+
+// B <- J^{-1} B for an N x N matrix J (column-major, overwritten) and NR right-hand sides
+// (B[i + N c]): Gaussian elimination with partial pivoting, in registers.
+template <int N, int NR>
+__device__ __forceinline__ void small_solve(double* J, double* B) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    int pr = k;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i)
+      if (fabs(J[i + N * k]) > fabs(J[pr + N * k])) pr = i;
+    if (pr != k) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double t = J[k + N * j];
+        J[k + N * j] = J[pr + N * j];
+        J[pr + N * j] = t;
+      }
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const double t = B[k + N * c];
+        B[k + N * c] = B[pr + N * c];
+        B[pr + N * c] = t;
+      }
+    }
+    const double inv = 1.0 / J[k + N * k];
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double l = J[i + N * k] * inv;
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) J[i + N * j] -= l * J[k + N * j];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) B[i + N * c] -= l * B[k + N * c];
+    }
+  }
+#pragma unroll
+  for (int k = N - 1; k >= 0; --k) {
+    const double inv = 1.0 / J[k + N * k];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      double t = B[k + N * c];
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) t -= J[k + N * j] * B[j + N * c];
+      B[k + N * c] = t * inv;
+    }
+  }
+}
+
+// The w equation at one point (wEquation, wequation.cpp:108-224): Newton on
+//   R(w) = Sourcew(x, uq, v, w) + alpha wsrc - (alpha dtfactor + beta) w = 0
+// from the guess in w, steps dw = J^{-1} R with J = (alpha dtfactor + beta) I - Sourcew_w, until
+// max |dw| < 1e-8 (at most 20 steps), then w_uq = J^{-1} Sourcew_uq ([j + NCW c]).  A point that
+// does not converge gets NaN, which the residual norm reports.
+template <class M>
+__device__ void w_point(const AsmArgs& a, const double* x, const double* uq, const double* v, const double* prm,
+                        const double* ws, double* w, double* w_uq) {
+  constexpr int NCW = model_ncw<M>::value, NC = M::NC;
+  const double scal = a.dae_alpha * a.dtfactor + a.dae_beta;
+  double sw[NCW], sw_uq[NCW * NC], sw_w[NCW * NCW], J[NCW * NCW];
+  bool conv = false;
+  for (int it = 0; it < 20 && !conv; ++it) {
+    M::sourcew(x, uq, v, w, prm, a.time, sw, sw_uq, sw_w);
+    double r[NCW];
+#pragma unroll
+    for (int i = 0; i < NCW; ++i) {
+      r[i] = sw[i] + a.dae_alpha * ws[i] - scal * w[i];
+#pragma unroll
+      for (int j = 0; j < NCW; ++j) J[i + NCW * j] = (i == j ? scal : 0.0) - sw_w[i + NCW * j];
+    }
+    small_solve<NCW, 1>(J, r);
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < NCW; ++i) {
+      w[i] += r[i];
+      mx = fmax(mx, fabs(r[i]));
+    }
+    conv = mx < 1e-8;
+  }
+  M::sourcew(x, uq, v, w, prm, a.time, sw, sw_uq, sw_w);
+#pragma unroll
+  for (int i = 0; i < NCW; ++i)
+#pragma unroll
+    for (int j = 0; j < NCW; ++j) J[i + NCW * j] = (i == j ? scal : 0.0) - sw_w[i + NCW * j];
+#pragma unroll
+  for (int k = 0; k < NCW * NC; ++k) w_uq[k] = sw_uq[k];
+  small_solve<NCW, NC>(J, w_uq);
+  if (!conv)
+#pragma unroll
+    for (int i = 0; i < NCW; ++i) w[i] = __builtin_nan("");
+}
+
 // physicsparam and tau of the instance element E belongs to (one instance: a.einst is null)
 __device__ __forceinline__ const double* inst_param(const AsmArgs& a, long E) {
   return a.einst ? a.param + static_cast<long>(a.einst[E]) * a.nparam : a.param;
@@ -1122,7 +1219,44 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
   double f[NCU * ND], s[NCU];
   double f_uq[JAC ? NCU * ND * NC : 1], s_uq[JAC ? NCU * NC : 1];
   const double* prm = inst_param(a, E);
-  if constexpr (is_soa<M>::value) {  // the model library's values at this point
+  if constexpr (model_ncw<M>::value > 0) {
+    // w at the point from its equation, and the chain rule through w_uq (uequation.cpp:105-165)
+    constexpr int NCW = model_ncw<M>::value;
+    double w[NCW], ws[NCW], w_uq[NCW * NC];
+#pragma unroll
+    for (int c = 0; c < NCW; ++c) w[c] = ws[c] = 0.0;
+    for (int i = 0; i < d.npe; ++i) {
+      const double phi = a.shapegt[g + d.nge * i];
+#pragma unroll
+      for (int c = 0; c < NCW; ++c) {
+        w[c] += phi * a.wdg[i + d.npe * (c + NCW * E)];
+        ws[c] += phi * a.wsrc[i + d.npe * (c + NCW * E)];
+      }
+    }
+    w_point<M>(a, x, uq, v, prm, ws, w, w_uq);
+    double f_w[JAC ? NCU * ND * NCW : 1], s_w[JAC ? NCU * NCW : 1];
+    M::template flux_w<JAC>(x, uq, v, w, prm, a.time, f, f_uq, f_w);
+    M::template source_w<JAC>(x, uq, v, w, prm, a.time, s, s_uq, s_w);
+    if (JAC) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int i = 0; i < NCU * ND; ++i) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < NCW; ++j) t += f_w[i + NCU * ND * j] * w_uq[j + NCW * c];
+          f_uq[i + NCU * ND * c] += t;
+        }
+#pragma unroll
+        for (int i = 0; i < NCU; ++i) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < NCW; ++j) t += s_w[i + NCU * j] * w_uq[j + NCW * c];
+          s_uq[i + NCU * c] += t;
+        }
+      }
+    }
+  } else if constexpr (is_soa<M>::value) {  // the model library's values at this point
     const double* o = a.pbuf + M::vol_in(d.nco) * P + p;
 #pragma unroll
     for (int i = 0; i < NCU * ND; ++i) f[i] = o[i * P];
@@ -1275,6 +1409,24 @@ __device__ __forceinline__ void face_state(const Dims& d, const AsmArgs& a, long
   for (int c = 0; c < d.nco && c < 8; ++c) v[c] = a.ogf[c * PF + pf];
 }
 
+// nodal w and wsrc of element E interpolated to point g of its face l
+template <class M>
+__device__ __forceinline__ void face_w(const Dims& d, const AsmArgs& a, long E, int l, int g, double* w,
+                                       double* ws) {
+  constexpr int NCW = model_ncw<M>::value;
+#pragma unroll
+  for (int c = 0; c < NCW; ++c) w[c] = ws[c] = 0.0;
+  for (int j = 0; j < d.npf; ++j) {
+    const double phi = a.shapfgt[g + d.ngf * j];
+    const int node = a.perm[j + d.npf * l];
+#pragma unroll
+    for (int c = 0; c < NCW; ++c) {
+      w[c] += phi * a.wdg[node + d.npe * (c + NCW * E)];
+      ws[c] += phi * a.wsrc[node + d.npe * (c + NCW * E)];
+    }
+  }
+}
+
 template <class M, bool JAC>
 __global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
   constexpr int ND = M::ND, NCU = M::NCU, NC = M::NC;
@@ -1296,7 +1448,27 @@ __global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
     uq[c] = uh[c];
   }
   double f[NCU * ND], f_uq[JAC ? NCU * ND * NC : 1];
-  if constexpr (is_soa<M>::value) {  // the model library's f(uhat, q) at this point
+  if constexpr (model_ncw<M>::value > 0) {
+    // w at the face point with (uhat, q) in the (u, q) slots (uequation.cpp:262-280): the u
+    // columns of w_uq are derivatives in uhat, as are those of f_uq here
+    constexpr int NCW = model_ncw<M>::value;
+    double w[NCW], ws[NCW], w_uq[NCW * NC];
+    face_w<M>(d, a, E, l, g, w, ws);
+    const double* prm = inst_param(a, E);
+    w_point<M>(a, x, uq, v, prm, ws, w, w_uq);
+    double f_w[JAC ? NCU * ND * NCW : 1];
+    M::template flux_w<JAC>(x, uq, v, w, prm, a.time, f, f_uq, f_w);
+    if (JAC)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < NCU * ND; ++i) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < NCW; ++j) t += f_w[i + NCU * ND * j] * w_uq[j + NCW * c];
+          f_uq[i + NCU * ND * c] += t;
+        }
+  } else if constexpr (is_soa<M>::value) {  // the model library's f(uhat, q) at this point
     const double* o = a.pbuf + M::face_in(d.nco) * P + p;
 #pragma unroll
     for (int i = 0; i < NCU * ND; ++i) f[i] = o[i * P];
@@ -1361,7 +1533,30 @@ __global__ __launch_bounds__(64) void k_face_bou_points(Dims d, AsmArgs a) {
   face_state<M>(d, a, E, l, g, uh, uq, x, nl, v, jac);
   const int ib = a.bf[l + d.nfe * E];
   double fb[NCU], fb_uq[JAC ? NCU * NC : 1], fb_uh[JAC ? NCU * NCU : 1];
-  if constexpr (is_soa<M>::value) {  // the model library's condition ib at this point
+  if constexpr (model_ncw<M>::value > 0) {
+    // the face point's w from (uhat, q), as in k_face_points; FbouHdg's derivative through w goes
+    // to uhat (u columns of w_uq) and to q
+    constexpr int NCW = model_ncw<M>::value;
+    double w[NCW], ws[NCW], w_uq[NCW * NC], uqh[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) uqh[c] = c < NCU ? uh[c] : uq[c];
+    face_w<M>(d, a, E, l, g, w, ws);
+    const double* prm = inst_param(a, E);
+    w_point<M>(a, x, uqh, v, prm, ws, w, w_uq);
+    double fb_w[JAC ? NCU * NCW : 1];
+    M::template fbou_w<JAC>(ib, x, uq, v, w, uh, nl, inst_tau(a, E), prm, a.time, fb, fb_uq, fb_uh, fb_w);
+    if (JAC)
+#pragma unroll
+      for (int r = 0; r < NCU; ++r)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < NCW; ++j) t += fb_w[r + NCU * j] * w_uq[j + NCW * c];
+          if (c < NCU) fb_uh[r + NCU * c] += t;
+          else fb_uq[r + NCU * c] += t;
+        }
+  } else if constexpr (is_soa<M>::value) {  // the model library's condition ib at this point
     const int nbc = a.plugin_nbc;
     const int b = (ib >= 1 && ib <= nbc) ? ib - 1 : 0;
     const double* o = a.pbuf + (M::face_in(d.nco) + NCU * ND + NCU * ND * NC + b * M::face_bc()) * P + p;
@@ -2498,6 +2693,45 @@ void plugin_points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStrea
   }
 }
 
+// GetW (wequation.cpp:226-240): the w equation at every element node with the nodal (u, q), the
+// nodal w as the first guess and the nodal wsrc; wdg is updated in place (one thread per node).
+template <class M>
+__global__ __launch_bounds__(256) void k_get_w(Dims d, AsmArgs a, const double* xdg, const double* odg,
+                                               double* wdg) {
+  constexpr int NCW = model_ncw<M>::value, NC = M::NC, ND = M::ND;
+  const long P = static_cast<long>(d.ne) * d.npe;
+  const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (p >= P) return;
+  const long E = p / d.npe;
+  const int i = static_cast<int>(p % d.npe);
+  double x[ND], uq[NC], v[8], w[NCW], ws[NCW], w_uq[NCW * NC];
+#pragma unroll
+  for (int b = 0; b < ND; ++b) x[b] = xdg[i + d.npe * (b + ND * E)];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) uq[c] = a.udg[i + d.npe * (c + NC * E)];
+  for (int c = 0; c < d.nco && c < 8; ++c) v[c] = odg[i + d.npe * (c + d.nco * E)];
+#pragma unroll
+  for (int c = 0; c < NCW; ++c) {
+    w[c] = wdg[i + d.npe * (c + NCW * E)];
+    ws[c] = a.wsrc[i + d.npe * (c + NCW * E)];
+  }
+  w_point<M>(a, x, uq, v, inst_param(a, E), ws, w, w_uq);
+#pragma unroll
+  for (int c = 0; c < NCW; ++c) wdg[i + d.npe * (c + NCW * E)] = w[c];
+}
+
+template <class G>
+void launch_get_w(const Dims& d, const AsmArgs& a, const double* xdg, const double* odg, double* wdg,
+                  hipStream_t s) {
+  if constexpr (model_ncw<G>::value > 0) {
+    const long P = static_cast<long>(d.ne) * d.npe;
+    k_get_w<G><<<nblocks(P, 256), 256, 0, s>>>(d, a, xdg, odg, wdg);
+    EXA_HIP(hipGetLastError());
+  } else {
+    throw Error(1, "the model has no w equation");
+  }
+}
+
 void points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) {
   if (d.nco > 8) throw Error(5, "at most 8 auxiliary components");
   switch (a.model) {
@@ -2519,6 +2753,28 @@ void points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStream_t s) 
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+void w_nodes(const Dims& d, const AsmArgs& a, const double* xdg, const double* odg, double* wdg, hipStream_t s) {
+  switch (a.model) {
+#define EXA_GEN_W(ID, NAME, G) \
+  case ID:                     \
+    return launch_get_w<G>(d, a, xdg, odg, wdg, s);
+    EXA_GENERATED_MODELS(EXA_GEN_W)
+#undef EXA_GEN_W
+    default: throw Error(1, "the model has no w equation");
+  }
+}
+
+int model_ncw_of(int model) {
+  switch (model) {
+#define EXA_GEN_NCW(ID, NAME, G) \
+  case ID:                       \
+    return model_ncw<G>::value;
+    EXA_GENERATED_MODELS(EXA_GEN_NCW)
+#undef EXA_GEN_NCW
+    default: return 0;
+  }
+}
+
 long plugin_buffer_doubles(const Dims& d, int nbc, int eb) {
   const long nd = d.nd, ncu = d.ncu, nc = d.nc, nco = d.nco;
   const long pv = static_cast<long>(eb) * d.nge, pf = static_cast<long>(eb) * d.nfe * d.ngf;

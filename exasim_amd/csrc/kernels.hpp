@@ -126,7 +126,15 @@ struct AsmArgs {
   const PluginFns* plugin;
   double* pbuf;
   int plugin_nbc;
+  // w equation (models with NCW > 0): nodal w (first guess) and its source, [npe, ncw, ne];
+  // alpha dw/dt + beta w = Sourcew with dw/dt ~ dtfactor w - wsrc
+  const double *wdg, *wsrc;
+  double dae_alpha, dae_beta;
 };
+// GetW: the w equation at every element node with the nodal (u, q) (wdg updated in place)
+void w_nodes(const Dims& d, const AsmArgs& a, const double* xdg, const double* odg, double* wdg, hipStream_t s);
+// the NCW of a generated model (0 when it has no w equation or is not a generated model)
+int model_ncw_of(int model);
 void vol_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
 void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s);
 // Ru[e*ldo + i + npe*k] = sum phiw * rv   (Ru block of element e0+e at out + e*ldo)

@@ -195,6 +195,16 @@ void Solver::init(SolData& sol) {
   newton_tol = app.solversparam[0];
   gmres_tol = app.solversparam[1];
   if (static_cast<int>(pr[27]) > 1) throw Error(5, "polynomial preconditioner (ppdegree > 1) is not supported");
+  // the w equation (wequation.cpp:31-240): ncw = ndims[13], dae_alpha = factor[1], dae_beta = factor[2]
+  ncw = app.ndims.size() > 13 ? static_cast<int>(app.ndims[13]) : 0;
+  dae_alpha = app.factor.size() > 1 ? app.factor[1] : 1.0;
+  dae_beta = app.factor.size() > 2 ? app.factor[2] : 0.0;
+  if (ncw != model_ncw_of(model))
+    throw Error(1, "app.bin has ncw = " + std::to_string(ncw) + " but the model '" + model_name + "' has " +
+                       std::to_string(model_ncw_of(model)) + " w components (a Sourcew function)");
+  if (ncw > 0 && static_cast<long>(sol.wdg.size()) != 0 &&
+      static_cast<long>(sol.wdg.size()) != static_cast<long>(d.npe) * ncw * d.ne)
+    throw Error(2, "wdg in sol.bin has the wrong size");
   if (static_cast<int>(pr[16]) > 0) throw Error(5, "reduced-basis initial guess (RBdim > 0) is not supported");
 
   // master tables, mesh, fields
@@ -266,6 +276,14 @@ void Solver::init(SolData& sol) {
   uh.upload(huh, st, s);
   udg0.upload(hudg, st, s);
   uh0.upload(huh, st, s);
+  if (ncw > 0) {  // initial w from sol.bin (zero when absent); wsrc starts at zero
+    std::vector<double> hw = sol.wdg;
+    if (hw.empty()) hw.assign(static_cast<size_t>(d.npe) * ncw * d.ne, 0.0);
+    wdg.upload(hw, st, s);
+    wdg0.upload(hw, st, s);
+    wsrc.alloc(hw.size());
+    wsrc.zero(s);
+  }
 
   // face-node gather sources for the atomic-free element -> face assembly
   long nn = static_cast<long>(d.npf) * d.nf;
@@ -351,6 +369,12 @@ void Solver::init(SolData& sol) {
     utmp.alloc(nu);
     sdg.zero(s);
     if (tscheme == 1) bdfhist.alloc(nu * torder);
+    if (ncw > 0) {
+      const size_t nw = static_cast<size_t>(d.npe) * ncw * d.ne;
+      wprev.alloc(nw);
+      wtmp.alloc(nw);
+      if (tscheme == 1) wbdfhist.alloc(nw * torder);
+    }
   }
 }
 
@@ -672,6 +696,10 @@ AsmArgs Solver::asm_args(int e0, int eb) {
     a.bf0 = lo;
     a.nbf = hi - lo;
   }
+  a.wdg = wdg.p;
+  a.wsrc = wsrc.p;
+  a.dae_alpha = dae_alpha;
+  a.dae_beta = dae_beta;
   a.plugin = model == MODEL_PLUGIN ? &plugin_fns : nullptr;
   a.pbuf = pbuf.p;
   a.plugin_nbc = plugin_fns.nbc;
@@ -753,6 +781,10 @@ int Solver::first_nonzero(const int* inf, long n) {
 void Solver::reset() {
   copy(udg.p, udg0.p, static_cast<long>(udg.n), s);
   copy(uh.p, uh0.p, static_cast<long>(uh.n), s);
+  if (ncw > 0) {
+    copy(wdg.p, wdg0.p, static_cast<long>(wdg.n), s);
+    wsrc.zero(s);
+  }
   tnow = time0;
   dtfactor = 0.0;
   if (tdep) sdg.zero(s);
@@ -760,6 +792,13 @@ void Solver::reset() {
 }
 
 void Solver::get_q() { getq(d, MCs.p, MEs.p, elemcon.p, uh.p, udg.p, s); }
+
+// GetW (solution.cpp:187-188): the nodal w from its equation with the nodal (u, q), after every
+// change of udg in the Newton loop and at each time stage
+void Solver::get_w() {
+  if (ncw == 0) return;
+  w_nodes(d, asm_args(0, d.ne), xdg.p, odg.p, wdg.p, s);
+}
 
 void Solver::assemble() {
   if (precond < 0 || precond > 2)
@@ -1061,6 +1100,11 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
     const double dt = dts[istep];
     u_axpby(d, udg.p, 1.0, 0.0, udgprev.p, s);          // udgprev = u
     axpby(utmp.p, udgprev.p, 1.0 - csum, 0.0, static_cast<long>(udgprev.n), s);
+    const long nw = static_cast<long>(wdg.n);
+    if (ncw > 0) {  // PreviousSolutions (previoussolutions.cpp:55-60): wprev = w, wtmp = (1 - sum c) w
+      copy(wprev.p, wdg.p, nw, s);
+      axpby(wtmp.p, wdg.p, 1.0 - csum, 0.0, nw, s);
+    }
     for (int j = 0; j < S; ++j) {
       tnow = t + dt * dirk_t[j];
       dtfactor = dirk_d[j * S + j] / dt;
@@ -1074,6 +1118,18 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
         u_axpby(d, udg.p, -dirk_d[S + 2] / dt, sc, sdg.p, s);
         axpby(sdg.p, udgprev.p, pc / dt, 1.0, static_cast<long>(sdg.n), s);
       }
+      if (ncw > 0) {  // the w equation's stage source (updatesource.cpp:78-100), as for u
+        if (j == 0) {
+          axpby(wsrc.p, wdg.p, dirk_d[0] / dt, 0.0, nw, s);
+        } else if (j == 1) {
+          axpby(wsrc.p, wdg.p, -dirk_d[1] / dt, (dirk_d[1] + dirk_d[S + 1]) / dirk_d[0], nw, s);
+        } else {
+          const double sc = dirk_d[2] / dirk_d[1];
+          const double pc = dirk_d[2] + dirk_d[S + 2] + dirk_d[2 * S + 2] - sc * (dirk_d[1] + dirk_d[S + 1]);
+          axpby(wsrc.p, wdg.p, -dirk_d[S + 2] / dt, sc, nw, s);
+          axpby(wsrc.p, wprev.p, pc / dt, 1.0, nw, s);
+        }
+      }
       int ni = 0, gi = 0;
       if (batch) newton_seg(&ni, &gi, &nrm);
       else newton(&ni, &gi, &nrm);
@@ -1081,6 +1137,11 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
       gtot += gi;
       u_axpby(d, udg.p, dirk_c[j], 1.0, utmp.p, s);       // utmp += c_j u
       if (j == S - 1) u_insert(d, utmp.p, udg.p, s);
+      if (ncw > 0) {  // UpdateSolution (updatesolution.cpp:142-172): GetW, wtmp += c_j w
+        get_w();
+        axpby(wtmp.p, wdg.p, dirk_c[j], 1.0, nw, s);
+        if (j == S - 1) copy(wdg.p, wtmp.p, nw, s);
+      }
       if (verbose) {
         std::printf("Timestep %zu, stage %d, time %g: Newton %d, GMRES %d, residual %g\n", istep + 1, j + 1,
                     tnow, ni, gi, nrm);
@@ -1117,6 +1178,15 @@ void Solver::solve_bdf(int* newton_iters, int* gmres_total, double* final_norm) 
     dtfactor = c[0] / dt;
     axpby(sdg.p, bdfhist.p, -c[1] / dt, 0.0, nu, s);
     for (int k = 1; k < q; ++k) axpby(sdg.p, bdfhist.p + k * nu, -c[k + 1] / dt, 1.0, nu, s);
+    if (ncw > 0) {  // the w equation's history source, as for u (the reference's UpdateSourceBDF
+                    // leaves wsrc unchanged for non-wave problems)
+      const long nw = static_cast<long>(wdg.n);
+      for (int k = std::min(nhist - 1, torder - 1); k >= 1; --k)
+        copy(wbdfhist.p + k * nw, wbdfhist.p + (k - 1) * nw, nw, s);
+      copy(wbdfhist.p, wdg.p, nw, s);
+      axpby(wsrc.p, wbdfhist.p, -c[1] / dt, 0.0, nw, s);
+      for (int k = 1; k < q; ++k) axpby(wsrc.p, wbdfhist.p + k * nw, -c[k + 1] / dt, 1.0, nw, s);
+    }
     tnow = t + dt;
     int ni = 0, gi = 0;
     if (batch) newton_seg(&ni, &gi, &nrm);
@@ -1162,6 +1232,7 @@ void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
     else dudg(d, RuS.p, DinvF.p, elemcon.p, x.p, du.p, s);
     update_udg(d, du.p, alpha, udg.p, s);
     get_q();
+    get_w();
     double nrm0 = nrmr;
     residual(&nb, &nru);
     nrmr = nb + nru;
@@ -1176,6 +1247,7 @@ void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
       copy(uh.p, u.p, N, s);
       update_udg(d, du.p, -alpha, udg.p, s);
       get_q();
+      get_w();
       residual(&nb, &nru);
       nrmr = nb + nru;
     }
@@ -1196,6 +1268,7 @@ void Solver::newton(int* newton_iters, int* gmres_total, double* final_norm) {
       dudg(d, RuS.p, DinvF32.p, elemcon.p, x.p, du.p, s);
       update_udg(d, du.p, 1.0, udg.p, s);
       get_q();
+      get_w();
       residual(&nb, &nru);
       nrmr = nb + nru;
       ++last_refinements;
@@ -1279,6 +1352,7 @@ long Solver::get_array(const std::string& name, double* dst, long len) {
   else if (name == "x") { src = x.p; size = static_cast<long>(x.n); }
   else if (name == "w") { src = wv.p; size = static_cast<long>(wv.n); }
   else if (name == "egeo") { src = egeo.p; size = static_cast<long>(egeo.n); }
+  else if (name == "wdg") { src = wdg.p; size = static_cast<long>(wdg.n); }
   else if (name == "fgeo") { src = fgeo.p; size = static_cast<long>(fgeo.n); }
   else if (name == "elemcon") from_int(conn.elemcon);
   else if (name == "f2e") from_int(conn.f2e);

@@ -60,6 +60,12 @@ struct Solver {
   double time0 = 0.0, tnow = 0.0, dtfactor = 0.0;
   double dirk_c[3] = {0, 0, 0}, dirk_d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dirk_t[3] = {0, 0, 0};
   DevBuf<double> sdg, udgprev, utmp, bdfhist;  // bdfhist: BDF history u^{n-k}, k < torder
+  // w equation (ncw > 0): nodal w, its source and initial state, the DIRK step's start and stage
+  // combination, the BDF history of w; alpha dw/dt + beta w = Sourcew (app.bin factor[1], [2])
+  int ncw = 0;
+  double dae_alpha = 1.0, dae_beta = 0.0;
+  DevBuf<double> wdg, wsrc, wdg0, wprev, wtmp, wbdfhist;
+  void get_w();
   bool timing = false;
   double ws_budget_bytes = 16.0e9;
 

@@ -606,9 +606,9 @@ def write_mesh(path, m, t, bf, boundary_conditions, cartgridpart=()):
         np.array([ne, 0, 0, 0], dtype=np.float64).tofile(f)          # intepartpts
 
 
-def write_sol(path, m, xdg, ncu, nc, udg=None, odg=None, uhat=None):
+def write_sol(path, m, xdg, ncu, nc, udg=None, odg=None, uhat=None, wdg=None):
     """writebinaryfiles.cpp:51-132 (``writesol``).  xdg[ne, npe, nd],
-    udg[ne, npe, ncu|nc], odg[ne, npe, nco] (natural order)."""
+    udg[ne, npe, ncu|nc], odg[ne, npe, nco], wdg[ne, npe, ncw] (natural order)."""
     ne, npe, nd = xdg.shape
     ndims = np.zeros(12)
     ndims[:12] = [ne, 0, m["nfe"], npe, m["npf"], nc, ncu, nc - ncu, 0,
@@ -618,6 +618,7 @@ def write_sol(path, m, xdg, ncu, nc, udg=None, odg=None, uhat=None):
     nsize[1] = xdg.size
     nsize[2] = 0 if udg is None else udg.size
     nsize[3] = 0 if odg is None else odg.size
+    nsize[4] = 0 if wdg is None else wdg.size
     nsize[5] = 0 if uhat is None else np.asarray(uhat).size
     with open(path, "wb") as f:
         np.array([nsize.size], dtype=np.float64).tofile(f)
@@ -629,6 +630,8 @@ def write_sol(path, m, xdg, ncu, nc, udg=None, odg=None, uhat=None):
             np.ascontiguousarray(np.transpose(udg, (0, 2, 1))).tofile(f)
         if odg is not None:
             np.ascontiguousarray(np.transpose(odg, (0, 2, 1))).tofile(f)
+        if wdg is not None:
+            np.ascontiguousarray(np.transpose(wdg, (0, 2, 1))).tofile(f)
         if uhat is not None:
             np.asarray(uhat, dtype=np.float64).tofile(f)
 

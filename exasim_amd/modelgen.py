@@ -409,8 +409,13 @@ def _walk(body):
 def emit_cpp(text, name, source="a pdemodel.txt"):
     header, funcs = parse(text)
     nd, ncu, nc, nco = model_sizes(header)
-    if header["vectors"].get("w", 0):
-        raise ModelSyntaxError("models with a w equation (ncw > 0) are not supported")
+    ncw = int(header["vectors"].get("w", 0))
+    if ncw and "Sourcew" not in funcs:
+        raise ModelSyntaxError("a model with w(ncw > 0) needs a Sourcew function (the w equation)")
+    if ncw > 5:
+        raise ModelSyntaxError("the w equation supports at most 5 components")
+    if ncw and funcs["Sourcew"].output[1] != ncw:
+        raise ModelSyntaxError("Sourcew must have ncw outputs")
     has_td = "Tdfunc" in funcs
     if has_td and funcs["Tdfunc"].output[1] != ncu:
         raise ModelSyntaxError("Tdfunc must have ncu outputs")
@@ -422,14 +427,15 @@ def emit_cpp(text, name, source="a pdemodel.txt"):
     ident = re.sub(r"\W", "_", name)
     parts = ["// Generated by exasim_amd/modelgen.py from %s -- do not edit." % source,
              "#pragma once", "#include \"dual.hpp\"", "", "namespace exa {", "namespace gen_%s {" % ident, ""]
-    for fname in _call_order(funcs, ["Flux", "Source", "FbouHdg"] + (["Tdfunc"] if has_td else [])):
+    for fname in _call_order(funcs, ["Flux", "Source", "FbouHdg"] + (["Tdfunc"] if has_td else []) +
+                             (["Sourcew"] if ncw else [])):
         parts.append(_Emitter(funcs, fname).emit(header))
         parts.append("")
     vec = header["vectors"]
 
-    def argv(fn, kind):
+    def argv(fn, kind, wname="W0"):
         """call arguments of a generated function from the kernel-side names."""
-        m = {"x": "X", "uq": "U", "v": "V", "w": "W0", "uhat": "UH", "n": "NL", "tau": "TAU", "mu": "MU",
+        m = {"x": "X", "uq": "U", "v": "V", "w": wname, "uhat": "UH", "n": "NL", "tau": "TAU", "mu": "MU",
              "eta": "ETA", "t": "T"}
         out = []
         for a in funcs[fn].args:
@@ -444,16 +450,17 @@ def emit_cpp(text, name, source="a pdemodel.txt"):
     nv = vec.get("v", 0)
     parts.append("// %s: nd = %d, ncu = %d, nc = %d, nv = %d, %d boundary conditions" % (name, nd, ncu, nc, nv, nbc))
     parts.append("struct Gen_%s {" % ident)
-    parts.append("  static constexpr int ND = %d, NCU = %d, NC = %d, NV = %d, NBC = %d;" % (nd, ncu, nc, nv, nbc))
-    prologue = """    S X[{nx}], V[{nv}], W0[1], MU[{nmu}], ETA[{neta}], TAU[{ntau}], NL[{nn}];
+    parts.append("  static constexpr int ND = %d, NCU = %d, NC = %d, NV = %d, NBC = %d, NCW = %d;"
+                 % (nd, ncu, nc, nv, nbc, ncw))
+    prologue = """    S X[{nx}], V[{nv}], W0[{nw}], MU[{nmu}], ETA[{neta}], TAU[{ntau}], NL[{nn}];
     for (int i = 0; i < {nx}; ++i) X[i] = S(x[i]);
     for (int i = 0; i < {nv}; ++i) V[i] = S(i < {nvr} ? v[i] : 0.0);
-    W0[0] = S(0.0);
+    for (int i = 0; i < {nw}; ++i) W0[i] = S(0.0);
     for (int i = 0; i < {nmu}; ++i) MU[i] = S(p[i]);
     for (int i = 0; i < {neta}; ++i) ETA[i] = S(0.0);
     for (int i = 0; i < {ntau}; ++i) TAU[i] = S(0.0);
     for (int i = 0; i < {nn}; ++i) NL[i] = S(0.0);
-    const S T = S(t);""".format(nx=sz("x"), nv=sz("v"), nvr=vec.get("v", 0), nmu=sz("mu"), neta=sz("eta"),
+    const S T = S(t);""".format(nx=sz("x"), nv=sz("v"), nvr=vec.get("v", 0), nw=sz("w"), nmu=sz("mu"), neta=sz("eta"),
                                 ntau=sz("tau"), nn=sz("n"))
     # flux
     parts.append("  template <bool JAC>")
@@ -547,10 +554,110 @@ def emit_cpp(text, name, source="a pdemodel.txt"):
     parts.append("    if constexpr (JAC) fbou_pass<0, KB>(ib, x, uq, v, uh, nl, tau, p, t, fb, fb_uq, fb_uh);")
     parts.append("    else fbou_pass<0, 0>(ib, x, uq, v, uh, nl, tau, p, t, fb, fb_uq, fb_uh);")
     parts.append("  }")
+    if ncw:
+        parts += _emit_w_methods(ident, prologue, argv, sz)
     parts.append("};")
     parts.append("")
     parts.append("}  // namespace exa")
     return "\n".join(parts) + "\n", dict(name=name, ident=ident, nd=nd, ncu=ncu, nc=nc, nv=nv, nbc=nbc)
+
+
+def _emit_w_methods(ident, prologue, argv, sz):
+    """The w-equation forms (wequation.cpp:31-240) of a model with ncw > 0: flux_w, source_w,
+    fbou_w take the point's w and give the derivatives in w as well (dual directions: uq, then w;
+    for fbou_w: uq, uhat, w), and sourcew evaluates Sourcew with its derivatives in uq and w."""
+    out = []
+    pw = prologue + """
+    S W[NCW];"""
+
+    def head(name, args):
+        out.append("  template <bool JAC>")
+        out.append("  __device__ static void %s(%s) {" % (name, args))
+
+    # flux_w
+    head("flux_w", "const double* x, const double* uq, const double* v, const double* w, const double* p, "
+                   "double t, double* f, double* f_uq, double* f_w")
+    out.append("    using S = typename std::conditional<JAC, Dual<NC + NCW>, Dual<0>>::type;")
+    out.append(pw)
+    out.append("    for (int k = 0; k < NCW; ++k) { W[k] = S(w[k]); if (JAC) W[k].seed(NC + k); }")
+    out.append("    S U[NC];")
+    out.append("    for (int k = 0; k < NC; ++k) { U[k] = S(uq[k]); if (JAC) U[k].seed(k); }")
+    out.append("    S F[NCU * ND];")
+    out.append("    gen_%s::fn_Flux<S>(%s, F);" % (ident, argv("Flux", "flux", "W")))
+    out.append("    for (int i = 0; i < NCU * ND; ++i) {")
+    out.append("      f[i] = F[i].v;")
+    out.append("      if (JAC) {")
+    out.append("        for (int k = 0; k < NC; ++k) f_uq[i + NCU * ND * k] = F[i].deriv(k);")
+    out.append("        for (int k = 0; k < NCW; ++k) f_w[i + NCU * ND * k] = F[i].deriv(NC + k);")
+    out.append("      }")
+    out.append("    }")
+    out.append("  }")
+    # source_w
+    head("source_w", "const double* x, const double* uq, const double* v, const double* w, const double* p, "
+                     "double t, double* s, double* s_uq, double* s_w")
+    out.append("    using S = typename std::conditional<JAC, Dual<NC + NCW>, Dual<0>>::type;")
+    out.append(pw)
+    out.append("    for (int k = 0; k < NCW; ++k) { W[k] = S(w[k]); if (JAC) W[k].seed(NC + k); }")
+    out.append("    S U[NC];")
+    out.append("    for (int k = 0; k < NC; ++k) { U[k] = S(uq[k]); if (JAC) U[k].seed(k); }")
+    out.append("    S F[NCU];")
+    out.append("    gen_%s::fn_Source<S>(%s, F);" % (ident, argv("Source", "source", "W")))
+    out.append("    for (int i = 0; i < NCU; ++i) {")
+    out.append("      s[i] = F[i].v;")
+    out.append("      if (JAC) {")
+    out.append("        for (int k = 0; k < NC; ++k) s_uq[i + NCU * k] = F[i].deriv(k);")
+    out.append("        for (int k = 0; k < NCW; ++k) s_w[i + NCU * k] = F[i].deriv(NC + k);")
+    out.append("      }")
+    out.append("    }")
+    out.append("  }")
+    # sourcew (always with its derivatives: the point Newton needs them)
+    out.append("  __device__ static void sourcew(const double* x, const double* uq, const double* v, const double* w,")
+    out.append("                                 const double* p, double t, double* sw, double* sw_uq, double* sw_w) {")
+    out.append("    using S = Dual<NC + NCW>;")
+    out.append(pw)
+    out.append("    for (int k = 0; k < NCW; ++k) { W[k] = S(w[k]); W[k].seed(NC + k); }")
+    out.append("    S U[NC];")
+    out.append("    for (int k = 0; k < NC; ++k) { U[k] = S(uq[k]); U[k].seed(k); }")
+    out.append("    S F[NCW];")
+    out.append("    gen_%s::fn_Sourcew<S>(%s, F);" % (ident, argv("Sourcew", "sourcew", "W")))
+    out.append("    for (int i = 0; i < NCW; ++i) {")
+    out.append("      sw[i] = F[i].v;")
+    out.append("      for (int k = 0; k < NC; ++k) sw_uq[i + NCW * k] = F[i].deriv(k);")
+    out.append("      for (int k = 0; k < NCW; ++k) sw_w[i + NCW * k] = F[i].deriv(NC + k);")
+    out.append("    }")
+    out.append("  }")
+    # fbou_w: one pass over all NC + NCU + NCW directions (the dual numbers stay within the limits
+    # of models small enough to have a w equation here)
+    head("fbou_w", "int ib, const double* x, const double* uq, const double* v, const double* w, "
+                   "const double* uh, const double* nl, const double* tau, const double* p, double t, double* fb, "
+                   "double* fb_uq, double* fb_uh, double* fb_w")
+    out.append("    constexpr int NTW = NC + NCU + NCW;")
+    out.append("    using S = typename std::conditional<JAC, Dual<NTW>, Dual<0>>::type;")
+    out.append(pw)
+    out.append("    for (int i = 0; i < %d; ++i) TAU[i] = S(tau[i]);" % sz("tau"))
+    out.append("    for (int i = 0; i < %d; ++i) NL[i] = S(i < ND ? nl[i] : 0.0);" % sz("n"))
+    out.append("    S U[NC], UH[NCU];")
+    out.append("    for (int k = 0; k < NC; ++k) { U[k] = S(uq[k]); if (JAC) U[k].seed(k); }")
+    out.append("    for (int k = 0; k < NCU; ++k) { UH[k] = S(uh[k]); if (JAC) UH[k].seed(NC + k); }")
+    out.append("    for (int k = 0; k < NCW; ++k) { W[k] = S(w[k]); if (JAC) W[k].seed(NC + NCU + k); }")
+    out.append("    S F[NCU * NBC];")
+    out.append("    gen_%s::fn_FbouHdg<S>(%s, F);" % (ident, argv("FbouHdg", "fbou", "W")))
+    out.append("    const int sel = (ib >= 1 && ib <= NBC) ? ib - 1 : 0;")
+    out.append("#pragma unroll")
+    out.append("    for (int b = 0; b < NBC; ++b) {")
+    out.append("      if (b != sel) continue;")
+    out.append("#pragma unroll")
+    out.append("      for (int r = 0; r < NCU; ++r) {")
+    out.append("        fb[r] = F[b * NCU + r].v;")
+    out.append("        if (JAC) {")
+    out.append("          for (int k = 0; k < NC; ++k) fb_uq[r + NCU * k] = F[b * NCU + r].deriv(k);")
+    out.append("          for (int k = 0; k < NCU; ++k) fb_uh[r + NCU * k] = F[b * NCU + r].deriv(NC + k);")
+    out.append("          for (int k = 0; k < NCW; ++k) fb_w[r + NCU * k] = F[b * NCU + r].deriv(NC + NCU + k);")
+    out.append("        }")
+    out.append("      }")
+    out.append("    }")
+    out.append("  }")
+    return out
 
 
 def write_registry():

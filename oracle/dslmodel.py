@@ -222,9 +222,14 @@ class DSLModel:
         if nd is not None and nd != self.nd or ncu is not None and ncu != self.ncu:
             raise ValueError("model file sizes differ from the problem's")
         self.nbc = self.funcs["FbouHdg"].output[1] // self.ncu
+        self.ncw = int(self.header["vectors"].get("w", 0))
+        if self.ncw and "Sourcew" not in self.funcs:
+            raise ValueError("a model with w(ncw > 0) needs a Sourcew function")
         self.name = "dsl"
 
-    def _args(self, fname, P, nv, x, uq, v, param, time, uh=None, nl=None, tau=None):
+    def _args(self, fname, P, nv, x, uq, v, param, time, uh=None, nl=None, tau=None, w=None, woff=0):
+        """Dual-number arguments: uq seeded in directions 0..nc-1, uhat in nc..nc+ncu-1, w (when
+        given) in woff..woff+ncw-1."""
         vec = self.header["vectors"]
         out = []
         for a in self.funcs[fname].args:
@@ -244,6 +249,8 @@ class DSLModel:
             elif a == "mu":
                 out.append([D.const(param[i] if i < len(param) else 0.0, nv, P)
                             for i in range(max(vec.get("mu", 0), 1))])
+            elif a == "w" and w is not None:
+                out.append([D(w[:, i].copy(), self._seed(nv, P, woff + i)) for i in range(self.ncw)])
             elif a in ("w", "eta"):
                 out.append([D.const(0.0, nv, P) for _ in range(max(vec.get(a, 0), 1))])
             elif a == "t":
@@ -258,19 +265,41 @@ class DSLModel:
             g[k] = 1.0
         return g
 
-    def flux(self, x, uq, v, param, time):
+    def flux(self, x, uq, v, param, time, w=None):
+        """f [P, ncu, nd], f_uq [P, ncu, nd, nc] (and f_w [P, ncu, nd, ncw] when w is given)."""
         P, nd, ncu, nc = x.shape[0], self.nd, self.ncu, self.nc
-        F = _Interp(self.funcs, self.header, nc, P).run("Flux", self._args("Flux", P, nc, x, uq, v, param, time))
+        nv = nc + (self.ncw if w is not None else 0)
+        F = _Interp(self.funcs, self.header, nv, P).run(
+            "Flux", self._args("Flux", P, nv, x, uq, v, param, time, w=w, woff=nc))
         f = np.stack([F[c + ncu * d].v for d in range(nd) for c in range(ncu)], 1).reshape(P, nd, ncu)
-        g = np.stack([F[c + ncu * d].g for d in range(nd) for c in range(ncu)], 0).reshape(nd, ncu, nc, P)
-        return f.transpose(0, 2, 1), g.transpose(3, 1, 0, 2)
+        g = np.stack([F[c + ncu * d].g for d in range(nd) for c in range(ncu)], 0).reshape(nd, ncu, nv, P)
+        g = g.transpose(3, 1, 0, 2)
+        if w is None:
+            return f.transpose(0, 2, 1), g
+        return f.transpose(0, 2, 1), g[..., :nc], g[..., nc:]
 
-    def source(self, x, uq, v, param, time):
+    def source(self, x, uq, v, param, time, w=None):
+        """s [P, ncu], s_uq [P, ncu, nc] (and s_w [P, ncu, ncw] when w is given)."""
         P, ncu, nc = x.shape[0], self.ncu, self.nc
-        S = _Interp(self.funcs, self.header, nc, P).run("Source", self._args("Source", P, nc, x, uq, v, param, time))
-        s = np.stack([S[c].v for c in range(ncu)], 1)
-        g = np.stack([S[c].g for c in range(ncu)], 0)          # [ncu, nc, P]
-        return s, g.transpose(2, 0, 1)
+        nv = nc + (self.ncw if w is not None else 0)
+        S = _Interp(self.funcs, self.header, nv, P).run(
+            "Source", self._args("Source", P, nv, x, uq, v, param, time, w=w, woff=nc))
+        s = np.stack([np.broadcast_to(S[c].v, (P,)) for c in range(ncu)], 1)
+        g = np.stack([S[c].g for c in range(ncu)], 0).transpose(2, 0, 1)          # [P, ncu, nv]
+        if w is None:
+            return s, g
+        return s, g[..., :nc], g[..., nc:]
+
+    def sourcew(self, x, uq, v, w, param, time):
+        """Sourcew (the w-equation's right-hand side): sw [P, ncw], sw_uq [P, ncw, nc],
+        sw_w [P, ncw, ncw]."""
+        P, nc, ncw = x.shape[0], self.nc, self.ncw
+        nv = nc + ncw
+        S = _Interp(self.funcs, self.header, nv, P).run(
+            "Sourcew", self._args("Sourcew", P, nv, x, uq, v, param, time, w=w, woff=nc))
+        sw = np.stack([np.broadcast_to(S[c].v, (P,)) for c in range(ncw)], 1)
+        g = np.stack([S[c].g for c in range(ncw)], 0).transpose(2, 0, 1)        # [P, ncw, nv]
+        return sw, g[..., :nc], g[..., nc:]
 
     def tdfunc(self, x, uq, v, param, time):
         """Tdfunc values [N, ncu] (the time-derivative coefficient; ones when the file has no
@@ -281,12 +310,17 @@ class DSLModel:
         T = _Interp(self.funcs, self.header, nc, P).run("Tdfunc", self._args("Tdfunc", P, nc, x, uq, v, param, time))
         return np.stack([np.broadcast_to(T[c].v, (P,)) for c in range(ncu)], 1)
 
-    def fbou(self, ib, x, uq, v, uh, nl, tau, param, time):
+    def fbou(self, ib, x, uq, v, uh, nl, tau, param, time, w=None):
+        """fb [P, ncu], fb_uq, fb_uh (and fb_w [P, ncu, ncw] when w is given)."""
         P, ncu, nc = x.shape[0], self.ncu, self.nc
-        nv = nc + ncu
+        nv = nc + ncu + (self.ncw if w is not None else 0)
         Fb = _Interp(self.funcs, self.header, nv, P).run(
-            "FbouHdg", self._args("FbouHdg", P, nv, x, uq, v, param, time, uh=uh, nl=nl, tau=tau))
+            "FbouHdg", self._args("FbouHdg", P, nv, x, uq, v, param, time, uh=uh, nl=nl, tau=tau, w=w,
+                                  woff=nc + ncu))
         o = (ib - 1 if 1 <= ib <= self.nbc else 0) * ncu
-        fb = np.stack([Fb[o + c].v for c in range(ncu)], 1)
+        fb = np.stack([np.broadcast_to(Fb[o + c].v, (P,)) for c in range(ncu)], 1)
         g = np.stack([Fb[o + c].g for c in range(ncu)], 0)    # [ncu, nv, P]
-        return fb, g[:, :nc].transpose(2, 0, 1), g[:, nc:].transpose(2, 0, 1)
+        out = fb, g[:, :nc].transpose(2, 0, 1), g[:, nc:nc + ncu].transpose(2, 0, 1)
+        if w is None:
+            return out
+        return out + (g[:, nc + ncu:].transpose(2, 0, 1),)

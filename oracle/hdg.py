@@ -113,6 +113,8 @@ def read_sol(path, npe, nc, ncu, ne):
         out["udg"] = None
     o = segs[3]
     out["odg"] = o.reshape(ne, o.size // (npe * ne), npe).transpose(0, 2, 1).copy() if o.size else None
+    w = segs[4]
+    out["wdg"] = w.reshape(ne, w.size // (npe * ne), npe).transpose(0, 2, 1).copy() if w.size else None
     out["uh"] = segs[5].copy() if segs[5].size else None
     return out
 
@@ -355,10 +357,21 @@ class HDGProblem:
         self.dt = np.asarray(app["dt"], dtype=float)
         self.time = float(app["factor"][0]) if len(app["factor"]) else 0.0
         self.dtfactor = 0.0
+        # w-equation (wequation.cpp:31-240): ncw = ndims[13], alpha dw/dt + beta w = Sourcew with
+        # dae_alpha = factor[1], dae_beta = factor[2] (setstructs.cpp:196-197)
+        self.ncw = int(nd_app[13]) if len(nd_app) > 13 else 0
+        fac = app["factor"]
+        self.dae_alpha = float(fac[1]) if len(fac) > 1 else 1.0
+        self.dae_beta = float(fac[2]) if len(fac) > 2 else 0.0
+        if int(flag[1]) != 0:
+            raise NotImplementedError("wave problems")
         self.xdg = sol["xdg"]
         self.udg = sol["udg"] if sol["udg"] is not None else np.zeros((self.ne, self.npe, self.nc))
         self.sdg = np.zeros((self.ne, self.npe, self.ncu))       # DIRK stage source (ncs = ncu)
         self.odg = sol["odg"] if sol["odg"] is not None else np.zeros((self.ne, self.npe, 0))
+        if self.ncw:
+            self.wdg = sol["wdg"] if sol["wdg"] is not None else np.zeros((self.ne, self.npe, self.ncw))
+            self.wsrc = np.zeros((self.ne, self.npe, self.ncw))
         self.xg, self.Xx, self.jac = elem_geom(master, self.xdg)
         self.xgf, self.nlf, self.jacf = elemface_geom(master, self.xdg, self.perm)
         self.Mass, self.Minv, self.MinvC, self.MinvE = q_equation(
@@ -395,6 +408,39 @@ class HDGProblem:
             udg[:, :, ncu * (d + 1):ncu * (d + 2)] = q
 
     # ------------------------------------------------------------------
+    def w_solve(self, x, uq, v, w0, wsrc):
+        """wEquation (wequation.cpp:108-224) at N points: Newton on
+        R(w) = Sourcew(u, q, w) + alpha wsrc - (alpha dtfactor + beta) w = 0 from w0, steps
+        dw = J^{-1} R with J = (alpha dtfactor + beta) I - dSourcew/dw, until max |dw| < 1e-8 (at
+        most 20 steps); then w_uq = J^{-1} dSourcew/duq.  Returns w [N, ncw], w_uq [N, ncw, nc]."""
+        scal = self.dae_alpha * self.dtfactor + self.dae_beta
+        w = np.array(w0, dtype=float, copy=True)
+        eye = np.eye(self.ncw)[None]
+        for _ in range(20):
+            sw, sw_uq, sw_w = self.model.sourcew(x, uq, v, w, self.param, self.time)
+            R = sw + self.dae_alpha * wsrc - scal * w
+            J = scal * eye - sw_w
+            dw = np.linalg.solve(J, R[..., None])[..., 0]
+            w = w + dw
+            if np.abs(dw).max() < 1e-8:
+                break
+        else:
+            raise FloatingPointError("Newton in the w equation does not converge to 1e-8")
+        sw, sw_uq, sw_w = self.model.sourcew(x, uq, v, w, self.param, self.time)
+        w_uq = np.linalg.solve(scal * eye - sw_w, sw_uq)
+        return w, w_uq
+
+    def get_w(self):
+        """GetW (wequation.cpp:226-240): the w equation at the element nodes with the nodal
+        (u, q), the nodal w as the first guess and the nodal wsrc; updates wdg."""
+        if not self.ncw:
+            return
+        N = self.ne * self.npe
+        w, _ = self.w_solve(self.xdg.reshape(N, self.nd), self.udg.reshape(N, self.nc),
+                            self.odg.reshape(N, -1), self.wdg.reshape(N, self.ncw),
+                            self.wsrc.reshape(N, self.ncw))
+        self.wdg = w.reshape(self.ne, self.npe, self.ncw)
+
     def _volume(self, udg, jacobian):
         """uEquationElemBlock (uequation.cpp:45-203) / RuEquationElemBlock (:790-880)."""
         m_ = self.master
@@ -405,7 +451,14 @@ class HDGProblem:
         x = self.xg.reshape(N, nd)
         uq = uqg.reshape(N, nc)
         v = self.og.reshape(N, -1)
-        s, s_uq = self.model.source(x, uq, v, self.param, self.time)
+        if self.ncw:  # w at the points, and the chain rule through w_uq (uequation.cpp:105-165)
+            w0 = np.einsum("gi,eic->egc", phi[0], self.wdg).reshape(N, self.ncw)
+            wsg = np.einsum("gi,eic->egc", phi[0], self.wsrc).reshape(N, self.ncw)
+            w, w_uq = self.w_solve(x, uq, v, w0, wsg)
+            s, s_uq, s_w = self.model.source(x, uq, v, self.param, self.time, w=w)
+            s_uq = s_uq + np.einsum("nkw,nwc->nkc", s_w, w_uq)
+        else:
+            s, s_uq = self.model.source(x, uq, v, self.param, self.time)
         if self.tdep:
             # uequation.cpp:126-151: s += (sdg - dtfactor*u) T, s_u -= dtfactor T, with T the
             # model's Tdfunc values when flag tdfunc is 1 and ones otherwise (the derivative of T
@@ -418,7 +471,11 @@ class HDGProblem:
             s_uq = s_uq.copy()
             for c in range(ncu):
                 s_uq[:, c, c] -= self.dtfactor * T[:, c]
-        f, f_uq = self.model.flux(x, uq, v, self.param, self.time)
+        if self.ncw:
+            f, f_uq, f_w = self.model.flux(x, uq, v, self.param, self.time, w=w)
+            f_uq = f_uq + np.einsum("nkdw,nwc->nkdc", f_w, w_uq)
+        else:
+            f, f_uq = self.model.flux(x, uq, v, self.param, self.time)
         ng = phi.shape[1]
         s = s.reshape(ne, ng, ncu)
         f = f.reshape(ne, ng, ncu, nd)
@@ -455,15 +512,28 @@ class HDGProblem:
         uhv = uhg.reshape(N, ncu)
         u_save = uq[:, :ncu].copy()
         uq[:, :ncu] = uhv
-        f, f_uq = self.model.flux(x, uq, v, self.param, self.time)
-        fh = np.einsum("nkd,nd->nk", f, nl) + self.tau[0] * (u_save - uhv)
         jacf = self.jacf.reshape(N)
         out = dict(x=x, nl=nl, v=v, uq=udgf.reshape(N, nc), uh=uhv, jac=jacf)
+        if self.ncw:
+            # w at the face points with (uhat, q) in place of (u, q) (uequation.cpp:262-280): its
+            # derivative in the u slot is a derivative in uhat
+            w0 = np.einsum("gj,eljc->elgc", phif[0], self.wdg[:, self.perm, :]).reshape(N, self.ncw)
+            wsf = np.einsum("gj,eljc->elgc", phif[0], self.wsrc[:, self.perm, :]).reshape(N, self.ncw)
+            w, w_uq = self.w_solve(x, uq, v, w0, wsf)
+            out["w"], out["w_uq"] = w, w_uq
+            f, f_uq, f_w = self.model.flux(x, uq, v, self.param, self.time, w=w)
+        else:
+            f, f_uq = self.model.flux(x, uq, v, self.param, self.time)
+        fh = np.einsum("nkd,nd->nk", f, nl) + self.tau[0] * (u_save - uhv)
         out["fh"] = fh * jacf[:, None]
         if jacobian:
             fh_uq = np.einsum("nkdc,nd->nkc", f_uq, nl)
             fh_uh = fh_uq[:, :, :ncu].copy()
             fh_uq[:, :, :ncu] = 0.0
+            if self.ncw:  # fh_uh += fh_w w_u, fh_q += fh_w w_q
+                fh_w = np.einsum("nkdw,nd->nkw", f_w, nl)
+                fh_uh += np.einsum("nkw,nwc->nkc", fh_w, w_uq[:, :, :ncu])
+                fh_uq[:, :, ncu:] += np.einsum("nkw,nwc->nkc", fh_w, w_uq[:, :, ncu:])
             for k in range(ncu):
                 fh_uq[:, k, k] += self.tau[0]
                 fh_uh[:, k, k] -= self.tau[0]
@@ -480,9 +550,18 @@ class HDGProblem:
         for ib in np.unique(bf[bf > 0]):
             ee, ll = np.nonzero(bf == ib)
             idx = ((ee * nfe + ll)[:, None] * ngf + np.arange(ngf)[None, :]).ravel()
-            fb, fb_uq, fb_uh = self.model.fbou(int(ib), ff["x"][idx], ff["uq"][idx], ff["v"][idx],
-                                               ff["uh"][idx], ff["nl"][idx], self.tau,
-                                               self.param, self.time)
+            if self.ncw:  # the face points' w (from (uhat, q)) and the same chain rule
+                fb, fb_uq, fb_uh, fb_w = self.model.fbou(int(ib), ff["x"][idx], ff["uq"][idx], ff["v"][idx],
+                                                         ff["uh"][idx], ff["nl"][idx], self.tau,
+                                                         self.param, self.time, w=ff["w"][idx])
+                wu = ff["w_uq"][idx]
+                fb_uh = fb_uh + np.einsum("nkw,nwc->nkc", fb_w, wu[:, :, :ncu])
+                fb_uq = fb_uq.copy()
+                fb_uq[:, :, ncu:] += np.einsum("nkw,nwc->nkc", fb_w, wu[:, :, ncu:])
+            else:
+                fb, fb_uq, fb_uh = self.model.fbou(int(ib), ff["x"][idx], ff["uq"][idx], ff["v"][idx],
+                                                   ff["uh"][idx], ff["nl"][idx], self.tau,
+                                                   self.param, self.time)
             jb = ff["jac"][idx]
             res[int(ib)] = (ee, ll, fb * jb[:, None], fb_uq * jb[:, None, None],
                             fb_uh * jb[:, None, None])
@@ -836,6 +915,7 @@ def newton_solve(prob, log=None):
         prob.update_udg(prob.udg, du, alpha)
         if ncq > 0:
             prob.get_q(prob.udg, prob.uh)
+        prob.get_w()
         nrm0 = nrmr
         b, Ru = prob.residual()
         nrmr = np.linalg.norm(b) + np.linalg.norm(Ru)
@@ -848,6 +928,7 @@ def newton_solve(prob, log=None):
             prob.update_udg(prob.udg, du, -alpha)
             if ncq > 0:
                 prob.get_q(prob.udg, prob.uh)
+            prob.get_w()
             b, Ru = prob.residual()
             nrmr = np.linalg.norm(b) + np.linalg.norm(Ru)
         hist["norms"].append(nrmr)
@@ -902,21 +983,28 @@ def dirk_coeffs(nstage, torder):
     return b @ Ai, Ai.T.reshape(-1), A.sum(1)
 
 
-def update_source_dirk(P, stage, dt, d, udgprev):
-    """UpdateSourceDIRK (updatesource.cpp:32-72) for ncs = ncu."""
-    S = P.tstages
-    Rq = P.udg[:, :, :P.ncu]
-    P.dtfactor = d[stage * S + stage] / dt
+def _dirk_source(src, cur, prev, stage, dt, d, S):
+    """The stage source of UpdateSourceDIRK for one field: src the previous stage's source, cur
+    the current solution, prev the step's starting solution."""
     if stage == 0:
-        P.sdg = (d[0] / dt) * Rq
-    elif stage == 1:
-        P.sdg = ((d[1] + d[S + 1]) / d[0]) * P.sdg - (d[1] / dt) * Rq
-    elif stage == 2:
+        return (d[0] / dt) * cur
+    if stage == 1:
+        return ((d[1] + d[S + 1]) / d[0]) * src - (d[1] / dt) * cur
+    if stage == 2:
         sc = d[2] / d[1]
         pc = d[2] + d[S + 2] + d[2 * S + 2] - sc * (d[1] + d[S + 1])
-        P.sdg = sc * P.sdg - (d[S + 2] / dt) * Rq + (pc / dt) * udgprev
-    else:
-        raise NotImplementedError("DIRK with more than 3 stages")
+        return sc * src - (d[S + 2] / dt) * cur + (pc / dt) * prev
+    raise NotImplementedError("DIRK with more than 3 stages")
+
+
+def update_source_dirk(P, stage, dt, d, udgprev, wprev=None):
+    """UpdateSourceDIRK (updatesource.cpp:32-100) for ncs = ncu, and for the w equation's
+    source wsrc from wdg (updatesource.cpp:78-100)."""
+    S = P.tstages
+    P.dtfactor = d[stage * S + stage] / dt
+    P.sdg = _dirk_source(P.sdg, P.udg[:, :, :P.ncu], udgprev, stage, dt, d, S)
+    if P.ncw:
+        P.wsrc = _dirk_source(P.wsrc, P.wdg, wprev, stage, dt, d, S)
 
 
 def bdf_coeffs(order):
@@ -929,10 +1017,12 @@ def bdf_solve(P, nsteps=None):
     updatesource.cpp:104-140): dtfactor = c0/dt, sdg = -sum_k (c_k/dt) u^{n+1-k}, one Newton solve
     per step at time t + dt.  The first steps use the highest order the stored history allows
     (BDF1, then BDF2, ...): the reference reads history it has not written in its first torder - 1
-    steps."""
+    steps.  The w equation gets wsrc = -sum_k (c_k/dt) w^{n+1-k} from its own history (the
+    reference's UpdateSourceBDF leaves wsrc unchanged for non-wave problems, which drops w's
+    time derivative history)."""
     nsteps = len(P.dt) if nsteps is None else nsteps
     time = P.time
-    hist = []                                       # hist[k] = u^{n-k}
+    hist, whist = [], []                            # hist[k] = u^{n-k}, whist[k] = w^{n-k}
     gmres_total = 0
     for istep in range(nsteps):
         dt = P.dt[istep]
@@ -942,6 +1032,10 @@ def bdf_solve(P, nsteps=None):
         c = bdf_coeffs(order)
         P.dtfactor = c[0] / dt
         P.sdg = sum(-(c[k + 1] / dt) * hist[k] for k in range(order))
+        if P.ncw:  # the w equation's history source, as for u (see the module docstring of bdf_solve)
+            whist.insert(0, P.wdg.copy())
+            del whist[P.torder:]
+            P.wsrc = sum(-(c[k + 1] / dt) * whist[k] for k in range(order))
         P.time = time + dt
         h = newton_solve(P)
         gmres_total += sum(h["gmres_iters"])
@@ -967,14 +1061,22 @@ def dirk_solve(P, nsteps=None):
         dt = P.dt[istep]
         udgprev = P.udg[:, :, :P.ncu].copy()
         utmp = (1.0 - c.sum()) * udgprev
+        if P.ncw:  # PreviousSolutions (previoussolutions.cpp:55-60)
+            wprev = P.wdg.copy()
+            wtmp = (1.0 - c.sum()) * wprev
         for j in range(P.tstages):
             P.time = time + dt * t[j]
-            update_source_dirk(P, j, dt, d, udgprev)
+            update_source_dirk(P, j, dt, d, udgprev, wprev if P.ncw else None)
             h = newton_solve(P)
             gmres_total += sum(h["gmres_iters"])
             utmp = utmp + c[j] * P.udg[:, :, :P.ncu]
             if j == P.tstages - 1:
                 P.udg[:, :, :P.ncu] = utmp
+            if P.ncw:  # UpdateSolution (updatesolution.cpp:142-172): GetW, then the stage combination
+                P.get_w()
+                wtmp = wtmp + c[j] * P.wdg
+                if j == P.tstages - 1:
+                    P.wdg = wtmp
         time += dt
     P.time = time
     return gmres_total

@@ -249,8 +249,10 @@ def run_serve(seconds, rate, check_alone=6):
     """The served workload: a Poisson stream of Navier-Stokes (70 %) and convection-diffusion
     solve requests (n x n quads, n in {32, 64, 128}, p in {2..5}; see cases.serve_stream)
     replayed in real time into the continuous-batching server (HBM budget 200 GB, mixed
-    precision).  Reports completed solves/s over the stream, p50/p99 time to solution (arrival
-    to result), and whether a sample of requests gets bitwise the same result alone."""
+    precision; a finished request's slot takes a waiting request of the same mesh between Newton
+    iterations or time steps).  Reports completed solves/s over the stream, p50/p99 time to
+    solution (arrival to result), the number of requests admitted into running batches, and
+    whether a sample of requests gets bitwise the same result alone."""
     from exasim_amd import cases, serve
     from exasim_amd.hdg import HDGSolver
     work = tempfile.mkdtemp(prefix="exa_serve_")

@@ -115,6 +115,21 @@ long exa_batch_get(exa_handle* h, int inst, const char* name, double* dst, long 
   return rc == EXA_OK ? out : -rc;
 }
 
+int exa_batch_step(exa_handle* h, int* status, int n, int* iterating) {
+  return guard(h, [&](exa::Solver& s) {
+    if (status && n < s.ninst) throw exa::Error(EXA_ERR_ARG, "exa_batch_step: status shorter than the batch");
+    const int left = s.newton_seg_step(status);
+    if (iterating) *iterating = left;
+  });
+}
+
+int exa_batch_refill(exa_handle* h, int inst, const char* datain_dir) {
+  return guard(h, [&](exa::Solver& s) {
+    if (!datain_dir) throw exa::Error(EXA_ERR_ARG, "null datain directory");
+    s.refill(inst, datain_dir);
+  });
+}
+
 void exa_destroy(exa_handle* h) { delete h; }
 
 const char* exa_last_error(const exa_handle* h) {

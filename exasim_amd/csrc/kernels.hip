@@ -1142,13 +1142,13 @@ __device__ __forceinline__ void small_solve(double* J, double* B) {
 // does not converge gets NaN, which the residual norm reports.
 template <class M>
 __device__ void w_point(const AsmArgs& a, const double* x, const double* uq, const double* v, const double* prm,
-                        const double* ws, double* w, double* w_uq) {
+                        double tm, const double* ws, double* w, double* w_uq) {
   constexpr int NCW = model_ncw<M>::value, NC = M::NC;
   const double scal = a.dae_alpha * a.dtfactor + a.dae_beta;
   double sw[NCW], sw_uq[NCW * NC], sw_w[NCW * NCW], J[NCW * NCW];
   bool conv = false;
   for (int it = 0; it < 20 && !conv; ++it) {
-    M::sourcew(x, uq, v, w, prm, a.time, sw, sw_uq, sw_w);
+    M::sourcew(x, uq, v, w, prm, tm, sw, sw_uq, sw_w);
     double r[NCW];
 #pragma unroll
     for (int i = 0; i < NCW; ++i) {
@@ -1165,7 +1165,7 @@ __device__ void w_point(const AsmArgs& a, const double* x, const double* uq, con
     }
     conv = mx < 1e-8;
   }
-  M::sourcew(x, uq, v, w, prm, a.time, sw, sw_uq, sw_w);
+  M::sourcew(x, uq, v, w, prm, tm, sw, sw_uq, sw_w);
 #pragma unroll
   for (int i = 0; i < NCW; ++i)
 #pragma unroll
@@ -1184,6 +1184,11 @@ __device__ __forceinline__ const double* inst_param(const AsmArgs& a, long E) {
 }
 __device__ __forceinline__ const double* inst_tau(const AsmArgs& a, long E) {
   return a.einst ? a.tau + static_cast<long>(a.einst[E]) * a.ntau : a.tau;
+}
+// the model time at element E: the instance's own time when the instances of a batch are at
+// different times (a.tinst, one entry per instance), else the common a.time
+__device__ __forceinline__ double inst_time(const AsmArgs& a, long E) {
+  return a.tinst ? a.tinst[a.einst[E]] : a.time;
 }
 
 // Element volume terms: uEquationElemBlock (uequation.cpp:45-203)
@@ -1219,6 +1224,7 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
   double f[NCU * ND], s[NCU];
   double f_uq[JAC ? NCU * ND * NC : 1], s_uq[JAC ? NCU * NC : 1];
   const double* prm = inst_param(a, E);
+  const double tm = inst_time(a, E);
   if constexpr (model_ncw<M>::value > 0) {
     // w at the point from its equation, and the chain rule through w_uq (uequation.cpp:105-165)
     constexpr int NCW = model_ncw<M>::value;
@@ -1233,10 +1239,10 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
         ws[c] += phi * a.wsrc[i + d.npe * (c + NCW * E)];
       }
     }
-    w_point<M>(a, x, uq, v, prm, ws, w, w_uq);
+    w_point<M>(a, x, uq, v, prm, tm, ws, w, w_uq);
     double f_w[JAC ? NCU * ND * NCW : 1], s_w[JAC ? NCU * NCW : 1];
-    M::template flux_w<JAC>(x, uq, v, w, prm, a.time, f, f_uq, f_w);
-    M::template source_w<JAC>(x, uq, v, w, prm, a.time, s, s_uq, s_w);
+    M::template flux_w<JAC>(x, uq, v, w, prm, tm, f, f_uq, f_w);
+    M::template source_w<JAC>(x, uq, v, w, prm, tm, s, s_uq, s_w);
     if (JAC) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
@@ -1272,15 +1278,15 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
 #pragma unroll
       for (int i = 0; i < NCU * NC; ++i) s_uq[i] = o[i * P];
   } else {
-    M::template flux<JAC>(x, uq, v, prm, a.time, f, f_uq);
-    M::template source<JAC>(x, uq, v, prm, a.time, s, s_uq);
+    M::template flux<JAC>(x, uq, v, prm, tm, f, f_uq);
+    M::template source<JAC>(x, uq, v, prm, tm, s, s_uq);
   }
   if (a.tdep) {  // uequation.cpp:126-151: s += (sdg - dtfactor*u) T, s_u -= dtfactor T
     double T[NCU];
 #pragma unroll
     for (int k = 0; k < NCU; ++k) T[k] = 1.0;
     if constexpr (has_tdfunc<M>::value)
-      if (a.tdfunc) M::tdfunc(x, uq, v, prm, a.time, T);
+      if (a.tdfunc) M::tdfunc(x, uq, v, prm, tm, T);
     double sg[NCU];
 #pragma unroll
     for (int k = 0; k < NCU; ++k) sg[k] = 0.0;
@@ -1455,9 +1461,10 @@ __global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
     double w[NCW], ws[NCW], w_uq[NCW * NC];
     face_w<M>(d, a, E, l, g, w, ws);
     const double* prm = inst_param(a, E);
-    w_point<M>(a, x, uq, v, prm, ws, w, w_uq);
+    const double tm = inst_time(a, E);
+    w_point<M>(a, x, uq, v, prm, tm, ws, w, w_uq);
     double f_w[JAC ? NCU * ND * NCW : 1];
-    M::template flux_w<JAC>(x, uq, v, w, prm, a.time, f, f_uq, f_w);
+    M::template flux_w<JAC>(x, uq, v, w, prm, tm, f, f_uq, f_w);
     if (JAC)
 #pragma unroll
       for (int c = 0; c < NC; ++c)
@@ -1476,7 +1483,7 @@ __global__ __launch_bounds__(256) void k_face_points(Dims d, AsmArgs a) {
 #pragma unroll
       for (int i = 0; i < NCU * ND * NC; ++i) f_uq[i] = o[(NCU * ND + i) * P];
   } else {
-    M::template flux<JAC>(x, uq, v, inst_param(a, E), a.time, f, f_uq);
+    M::template flux<JAC>(x, uq, v, inst_param(a, E), inst_time(a, E), f, f_uq);
   }
 #pragma unroll
   for (int c = 0; c < NCU; ++c) uq[c] = usave[c];
@@ -1542,9 +1549,10 @@ __global__ __launch_bounds__(64) void k_face_bou_points(Dims d, AsmArgs a) {
     for (int c = 0; c < NC; ++c) uqh[c] = c < NCU ? uh[c] : uq[c];
     face_w<M>(d, a, E, l, g, w, ws);
     const double* prm = inst_param(a, E);
-    w_point<M>(a, x, uqh, v, prm, ws, w, w_uq);
+    const double tm = inst_time(a, E);
+    w_point<M>(a, x, uqh, v, prm, tm, ws, w, w_uq);
     double fb_w[JAC ? NCU * NCW : 1];
-    M::template fbou_w<JAC>(ib, x, uq, v, w, uh, nl, inst_tau(a, E), prm, a.time, fb, fb_uq, fb_uh, fb_w);
+    M::template fbou_w<JAC>(ib, x, uq, v, w, uh, nl, inst_tau(a, E), prm, tm, fb, fb_uq, fb_uh, fb_w);
     if (JAC)
 #pragma unroll
       for (int r = 0; r < NCU; ++r)
@@ -1569,7 +1577,8 @@ __global__ __launch_bounds__(64) void k_face_bou_points(Dims d, AsmArgs a) {
       for (int i = 0; i < NCU * NCU; ++i) fb_uh[i] = o[(NCU + NCU * NC + i) * P];
     }
   } else {
-    M::template fbou<JAC>(ib, x, uq, v, uh, nl, inst_tau(a, E), inst_param(a, E), a.time, fb, fb_uq, fb_uh);
+    M::template fbou<JAC>(ib, x, uq, v, uh, nl, inst_tau(a, E), inst_param(a, E), inst_time(a, E), fb, fb_uq,
+                          fb_uh);
   }
 #pragma unroll
   for (int k = 0; k < NCU; ++k) a.fpt[(NF1 + k) * P + p] = fb[k] * jac;
@@ -2695,6 +2704,8 @@ void plugin_points(const Dims& d, const AsmArgs& a, bool jac, bool vol, hipStrea
 
 // GetW (wequation.cpp:226-240): the w equation at every element node with the nodal (u, q), the
 // nodal w as the first guess and the nodal wsrc; wdg is updated in place (one thread per node).
+// A pass from a converged w still moves it in the last bits, so a batch passes a.wmask to update
+// only the instances whose (u, q) just changed.
 template <class M>
 __global__ __launch_bounds__(256) void k_get_w(Dims d, AsmArgs a, const double* xdg, const double* odg,
                                                double* wdg) {
@@ -2703,6 +2714,7 @@ __global__ __launch_bounds__(256) void k_get_w(Dims d, AsmArgs a, const double* 
   const long p = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (p >= P) return;
   const long E = p / d.npe;
+  if (a.wmask && a.wmask[a.einst[E]] == 0.0) return;  // an instance whose state did not change
   const int i = static_cast<int>(p % d.npe);
   double x[ND], uq[NC], v[8], w[NCW], ws[NCW], w_uq[NCW * NC];
 #pragma unroll
@@ -2715,7 +2727,7 @@ __global__ __launch_bounds__(256) void k_get_w(Dims d, AsmArgs a, const double* 
     w[c] = wdg[i + d.npe * (c + NCW * E)];
     ws[c] = a.wsrc[i + d.npe * (c + NCW * E)];
   }
-  w_point<M>(a, x, uq, v, inst_param(a, E), ws, w, w_uq);
+  w_point<M>(a, x, uq, v, inst_param(a, E), inst_time(a, E), ws, w, w_uq);
 #pragma unroll
   for (int c = 0; c < NCW; ++c) wdg[i + d.npe * (c + NCW * E)] = w[c];
 }

@@ -106,6 +106,7 @@ struct AsmArgs {
   const double* param;  // device, [nparam, ninst]
   const double* tau;    // device, [ntau, ninst]
   const int* einst;     // device, instance of each element (null: one instance)
+  const double* tinst;  // device, [ninst]: each instance's model time (null: all at time)
   int nparam, ntau;
   int e0, eb;
   const double *shapegt, *shapegw, *shapfgt, *shapfgw;
@@ -130,6 +131,7 @@ struct AsmArgs {
   // alpha dw/dt + beta w = Sourcew with dw/dt ~ dtfactor w - wsrc
   const double *wdg, *wsrc;
   double dae_alpha, dae_beta;
+  const double* wmask;  // GetW in a batch: device [ninst], nonzero for the instances to update (null: all)
 };
 // GetW: the w equation at every element node with the nodal (u, q) (wdg updated in place)
 void w_nodes(const Dims& d, const AsmArgs& a, const double* xdg, const double* odg, double* wdg, hipStream_t s);

@@ -321,26 +321,8 @@ void Solver::init(SolData& sol) {
   MCs.alloc(static_cast<size_t>(d.ne) * d.npe * d.nd * d.npe);
   MEs.alloc(static_cast<size_t>(d.ne) * d.npe * d.nd * d.ndf);
   Mass.alloc(static_cast<size_t>(d.ne) * d.npe * d.npe);
-  {
-    long ncq = static_cast<long>(d.npe) + d.npe * d.nd + static_cast<long>(d.ndf) * d.nd;
-    long per = d.npe * ncq;
-    int qb = static_cast<int>(std::max(1L, std::min<long>(d.ne, static_cast<long>(1.0e9 / 8 / per))));
-    DevBuf<double> qaug, qA;
-    qaug.alloc(static_cast<size_t>(qb) * per);
-    qA.alloc(static_cast<size_t>(qb) * (d.nd + 1) * d.npe * d.nge);
-    info.alloc(std::max(d.ne, d.nf) + 1);
-    for (int e0 = 0; e0 < d.ne; e0 += qb) {
-      int eb = std::min(qb, d.ne - e0);
-      EXA_HIP(hipMemsetAsync(qaug.p, 0, static_cast<size_t>(eb) * per * sizeof(double), s));
-      qsetup(d, e0, eb, shapegt.p, shapegw.p, shapfgt.p, shapfgw.p, perm.p, egeo.p, fgeo.p, qaug.p, Mass.p,
-             s, qA.p);
-      lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
-      const int bad = first_nonzero(info.p, eb);
-      if (bad >= 0) throw Error(4, "singular mass matrix at element " + std::to_string(e0 + bad));
-      qpack(d, e0, eb, qaug.p, MCs.p, MEs.p, s);
-    }
-    EXA_HIP(hipStreamSynchronize(s));
-  }
+  info.alloc(std::max(d.ne, d.nf) + 1);
+  setup_q(0, d.ne);
 
   // persistent system arrays
   H.alloc(static_cast<size_t>(d.ne) * d.m * d.m);
@@ -376,6 +358,26 @@ void Solver::init(SolData& sol) {
       if (tscheme == 1) wbdfhist.alloc(nw * torder);
     }
   }
+}
+
+void Solver::setup_q(long e_begin, long ne) {
+  const long ncq = static_cast<long>(d.npe) + d.npe * d.nd + static_cast<long>(d.ndf) * d.nd;
+  const long per = d.npe * ncq;
+  const int qb = static_cast<int>(std::max(1L, std::min<long>(ne, static_cast<long>(1.0e9 / 8 / per))));
+  DevBuf<double> qaug, qA;
+  qaug.alloc(static_cast<size_t>(qb) * per);
+  qA.alloc(static_cast<size_t>(qb) * (d.nd + 1) * d.npe * d.nge);
+  for (long e0 = e_begin; e0 < e_begin + ne; e0 += qb) {
+    const int eb = static_cast<int>(std::min<long>(qb, e_begin + ne - e0));
+    EXA_HIP(hipMemsetAsync(qaug.p, 0, static_cast<size_t>(eb) * per * sizeof(double), s));
+    qsetup(d, static_cast<int>(e0), eb, shapegt.p, shapegw.p, shapfgt.p, shapfgw.p, perm.p, egeo.p, fgeo.p,
+           qaug.p, Mass.p, s, qA.p);
+    lu_solve(qaug.p, d.npe, static_cast<int>(ncq), per, eb, info.p, s, nullptr, lu_ws(eb), lu_tri(eb));
+    const int bad = first_nonzero(info.p, eb);
+    if (bad >= 0) throw Error(4, "singular mass matrix at element " + std::to_string(e0 + bad));
+    qpack(d, static_cast<int>(e0), eb, qaug.p, MCs.p, MEs.p, s);
+  }
+  EXA_HIP(hipStreamSynchronize(s));
 }
 
 // DIRKcoeff (timestepcoeff.cpp:35-104) computed from the Butcher tableaux of the
@@ -673,6 +675,7 @@ AsmArgs Solver::asm_args(int e0, int eb) {
   a.sdg = sdg.p;
   a.model = model;
   a.time = tnow;
+  a.tinst = use_tinst ? tinst_d.p : nullptr;
   a.param = param.p;
   a.tau = tau.p;
   a.einst = einst.p;
@@ -789,15 +792,19 @@ void Solver::reset() {
   dtfactor = 0.0;
   if (tdep) sdg.zero(s);
   assembled = false;
+  step_act.clear();
+  step_valid = false;
 }
 
 void Solver::get_q() { getq(d, MCs.p, MEs.p, elemcon.p, uh.p, udg.p, s); }
 
 // GetW (solution.cpp:187-188): the nodal w from its equation with the nodal (u, q), after every
 // change of udg in the Newton loop and at each time stage
-void Solver::get_w() {
+void Solver::get_w(const double* mask) {
   if (ncw == 0) return;
-  w_nodes(d, asm_args(0, d.ne), xdg.p, odg.p, wdg.p, s);
+  AsmArgs a = asm_args(0, d.ne);
+  a.wmask = mask;
+  w_nodes(d, a, xdg.p, odg.p, wdg.p, s);
 }
 
 void Solver::assemble() {
@@ -1080,74 +1087,29 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
     inst_gmres.assign(ninst, 0);
     inst_norm.assign(ninst, 0.0);
     inst_failed.assign(ninst, 0);
+    step_act.clear();  // a full solve ends an exa_batch_step sequence
+    step_valid = false;
   }
   if (!tdep) {
     if (batch) newton_seg(newton_iters, gmres_total, final_norm);
     else newton(newton_iters, gmres_total, final_norm);
     return;
   }
+  if (batch)
+    for (int n : inst_nsteps)
+      if (n != static_cast<int>(dts.size()))
+        throw Error(1, "the instances of this batch take different numbers of time steps: exa_batch_step "
+                       "runs them, exa_solve needs one list of steps");
   if (tscheme == 1) {
     solve_bdf(newton_iters, gmres_total, final_norm);
     return;
   }
-  const int S = tstages;
-  double csum = 0.0;
-  for (int j = 0; j < S; ++j) csum += dirk_c[j];
   int nit = 0, gtot = 0;
   double nrm = 0.0;
   double t = tnow;
   for (size_t istep = 0; istep < dts.size(); ++istep) {
     const double dt = dts[istep];
-    u_axpby(d, udg.p, 1.0, 0.0, udgprev.p, s);          // udgprev = u
-    axpby(utmp.p, udgprev.p, 1.0 - csum, 0.0, static_cast<long>(udgprev.n), s);
-    const long nw = static_cast<long>(wdg.n);
-    if (ncw > 0) {  // PreviousSolutions (previoussolutions.cpp:55-60): wprev = w, wtmp = (1 - sum c) w
-      copy(wprev.p, wdg.p, nw, s);
-      axpby(wtmp.p, wdg.p, 1.0 - csum, 0.0, nw, s);
-    }
-    for (int j = 0; j < S; ++j) {
-      tnow = t + dt * dirk_t[j];
-      dtfactor = dirk_d[j * S + j] / dt;
-      if (j == 0) {
-        u_axpby(d, udg.p, dirk_d[0] / dt, 0.0, sdg.p, s);
-      } else if (j == 1) {
-        u_axpby(d, udg.p, -dirk_d[1] / dt, (dirk_d[1] + dirk_d[S + 1]) / dirk_d[0], sdg.p, s);
-      } else {
-        const double sc = dirk_d[2] / dirk_d[1];
-        const double pc = dirk_d[2] + dirk_d[S + 2] + dirk_d[2 * S + 2] - sc * (dirk_d[1] + dirk_d[S + 1]);
-        u_axpby(d, udg.p, -dirk_d[S + 2] / dt, sc, sdg.p, s);
-        axpby(sdg.p, udgprev.p, pc / dt, 1.0, static_cast<long>(sdg.n), s);
-      }
-      if (ncw > 0) {  // the w equation's stage source (updatesource.cpp:78-100), as for u
-        if (j == 0) {
-          axpby(wsrc.p, wdg.p, dirk_d[0] / dt, 0.0, nw, s);
-        } else if (j == 1) {
-          axpby(wsrc.p, wdg.p, -dirk_d[1] / dt, (dirk_d[1] + dirk_d[S + 1]) / dirk_d[0], nw, s);
-        } else {
-          const double sc = dirk_d[2] / dirk_d[1];
-          const double pc = dirk_d[2] + dirk_d[S + 2] + dirk_d[2 * S + 2] - sc * (dirk_d[1] + dirk_d[S + 1]);
-          axpby(wsrc.p, wdg.p, -dirk_d[S + 2] / dt, sc, nw, s);
-          axpby(wsrc.p, wprev.p, pc / dt, 1.0, nw, s);
-        }
-      }
-      int ni = 0, gi = 0;
-      if (batch) newton_seg(&ni, &gi, &nrm);
-      else newton(&ni, &gi, &nrm);
-      nit += ni;
-      gtot += gi;
-      u_axpby(d, udg.p, dirk_c[j], 1.0, utmp.p, s);       // utmp += c_j u
-      if (j == S - 1) u_insert(d, utmp.p, udg.p, s);
-      if (ncw > 0) {  // UpdateSolution (updatesolution.cpp:142-172): GetW, wtmp += c_j w
-        get_w();
-        axpby(wtmp.p, wdg.p, dirk_c[j], 1.0, nw, s);
-        if (j == S - 1) copy(wdg.p, wtmp.p, nw, s);
-      }
-      if (verbose) {
-        std::printf("Timestep %zu, stage %d, time %g: Newton %d, GMRES %d, residual %g\n", istep + 1, j + 1,
-                    tnow, ni, gi, nrm);
-        std::fflush(stdout);
-      }
-    }
+    dirk_step(dt, t, nullptr, &nit, &gtot, &nrm, static_cast<int>(istep));
     t += dt;
   }
   tnow = t;
@@ -1155,6 +1117,73 @@ void Solver::solve(int* newton_iters, int* gmres_total, double* final_norm) {
   if (newton_iters) *newton_iters = nit;
   if (gmres_total) *gmres_total = gtot;
   if (final_norm) *final_norm = nrm;
+}
+
+// One DIRK time step of length dt from time t (the stages of the reference's time loop,
+// solution.cpp:244-330, with PreviousSolutions, UpdateSource and UpdateSolution): per stage j the
+// model time t + dt t_j, dtfactor = d_jj / dt, the stage source in sdg and a Newton solve.  With
+// only (a batch) the Newton solves iterate the instances it marks; with use_tinst each instance
+// runs at its own time inst_t[i] + dt t_j.  Adds the Newton and GMRES iterations to *nit, *gtot.
+void Solver::dirk_step(double dt, double t, const std::vector<char>* only, int* nit, int* gtot, double* nrm,
+                       int istep) {
+  const int S = tstages;
+  double csum = 0.0;
+  for (int j = 0; j < S; ++j) csum += dirk_c[j];
+  u_axpby(d, udg.p, 1.0, 0.0, udgprev.p, s);          // udgprev = u
+  axpby(utmp.p, udgprev.p, 1.0 - csum, 0.0, static_cast<long>(udgprev.n), s);
+  const long nw = static_cast<long>(wdg.n);
+  if (ncw > 0) {  // PreviousSolutions (previoussolutions.cpp:55-60): wprev = w, wtmp = (1 - sum c) w
+    copy(wprev.p, wdg.p, nw, s);
+    axpby(wtmp.p, wdg.p, 1.0 - csum, 0.0, nw, s);
+  }
+  for (int j = 0; j < S; ++j) {
+    tnow = t + dt * dirk_t[j];
+    if (use_tinst) {
+      std::vector<double> ti(ninst);
+      for (int i = 0; i < ninst; ++i) ti[i] = inst_t[i] + dt * dirk_t[j];
+      st.h2d(tinst_d.p, ti.data(), sizeof(double) * ninst, s);
+    }
+    dtfactor = dirk_d[j * S + j] / dt;
+    if (j == 0) {
+      u_axpby(d, udg.p, dirk_d[0] / dt, 0.0, sdg.p, s);
+    } else if (j == 1) {
+      u_axpby(d, udg.p, -dirk_d[1] / dt, (dirk_d[1] + dirk_d[S + 1]) / dirk_d[0], sdg.p, s);
+    } else {
+      const double sc = dirk_d[2] / dirk_d[1];
+      const double pc = dirk_d[2] + dirk_d[S + 2] + dirk_d[2 * S + 2] - sc * (dirk_d[1] + dirk_d[S + 1]);
+      u_axpby(d, udg.p, -dirk_d[S + 2] / dt, sc, sdg.p, s);
+      axpby(sdg.p, udgprev.p, pc / dt, 1.0, static_cast<long>(sdg.n), s);
+    }
+    if (ncw > 0) {  // the w equation's stage source (updatesource.cpp:78-100), as for u
+      if (j == 0) {
+        axpby(wsrc.p, wdg.p, dirk_d[0] / dt, 0.0, nw, s);
+      } else if (j == 1) {
+        axpby(wsrc.p, wdg.p, -dirk_d[1] / dt, (dirk_d[1] + dirk_d[S + 1]) / dirk_d[0], nw, s);
+      } else {
+        const double sc = dirk_d[2] / dirk_d[1];
+        const double pc = dirk_d[2] + dirk_d[S + 2] + dirk_d[2 * S + 2] - sc * (dirk_d[1] + dirk_d[S + 1]);
+        axpby(wsrc.p, wdg.p, -dirk_d[S + 2] / dt, sc, nw, s);
+        axpby(wsrc.p, wprev.p, pc / dt, 1.0, nw, s);
+      }
+    }
+    int ni = 0, gi = 0;
+    if (batch) newton_seg(&ni, &gi, nrm, only);
+    else newton(&ni, &gi, nrm);
+    *nit += ni;
+    *gtot += gi;
+    u_axpby(d, udg.p, dirk_c[j], 1.0, utmp.p, s);       // utmp += c_j u
+    if (j == S - 1) u_insert(d, utmp.p, udg.p, s);
+    if (ncw > 0) {  // UpdateSolution (updatesolution.cpp:142-172): GetW, wtmp += c_j w
+      get_w();
+      axpby(wtmp.p, wdg.p, dirk_c[j], 1.0, nw, s);
+      if (j == S - 1) copy(wdg.p, wtmp.p, nw, s);
+    }
+    if (verbose) {
+      std::printf("Timestep %d, stage %d, time %g: Newton %d, GMRES %d, residual %g\n", istep + 1, j + 1, tnow, ni,
+                  gi, *nrm);
+      std::fflush(stdout);
+    }
+  }
 }
 
 // BDF1-3 (PreviousSolutions, previoussolutions.cpp:62-100; UpdateSourceBDF, updatesource.cpp:104-140;

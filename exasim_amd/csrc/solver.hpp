@@ -65,7 +65,8 @@ struct Solver {
   int ncw = 0;
   double dae_alpha = 1.0, dae_beta = 0.0;
   DevBuf<double> wdg, wsrc, wdg0, wprev, wtmp, wbdfhist;
-  void get_w();
+  // mask (device, [ninst], may be null): update only the instances with a nonzero entry
+  void get_w(const double* mask = nullptr);
   bool timing = false;
   double ws_budget_bytes = 16.0e9;
 
@@ -167,6 +168,19 @@ struct Solver {
   std::vector<int> inst_newton, inst_gmres;
   std::vector<char> inst_failed;  // instances stopped by a Newton breakdown in this solve
   std::vector<double> inst_norm;
+  // exa_batch_step state: instances still iterating (empty until the first step or refill), the
+  // residual norms of the current state (valid unless a refill or a full solve came between) and
+  // the Newton iterate
+  std::vector<char> step_act;
+  std::vector<double> step_nr;
+  bool step_valid = false;
+  DevBuf<double> step_u;
+  // time-dependent steps: each instance's time and number of time steps done; while use_tinst the
+  // assembly reads each instance's model time from tinst_d
+  std::vector<double> inst_t;
+  std::vector<int> inst_steps, inst_nsteps;  // time steps done, and each instance's number of steps
+  DevBuf<double> tinst_d;
+  bool use_tinst = false;
 
   // timing
   std::array<std::vector<std::pair<hipEvent_t, hipEvent_t>>, T_NCLASS> ev;
@@ -181,6 +195,7 @@ struct Solver {
   int* lu_ws(long batch);
   void* lu_tri(long batch);  // triangle-inverse workspace of the blocked LU (LU_TRI_WS doubles per matrix)
   void init(SolData& sol);
+  void setup_q(long e0, long ne);  // q-equation matrices (MCs, MEs, Mass) of elements [e0, e0 + ne)
   // batch (segmented) Newton-GMRES
   void seg_setup();
   void build_seg(SegDev& sd, const std::vector<long>& off);
@@ -189,7 +204,15 @@ struct Solver {
   void seg_norms(const SegDev& sd, const double* v, double* out_dev);
   std::vector<double> residual_seg();
   int gmres_seg(const double* bvec, double* xvec, double tol_rel, std::vector<int>& iters, int maxit = -1);
-  void newton_seg(int* newton_iters, int* gmres_iters, double* final_norm);
+  // only (may be null): the instances to iterate (the others keep their state)
+  void newton_seg(int* newton_iters, int* gmres_iters, double* final_norm, const std::vector<char>* only = nullptr);
+  void newton_seg_iter(std::vector<char>& act, std::vector<double>& nr, double* u, int* gmres_total);
+  // iteration-level batching: one Newton iteration per call; a finished slot takes a new problem
+  void step_init();
+  int newton_seg_step(int* status);
+  int dirk_seg_step(int* status);
+  void dirk_step(double dt, double t, const std::vector<char>* only, int* nit, int* gtot, double* nrm, int istep);
+  void refill(int inst, const std::string& datain);
   long get_inst_array(int inst, const std::string& name, double* dst, long len);
   void restore_instance(int inst, double* newton_iterate);
 

@@ -48,6 +48,8 @@ SIGNATURES = [
     ("exa_batch_size", ctypes.c_int, [_P]),
     ("exa_batch_stats", ctypes.c_int, [_P, _I, _I, _D, ctypes.c_int]),
     ("exa_batch_get", ctypes.c_long, [_P, ctypes.c_int, ctypes.c_char_p, _D, ctypes.c_long]),
+    ("exa_batch_step", ctypes.c_int, [_P, _I, ctypes.c_int, _I]),
+    ("exa_batch_refill", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_char_p]),
     ("libmain", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
 ]
 TIMER_CLASSES = ["matvec", "precond", "ortho", "assembly", "lu", "other"]
@@ -142,6 +144,20 @@ class HDGSolver:
         if r < 0:
             self._check(-r)
         return out
+
+    def step(self):
+        """One Newton iteration (steady) or one DIRK time step of every instance still iterating
+        (``exa_batch_step``): returns the status per instance (1 iterating, 0 finished, -1 broken
+        down)."""
+        n = self.ninst
+        st = np.zeros(n, dtype=np.int32)
+        left = ctypes.c_int(0)
+        self._check(self._lib.exa_batch_step(self._h, st.ctypes.data_as(_I), n, ctypes.byref(left)))
+        return st
+
+    def refill(self, inst, datain):
+        """Put the problem of ``datain`` into the finished slot ``inst`` (``exa_batch_refill``)."""
+        self._check(self._lib.exa_batch_refill(self._h, int(inst), datain.encode()))
 
     SIZE_KEYS = ["nd", "ncu", "nc", "npe", "npf", "nfe", "ne", "nf", "n", "m", "N", "nge", "ngf",
                  "nco", "restart", "preconditioner", "precision", "refinements", "neb"]

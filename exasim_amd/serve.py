@@ -6,25 +6,34 @@ into lanes.  A dispatcher starts a batch as soon as requests of a lane wait (aft
 batching window) and the HBM budget admits them, while fewer than ``max_concurrent`` batches
 run: it takes every waiting request of the lane that fits (up to ``max_batch``), builds one
 batch handle (``exa_batch_create``: the instances' meshes, Jacobians and Krylov vectors
-resident on the device for the whole solve, one HIP stream), solves it on a thread of its own
-(shared launches, per-instance Newton/GMRES, see exasim_amd/csrc/batch.cpp), hands each request
-its result and frees the memory.  A request that arrives while batches of its lane are running
-does not wait for them: it is admitted into a new batch as soon as memory and a slot free up,
-and the batches run concurrently on their own streams, so their work overlaps on the device.
-An instance's result does not depend on the batch it was solved in (the batch keeps every
-Krylov/Newton scalar per instance), so a request gets the same answer alone and in the mix.
+resident on the device for the whole solve, one HIP stream) and runs it on a thread of its own
+(shared launches, per-instance Newton/GMRES, see exasim_amd/csrc/batch.cpp).
+
+The batch advances one Newton iteration (steady requests) or one time step (DIRK requests) at a
+time (``exa_batch_step``).  After each step every request that has finished gets its result at
+once, and its slot is admitted the oldest waiting request of the lane with the same mesh
+(``exa_batch_refill``: same element shape and mesh.bin, any coordinates, parameters and initial
+state): the new request's Newton iterations run in the same launches as the requests already
+under way, in the slot and memory the finished one leaves.  A request with no free slot of its
+mesh is admitted into a new batch as soon as memory and a batch slot free up, and the batches run
+concurrently on their own streams, so their work overlaps on the device.  An instance's result
+does not depend on the batch it was solved in, nor on when it joined (the batch keeps every
+Krylov/Newton scalar and the model time per instance), so a request gets the same answer alone
+and in the mix.
 
 No counterpart in the reference (one problem per process there); each solve follows
 exa_solve's algorithm.
 """
 import collections
+import hashlib
+import os
 import threading
 import time
 
 import numpy as np
 
 from .datain import read_sizes
-from .hdg import HDGSolver
+from .hdg import ExaError, HDGSolver
 
 
 def shape_key(path, model):
@@ -33,6 +42,14 @@ def shape_key(path, model):
     z = read_sizes(path)
     return (model, z["nd"], z["ncu"], z["elemtype"], z["porder"], z["pgauss"], z["npe"], z["npf"], z["nge"],
             z["ngf"])
+
+
+def mesh_key(path):
+    """Requests with equal keys (and equal shape keys) can take each other's slot in a running
+    batch: the digest of mesh.bin (connectivity, boundary faces and face numbering; the node
+    coordinates are in sol.bin)."""
+    with open(os.path.join(path, "mesh.bin"), "rb") as f:
+        return hashlib.sha1(f.read()).hexdigest()
 
 
 def estimate_bytes(ne, npe, npf, nfe, ncu, nd, restart, precision=1):
@@ -56,7 +73,7 @@ class Server:
     """Continuous-batching request loop over batch handles."""
 
     def __init__(self, hbm_budget_bytes=200e9, max_batch=64, max_concurrent=8, precision=1, options=None,
-                 keep_results=False, log=None, window=0.0):
+                 keep_results=False, log=None, window=0.0, refill=True):
         self.budget = float(hbm_budget_bytes)
         self.max_batch = max_batch
         self.max_concurrent = int(max_concurrent)  # batches solving at once (a host thread and a HIP stream each)
@@ -84,11 +101,19 @@ class Server:
         self.fallbacks = []      # (lane, size) of batches solved again in FP64
         self.t0 = None
         self._est = {}
+        # iteration-level admission: batches step (exa_batch_step) and finished slots take waiting
+        # requests of the same mesh (exa_batch_refill); False: each batch solves to the end
+        self.refill = bool(refill)
+        self.refills = 0
+        self._mesh = {}          # datain -> mesh_key
+        self._no_refill = set()  # requests a running batch refused (they wait for a batch of their own)
 
     # -- requests --------------------------------------------------------
     def submit(self, rid, datain, model, arrival=None):
         key = shape_key(datain, model)
+        mk = mesh_key(datain) if self.refill else None
         with self.lock:
+            self._mesh[datain] = mk
             if self.t0 is None:
                 self.t0 = time.perf_counter()
             t = time.perf_counter() - self.t0 if arrival is None else arrival
@@ -169,6 +194,17 @@ class Server:
                     return
                 self.lock.wait(0.005)
 
+    def _take_same_mesh(self, key, mk):
+        """The oldest waiting request of the lane whose mesh is mk (removed from the queue), or None."""
+        with self.lock:
+            q = self.queues[key]
+            for idx, req in enumerate(q):
+                if self._mesh.get(req.datain) == mk and req.rid not in self._no_refill:
+                    del q[idx]
+                    self.records[req.rid]["admit"] = time.perf_counter() - self.t0
+                    return req
+        return None
+
     def _run_batch(self, key, take, need):
         try:
             try:
@@ -183,20 +219,22 @@ class Server:
                 # instances whose Newton iteration broke down with the FP32 factors: again, in FP64
                 with self.lock:
                     self.fallbacks.append((key, len(failed)))
-                failed = self._solve(key, failed, 0)
+                failed = self._solve(key, failed, 0, admit=False)
             with self.lock:
                 for r in failed:
                     self.records[r.rid]["failed"] = True
         except Exception as e:  # the requests fail, the server goes on
             with self.lock:
-                self.errors.append((key, [r.rid for r in take], repr(e)))
+                self.errors.append((key, [r.rid for r in getattr(e, "requests", take)], repr(e)))
         finally:
             with self.lock:
                 self.in_flight -= need
                 self.running -= 1
                 self.lock.notify_all()
 
-    def _solve(self, key, take, precision):
+    def _solve(self, key, take, precision, admit=True):
+        if self.refill:
+            return self._solve_steps(key, take, precision, admit)
         t1 = time.perf_counter()
         S = HDGSolver.batch([r.datain for r in take], take[0].model)
         try:
@@ -224,6 +262,71 @@ class Server:
                 self.records[r.rid].pop("failed", None)
                 if uhs is not None:
                     self.results[r.rid] = uhs[i]
+        return failed
+
+    def _solve_steps(self, key, take, precision, admit):
+        """Step one batch to the end; after each step hand out the finished requests' results and
+        refill their slots from the lane's queue (admit) -- requests of the same mesh."""
+        t1 = time.perf_counter()
+        S = HDGSolver.batch([r.datain for r in take], take[0].model)
+        failed, slots, served = [], list(take), len(take)
+        try:
+            S.set_option("precision", precision)
+            for k, v in self.options.items():
+                S.set_option(k, v)
+            while any(r is not None for r in slots):
+                try:
+                    st = S.step()
+                except ExaError as e:
+                    # a numerical failure of the whole batch (EXA_ERR_NUMERIC) in the mixed form:
+                    # the requests under way are solved again in FP64
+                    if precision == 0 or e.code != 4:
+                        e.requests = [r for r in slots if r is not None]
+                        raise
+                    failed += [r for r in slots if r is not None]
+                    break
+                fin = [i for i, r in enumerate(slots) if r is not None and st[i] != 1]
+                if not fin:
+                    continue
+                newton, gmres, norm = S.instance_stats()
+                done = time.perf_counter() - self.t0
+                for i in fin:
+                    r = slots[i]
+                    uh = S.get_instance(i, "uh") if self.keep_results else None
+                    with self.lock:
+                        if st[i] < 0 or not np.isfinite(norm[i]):   # its Newton iteration broke down
+                            failed.append(r)
+                        else:
+                            self.records[r.rid].update(done=done, batch=len(take), newton=int(newton[i]),
+                                                       gmres=int(gmres[i]), norm=float(norm[i]),
+                                                       precision=precision, refilled=r not in take)
+                            self.records[r.rid].pop("failed", None)
+                            if uh is not None:
+                                self.results[r.rid] = uh
+                    slots[i] = None
+                    nxt = self._take_same_mesh(key, self._mesh.get(r.datain)) if admit else None
+                    if nxt is None:
+                        continue
+                    try:
+                        S.refill(i, nxt.datain)
+                    except ExaError:  # other options or time steps: it waits for a batch of its own
+                        with self.lock:
+                            self._no_refill.add(nxt.rid)
+                            self.queues[key].appendleft(nxt)
+                            self.lock.notify_all()
+                        continue
+                    slots[i] = nxt
+                    served += 1
+                    with self.lock:
+                        self.refills += 1
+        finally:
+            S.close()
+        if self.log:
+            self.log("batch of %d (%d admitted while it ran) %s p=%d requests solved in %.2f s (%s), t = %.1f s"
+                     % (served, served - len(take), key[0], key[4], time.perf_counter() - t1,
+                        "mixed" if precision else "fp64", time.perf_counter() - self.t0))
+        with self.lock:
+            self.batches.append((key, served, time.perf_counter() - t1))
         return failed
 
     # -- replay of a timed stream ------------------------------------------
@@ -254,6 +357,7 @@ class Server:
                     batches=len(self.batches),
                     mean_batch=float(np.mean([b[1] for b in self.batches])) if self.batches else None,
                     peak_in_flight_gb=self.peak_bytes / 1e9, errors=[e[2] for e in self.errors[:3]],
+                    refills=self.refills,
                     fp64_fallback_batches=len(self.fallbacks),
                     fp64_fallback_requests=sum(f[1] for f in self.fallbacks),
                     lanes={"%s p=%d" % (k[0], k[4]): dict(batches=len(v), requests=int(sum(b[1] for b in v)),

@@ -166,6 +166,27 @@ int exa_batch_stats(exa_handle* h, int* newton_iters, int* gmres_iters, double* 
 /* Instance inst's "uh" [ncu,npf,nf_i] or "udg" [npe,nc,ne_i]: copies it to dst (len >= size)
  * and returns its size; with dst NULL returns the size only.  Negative EXA_ERR_* on failure. */
 long exa_batch_get(exa_handle* h, int inst, const char* name, double* dst, long len);
+/* Iteration-level batching (no counterpart in the reference): for steady problems one Newton
+ * iteration (the body of NewtonSolver's loop, backend/Solution/solution.cpp:125-242, with its
+ * GMRES, damping and the mixed form's refinement), for DIRK problems one time step (its stages,
+ * each with its Newton solve, solution.cpp:244-330), of every instance that still iterates, in
+ * shared launches; each instance runs at its own model time.  status[i] (n >= ninst entries, may
+ * be NULL): 1 still iterating, 0 converged, at the Newton iteration limit or through its time
+ * steps, -1 broken down; *iterating (may be NULL) = the number still iterating.  Repeated steps
+ * give each instance exa_solve's iterates bit for bit; exa_batch_stats reports the counts of the
+ * problem now in each slot.  Read a finished slot's arrays (exa_batch_get) before the next step:
+ * a time step's stage updates also touch finished slots.  exa_solve and exa_reset end the step
+ * sequence (the next step starts every instance afresh).  Time-dependent batches need one
+ * time-step length; BDF batches and model libraries ("plugin:") run through exa_solve only. */
+int exa_batch_step(exa_handle* h, int* status, int n, int* iterating);
+/* Put the problem of datain_dir into slot inst of a batch whose problem in that slot has
+ * finished (status 0 or -1): the same element shape, model options, time scheme and steps and
+ * mesh (mesh.bin connectivity and boundary faces) as the slot's problem, any coordinates, initial
+ * state, auxiliary fields, physicsparam and tau (no longer than the batch's).  The slot's
+ * geometry and q-equation matrices are set up again and its Newton iteration (or its first time
+ * step, from the initial time) starts at the next exa_batch_step, beside the instances already
+ * under way. */
+int exa_batch_refill(exa_handle* h, int inst, const char* datain_dir);
 
 /* ---- the reference library's entry point ----
  * libmain replaces backend/Main/libexasim.h:5 (int libmain(int argc, char** argv), defined in

@@ -41,3 +41,25 @@ def test_served_results_equal_alone(tmp_path):
         worst = max(worst, err)
         assert err < 1e-8, (rid, model, err)
     print("served results vs FP64 oracle: largest relative error %.2e" % worst, flush=True)
+
+
+def test_served_refills_equal_alone(tmp_path):
+    """One batch slot at a time: every request of the slot's mesh that waits when the slot's
+    request finishes is admitted into the running batch (exa_batch_refill), and each result is
+    still bitwise the request solved alone."""
+    from exasim_amd import cases, serve
+    from exasim_amd.hdg import HDGSolver
+    reqs = cases.serve_stream(str(tmp_path), seconds=0.4, rate=40.0, sizes=(4, 6), orders=(2, 3), ns_steps=1)
+    print("served stream of %d requests, one slot" % len(reqs), flush=True)
+    srv = serve.Server(hbm_budget_bytes=8e9, max_batch=1, max_concurrent=1, precision=1, keep_results=True)
+    summ = srv.run_stream(reqs)
+    print("refills: %d, batches: %d" % (summ["refills"], summ["batches"]), flush=True)
+    assert summ["completed"] == len(reqs) and summ["failed"] == 0, summ
+    assert summ["refills"] > 0
+    assert any(r.get("refilled") for r in srv.records.values())
+    for rid, path, model, t in reqs:
+        with HDGSolver.batch([path], model) as S:
+            S.set_option("precision", srv.records[rid]["precision"])
+            S.solve()
+            uh = S.get_instance(0, "uh")
+        assert np.array_equal(uh, srv.results[rid]), rid

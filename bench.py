@@ -245,7 +245,7 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
                 mixed_refinement=BATCH_REFINE[kind])
 
 
-def run_serve(seconds, rate, check_alone=6):
+def run_serve(seconds, rate, check_alone=6, refill=True):
     """The served workload: a Poisson stream of Navier-Stokes (70 %) and convection-diffusion
     solve requests (n x n quads, n in {32, 64, 128}, p in {2..5}; see cases.serve_stream)
     replayed in real time into the continuous-batching server (HBM budget 200 GB, mixed
@@ -260,11 +260,11 @@ def run_serve(seconds, rate, check_alone=6):
     reqs = cases.serve_stream(work, seconds=seconds, rate=rate)
     print("# serve: %d requests written in %.1f s" % (len(reqs), time.perf_counter() - t0), file=sys.stderr,
           flush=True)
-    srv = serve.Server(hbm_budget_bytes=200e9, max_batch=64, max_concurrent=8, precision=1,
+    srv = serve.Server(hbm_budget_bytes=200e9, max_batch=64, max_concurrent=8, precision=1, refill=refill,
                        keep_results=check_alone > 0, window=0.02,
                        log=lambda msg: print("# serve: " + msg, file=sys.stderr, flush=True))
     summ = srv.run_stream(reqs)
-    print("# serve: %s" % json.dumps({k: summ[k] for k in ("completed", "solves_per_s", "p50_ms", "p99_ms")}),
+    print("# serve: %s" % json.dumps({k: summ[k] for k in ("completed", "solves_per_s", "p50_ms", "p99_ms", "refills")}),
           file=sys.stderr, flush=True)
     same, err64, by_model = [], 0.0, {}
     for rid, path, model, t in reqs[:check_alone]:
@@ -325,6 +325,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of the served request stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream in the c2 line")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
+    ap.add_argument("--serve-refill", type=int, default=1,
+                    help="1: running batches admit waiting requests between steps; 0: each batch solves to the end")
     ap.add_argument("--workload", default="c2", choices=["c2", "batch", "batch3d", "serve"],
                     help="c2: the steady 3D convection-diffusion solve (default, with the batch "
                          "workloads' results as sub-objects); batch: the 2D batch workload alone; "
@@ -348,7 +350,7 @@ def main():
     from exasim_amd.hdg import HDGSolver
 
     if args.workload == "serve":
-        sv = run_serve(args.serve_seconds, args.serve_rate)
+        sv = run_serve(args.serve_seconds, args.serve_rate, refill=bool(args.serve_refill))
         print(json.dumps({
             "metric": METRIC, "value": sv["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1,
             "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -552,7 +554,7 @@ def main():
             out["batch3d"] = {"error": repr(e)}
     if not args.no_serve:
         try:
-            out["serve"] = run_serve(args.serve_seconds, args.serve_rate)
+            out["serve"] = run_serve(args.serve_seconds, args.serve_rate, refill=bool(args.serve_refill))
         except Exception as e:
             out["serve"] = {"error": repr(e)}
     print(json.dumps(out))

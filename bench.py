@@ -156,10 +156,13 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
     groups = collections.defaultdict(list)
     for i, d in enumerate(inst):
         groups[d["porder"]].append(i)
-    # plain FP64 form: each instance alone, inputs resident (set-up outside the timed solve)
+    # plain FP64 form: each instance alone, inputs resident (set-up outside the timed solve; an
+    # untimed solve first, so that the GEMM tile tuner has timed the instance's product shapes)
     plain_t, uh_ref = [], {}
     for i, d in enumerate(inst):
         with HDGSolver(d["path"], model) as S:
+            S.solve()
+            S.reset()
             S.sync()
             t1 = time.perf_counter()
             S.solve()
@@ -325,6 +328,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of the served request stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream in the c2 line")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
+    ap.add_argument("--gemm-autotune", type=int, default=1,
+                    help="1: time the GEMM workgroup tiles per product shape (first call) and keep the fastest; "
+                         "0: the default 64x64 tile")
     ap.add_argument("--serve-refill", type=int, default=1,
                     help="1: running batches admit waiting requests between steps; 0: each batch solves to the end")
     ap.add_argument("--workload", default="c2", choices=["c2", "batch", "batch3d", "serve"],
@@ -333,6 +339,7 @@ def main():
                          "batch3d: the 3D convection-diffusion batch alone")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch workload in the c2 line")
     args = ap.parse_args()
+    os.environ["EXA_GEMM_AUTOTUNE"] = str(args.gemm_autotune)   # read by the library at its first GEMM
 
     if args.cpu_child:
         print(json.dumps(cpu_sample(args.porder, args.restart, args.cpu_sample_n, args.precond, args.se)))
@@ -373,7 +380,8 @@ def main():
                                    "{4,6,8,12}, p in {2,3,4}, 2 components, kappa log-uniform [0.01, 0.1]), "
                                    "one batch per p, mixed FP32/FP64 with refinement, elemental Schwarz, "
                                    "GMRES(100) CGS tol 1e-10"},
-            "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "batch3d": b, "roofline": None, "cpu_baseline": None}))
+            "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "batch3d": b, "roofline": None, "cpu_baseline": None,
+            "gemm_autotune": gemm_summary()}))
         return 0
     if args.workload == "batch":
         b = run_batch(args.steps, args.warmup, cpu_baseline=not args.no_cpu_baseline,
@@ -387,7 +395,7 @@ def main():
                                    "p in {2..5}), one batch per p, mixed FP32/FP64 with refinement, "
                                    "elemental Schwarz, GMRES(100) CGS tol 1e-10"},
             "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "batch": b, "roofline": None,
-            "cpu_baseline": b["cpu_baseline"]}))
+            "cpu_baseline": b["cpu_baseline"], "gemm_autotune": gemm_summary()}))
         return 0
 
     if args.copy_calibration:
@@ -557,8 +565,32 @@ def main():
             out["serve"] = run_serve(args.serve_seconds, args.serve_rate, refill=bool(args.serve_refill))
         except Exception as e:
             out["serve"] = {"error": repr(e)}
+    out["gemm_autotune"] = gemm_summary()
     print(json.dumps(out))
     return 0
+
+
+def gemm_summary(top=6):
+    """The GEMM tile autotuner's record for this run (exa_gemm_tuning): shapes timed, the tiles
+    chosen, and the time of the chosen tile against the default 64x64 tile, summed over the shapes
+    (each shape once); the largest shapes listed."""
+    from exasim_amd.hdg import gemm_tuning
+    rows = gemm_tuning()
+    if not rows:
+        return {"shapes": 0, "mode": os.environ.get("EXA_GEMM_AUTOTUNE", "1")}
+    hist = {}
+    for r in rows:
+        hist[r["tile"]] = hist.get(r["tile"], 0) + 1
+    def best(r):      # (-1: a tile the product cannot take)
+        return min(x for x in r["us"] if x > 0.0)
+    t_def = sum(r["us"][0] for r in rows)
+    t_best = sum(best(r) for r in rows)
+    big = sorted(rows, key=lambda r: -r["us"][0])[:top]
+    return {"shapes": len(rows), "tiles_chosen": hist, "default_us_sum": t_def, "chosen_us_sum": t_best,
+            "default_over_chosen": t_def / t_best if t_best > 0 else None,
+            "largest": [{"M": r["M"], "N": r["N"], "K": r["K"], "batch_log2": r["log2_batch"],
+                         "fp32": r["fp32"], "tile": r["tile"], "us_default": r["us"][0], "us_chosen": best(r)}
+                        for r in big]}
 
 
 if __name__ == "__main__":

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexahip.so")
 EXE = os.path.join(HERE, "gput2cEXASIM")
-SOURCES = ["kernels.hip", "datain.cpp", "conn.cpp", "bilu0.cpp", "solver.cpp", "gmres.cpp", "batch.cpp", "api.cpp", "noalloc.hip"]
+SOURCES = ["kernels.hip", "gemm.hip", "datain.cpp", "conn.cpp", "bilu0.cpp", "solver.cpp", "gmres.cpp", "batch.cpp", "api.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

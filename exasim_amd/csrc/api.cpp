@@ -4,7 +4,9 @@
 // there is no handle yet, in a thread-local slot).
 #include "exasim_hdg.h"
 #include "solver.hpp"
+#include "kernels.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -226,6 +228,15 @@ int exa_timings(exa_handle* h, double* ms, long* counts, int len) {
 }
 
 int exa_reset_timings(exa_handle* h) { return guard(h, [&](exa::Solver& s) { s.reset_timings(); }); }
+
+int exa_gemm_tuning(char* buf, int len) {
+  if (!buf || len < 1) return EXA_ERR_ARG;
+  const std::string r = exa::gemm_tuning_report();
+  const size_t n = std::min(r.size(), static_cast<size_t>(len - 1));
+  std::memcpy(buf, r.data(), n);
+  buf[n] = 0;
+  return static_cast<int>(r.size());
+}
 
 int exa_sync(exa_handle* h) {
   return guard(h, [&](exa::Solver& s) { EXA_HIP(hipStreamSynchronize(s.s)); });

@@ -25,82 +25,9 @@ namespace {
 
 inline unsigned nblocks(long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
-// ---------------------------------------------------------------------------
-// Batched FP64 GEMM: LDS-tiled, 256 threads, each thread a TM x TN micro-tile.
-// ---------------------------------------------------------------------------
-template <int BM, int BN, int BK, int TM, int TN>
-__global__ __launch_bounds__(256) void k_gemm(Gemm g, int tilesM) {
-  const int tile = blockIdx.x % tilesM;
-  const long b = blockIdx.x / tilesM;
-  const int b0 = static_cast<int>(b % g.nb0);
-  const long t1 = b / g.nb0;
-  const int b1 = static_cast<int>(t1 % g.nb1);
-  const int b2 = static_cast<int>(t1 / g.nb1);
-  const long a0 = g.iA ? g.iA[b0] : b0, bb0 = g.iB ? g.iB[b0] : b0, c0 = g.iC ? g.iC[b0] : b0;
-  const double* A = g.A + g.sA0 * a0 + g.sA1 * b1 + g.sA2 * b2;
-  const double* B = g.B + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
-  double* C = g.C + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
-  const int row0 = tile * BM, col0 = blockIdx.y * BN;
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    for (int idx = threadIdx.x; idx < BM * BK; idx += 256) {
-      int r = idx % BM, kk = idx / BM;
-      int gr = row0 + r, gk = k0 + kk;
-      As[kk][r] = (gr < g.M && gk < g.K) ? A[gr + g.lda * gk] : 0.0;
-    }
-    for (int idx = threadIdx.x; idx < BK * BN; idx += 256) {
-      int kk = idx % BK, c = idx / BK;
-      int gk = k0 + kk, gc = col0 + c;
-      Bs[kk][c] = (gk < g.K && gc < g.N) ? B[gk + g.ldb * gc] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; ++kk) {
-      double a[TM], bb[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-      for (int j = 0; j < TN; ++j) bb[j] = Bs[kk][ty + 16 * j];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      int r = row0 + tx + 16 * i, c = col0 + ty + 16 * j;
-      if (r < g.M && c < g.N) {
-        double v = g.alpha * acc[i][j];
-        if (g.beta != 0.0) v = fma(g.beta, C[r + g.ldc * c], v);
-        C[r + g.ldc * c] = v;
-      }
-    }
-}
-
-// FP64 batched GEMM on the matrix cores (v_mfma_f64_16x16x4f64).  Workgroup tile
-// 64 x 64, four waves of 32 x 32 (2 x 2 MFMA tiles), K staged through LDS 16 at a
-// time with the next K-tile prefetched into registers.  Lane layout of the
-// instruction: lane l supplies A[l%16][l/16] and B[l/16][l%16] of a 16x4 / 4x16
-// step; accumulator register i of lane l is D[l/16 + 4i][l%16].
+// MFMA operand vectors (the batched GEMMs themselves are in gemm.hip)
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
-constexpr int MG_BM = 64, MG_BN = 64, MG_BK = 32;
-
-// LDS is fragment-major: element (row r, k) of the A tile sits at
-// (r/16)*16*MG_BK + k*16 + r%16, so the 64 lanes of one MFMA operand read 64
-// consecutive values (no bank conflicts); B likewise with its columns.
-__device__ inline int mg_off(int rc, int k) { return (rc >> 4) * (16 * MG_BK) + k * 16 + (rc & 15); }
 
 // One 16x16x4 step.  FP64: register i of lane l holds D[l/16 + 4i][l%16];
 // FP32 (inputs rounded to FP32, FP32 accumulation): D[4(l/16) + i][l%16].
@@ -109,108 +36,6 @@ __device__ inline dbl4 mfma_step(double a, double b, dbl4 c) {
 }
 __device__ inline flt4 mfma_step(float a, float b, flt4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ inline int mfma_row(double, int lk, int i) { return lk + 4 * i; }
-__device__ inline int mfma_row(float, int lk, int i) { return 4 * lk + i; }
-
-// Workgroup w is dispatched to XCD w % 8 (each XCD has its own L2).  The logical
-// tile id L = (w % 8) * (nwg / 8) + w / 8 gives every XCD a contiguous range of L,
-// and L runs over the M/N tiles of one batch entry before the next, so the tiles
-// that share an operand run together on one XCD and reuse it from its L2.
-constexpr int NXCD = 8;
-
-// Operand storage: the double pointer, or the float one when the operand is stored in FP32.
-template <typename T> __device__ __forceinline__ const T* gsel(const double* d, const float* f);
-template <> __device__ __forceinline__ const double* gsel<double>(const double* d, const float*) { return d; }
-template <> __device__ __forceinline__ const float* gsel<float>(const double*, const float* f) { return f; }
-
-// CT: MFMA type; TA, TB, TC: storage of A, B, C
-template <typename CT, typename TA, typename TB, typename TC>
-__global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tilesN, long ntiles) {
-  typedef CT acc4 __attribute__((ext_vector_type(4)));
-  const long nwg = static_cast<long>(gridDim.x);   // a multiple of NXCD
-  const long L = (blockIdx.x % NXCD) * (nwg / NXCD) + blockIdx.x / NXCD;
-  if (L >= ntiles) return;
-  const int tpb = tilesM * tilesN;
-  const long b = L / tpb;
-  const int tt = static_cast<int>(L % tpb);
-  const int tile = tt % tilesM, tcol = tt / tilesM;
-  const int b0 = static_cast<int>(b % g.nb0);
-  const long t1 = b / g.nb0;
-  const int b1 = static_cast<int>(t1 % g.nb1);
-  const int b2 = static_cast<int>(t1 / g.nb1);
-  const long a0 = g.iA ? g.iA[b0] : b0, bb0 = g.iB ? g.iB[b0] : b0, c0 = g.iC ? g.iC[b0] : b0;
-  const TA* A = gsel<TA>(g.A, g.Af) + g.sA0 * a0 + g.sA1 * b1 + g.sA2 * b2;
-  const TB* B = gsel<TB>(g.B, g.Bf) + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
-  TC* C = const_cast<TC*>(gsel<TC>(g.C, g.Cf)) + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
-  const int row0 = tile * MG_BM, col0 = tcol * MG_BN;
-  __shared__ CT As[MG_BM * MG_BK];
-  __shared__ CT Bs[MG_BN * MG_BK];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave & 1, wc = wave >> 1;
-  // global -> register staging: A row tid%64, k = tid/64 + 4j; B k = tid%32, column tid/32 + 8j
-  const int ar = tid & 63, ak = tid >> 6;
-  const int bk = tid & 31, bc = tid >> 5;
-  constexpr int NL = MG_BM * MG_BK / 256;
-  TA ra[NL];
-  TB rb[NL];
-  bool ma[NL], mb[NL];
-  // out-of-range elements read a clamped (valid) address and are zeroed when stored to
-  // LDS, so the loads need no branches and stay in flight through the MFMA loop
-  const bool rowok = row0 + ar < g.M;
-  const long arow = min(row0 + ar, g.M - 1);
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int j = 0; j < NL; ++j) {
-      const int gk = k0 + ak + 4 * j;
-      ra[j] = A[arow + g.lda * min(gk, g.K - 1)];
-      ma[j] = rowok & (gk < g.K);
-      const int gk2 = k0 + bk, gc = col0 + bc + 8 * j;
-      rb[j] = B[min(gk2, g.K - 1) + g.ldb * static_cast<long>(min(gc, g.N - 1))];
-      mb[j] = (gk2 < g.K) & (gc < g.N);
-    }
-  };
-  acc4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc4{0, 0, 0, 0};
-  load(0);
-  const int li = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < g.K; k0 += MG_BK) {
-#pragma unroll
-    for (int j = 0; j < NL; ++j) {
-      As[mg_off(ar, ak + 4 * j)] = ma[j] ? static_cast<CT>(ra[j]) : CT(0);
-      Bs[mg_off(bc + 8 * j, bk)] = mb[j] ? static_cast<CT>(rb[j]) : CT(0);
-    }
-    __syncthreads();
-    load(k0 + MG_BK);  // unconditional: past the end it reads clamped, unused values
-#pragma unroll
-    for (int kk = 0; kk < MG_BK; kk += 4) {
-      const int kr = kk + lk;
-      const CT a0v = As[mg_off(32 * wr + li, kr)], a1v = As[mg_off(32 * wr + 16 + li, kr)];
-      const CT b0v = Bs[mg_off(32 * wc + li, kr)], b1v = Bs[mg_off(32 * wc + 16 + li, kr)];
-      acc[0][0] = mfma_step(a0v, b0v, acc[0][0]);
-      acc[0][1] = mfma_step(a0v, b1v, acc[0][1]);
-      acc[1][0] = mfma_step(a1v, b0v, acc[1][0]);
-      acc[1][1] = mfma_step(a1v, b1v, acc[1][1]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = row0 + 32 * wr + 16 * ti + mfma_row(CT(0), lk, i);
-        const int c = col0 + 32 * wc + 16 * tj + li;
-        if (r < g.M && c < g.N) {
-          double v = g.alpha * static_cast<double>(acc[ti][tj][i]);
-          if (g.beta != 0.0) v = fma(g.beta, static_cast<double>(C[r + g.ldc * static_cast<long>(c)]), v);
-          C[r + g.ldc * static_cast<long>(c)] = static_cast<TC>(v);
-        }
-      }
 }
 
 // ---------------------------------------------------------------------------
@@ -2806,43 +2631,6 @@ int generated_model(const std::string& name, int* nd, int* ncu, int* nv) {
   EXA_GENERATED_MODELS(EXA_GEN_FIND)
 #undef EXA_GEN_FIND
   return -1;
-}
-
-void gemm(const Gemm& g, hipStream_t s) {
-  if (g.M <= 0 || g.N <= 0) return;
-  const long batch = static_cast<long>(g.nb0) * g.nb1 * g.nb2;
-  if (batch <= 0) return;
-  const int store = (g.Af ? 1 : 0) | (g.Bf ? 2 : 0) | (g.Cf ? 4 : 0);
-  if (g.M <= 32 && g.N <= 32 && store == 0) {
-    int tm = (g.M + 31) / 32, tn = (g.N + 31) / 32;
-    dim3 grid(static_cast<unsigned>(tm * batch), tn);
-    k_gemm<32, 32, 16, 2, 2><<<grid, 256, 0, s>>>(g, tm);
-  } else {
-    int tm = (g.M + MG_BM - 1) / MG_BM, tn = (g.N + MG_BN - 1) / MG_BN;
-    const long ntiles = static_cast<long>(tm) * tn * batch;
-    const long nwg = (ntiles + NXCD - 1) / NXCD * NXCD;
-    if (nwg > 0x7fffffffL) throw Error(5, "GEMM batch too large for one launch");
-    const unsigned grid = static_cast<unsigned>(nwg);
-    switch (store) {
-#define EXA_GEMM_CASE(CODE, TA, TB, TC)                                                                   \
-  case CODE:                                                                                              \
-    if (g.fp32) k_gemm_mfma<float, TA, TB, TC><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);                  \
-    else k_gemm_mfma<double, TA, TB, TC><<<grid, 256, 0, s>>>(g, tm, tn, ntiles);                        \
-    break;
-      EXA_GEMM_CASE(0, double, double, double)
-      EXA_GEMM_CASE(1, float, double, double)
-      EXA_GEMM_CASE(2, double, float, double)
-      EXA_GEMM_CASE(3, float, float, double)
-      EXA_GEMM_CASE(4, double, double, float)
-      EXA_GEMM_CASE(5, float, double, float)
-      EXA_GEMM_CASE(6, double, float, float)
-      EXA_GEMM_CASE(7, float, float, float)
-#undef EXA_GEMM_CASE
-      default:
-        throw Error(5, "GEMM operand storage combination");
-    }
-  }
-  EXA_HIP(hipGetLastError());
 }
 
 // operands of a GEMM inside the LU: FP64 storage and products, or FP32 storage and products

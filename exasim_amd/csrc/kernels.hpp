@@ -37,6 +37,13 @@ struct Gemm {
   float* Cf = nullptr;
 };
 void gemm(const Gemm& g, hipStream_t s);
+// the tile autotuner of gemm (gemm.hip): mode 1 times the workgroup tiles on the first call of each
+// product shape, 0 always takes the default tile, -1 follows EXA_GEMM_AUTOTUNE (default 1)
+void set_gemm_autotune(int mode);
+// every MFMA product on tile 0..4 (64x64 32x64 64x32 128x64 64x128), or -1: the tuner's choice
+void set_gemm_tile(int tile);
+// one line per tuned shape: M N K log2(batch) fp32 storage chosen-tile and the microseconds of each tile
+std::string gemm_tuning_report();
 
 // In-place blocked LU with partial pivoting of the leading n x n block of each
 // augmented matrix A_b (n x ncols, ld n, batch stride `stride`), then back

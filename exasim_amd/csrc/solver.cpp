@@ -635,7 +635,12 @@ void Solver::set_option(const std::string& name, double v) {
     gm_lag = static_cast<int>(v);
   }
   else if (name == "verbose") verbose = static_cast<int>(v);
-  else if (name == "precision") {
+  else if (name == "gemm_autotune") {  // process-wide: 1 time the GEMM tiles per shape, 0 default tile
+    if (v != 0.0 && v != 1.0 && v != -1.0) throw Error(1, "gemm_autotune: 1, 0 or -1 (the environment's choice)");
+    set_gemm_autotune(static_cast<int>(v));
+  } else if (name == "gemm_tile") {  // process-wide: every MFMA product on tile v (0..4), -1 the tuner's
+    set_gemm_tile(static_cast<int>(v));
+  } else if (name == "precision") {
     int pv = static_cast<int>(v);
     if (pv != 0 && pv != 1) throw Error(1, "precision: 0 (FP64) or 1 (mixed FP32/FP64)");
     if (pv != precision) {

@@ -43,6 +43,7 @@ SIGNATURES = [
     ("exa_timings", ctypes.c_int, [_P, _D, _L, ctypes.c_int]),
     ("exa_reset_timings", ctypes.c_int, [_P]),
     ("exa_sync", ctypes.c_int, [_P]),
+    ("exa_gemm_tuning", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     ("exa_batch_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p,
                                         ctypes.c_int, ctypes.POINTER(_P)]),
     ("exa_batch_size", ctypes.c_int, [_P]),
@@ -53,6 +54,21 @@ SIGNATURES = [
     ("libmain", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
 ]
 TIMER_CLASSES = ["matvec", "precond", "ortho", "assembly", "lu", "other"]
+
+
+def gemm_tuning():
+    """The GEMM tile autotuner's choices so far (``exa_gemm_tuning``): a list of dicts with the
+    product shape, the chosen tile and each tile's time in microseconds."""
+    lib = load()
+    n = lib.exa_gemm_tuning(ctypes.create_string_buffer(1), 1)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.exa_gemm_tuning(buf, n + 1)
+    out = []
+    for line in buf.value.decode().splitlines():
+        f = line.split()
+        out.append(dict(M=int(f[0]), N=int(f[1]), K=int(f[2]), log2_batch=int(f[3]), fp32=int(f[4]),
+                        storage=int(f[5]), tile=f[6], us=[float(x) for x in f[7:]]))
+    return out
 
 
 class ExaError(RuntimeError):

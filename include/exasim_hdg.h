@@ -144,6 +144,17 @@ int exa_reset_timings(exa_handle* h);
 /* Synchronise the handle's stream. */
 int exa_sync(exa_handle* h);
 
+/* The batched GEMMs' tile autotuner (no counterpart in the reference, whose batched products are
+ * library calls): the workgroup tile of every product shape timed so far in this process, one line
+ * per shape, "M N K log2(batch) fp32 storage BMxBN" followed by the microseconds of each of the five
+ * tiles (64x64 32x64 64x32 128x64 64x128; -1 for a tile the product cannot take: a product
+ * computed in place needs a tile that holds all its rows).  Writes at most len - 1 characters and a
+ * terminating zero; returns the length of the whole report.  exa_set_option(h, "gemm_autotune", 0)
+ * turns the timing off for the process (1 on, the default; the environment variable
+ * EXA_GEMM_AUTOTUNE sets the start value); "gemm_tile" 0..4 puts every product on one tile (-1: the
+ * tuner's choice).  Every tile gives bitwise the same products. */
+int exa_gemm_tuning(char* buf, int len);
+
 /* ---- batches of independent problems (no counterpart in the reference, which runs one
  * problem per process; each instance follows exa_solve's algorithm: NewtonSolver,
  * backend/Solution/solution.cpp:125-242, and GMRES, backend/Solver/gmres.cpp:361-597).

@@ -53,6 +53,24 @@ PREC_NAME = {0: "fp64", 1: "mixed"}
 PREC_ID = {"fp64": 0, "mixed": 1}
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "convdiff3d_32cube_p4_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the matvec and the block-ILU(0) apply on the 32^3 p=4 workload, from the
+    two counter passes (FETCH_SIZE, WRITE_SIZE) summarised in the profile file, each scaled by what the
+    counter reads for an elementwise copy of known size. Empty when the file is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return {k: float(v) for k, v in json.loads(line.split(":", 1)[1]).items()}
+    except OSError:
+        pass
+    return {}
+
+
 def build_case(path, n, porder, precond, restart, tol, maxit, se):
     from exasim_amd import cases
     cart = [3, n, n, n, se, se, se, 0] if precond == "bilu0" else None
@@ -428,6 +446,7 @@ def main():
     print("# setup (read, connectivity, geometry, q-equation) %.2f s; sizes %s" % (t_setup, z),
           file=sys.stderr, flush=True)
     ne, m, n, ncu, N = z["ne"], z["m"], z["n"], z["ncu"], z["N"]
+    traffic = measured_traffic() if (args.n, args.porder, args.se, args.precond) == (32, 4, 2, "bilu0") else {}
 
     def leg(precision):
         """W untimed + K timed solves from the input state at one precision."""
@@ -463,11 +482,26 @@ def main():
         bytes_mv = sH * m * m * ne + 8.0 * m * ne + 4.0 * (m // ncu) * ne + 8.0 * m * ne
         achieved = bytes_mv / mv_avg_s / 1e9 if mv_cnt else None
         lat_ms = np.array(lat) * 1e3
+        kmv = "k_matvec_elem<%s>" % ("float" if precision else "double")
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic.get(kmv),
+                "kernel": kmv, "bytes_per_launch": bytes_mv, "avg_launch_ms": mv_avg_s * 1e3}
+        st = S.state()
+        pc_roof = None
+        if st["preconditioner"] == 2 and st["ilu_nnz"] > 0:
+            # the block-ILU(0) apply: its factored blocks are read once per application
+            pc_ms, pc_cnt = tm["precond"]
+            ncf = z["npf"] * ncu
+            s_f = (2.0 if not args.precond_fp32 else 4.0) if precision else 8.0
+            bytes_pc = s_f * ncf * ncf * st["ilu_nnz"] * st["ilu_nse"]
+            pc_avg_s = pc_ms / max(pc_cnt, 1) / 1e3
+            kpc = "k_bilu_apply<%s>" % (("bf16" if not args.precond_fp32 else "float") if precision else "double")
+            ach = bytes_pc / pc_avg_s / 1e9 if pc_cnt else None
+            pc_roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": traffic.get(kpc),
+                       "kernel": kpc, "bytes_per_launch": bytes_pc, "avg_launch_ms": pc_avg_s * 1e3}
         return dict(solves_s=args.steps / t_all, lat_ms=lat_ms, gm=gm, res=res, tm=tm, uh=uh, refs=refs,
-                    roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                              "kernel": "k_matvec_elem<%s>" % ("float" if precision else "double"),
-                              "bytes_per_launch": bytes_mv, "avg_launch_ms": mv_avg_s * 1e3})
+                    roofline=roof, precond_roofline=pc_roof)
 
     legs = {}
     for p in ([0, 1] if args.precision == "both" else [PREC_ID[args.precision]]):
@@ -540,6 +574,7 @@ def main():
         "final_residual": head["res"],
         "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in tm.items()},
         "roofline": head["roofline"],
+        "precond_roofline": head["precond_roofline"],
         "cpu_baseline": cpu,
     }
     if 0 in legs:
@@ -547,6 +582,7 @@ def main():
         out["fp64"] = {"solves_per_s": L["solves_s"], "p50_ms": float(np.percentile(L["lat_ms"], 50)),
                        "p99_ms": float(np.percentile(L["lat_ms"], 99)), "gmres_iterations": L["gm"],
                        "final_residual": L["res"], "roofline": L["roofline"],
+                       "precond_roofline": L["precond_roofline"],
                        "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in L["tm"].items()}}
     if mixed and 0 in legs:
         out["mixed_over_fp64"] = solves_s / legs[0]["solves_s"]

@@ -143,9 +143,11 @@ int exa_sizes(const exa_handle* h, int* out, int len) {
   if (!h || !h->solver || !out) return 0;
   const exa::Solver& s = *h->solver;
   const exa::Dims& d = s.d;
-  int v[19] = {d.nd, d.ncu, d.nc, d.npe, d.npf, d.nfe, d.ne, d.nf, d.n, d.m, static_cast<int>(d.N),
-               d.nge, d.ngf, d.nco, s.restart, s.precond, s.precision, s.last_refinements, s.neb};
-  int k = len < 19 ? len : 19;
+  const bool ilu = s.ilu_ready;
+  int v[22] = {d.nd, d.ncu, d.nc, d.npe, d.npf, d.nfe, d.ne, d.nf, d.n, d.m, static_cast<int>(d.N),
+               d.nge, d.ngf, d.nco, s.restart, s.precond, s.precision, s.last_refinements, s.neb,
+               ilu ? s.ilu.nse : 0, ilu ? s.ilu.nfse : 0, ilu ? s.ilu.nnz : 0};
+  int k = len < 22 ? len : 22;
   for (int i = 0; i < k; ++i) out[i] = v[i];
   return k;
 }

@@ -445,7 +445,7 @@ std::string gemm_tuning_report() {
                   std::get<4>(k), std::get<5>(k), std::get<6>(k), TILES[e.tile].bm, TILES[e.tile].bn);
     out += line;
     for (int t = 0; t < NTILE; ++t) {
-      std::snprintf(line, sizeof line, " %.1f", e.us[t]);
+      std::snprintf(line, sizeof line, " %.3f", e.us[t]);
       out += line;
     }
     out += "\n";

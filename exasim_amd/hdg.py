@@ -176,7 +176,8 @@ class HDGSolver:
         self._check(self._lib.exa_batch_refill(self._h, int(inst), datain.encode()))
 
     SIZE_KEYS = ["nd", "ncu", "nc", "npe", "npf", "nfe", "ne", "nf", "n", "m", "N", "nge", "ngf",
-                 "nco", "restart", "preconditioner", "precision", "refinements", "neb"]
+                 "nco", "restart", "preconditioner", "precision", "refinements", "neb",
+                 "ilu_nse", "ilu_nfse", "ilu_nnz"]
 
     def state(self):
         """exa_sizes: the sizes plus the current preconditioner, precision and the number of

@@ -59,10 +59,12 @@ int exa_create(const char* datain_dir, const char* dataout_prefix, const char* m
 void exa_destroy(exa_handle* h);
 const char* exa_last_error(const exa_handle* h);
 
-/* Sizes and state: writes up to 19 ints: nd, ncu, nc, npe, npf, nfe, ne, nf, n, m,
+/* Sizes and state: writes up to 22 ints: nd, ncu, nc, npe, npf, nfe, ne, nf, n, m,
  * N (= ndof uhat), nge, ngf, nco, restart, preconditioner, precision, refinements
  * (corrections made in the last Newton step of a mixed-precision solve), neb (elements per
- * assembly block, set by the workspace_bytes option).  Returns the count written. */
+ * assembly block, set by the workspace_bytes option), and the block-ILU(0) structure once it
+ * is built: superelements, face slots per superelement, blocks per superelement (0 before).
+ * Returns the count written. */
 int exa_sizes(const exa_handle* h, int* out, int len);
 
 /* Options (name, value): "gmres_tol", "gmres_restart" (1..2000), "gmres_maxit",

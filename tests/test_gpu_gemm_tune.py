@@ -37,8 +37,9 @@ def test_tiles_give_identical_solves(tmp_path, precision):
     for r in rows:
         us = np.array(r["us"])          # -1: a tile the product cannot take (computed in place)
         assert r["tile"] in tiles and len(us) == 5 and us[0] > 0.0 and np.all((us > 0.0) | (us == -1.0)), r
-        best = ["64x64", "32x64", "64x32", "128x64", "64x128"][int(np.argmin(np.where(us > 0.0, us, np.inf)))]
-        assert r["tile"] == best, r
+        # the chosen tile is the fastest (times are reported to 1 ns, so two tiles may print alike)
+        chosen = us[["64x64", "32x64", "64x32", "128x64", "64x128"].index(r["tile"])]
+        assert chosen > 0.0 and chosen <= us[us > 0.0].min() + 1.5e-3, r
     say("tuned shapes: %d, tiles %s" % (len(rows), sorted({r["tile"] for r in rows})))
 
 

@@ -17,6 +17,8 @@
 #include "generated/registry.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <type_traits>
 
 namespace exa {
@@ -2151,6 +2153,121 @@ __global__ __launch_bounds__(256) void k_bilu_apply(Dims d, IluDev I, const T* B
   }
 }
 
+// The same sweeps with one wave per superelement (BF16 factors; ncf = nparts CW with
+// nparts = 64 / (ncf / VW)): lane (rr, part) accumulates rows VW*rr .. VW*rr+VW-1 over columns
+// part*CW .. part*CW+CW-1 of every block, the parts are added by lane shuffles in a fixed order,
+// and no workgroup barrier is needed (the wave's LDS operations complete in order). The sweeps
+// are one flat program of (block, vector offset) pairs in 3 nfse segments (forward rows
+// 0..nfse-1, then for i = nfse-1..0 the U row and the diagonal block, whose vector z sits after
+// v), read with uniform indices (scalar loads), so the LDS holds only the superelement's vector
+// and 16 superelements share a CU. Blocks are walked one at a time (CW loads per lane), the next
+// block's loads issued before the current block's FMAs, across segment ends: the loads never wait
+// for a row's result.
+template <typename T, int VW, int NCF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_bilu_apply_wave(Dims d, IluDev I, const T* BE, const double* x, double* Xo, const double* act) {
+  if (act && act[0] == 0.0) return;  // a finished solve (grid-uniform)
+  typedef T vecT __attribute__((ext_vector_type(VW)));
+  constexpr int nrow = NCF / VW;
+  constexpr int nparts = 64 / nrow;
+  constexpr int CW = NCF / nparts;
+  static_assert(NCF % VW == 0 && nparts * CW == NCF, "the wave's parts must tile the block's columns");
+  constexpr int ncf = NCF;
+  const int nfse = I.nfse;
+  const long n = blockIdx.x;
+  constexpr long per = static_cast<long>(NCF) * NCF;
+  const long bstride = static_cast<long>(I.nse) * per;
+  const int lane = threadIdx.x;
+  const int rr = lane % nrow, part = lane / nrow;
+  const bool active = part < nparts;
+  const int r0 = VW * rr;
+  const int c0 = active ? part * CW : 0;   // idle lanes (part >= nparts) read part 0's columns
+  // the program is read-only for the kernel's lifetime: constant address space (scalar loads)
+  typedef const int __attribute__((address_space(4)))* cptr;
+  const cptr wseg = reinterpret_cast<cptr>(reinterpret_cast<uintptr_t>(I.wseg));
+  const cptr wblk = reinterpret_cast<cptr>(reinterpret_cast<uintptr_t>(I.wblk));
+  const cptr wvec = reinterpret_cast<cptr>(reinterpret_cast<uintptr_t>(I.wvec));
+  extern __shared__ double sm[];
+  float* v = reinterpret_cast<float*>(sm);  // nfse * ncf, then z (ncf)
+  for (int idx = lane; idx < nfse * ncf; idx += 64) {
+    const int i = idx / ncf, q = idx % ncf;
+    v[idx] = static_cast<float>(x[static_cast<long>(I.face[n + static_cast<long>(I.nse) * i]) * ncf + q]);
+  }
+  __syncthreads();
+  // this lane's column c0 of block 0 (the columns' offsets are compile-time constants)
+  const T* base = BE + n * per + r0 + static_cast<long>(ncf) * c0;
+  const int nseg = 3 * nfse;
+  const int total = wseg[nseg];   // blocks in the program (>= nfse: every row has its diagonal)
+  auto issue = [&](int q, vecT (&raw)[CW]) {
+    const T* p = base + wblk[min(q, total - 1)] * bstride;
+#pragma unroll
+    for (int u = 0; u < CW; ++u) raw[u] = *reinterpret_cast<const vecT*>(p + ncf * u);
+  };
+  double acc[VW];
+#pragma unroll
+  for (int j = 0; j < VW; ++j) acc[j] = 0.0;
+  auto consume = [&](int q, const vecT (&raw)[CW]) {
+    const float* y = v + wvec[q] + c0;
+#pragma unroll
+    for (int u = 0; u < CW; ++u) {
+      const double yc = static_cast<double>(y[u]);
+#pragma unroll
+      for (int j = 0; j < VW; ++j) acc[j] = fma(blk_val<T>(raw[u][j]), yc, acc[j]);
+    }
+  };
+  // segment s is complete: add the parts and store its row's result
+  auto finish = [&](int sg) {
+    double tot[VW];
+#pragma unroll
+    for (int j = 0; j < VW; ++j) {
+      const double a = active ? acc[j] : 0.0;
+      acc[j] = a;
+      tot[j] = 0.0;
+    }
+    for (int p = 0; p < nparts; ++p)
+#pragma unroll
+      for (int j = 0; j < VW; ++j) tot[j] += __shfl(acc[j], rr + p * nrow, 64);
+    const int kind = sg < nfse ? 0 : 1 + ((sg - nfse) & 1);
+    const int row = sg < nfse ? sg : nfse - 1 - (sg - nfse) / 2;
+    if (lane < nrow) {
+      float* vi = v + row * ncf + r0;
+      float* zi = v + nfse * ncf + r0;
+#pragma unroll
+      for (int j = 0; j < VW; ++j) {
+        if (kind == 0) vi[j] = static_cast<float>(static_cast<double>(vi[j]) - tot[j]);
+        else if (kind == 1) zi[j] = static_cast<float>(static_cast<double>(vi[j]) - tot[j]);
+        else vi[j] = static_cast<float>(tot[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < VW; ++j) acc[j] = 0.0;
+  };
+  int sg = 0;
+  auto finish_through = [&](int q) {  // every segment that ends at or before block q
+    while (sg < nseg && wseg[sg + 1] <= q) finish(sg++);
+  };
+  finish_through(0);  // segments with no blocks before the first one
+  vecT ra[CW], rb[CW];
+  issue(0, ra);
+  // (the scheduling barriers keep the next block's loads ahead of the current block's FMAs)
+  for (int q = 0; q < total; q += 2) {
+    issue(q + 1, rb);   // past the last block: loads of the last block again, not used
+    __builtin_amdgcn_sched_barrier(0);
+    consume(q, ra);
+    finish_through(q + 1);
+    if (q + 1 >= total) break;
+    issue(q + 2, ra);
+    __builtin_amdgcn_sched_barrier(0);
+    consume(q + 1, rb);
+    finish_through(q + 2);
+  }
+  __syncthreads();
+  for (int idx = lane; idx < nfse * ncf; idx += 64) {
+    const int i = idx / ncf, qv = idx % ncf;
+    Xo[(n + static_cast<long>(I.nse) * i) * ncf + qv] = v[idx];
+  }
+}
+
 // x[f] = Xo[slot1[f]] + Xo[slot2[f]] (faces shared by two superelements get both)
 __global__ void k_bilu_scatter(Dims d, IluDev I, const double* Xo, double* out, const double* act) {
   if (act && act[0] == 0.0) return;
@@ -3125,6 +3242,20 @@ void bilu_put_l(const Dims& d, const IluDev& I, int k0, int ne_i, const float* L
   bilu_put_l_t(d, I, k0, ne_i, Ltmp, BE, s);
 }
 
+// the block-ILU(0) apply for BF16 factors: one wave per superelement (1) or the 256-thread kernel
+// (0); -1 follows EXA_BILU_WAVE (default 1)
+std::atomic<int> g_bilu_wave{-1};
+void set_bilu_wave(int mode) { g_bilu_wave.store(mode); }
+bool bilu_wave_enabled() {
+  const int m = g_bilu_wave.load();
+  if (m >= 0) return m != 0;
+  static const bool env_on = [] {
+    const char* e = std::getenv("EXA_BILU_WAVE");
+    return !e || std::atoi(e) != 0;
+  }();
+  return env_on;
+}
+
 template <typename T>
 void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, double* Y, double* Xo,
                 double* out, hipStream_t s, const double* act) {
@@ -3142,8 +3273,23 @@ void bilu_apply(const Dims& d, const IluDev& I, const T* BE, const double* x, do
                                   static_cast<int>(shm)));
     kern<<<I.nse, 256, shm, s>>>(d, I, BE, x, Xo, act);
   };
-  if (vw == 2) launch(&k_bilu_apply<T, 2>);
-  else launch(&k_bilu_apply<T, 1>);
+  // one wave per superelement for FP32 / BF16 factors when a wave covers a block's rows
+  const size_t shm_w = (static_cast<size_t>(I.nfse) * d.ncf + d.ncf) * sizeof(float);
+  bool wave = false;
+  if constexpr (sizeof(T) == 2) {
+    // the one-wave kernel's shapes: ncf = 50 (p = 4, two components on hex faces): 25 row pairs,
+    // 2 parts of 25 columns
+    if (d.ncf == 50 && shm_w <= 64 * 1024 && I.wseg && bilu_wave_enabled()) {
+      wave = true;
+      k_bilu_apply_wave<T, 2, 50><<<I.nse, 64, shm_w, s>>>(d, I, BE, x, Xo, act);
+    }
+  }
+  if (wave) {
+  } else if (vw == 2) {
+    launch(&k_bilu_apply<T, 2>);
+  } else {
+    launch(&k_bilu_apply<T, 1>);
+  }
   k_bilu_scatter<<<nblocks(static_cast<long>(d.nf) * d.ncf, 256), 256, 0, s>>>(d, I, Xo, out, act);
   EXA_HIP(hipGetLastError());
 }

@@ -40,6 +40,9 @@ void gemm(const Gemm& g, hipStream_t s);
 // the tile autotuner of gemm (gemm.hip): mode 1 times the workgroup tiles on the first call of each
 // product shape, 0 always takes the default tile, -1 follows EXA_GEMM_AUTOTUNE (default 1)
 void set_gemm_autotune(int mode);
+// the block-ILU(0) apply of FP32 / BF16 factors: 1 one wave per superelement, 0 the 256-thread
+// kernel, -1 follows EXA_BILU_WAVE (default 1)
+void set_bilu_wave(int mode);
 // every MFMA product on tile 0..4 (64x64 32x64 64x32 128x64 64x128), or -1: the tuner's choice
 void set_gemm_tile(int tile);
 // one line per tuned shape: M N K log2(batch) fp32 storage chosen-tile and the microseconds of each tile
@@ -218,6 +221,10 @@ struct IluDev {
   const int *eptr, *elim_ji, *uptr, *upd_jl, *upd_il, *upd_ji;
   const int *lptr, *lidx, *lcol, *uuptr, *uidx, *ucol;
   const int *slot1, *slot2;
+  // the apply's sweeps as one program of (block, vector offset) pairs in 3 nfse segments:
+  // forward rows 0..nfse-1 (L blocks, offset col*ncf), then for i = nfse-1..0 the U row and
+  // the diagonal block (offset nfse*ncf: the intermediate vector)
+  const int *wseg, *wblk, *wvec;
 };
 void bilu_assemble(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const double* H,
                    double* BE, hipStream_t s);

@@ -552,6 +552,29 @@ void Solver::setup_bilu0() {
   ilud.ucol = up(iucol, ilu.ucol);
   ilud.slot1 = up(islot1, ilu.slot1);
   ilud.slot2 = up(islot2, ilu.slot2);
+  {  // the apply's sweeps as one flat program (ApplyBlockILU0's loop order)
+    std::vector<int> wseg(1, 0), wblk, wvec;
+    for (int i = 0; i < ilu.nfse; ++i) {
+      for (int q = ilu.lptr[i]; q < ilu.lptr[i + 1]; ++q) {
+        wblk.push_back(ilu.lidx[q]);
+        wvec.push_back(ilu.lcol[q] * d.ncf);
+      }
+      wseg.push_back(static_cast<int>(wblk.size()));
+    }
+    for (int i = ilu.nfse - 1; i >= 0; --i) {
+      for (int q = ilu.uuptr[i]; q < ilu.uuptr[i + 1]; ++q) {
+        wblk.push_back(ilu.uidx[q]);
+        wvec.push_back(ilu.ucol[q] * d.ncf);
+      }
+      wseg.push_back(static_cast<int>(wblk.size()));
+      wblk.push_back(ilu.diag[i]);
+      wvec.push_back(ilu.nfse * d.ncf);
+      wseg.push_back(static_cast<int>(wblk.size()));
+    }
+    ilud.wseg = up(iwseg, wseg);
+    ilud.wblk = up(iwblk, wblk);
+    ilud.wvec = up(iwvec, wvec);
+  }
   BEilu.alloc(static_cast<size_t>(ilu.nnz) * ilu.nse * d.ncf * d.ncf);
   Yilu.alloc(static_cast<size_t>(ilu.nfse) * ilu.nse * d.ncf);
   Xilu.alloc(static_cast<size_t>(ilu.nfse) * ilu.nse * d.ncf);
@@ -638,6 +661,9 @@ void Solver::set_option(const std::string& name, double v) {
   else if (name == "gemm_autotune") {  // process-wide: 1 time the GEMM tiles per shape, 0 default tile
     if (v != 0.0 && v != 1.0 && v != -1.0) throw Error(1, "gemm_autotune: 1, 0 or -1 (the environment's choice)");
     set_gemm_autotune(static_cast<int>(v));
+  } else if (name == "ilu_wave") {  // process-wide: FP32/BF16 block-ILU(0) apply, 1 one wave per superelement
+    if (v != 0.0 && v != 1.0 && v != -1.0) throw Error(1, "ilu_wave: 1, 0 or -1 (the environment's choice)");
+    set_bilu_wave(static_cast<int>(v));
   } else if (name == "gemm_tile") {  // process-wide: every MFMA product on tile v (0..4), -1 the tuner's
     set_gemm_tile(static_cast<int>(v));
   } else if (name == "precision") {

@@ -109,7 +109,7 @@ struct Solver {
   Bilu0Struct ilu;
   bool ilu_ready = false;
   DevBuf<int> iface, irow_ptr, icol_ind, irow_of, idiag, ieptr, ielim_ji, iuptr, iupd_jl, iupd_il, iupd_ji,
-      ilptr, ilidx, ilcol, iuuptr, iuidx, iucol, islot1, islot2;
+      ilptr, ilidx, ilcol, iuuptr, iuidx, iucol, islot1, islot2, iwseg, iwblk, iwvec;
   DevBuf<double> BEilu, Yilu, Xilu, Lilu;
   IluDev ilud{};
   // GMRES: Krylov basis V, vectors, dot partials; the Krylov scalars of every instance live on the

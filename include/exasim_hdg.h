@@ -87,7 +87,10 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * "gmres_lag" (0..30, default 3): GMRES steps queued ahead of the host's convergence check (the
  * Krylov scalars stay on the device; steps queued after convergence do no work);
  * "gmres_graph" (0/1, default 0): replay the Newton loop's GMRES steps from captured HIP graphs;
- * "noalloc_chunk" (elements, default 0 = as many as fit): element chunk of exa_matvec_noalloc. */
+ * "noalloc_chunk" (elements, default 0 = as many as fit): element chunk of exa_matvec_noalloc;
+ * "ilu_wave" (process-wide; 1 default, 0, or -1 for the EXA_BILU_WAVE environment value): the
+ * block-ILU(0) apply of FP32 / BF16 factors with one wave per superelement (1) or with a
+ * 256-thread workgroup per superelement (0). */
 int exa_set_option(exa_handle* h, const char* name, double value);
 
 /* Full steady solve (NewtonSolver).  newton_iters/gmres_iters/final_norm may be NULL. */

@@ -65,7 +65,7 @@ def c2cfg(case_dir):
     Q = ora.HDGProblem(path, "convdiff")
     ora.newton_solve(Q)
     return dict(path=path, P=P, S=S_, b=b, BE=BE, v=v, pv=pv, xo=xo, its=its_o, uh=Q.uh.reshape(-1),
-                udg=Q.udg)
+                udg=Q.udg, ilu=(ind_ii, L, U, face), ncf=ncf)
 
 
 def test_c2cfg_fp64_stages(c2cfg):
@@ -102,6 +102,47 @@ def test_c2cfg_fp64_stages(c2cfg):
     say("  gmres iterations: gpu %d oracle %d" % (its, o["its"]))
     assert its == o["its"]
     assert rel(xg, o["xo"]) < 1e-9
+
+
+def bf16_round(a):
+    """Round to BF16 (nearest, ties to even) as the library's FP32 -> BF16 conversion does."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("bf16", [0, 1])
+def test_c2cfg_mixed_ilu_apply(c2cfg, bf16):
+    """The block-ILU(0) apply of the mixed form's FP32 / BF16 factors, with one wave per
+    superelement and with the 256-thread kernel, against the oracle's sweeps (ApplyBlockILU0,
+    preconditioner.cpp:184-270) in FP64 on the same rounded factors. The kernels keep the
+    superelement's vector in FP32 between row steps, hence the 1e-5 bound."""
+    from exasim_amd.hdg import HDGSolver
+    say("case c2cfg: mixed form block-ILU(0) apply, %s factors, wave and 256-thread kernels"
+        % ("BF16" if bf16 else "FP32"))
+    o = c2cfg
+    with HDGSolver(o["path"], "convdiff") as S:
+        S.set_option("precision", 1)
+        S.set_option("precond_bf16", bf16)
+        S.get_q()
+        S.assemble()
+        BEg = S.get("BEilu").reshape(o["BE"].shape[0], o["BE"].shape[1], 50, 50).transpose(0, 1, 3, 2)
+        pw, _ = S.device_ptr("w")
+        out = {}
+        for wave in (1, 0):
+            S.set_option("ilu_wave", wave)
+            S.set("w", o["v"])
+            S.precond(pw)
+            out[wave] = S.get("w")
+        S.set_option("ilu_wave", -1)
+    # the FP32 factors are the FP32 factorisation of the FP32 condensed matrices: close to FP64's
+    assert rel(BEg, o["BE"]) < 1e-3
+    BEr = bf16_round(BEg) if bf16 else BEg.copy()
+    ind_ii, L, U, face = o["ilu"]
+    pv = obilu.apply(BEr, ind_ii, L, U, face, o["v"], o["ncf"], o["P"].nf)
+    e1, e0, e10 = rel(out[1], pv), rel(out[0], pv), rel(out[1], out[0])
+    say("  rel err vs oracle: wave %.2e, 256-thread %.2e; between kernels %.2e" % (e1, e0, e10))
+    assert e1 < 1e-5 and e0 < 1e-5 and e10 < 1e-5
 
 
 def test_c2cfg_fp64_solve(c2cfg):

@@ -1177,31 +1177,43 @@ __global__ void k_vol_res(Dims d, AsmArgs a, T* out, long ldo) {
 }
 
 // W[(e*NC + c)*NCU + k][i + npe*g] = sum_dp shapegw[i, g, dp] * rJ[dp, k, c]
-// One thread per (i, g, element) for all ncu*nc (k, c) blocks: the shape weights shapegw[i, g, :]
-// are loaded once and reused (the sum over dp runs in the same order as per block).
+// One workgroup per (element, gp consecutive points): the points' rJ values (nd+1) ncu nc each are
+// staged in LDS once, then one thread per (i, g) forms all ncu*nc (k, c) blocks from LDS (the sum
+// over dp runs in the same order for every block); the stores run along i (coalesced).
 template <typename T>
-__global__ void k_vol_W(Dims d, AsmArgs a, T* W) {
+__global__ __launch_bounds__(256) void k_vol_W(Dims d, AsmArgs a, T* W, int gp) {
   constexpr int MAXD = 4;  // nd + 1 <= 4
+  extern __shared__ double vw_sm[];
   const int npe = d.npe, nge = d.nge, ncu = d.ncu, nc = d.nc, nd = d.nd;
-  const int blk = npe * nge;
-  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (q >= static_cast<long>(blk) * a.eb) return;
-  const long e = q / blk;
-  const int ig = static_cast<int>(q - e * blk);
-  const int g = ig / npe, i = ig - g * npe;
+  const int blk = npe * nge, L = (nd + 1) * ncu * nc;
+  const int ngrp = (nge + gp - 1) / gp;
+  const long e = blockIdx.x / ngrp;
+  const int g0 = static_cast<int>(blockIdx.x % ngrp) * gp;
+  const int ng = min(gp, nge - g0);
   const long P = static_cast<long>(a.eb) * nge;
-  double w[MAXD];
+  const int tid = threadIdx.x;
+  // vw_sm[gl * L + (dp*ncu + k)*nc + c] = rJ[((dp*ncu + k)*nc + c)*P + e*nge + g0 + gl]
+  for (int idx = tid; idx < L * ng; idx += 256) {
+    const int gl = idx % ng, j = idx / ng;
+    vw_sm[gl * L + j] = a.rJ[j * P + e * nge + g0 + gl];
+  }
+  __syncthreads();
+  T* We = W + e * static_cast<long>(nc) * ncu * blk;
+  for (int idx = tid; idx < npe * ng; idx += 256) {
+    const int i = idx % npe, gl = idx / npe, g = g0 + gl;
+    double w[MAXD];
 #pragma unroll
-  for (int dp = 0; dp < MAXD; ++dp) w[dp] = dp <= nd ? a.shapegw[i + npe * (g + nge * dp)] : 0.0;
-  const double* rJ = a.rJ + e * nge + g;
-  T* We = W + e * static_cast<long>(nc) * ncu * blk + ig;
-  for (int kc = 0; kc < ncu * nc; ++kc) {
-    const int k = kc % ncu, c = kc / ncu;
-    double s = 0.0;
+    for (int dp = 0; dp < MAXD; ++dp) w[dp] = dp <= nd ? a.shapegw[i + npe * (g + nge * dp)] : 0.0;
+    const double* r = vw_sm + gl * L;
+    const int ig = i + npe * g;
+    for (int kc = 0; kc < ncu * nc; ++kc) {
+      const int k = kc % ncu, c = kc / ncu;
+      double sum = 0.0;
 #pragma unroll
-    for (int dp = 0; dp < MAXD; ++dp)
-      if (dp <= nd) s += w[dp] * rJ[((dp * ncu + k) * nc + c) * P];
-    We[static_cast<long>(c * ncu + k) * blk] = static_cast<T>(s);
+      for (int dp = 0; dp < MAXD; ++dp)
+        if (dp <= nd) sum += w[dp] * r[(dp * ncu + k) * nc + c];
+      We[static_cast<long>(c * ncu + k) * blk + ig] = static_cast<T>(sum);
+    }
   }
 }
 
@@ -1879,47 +1891,60 @@ __global__ void k_update_udg(Dims d, const double* du, double alpha, double* udg
 // hdgBlockILU0 (matvec.cpp:119-166), ApplyBlockILU0 (preconditioner.cpp:184-270).
 // BE[((t*nse) + n)*ncf*ncf + r + ncf*c]: CRS entry t of superelement n.
 // ---------------------------------------------------------------------------
-// grid: x over (entry rc, superelement n) of CRS entry t = blockIdx.y (32-bit index math)
+// grid: (superelement n, CRS entry t)
 template <typename T, typename TH>
-__global__ void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon, const TH* H, T* BE) {
+__global__ __launch_bounds__(256) void k_bilu_assemble(Dims d, IluDev I, const int* f2e, const int* elemcon,
+                                                       const TH* H, T* BE) {
+  // one workgroup per (superelement n, block slot t): the block's element(s), local faces and node
+  // permutations are found once and kept in LDS as H row / column offsets; the ncf x ncf entries are
+  // then gathered column by column (the diagonal block of a face shared by two elements adds both)
+  __shared__ int rowoff[2][256], coloff[2][256];
   const int ncf = d.ncf, ncu = d.ncu, npf = d.npf, ndf = d.ndf, m = d.m;
   const int per = ncf * ncf;
-  const unsigned qn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (qn >= static_cast<unsigned>(per) * I.nse) return;
-  const int t = blockIdx.y;
-  const int n = static_cast<int>(qn / per);
-  const int rc = static_cast<int>(qn - static_cast<unsigned>(n) * per);
-  const long q = (static_cast<long>(t) * I.nse + n) * per + rc;
-  const int rr = rc % ncf, cc = rc / ncf;
-  const int am = rr % ncu, bm = rr / ncu, an = cc % ncu, bn = cc / ncu;
+  const int n = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
   const int i = I.row_of[t];
   const int fi = I.face[n + I.nse * i];
   const int e1 = f2e[4 * fi], l1 = f2e[4 * fi + 1], e2 = f2e[4 * fi + 2], l2 = f2e[4 * fi + 3];
   const int nfi = npf * fi;
-  double v;
-  if (t == I.row_ptr[i]) {
-    int m1 = am + ncu * (elemcon[bm + npf * l1 + ndf * e1] - nfi);
-    int n1 = an + ncu * (elemcon[bn + npf * l1 + ndf * e1] - nfi);
-    v = H[static_cast<long>(e1) * m * m + (m1 + ncf * l1) + static_cast<long>(m) * (n1 + ncf * l1)];
-    if (e2 >= 0) {
-      int m2 = am + ncu * (elemcon[bm + npf * l2 + ndf * e2] - nfi);
-      int n2 = an + ncu * (elemcon[bn + npf * l2 + ndf * e2] - nfi);
-      v += H[static_cast<long>(e2) * m * m + (m2 + ncf * l2) + static_cast<long>(m) * (n2 + ncf * l2)];
+  const bool diag = t == I.row_ptr[i];
+  int ea, eb = -1;  // the element of each term
+  if (diag) {
+    ea = e1;
+    eb = e2;
+    for (int r = tid; r < ncf; r += 256) {
+      const int am = r % ncu, bm = r / ncu;
+      rowoff[0][r] = am + ncu * (elemcon[bm + npf * l1 + ndf * e1] - nfi) + ncf * l1;
+      coloff[0][r] = rowoff[0][r];
+      if (e2 >= 0) {
+        rowoff[1][r] = am + ncu * (elemcon[bm + npf * l2 + ndf * e2] - nfi) + ncf * l2;
+        coloff[1][r] = rowoff[1][r];
+      }
     }
   } else {
     const int j = I.col_ind[t];
     const int fj = I.face[n + I.nse * j];
     const int je1 = f2e[4 * fj], je2 = f2e[4 * fj + 2];
-    int e, k1, k2;
-    if (je1 == e1) { e = e1; k1 = l1; k2 = f2e[4 * fj + 1]; }
-    else if (je1 == e2) { e = e2; k1 = l2; k2 = f2e[4 * fj + 1]; }
-    else if (je2 == e1) { e = e1; k1 = l1; k2 = f2e[4 * fj + 3]; }
-    else { e = e2; k1 = l2; k2 = f2e[4 * fj + 3]; }
-    int m1 = am + ncu * (elemcon[bm + npf * k1 + ndf * e] - nfi);
-    int n2 = an + ncu * (elemcon[bn + npf * k2 + ndf * e] - npf * fj);
-    v = H[static_cast<long>(e) * m * m + (m1 + ncf * k1) + static_cast<long>(m) * (n2 + ncf * k2)];
+    int k1, k2;
+    if (je1 == e1) { ea = e1; k1 = l1; k2 = f2e[4 * fj + 1]; }
+    else if (je1 == e2) { ea = e2; k1 = l2; k2 = f2e[4 * fj + 1]; }
+    else if (je2 == e1) { ea = e1; k1 = l1; k2 = f2e[4 * fj + 3]; }
+    else { ea = e2; k1 = l2; k2 = f2e[4 * fj + 3]; }
+    for (int r = tid; r < ncf; r += 256) {
+      const int am = r % ncu, bm = r / ncu;
+      rowoff[0][r] = am + ncu * (elemcon[bm + npf * k1 + ndf * ea] - nfi) + ncf * k1;
+      coloff[0][r] = am + ncu * (elemcon[bm + npf * k2 + ndf * ea] - npf * fj) + ncf * k2;
+    }
   }
-  BE[q] = static_cast<T>(v);
+  __syncthreads();
+  const TH* Ha = H + static_cast<long>(ea) * m * m;
+  const TH* Hb = H + static_cast<long>(eb >= 0 ? eb : ea) * m * m;
+  T* out = BE + (static_cast<long>(t) * I.nse + n) * per;
+  for (int rc = tid; rc < per; rc += 256) {
+    const int rr = rc % ncf, cc = rc / ncf;
+    double v = Ha[rowoff[0][rr] + static_cast<long>(m) * coloff[0][cc]];
+    if (eb >= 0) v += Hb[rowoff[1][rr] + static_cast<long>(m) * coloff[1][cc]];
+    out[rc] = static_cast<T>(v);
+  }
 }
 
 // Inverse of the diagonal block of face slot i of every superelement (one
@@ -2972,8 +2997,11 @@ void vol_res(const Dims& d, const AsmArgs& a, float* out, long ldo, hipStream_t 
 template <typename T>
 void vol_W_t(const Dims& d, const AsmArgs& a, T* W, hipStream_t s) {
   if (d.nd > 3) throw Error(5, "volume Jacobian for nd > 3");
-  long n1 = static_cast<long>(d.npe) * d.nge * a.eb;
-  k_vol_W<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, W);
+  const int gp = std::max(1, 256 / d.npe);  // points per workgroup
+  const long nwg = static_cast<long>(a.eb) * ((d.nge + gp - 1) / gp);
+  const size_t shm = static_cast<size_t>(gp) * (d.nd + 1) * d.ncu * d.nc * sizeof(double);
+  if (shm > 64 * 1024) throw Error(5, "volume Jacobian: too many components for the LDS");
+  k_vol_W<T><<<static_cast<unsigned>(nwg), 256, shm, s>>>(d, a, W, gp);
   EXA_HIP(hipGetLastError());
 }
 void vol_W(const Dims& d, const AsmArgs& a, hipStream_t s) { vol_W_t(d, a, a.W, s); }
@@ -3196,9 +3224,8 @@ void update_udg(const Dims& d, const double* du, double alpha, double* udg, hipS
 template <typename T, typename TH>
 void bilu_assemble_t(const Dims& d, const IluDev& I, const int* f2e, const int* elemcon, const TH* H, T* BE,
                      hipStream_t s) {
-  const long n1 = static_cast<long>(d.ncf) * d.ncf * I.nse;
-  if (n1 >= (1L << 32) - 256 || I.nnz > 65535) throw Error(5, "block-ILU(0) assembly grid too large");
-  k_bilu_assemble<T, TH><<<dim3(nblocks(n1, 256), I.nnz), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
+  if (d.ncf > 256 || I.nnz > 65535) throw Error(5, "block-ILU(0) assembly grid too large");
+  k_bilu_assemble<T, TH><<<dim3(I.nse, I.nnz), 256, 0, s>>>(d, I, f2e, elemcon, H, BE);
   EXA_HIP(hipGetLastError());
 }
 

@@ -822,38 +822,47 @@ __global__ void k_qpack(Dims d, int e0, int eb, const double* aug, double* MCs, 
 }
 
 // hdgGetQ (qequation.cpp:372-487): q_d = MinvC_d u - MinvE_d uh, stored in udg.
-// One thread per (row kd of M^-1 C / M^-1 E, element), all NCU components: the two matrices are
-// read once per element (the sums for each component run in the same order as per component).
+// One workgroup per element, one thread per row kd of M^-1 C / M^-1 E, all NCU components: the
+// element's u and its gathered uh are staged in LDS, and the two matrices are read once, along
+// their rows (the sums for each component run in the same order as per component).
 template <int NCU>
-__global__ void k_getq(Dims d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
-                       double* udg) {
+__global__ __launch_bounds__(1024) void k_getq(Dims d, const double* MCs, const double* MEs, const int* elemcon,
+                                               const double* uh, double* udg) {
+  extern __shared__ double gq_sm[];
   const int npe = d.npe, nd = d.nd, nc = d.nc, ndf = d.ndf;
   const int per = npe * nd;
-  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (q >= static_cast<long>(per) * d.ne) return;
-  const long e = q / per;
-  const int kd = static_cast<int>(q - e * per);
-  const int k = kd % npe, dd = kd / npe;
+  const long e = blockIdx.x;
+  const int tid = threadIdx.x;
+  double* us = gq_sm;               // u[j + npe*c]
+  double* uhs = gq_sm + npe * NCU;  // uh of the element's trace nodes, [t*NCU + c]
+  const double* u = udg + static_cast<long>(npe) * nc * e;
+  const int* ec = elemcon + static_cast<long>(ndf) * e;
+  for (int idx = tid; idx < npe * NCU; idx += blockDim.x) us[idx] = u[idx];
+  for (int idx = tid; idx < ndf * NCU; idx += blockDim.x)
+    uhs[idx] = uh[NCU * static_cast<long>(ec[idx / NCU]) + idx % NCU];
+  __syncthreads();
   const double* mc = MCs + e * npe * nd * npe;
   const double* me = MEs + e * npe * nd * ndf;
-  const double* u = udg + static_cast<long>(npe) * nc * e;
-  double s[NCU];
+  for (int kd = tid; kd < per; kd += blockDim.x) {
+    const int k = kd % npe, dd = kd / npe;
+    double s[NCU];
 #pragma unroll
-  for (int c = 0; c < NCU; ++c) s[c] = 0.0;
-  for (int j = 0; j < npe; ++j) {
-    const double mv = mc[kd + static_cast<long>(npe) * nd * j];
+    for (int c = 0; c < NCU; ++c) s[c] = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < npe; ++j) {
+      const double mv = mc[kd + static_cast<long>(npe) * nd * j];
 #pragma unroll
-    for (int c = 0; c < NCU; ++c) s[c] += mv * u[j + npe * c];
+      for (int c = 0; c < NCU; ++c) s[c] += mv * us[j + npe * c];
+    }
+#pragma unroll 4
+    for (int t = 0; t < ndf; ++t) {
+      const double mv = me[kd + static_cast<long>(npe) * nd * t];
+#pragma unroll
+      for (int c = 0; c < NCU; ++c) s[c] -= mv * uhs[t * NCU + c];
+    }
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) udg[k + npe * (NCU + c + NCU * dd) + static_cast<long>(npe) * nc * e] = s[c];
   }
-  const int* ec = elemcon + static_cast<long>(ndf) * e;
-  for (int t = 0; t < ndf; ++t) {
-    const double mv = me[kd + static_cast<long>(npe) * nd * t];
-    const double* uht = uh + NCU * static_cast<long>(ec[t]);
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) s[c] -= mv * uht[c];
-  }
-#pragma unroll
-  for (int c = 0; c < NCU; ++c) udg[k + npe * (NCU + c + NCU * dd) + static_cast<long>(npe) * nc * e] = s[c];
 }
 
 // the same for any ncu, one component per thread
@@ -1859,20 +1868,26 @@ __global__ __launch_bounds__(256) void k_matvec_small(Dims d, const T* H, const 
   we[q] = acc;
 }
 
-// hdgGetDUDG (matvec.cpp:219-238): du_e = Ru_e + DinvF_e * x_e.
+// hdgGetDUDG (matvec.cpp:219-238): du_e = Ru_e + DinvF_e * x_e.  One workgroup per element: the
+// element's trace values x_e are gathered into LDS once, then one thread per row reads DinvF_e
+// along its columns (the sum runs over t in order).
 template <typename T>
-__global__ void k_dudg(Dims d, const double* RuS, const T* DinvF, const int* elemcon,
-                       const double* x, double* du) {
+__global__ __launch_bounds__(1024) void k_dudg(Dims d, const double* RuS, const T* DinvF, const int* elemcon,
+                                               const double* x, double* du) {
+  extern __shared__ double dd_sm[];
   const int n = d.n, m = d.m, ncu = d.ncu, ndf = d.ndf;
-  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (q >= static_cast<long>(n) * d.ne) return;
-  const long e = q / n;
-  const int r = static_cast<int>(q % n);
+  const long e = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < m; t += blockDim.x)
+    dd_sm[t] = x[(t % ncu) + ncu * static_cast<long>(elemcon[t / ncu + ndf * e])];
+  __syncthreads();
   const T* F = DinvF + e * n * static_cast<long>(m);
-  double s = RuS[q];
-  for (int t = 0; t < m; ++t)
-    s += static_cast<double>(F[r + static_cast<long>(n) * t]) * x[(t % ncu) + ncu * static_cast<long>(elemcon[t / ncu + ndf * e])];
-  du[q] = s;
+  for (int r = tid; r < n; r += blockDim.x) {
+    double s = RuS[e * n + r];
+#pragma unroll 4
+    for (int t = 0; t < m; ++t) s += static_cast<double>(F[r + static_cast<long>(n) * t]) * dd_sm[t];
+    du[e * n + r] = s;
+  }
 }
 
 // UpdateUDG (kokkosimpl.h:322-338) on the u components.
@@ -2969,13 +2984,17 @@ void qpack(const Dims& d, int e0, int eb, const double* aug, double* MCs, double
 void getq(const Dims& d, const double* MCs, const double* MEs, const int* elemcon, const double* uh,
           double* udg, hipStream_t s) {
   const long n1 = static_cast<long>(d.npe) * d.nd * d.ne;
-  const unsigned nb = nblocks(n1, 256);
-  switch (d.ncu) {
-    case 1: k_getq<1><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
-    case 2: k_getq<2><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
-    case 3: k_getq<3><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
-    case 4: k_getq<4><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
-    case 5: k_getq<5><<<nb, 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+  // one workgroup per element: the rows of M^-1 C rounded up to whole waves (at most 1024 threads)
+  const int nt = static_cast<int>(std::min<long>(1024, (static_cast<long>(d.npe) * d.nd + 63) / 64 * 64));
+  const size_t shm = static_cast<size_t>(d.npe + d.ndf) * d.ncu * sizeof(double);
+  const unsigned ne = static_cast<unsigned>(d.ne);
+  if (d.ne <= 0) return;
+  switch (shm <= 48 * 1024 ? d.ncu : 0) {
+    case 1: k_getq<1><<<ne, nt, shm, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 2: k_getq<2><<<ne, nt, shm, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 3: k_getq<3><<<ne, nt, shm, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 4: k_getq<4><<<ne, nt, shm, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
+    case 5: k_getq<5><<<ne, nt, shm, s>>>(d, MCs, MEs, elemcon, uh, udg); break;
     default:
       k_getq_any<<<nblocks(n1 * d.ncu, 256), 256, 0, s>>>(d, MCs, MEs, elemcon, uh, udg);
   }
@@ -3089,19 +3108,41 @@ void to_float(float* dst, const float* src, int rows, int cols, long ld, long ss
 }
 
 // BF16 (round to nearest even of the FP32 value), kept as unsigned short
+__device__ __forceinline__ unsigned short bf16_round(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(u >> 16);
+}
+// four values per thread (the count's tail one by one)
 template <typename S>
 __global__ void k_to_bf16(unsigned short* dst, const S* src, long count) {
-  const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (q >= count) return;
-  unsigned int u = __float_as_uint(static_cast<float>(src[q]));
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  dst[q] = static_cast<unsigned short>(u >> 16);
+  const long q = (blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x) * 4;
+  if (q + 4 <= count) {
+    S v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = src[q + k];
+    ushort4 o;
+    o.x = bf16_round(static_cast<float>(v[0]));
+    o.y = bf16_round(static_cast<float>(v[1]));
+    o.z = bf16_round(static_cast<float>(v[2]));
+    o.w = bf16_round(static_cast<float>(v[3]));
+    if ((reinterpret_cast<uintptr_t>(dst + q) & 7) == 0) {
+      *reinterpret_cast<ushort4*>(dst + q) = o;
+    } else {
+      dst[q] = o.x;
+      dst[q + 1] = o.y;
+      dst[q + 2] = o.z;
+      dst[q + 3] = o.w;
+    }
+  } else {
+    for (long k = q; k < count; ++k) dst[k] = bf16_round(static_cast<float>(src[k]));
+  }
 }
 
 template <typename S>
 void to_bf16_t(unsigned short* dst, const S* src, long count, hipStream_t s) {
   if (count <= 0) return;
-  k_to_bf16<S><<<nblocks(count, 256), 256, 0, s>>>(dst, src, count);
+  k_to_bf16<S><<<nblocks((count + 3) / 4, 256), 256, 0, s>>>(dst, src, count);
   EXA_HIP(hipGetLastError());
 }
 void to_bf16(unsigned short* dst, const double* src, long count, hipStream_t s) { to_bf16_t(dst, src, count, s); }
@@ -3210,8 +3251,10 @@ void matvec_elem(const Dims& d, const T* H, const int* elemcon, const double* v,
 template <typename T>
 void dudg(const Dims& d, const double* RuS, const T* DinvF, const int* elemcon, const double* x,
           double* du, hipStream_t s) {
-  long n1 = static_cast<long>(d.n) * d.ne;
-  k_dudg<T><<<nblocks(n1, 256), 256, 0, s>>>(d, RuS, DinvF, elemcon, x, du);
+  if (d.ne <= 0) return;
+  const int nt = std::min(1024, (d.n + 63) / 64 * 64);  // one workgroup per element, a thread per row
+  k_dudg<T><<<static_cast<unsigned>(d.ne), nt, static_cast<size_t>(d.m) * sizeof(double), s>>>(d, RuS, DinvF,
+                                                                                             elemcon, x, du);
   EXA_HIP(hipGetLastError());
 }
 

@@ -1165,9 +1165,10 @@ __global__ __launch_bounds__(256) void k_vol_points(Dims d, AsmArgs a) {
   }
 }
 
-// Ru[i + npe*k] = sum_dp sum_g shapegw[i, g, dp] * rv  (Gauss2Node with shapegw)
+// Ru[i + npe*k] = sum_dp sum_g shapegw[i, g, dp] * rv  (Gauss2Node with shapegw); one thread per
+// (i, k, element), for point sets too large for the LDS of k_vol_res
 template <typename T>
-__global__ void k_vol_res(Dims d, AsmArgs a, T* out, long ldo) {
+__global__ void k_vol_res_direct(Dims d, AsmArgs a, T* out, long ldo) {
   const int npe = d.npe, nge = d.nge, ncu = d.ncu, nd = d.nd;
   const long per = static_cast<long>(npe) * ncu;
   const long q = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1183,6 +1184,36 @@ __global__ void k_vol_res(Dims d, AsmArgs a, T* out, long ldo) {
     for (int g = 0; g < nge; ++g) s += w[i + npe * g] * rv[g];
   }
   out[e * ldo + r] = static_cast<T>(s);
+}
+
+// One workgroup per element: the element's point values rv (nd+1) ncu nge are staged in LDS once;
+// thread (i, k) sums over dp and the points in the same order as before (dp outer, g inner).
+template <typename T>
+__global__ __launch_bounds__(256) void k_vol_res(Dims d, AsmArgs a, T* out, long ldo) {
+  extern __shared__ double vr_sm[];
+  const int npe = d.npe, nge = d.nge, ncu = d.ncu, nd = d.nd;
+  const int per = npe * ncu;
+  const long e = blockIdx.x;
+  const int tid = threadIdx.x;
+  const long P = static_cast<long>(a.eb) * nge;
+  const int L = (nd + 1) * ncu;
+  // vr_sm[(dp*ncu + k)*nge + g] = rv[(dp*ncu + k)*P + e*nge + g]
+  for (int idx = tid; idx < L * nge; idx += blockDim.x) {
+    const int j = idx / nge, g = idx - j * nge;
+    vr_sm[idx] = a.rv[j * P + e * nge + g];
+  }
+  __syncthreads();
+  for (int r = tid; r < per; r += blockDim.x) {
+    const int i = r % npe, k = r / npe;
+    double sum = 0.0;
+    for (int dp = 0; dp <= nd; ++dp) {
+      const double* rv = vr_sm + (dp * ncu + k) * nge;
+      const double* w = a.shapegw + static_cast<long>(npe) * nge * dp;
+#pragma unroll 4
+      for (int g = 0; g < nge; ++g) sum += w[i + npe * g] * rv[g];
+    }
+    out[e * ldo + r] = static_cast<T>(sum);
+  }
 }
 
 // W[(e*NC + c)*NCU + k][i + npe*g] = sum_dp shapegw[i, g, dp] * rJ[dp, k, c]
@@ -3006,8 +3037,15 @@ void face_points(const Dims& d, const AsmArgs& a, bool jac, hipStream_t s) { poi
 
 template <typename T>
 void vol_res_t(const Dims& d, const AsmArgs& a, T* out, long ldo, hipStream_t s) {
-  long n1 = static_cast<long>(d.npe) * d.ncu * a.eb;
-  k_vol_res<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, out, ldo);
+  if (a.eb <= 0) return;
+  const size_t shm = static_cast<size_t>(d.nd + 1) * d.ncu * d.nge * sizeof(double);
+  if (shm > 48 * 1024) {
+    const long n1 = static_cast<long>(d.npe) * d.ncu * a.eb;
+    k_vol_res_direct<T><<<nblocks(n1, 256), 256, 0, s>>>(d, a, out, ldo);
+  } else {
+    const int nt = std::min(256, (d.npe * d.ncu + 63) / 64 * 64);
+    k_vol_res<T><<<static_cast<unsigned>(a.eb), nt, shm, s>>>(d, a, out, ldo);
+  }
   EXA_HIP(hipGetLastError());
 }
 void vol_res(const Dims& d, const AsmArgs& a, double* out, long ldo, hipStream_t s) { vol_res_t(d, a, out, ldo, s); }
@@ -3252,6 +3290,7 @@ template <typename T>
 void dudg(const Dims& d, const double* RuS, const T* DinvF, const int* elemcon, const double* x,
           double* du, hipStream_t s) {
   if (d.ne <= 0) return;
+  if (d.m > 8192) throw Error(5, "element trace block larger than 8192 (the LDS of k_dudg)");
   const int nt = std::min(1024, (d.n + 63) / 64 * 64);  // one workgroup per element, a thread per row
   k_dudg<T><<<static_cast<unsigned>(d.ne), nt, static_cast<size_t>(d.m) * sizeof(double), s>>>(d, RuS, DinvF,
                                                                                              elemcon, x, du);

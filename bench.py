@@ -269,7 +269,7 @@ def run_batch(steps, warmup, cpu_baseline=True, count=256, verbose=True, concurr
                 mixed_refinement=BATCH_REFINE[kind])
 
 
-def run_serve(seconds, rate, check_alone=6, refill=True):
+def run_serve(seconds, rate, check_alone=6, refill=True, lu_inverses=1):
     """The served workload: a Poisson stream of Navier-Stokes (70 %) and convection-diffusion
     solve requests (n x n quads, n in {32, 64, 128}, p in {2..5}; see cases.serve_stream)
     replayed in real time into the continuous-batching server (HBM budget 200 GB, mixed
@@ -285,6 +285,7 @@ def run_serve(seconds, rate, check_alone=6, refill=True):
     print("# serve: %d requests written in %.1f s" % (len(reqs), time.perf_counter() - t0), file=sys.stderr,
           flush=True)
     srv = serve.Server(hbm_budget_bytes=200e9, max_batch=64, max_concurrent=8, precision=1, refill=refill,
+                       options={} if lu_inverses else {"lu_inverses": 0},
                        keep_results=check_alone > 0, window=0.02,
                        log=lambda msg: print("# serve: " + msg, file=sys.stderr, flush=True))
     summ = srv.run_stream(reqs)
@@ -354,6 +355,8 @@ def main():
     ap.add_argument("--gemm-autotune", type=int, default=1,
                     help="1: time the GEMM workgroup tiles per product shape (first call) and keep the fastest; "
                          "0: the default 64x64 tile")
+    ap.add_argument("--serve-lu-inverses", type=int, default=1, choices=[0, 1],
+                    help="served stream, mixed form: 1 FP32 LU with triangle inverses, 0 by substitution")
     ap.add_argument("--serve-refill", type=int, default=1,
                     help="1: running batches admit waiting requests between steps; 0: each batch solves to the end")
     ap.add_argument("--workload", default="c2", choices=["c2", "batch", "batch3d", "serve"],
@@ -380,7 +383,8 @@ def main():
     from exasim_amd.hdg import HDGSolver
 
     if args.workload == "serve":
-        sv = run_serve(args.serve_seconds, args.serve_rate, refill=bool(args.serve_refill))
+        sv = run_serve(args.serve_seconds, args.serve_rate, refill=bool(args.serve_refill),
+                       lu_inverses=args.serve_lu_inverses)
         print(json.dumps({
             "metric": METRIC, "value": sv["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1,
             "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -616,7 +620,8 @@ def main():
             out["batch3d"] = {"error": repr(e)}
     if not args.no_serve:
         try:
-            out["serve"] = run_serve(args.serve_seconds, args.serve_rate, refill=bool(args.serve_refill))
+            out["serve"] = run_serve(args.serve_seconds, args.serve_rate, refill=bool(args.serve_refill),
+                                     lu_inverses=args.serve_lu_inverses)
         except Exception as e:
             out["serve"] = {"error": repr(e)}
     out["gemm_autotune"] = gemm_summary()

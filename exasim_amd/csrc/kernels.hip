@@ -2050,9 +2050,11 @@ __global__ __launch_bounds__(256) void k_bilu_inv(Dims d, IluDev I, int i, T* BE
     __syncthreads();
     for (int k = tid; k < w2; k += 256) A[c * w2 + k] *= inv;
     __syncthreads();
-    for (int idx = tid; idx < ncf * w2; idx += 256) {
-      int r = idx / w2, k = idx % w2;
-      if (r != c) A[idx] = fma(-fac[r], A[c * w2 + k], A[idx]);
+    // a wave per row at a time, a lane per column (no index division)
+    for (int r = wave; r < ncf; r += 4) {
+      if (r == c) continue;
+      const T f = fac[r];
+      for (int k = lane; k < w2; k += 64) A[r * w2 + k] = fma(-f, A[c * w2 + k], A[r * w2 + k]);
     }
     __syncthreads();
   }

@@ -97,6 +97,7 @@ void Solver::setup() {
 
 // pivot workspace of the batched LU for `batch` matrices (grown with the stream idle)
 void* Solver::lu_tri(long batch) {
+  if (!lu_inverses) return nullptr;
   const size_t need = static_cast<size_t>(std::max(1L, batch)) * LU_TRI_WS;
   if (lutri.n < need) {
     EXA_HIP(hipStreamSynchronize(s));
@@ -661,6 +662,8 @@ void Solver::set_option(const std::string& name, double v) {
   else if (name == "gemm_autotune") {  // process-wide: 1 time the GEMM tiles per shape, 0 default tile
     if (v != 0.0 && v != 1.0 && v != -1.0) throw Error(1, "gemm_autotune: 1, 0 or -1 (the environment's choice)");
     set_gemm_autotune(static_cast<int>(v));
+  } else if (name == "lu_inverses") {  // FP32 LU: 1 triangle inverses on MFMA, 0 substitution
+    lu_inverses = v != 0.0;
   } else if (name == "ilu_wave") {  // process-wide: FP32/BF16 block-ILU(0) apply, 1 one wave per superelement
     if (v != 0.0 && v != 1.0 && v != -1.0) throw Error(1, "ilu_wave: 1, 0 or -1 (the environment's choice)");
     set_bilu_wave(static_cast<int>(v));

@@ -193,6 +193,9 @@ struct Solver {
   ~Solver();
   void set_dims();
   int* lu_ws(long batch);
+  // FP32 LU of D' and of the Schwarz blocks: explicit triangle inverses applied by GEMMs (true) or
+  // substitution (false, option lu_inverses = 0)
+  bool lu_inverses = true;
   void* lu_tri(long batch);  // triangle-inverse workspace of the blocked LU (LU_TRI_WS doubles per matrix)
   void init(SolData& sol);
   void setup_q(long e0, long ne);  // q-equation matrices (MCs, MEs, Mass) of elements [e0, e0 + ne)

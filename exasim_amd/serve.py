@@ -73,7 +73,7 @@ class Server:
     """Continuous-batching request loop over batch handles."""
 
     def __init__(self, hbm_budget_bytes=200e9, max_batch=64, max_concurrent=8, precision=1, options=None,
-                 keep_results=False, log=None, window=0.0, refill=True):
+                 keep_results=False, log=None, window=0.0, refill=True, accept_norm=1e-8):
         self.budget = float(hbm_budget_bytes)
         self.max_batch = max_batch
         self.max_concurrent = int(max_concurrent)  # batches solving at once (a host thread and a HIP stream each)
@@ -81,6 +81,10 @@ class Server:
         self.threads = []
         self.dispatcher = None
         self.precision = precision
+        # a request is solved when its final Newton residual norm is finite and at most accept_norm;
+        # otherwise the mixed form's result is solved again in FP64, and an FP64 solve that ends
+        # above it (Newton stagnated or diverged) is reported as failed
+        self.accept_norm = float(accept_norm)
         self.options = dict(options or {})
         # each batch handle's assembly workspace, counted against the budget per batch
         self.options.setdefault("workspace_bytes", 1e9)
@@ -194,6 +198,9 @@ class Server:
                     return
                 self.lock.wait(0.005)
 
+    def _solved(self, norm):
+        return bool(np.isfinite(norm)) and norm <= self.accept_norm
+
     def _take_same_mesh(self, key, mk):
         """The oldest waiting request of the lane whose mesh is mk (removed from the queue), or None."""
         with self.lock:
@@ -254,7 +261,8 @@ class Server:
         with self.lock:
             self.batches.append((key, len(take), time.perf_counter() - t1))
             for i, r in enumerate(take):
-                if not np.isfinite(norm[i]):       # the instance's Newton iteration broke down
+                if not self._solved(norm[i]):       # the instance's Newton iteration broke down
+                    self.records[r.rid].update(norm=float(norm[i]), precision=precision)
                     failed.append(r)
                     continue
                 self.records[r.rid].update(done=done, batch=len(take), newton=int(newton[i]), gmres=int(gmres[i]),
@@ -294,7 +302,8 @@ class Server:
                     r = slots[i]
                     uh = S.get_instance(i, "uh") if self.keep_results else None
                     with self.lock:
-                        if st[i] < 0 or not np.isfinite(norm[i]):   # its Newton iteration broke down
+                        if st[i] < 0 or not self._solved(norm[i]):   # its Newton iteration broke down
+                            self.records[r.rid].update(norm=float(norm[i]), precision=precision)
                             failed.append(r)
                         else:
                             self.records[r.rid].update(done=done, batch=len(take), newton=int(newton[i]),

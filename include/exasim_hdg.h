@@ -88,6 +88,8 @@ int exa_sizes(const exa_handle* h, int* out, int len);
  * Krylov scalars stay on the device; steps queued after convergence do no work);
  * "gmres_graph" (0/1, default 0): replay the Newton loop's GMRES steps from captured HIP graphs;
  * "noalloc_chunk" (elements, default 0 = as many as fit): element chunk of exa_matvec_noalloc;
+ * "lu_inverses" (1 default, 0): the mixed form's FP32 LU of D' and of the Schwarz blocks applies
+ * explicit triangle inverses by MFMA products (1) or solves by substitution (0);
  * "ilu_wave" (process-wide; 1 default, 0, or -1 for the EXA_BILU_WAVE environment value): the
  * block-ILU(0) apply of FP32 / BF16 factors with one wave per superelement (1) or with a
  * 256-thread workgroup per superelement (0). */

@@ -27,6 +27,12 @@ CASES = {
                                         inflow_dirichlet=True), "convdiff", {"precond_bf16": 1}),
     "cd4_eas_bf16": ("convdiff3d", dict(n=2, porder=4, ncu=2, restart=100, gmres_tol=1e-12, precond="eas"),
                      "convdiff", {"precond_bf16": 1}),
+    # the FP32 LU of D' (n = 250, 64-wide panels) and of the Schwarz blocks (m = 300) by substitution
+    # instead of triangle inverses (option lu_inverses = 0)
+    "cd_bj_subst": ("convdiff3d", dict(n=2, porder=4, ncu=2, restart=100, gmres_tol=1e-12), "convdiff",
+                    {"lu_inverses": 0}),
+    "cd4_eas_subst": ("convdiff3d", dict(n=2, porder=4, ncu=2, restart=100, gmres_tol=1e-12, precond="eas"),
+                      "convdiff", {"lu_inverses": 0}),
     # each solve capped at one restart cycle of 10: the refinement loop carries the convergence
     "cd_ilu_cycles": ("convdiff3d", dict(n=4, porder=2, ncu=2, restart=10, gmres_tol=1e-12, precond="bilu0",
                                          cartgridpart=[3, 4, 4, 4, 2, 2, 2, 0]), "convdiff",

@@ -49,6 +49,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s + p50/p99 latency, mixed vs FP64 GPU at rel err ≤1e-8; HBM/MFMA %"
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_FP32 = 157.3   # TFLOP/s, dense
+MFMA_PEAK_FP64 = 78.6
 PREC_NAME = {0: "fp64", 1: "mixed"}
 PREC_ID = {"fp64": 0, "mixed": 1}
 
@@ -529,6 +531,8 @@ def main():
     for p in ([0, 1] if args.precision == "both" else [PREC_ID[args.precision]]):
         legs[p] = leg(p)
     S.close()
+    # the C2 legs' products only (before the batch workloads); FP32 ones when the mixed leg is `value`
+    mfma = mfma_roofline(gemm_summary(top=40), fp32=1 if 1 in legs else 0)
     head = legs[1] if 1 in legs else legs[0]
     err = None
     if 0 in legs and 1 in legs:
@@ -608,6 +612,7 @@ def main():
         "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in tm.items()},
         "roofline": head["roofline"],
         "precond_roofline": head["precond_roofline"],
+        "mfma_roofline": mfma,
         "cpu_baseline": cpu,
     }
     if 0 in legs:
@@ -656,11 +661,31 @@ def gemm_summary(top=6):
     t_def = sum(r["us"][0] for r in rows)
     t_best = sum(best(r) for r in rows)
     big = sorted(rows, key=lambda r: -r["us"][0])[:top]
+
+    def tflops(r):    # 2 M N K per product over the chosen tile's time
+        return 2.0 * r["M"] * r["N"] * r["K"] * r["batch"] / (best(r) * 1e-6) / 1e12
     return {"shapes": len(rows), "tiles_chosen": hist, "default_us_sum": t_def, "chosen_us_sum": t_best,
             "default_over_chosen": t_def / t_best if t_best > 0 else None,
-            "largest": [{"M": r["M"], "N": r["N"], "K": r["K"], "batch_log2": r["log2_batch"],
-                         "fp32": r["fp32"], "tile": r["tile"], "us_default": r["us"][0], "us_chosen": best(r)}
+            "largest": [{"M": r["M"], "N": r["N"], "K": r["K"], "batch": r["batch"], "fp32": r["fp32"],
+                         "tile": r["tile"], "us_default": r["us"][0], "us_chosen": best(r),
+                         "tflops": tflops(r),
+                         "mfma_frac": tflops(r) / (MFMA_PEAK_FP32 if r["fp32"] else MFMA_PEAK_FP64)}
                         for r in big]}
+
+
+def mfma_roofline(summary, fp32=None):
+    """The MFMA roofline of the largest product shape of the line's GEMMs (of FP32 or FP64 products
+    when fp32 is given): TFLOP/s of the chosen tile in the tuner's timing against the dense MFMA peak
+    of the product's type."""
+    big = [r for r in (summary.get("largest") or []) if fp32 is None or r["fp32"] == fp32]
+    if not big:
+        return None
+    r = big[0]
+    return {"bound": "mfma", "achieved": r["tflops"], "peak": MFMA_PEAK_FP32 if r["fp32"] else MFMA_PEAK_FP64,
+            "unit": "TFLOP/s", "frac": r["mfma_frac"], "traffic": None,
+            "kernel": "k_gemm_mfma<%s> %dx%dx%d, %d products, tile %s" % (
+                "float" if r["fp32"] else "double", r["M"], r["N"], r["K"], r["batch"], r["tile"]),
+            "flop_per_launch": 2.0 * r["M"] * r["N"] * r["K"] * r["batch"], "avg_launch_ms": r["us_chosen"] / 1e3}
 
 
 if __name__ == "__main__":

@@ -274,6 +274,7 @@ MfmaLaunch mfma_launcher(int fp32, int store, int tile) {
 typedef std::tuple<int, int, int, int, int, int, int> TuneKey;  // device, M, N, K, log2 batch, fp32, store
 struct TuneEntry {
   int tile = 0;
+  long batch = 0;  // products in the timed call
   float us[NTILE] = {0};
 };
 std::mutex tune_mu;
@@ -369,6 +370,7 @@ int pick_tile(const Gemm& g, int store, long batch, hipStream_t s) {
     else gt.C = static_cast<double*>(scratch);
   }
   TuneEntry ent;
+  ent.batch = batch;
   hipEvent_t e0, e1;
   EXA_HIP(hipEventCreate(&e0));
   EXA_HIP(hipEventCreate(&e1));
@@ -441,8 +443,8 @@ std::string gemm_tuning_report() {
   for (const auto& kv : tune_table()) {
     const TuneKey& k = kv.first;
     const TuneEntry& e = kv.second;
-    std::snprintf(line, sizeof line, "%d %d %d %d %d %d %dx%d", std::get<1>(k), std::get<2>(k), std::get<3>(k),
-                  std::get<4>(k), std::get<5>(k), std::get<6>(k), TILES[e.tile].bm, TILES[e.tile].bn);
+    std::snprintf(line, sizeof line, "%d %d %d %d %d %d %dx%d %ld", std::get<1>(k), std::get<2>(k), std::get<3>(k),
+                  std::get<4>(k), std::get<5>(k), std::get<6>(k), TILES[e.tile].bm, TILES[e.tile].bn, e.batch);
     out += line;
     for (int t = 0; t < NTILE; ++t) {
       std::snprintf(line, sizeof line, " %.3f", e.us[t]);

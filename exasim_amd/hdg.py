@@ -58,7 +58,8 @@ TIMER_CLASSES = ["matvec", "precond", "ortho", "assembly", "lu", "other"]
 
 def gemm_tuning():
     """The GEMM tile autotuner's choices so far (``exa_gemm_tuning``): a list of dicts with the
-    product shape, the chosen tile and each tile's time in microseconds."""
+    product shape, the chosen tile, the number of products in the timed call and each tile's time
+    in microseconds."""
     lib = load()
     n = lib.exa_gemm_tuning(ctypes.create_string_buffer(1), 1)
     buf = ctypes.create_string_buffer(n + 1)
@@ -67,7 +68,7 @@ def gemm_tuning():
     for line in buf.value.decode().splitlines():
         f = line.split()
         out.append(dict(M=int(f[0]), N=int(f[1]), K=int(f[2]), log2_batch=int(f[3]), fp32=int(f[4]),
-                        storage=int(f[5]), tile=f[6], us=[float(x) for x in f[7:]]))
+                        storage=int(f[5]), tile=f[6], batch=int(f[7]), us=[float(x) for x in f[8:]]))
     return out
 
 

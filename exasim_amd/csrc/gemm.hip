@@ -23,6 +23,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 namespace exa {
@@ -150,8 +151,14 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tiles
   const TB* B = gsel<TB>(g.B, g.Bf) + g.sB0 * bb0 + g.sB1 * b1 + g.sB2 * b2;
   TC* C = const_cast<TC*>(gsel<TC>(g.C, g.Cf)) + g.sC0 * c0 + g.sC1 * b1 + g.sC2 * b2;
   const int row0 = tile * BM, col0 = tcol * BN;
-  __shared__ CT As[BM * MG_BK];
-  __shared__ CT Bs[BN * MG_BK];
+  // the K tiles of A and B; the output tile is staged in the same LDS afterwards when it fits in
+  // 48 KB (all FP32 tiles; the FP64 64x64, 32x64 and 64x32 tiles)
+  constexpr int CLD = BM + 1;  // leading dimension of the staged output tile
+  constexpr bool STAGE_C = static_cast<long>(BN) * CLD * sizeof(CT) <= 48 * 1024;
+  constexpr int NAB = (BM + BN) * MG_BK, NCS = STAGE_C ? BN * CLD : 0;
+  __shared__ CT smem[NAB > NCS ? NAB : NCS];
+  CT* As = smem;
+  CT* Bs = smem + BM * MG_BK;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave & 1, wc = wave >> 1;
   // global -> register staging of element e = tid + 256 j: A row e % BM, k = e / BM;
@@ -209,20 +216,43 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(Gemm g, int tilesM, int tiles
     }
     __syncthreads();
   }
+  if constexpr (STAGE_C) {
+    // the output tile through LDS (the K loop ended with a barrier): each column's rows are then
+    // read and written by consecutive threads, where the MFMA layout puts 16 columns in a store
+    CT* Cs = smem;
 #pragma unroll
-  for (int ti = 0; ti < FM; ++ti)
+    for (int ti = 0; ti < FM; ++ti)
 #pragma unroll
-    for (int tj = 0; tj < FN; ++tj)
+      for (int tj = 0; tj < FN; ++tj)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = row0 + (BM / 2) * wr + 16 * ti + mfma_row(CT(0), lk, i);
-        const int c = col0 + (BN / 2) * wc + 16 * tj + li;
-        if (r < g.M && c < g.N) {
-          double v = g.alpha * static_cast<double>(acc[ti][tj][i]);
-          if (g.beta != 0.0) v = fma(g.beta, static_cast<double>(C[r + g.ldc * static_cast<long>(c)]), v);
-          C[r + g.ldc * static_cast<long>(c)] = static_cast<TC>(v);
-        }
+        for (int i = 0; i < 4; ++i)
+          Cs[((BN / 2) * wc + 16 * tj + li) * CLD + (BM / 2) * wr + 16 * ti + mfma_row(CT(0), lk, i)] = acc[ti][tj][i];
+    __syncthreads();
+    for (int idx = tid; idx < BM * BN; idx += 256) {
+      const int rl = idx % BM, cl = idx / BM;
+      const int r = row0 + rl, c = col0 + cl;
+      if (r < g.M && c < g.N) {
+        double v = g.alpha * static_cast<double>(Cs[cl * CLD + rl]);
+        if (g.beta != 0.0) v = fma(g.beta, static_cast<double>(C[r + g.ldc * static_cast<long>(c)]), v);
+        C[r + g.ldc * static_cast<long>(c)] = static_cast<TC>(v);
       }
+    }
+  } else {
+#pragma unroll
+    for (int ti = 0; ti < FM; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < FN; ++tj)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = row0 + (BM / 2) * wr + 16 * ti + mfma_row(CT(0), lk, i);
+          const int c = col0 + (BN / 2) * wc + 16 * tj + li;
+          if (r < g.M && c < g.N) {
+            double v = g.alpha * static_cast<double>(acc[ti][tj][i]);
+            if (g.beta != 0.0) v = fma(g.beta, static_cast<double>(C[r + g.ldc * static_cast<long>(c)]), v);
+            C[r + g.ldc * static_cast<long>(c)] = static_cast<TC>(v);
+          }
+        }
+  }
 }
 
 // ---------------------------------------------------------------------------

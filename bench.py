@@ -519,7 +519,10 @@ def main():
             s_f = (2.0 if not args.precond_fp32 else 4.0) if precision else 8.0
             bytes_pc = s_f * ncf * ncf * st["ilu_nnz"] * st["ilu_nse"]
             pc_avg_s = pc_ms / max(pc_cnt, 1) / 1e3
-            kpc = "k_bilu_apply<%s>" % (("bf16" if not args.precond_fp32 else "float") if precision else "double")
+            bf16 = precision and not args.precond_fp32
+            wave = bf16 and ncf == 50 and os.environ.get("EXA_BILU_WAVE", "1") != "0"  # the library's choice
+            kpc = "k_bilu_apply%s<%s>" % ("_wave" if wave else "",
+                                          ("bf16" if bf16 else "float") if precision else "double")
             ach = bytes_pc / pc_avg_s / 1e9 if pc_cnt else None
             pc_roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": traffic.get(kpc),

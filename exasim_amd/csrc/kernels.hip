@@ -1661,6 +1661,19 @@ __global__ void k_post_schur(Dims d, int e0, int eb, const TA* aug, long augs, T
   else RuS[E * n + (r - static_cast<long>(n) * m)] = v;
 }
 
+// The same with a 2-D grid (x: entries of one element, y: element): no index division.
+template <typename TA, typename TD>
+__global__ void k_post_schur_2d(Dims d, int e0, const TA* aug, long augs, TD* DinvF, double* RuS) {
+  const int n = d.n, m = d.m;
+  const long r = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (r >= static_cast<long>(n) * (m + 1)) return;
+  const long e = blockIdx.y;
+  const TA v = aug[e * augs + static_cast<long>(n) * n + r];
+  const long E = e0 + e;
+  if (r < static_cast<long>(n) * m) DinvF[E * n * static_cast<long>(m) + r] = -v;
+  else RuS[E * n + (r - static_cast<long>(n) * m)] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Global system
 // ---------------------------------------------------------------------------
@@ -2390,6 +2403,15 @@ __global__ void k_to_float(float* dst, const S* src, int rows, int cols, long ld
   dst[b * dstride + rc] = static_cast<float>(src[b * sstride + r + ld * c]);
 }
 
+// Contiguous matrices (ld = rows) with a 2-D grid (x: entries of one matrix, y: matrix)
+template <typename S>
+__global__ void k_to_float_2d(float* dst, const S* src, long per, long sstride, long dstride) {
+  const long rc = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (rc >= per) return;
+  const long b = blockIdx.y;
+  dst[b * dstride + rc] = static_cast<float>(src[b * sstride + rc]);
+}
+
 // One workgroup per element: RuS = D'^{-1} Ru from the stored FP32 LU factors of
 // D' (replaying the blocked factorisation: each panel's row exchanges, then the
 // unit-lower elimination), the upper-triangular solve, then Rh -= K' RuS.  The
@@ -3107,15 +3129,23 @@ void face_jac(const Dims& d, const AsmArgs& a, float* aug, float* Bfull, float* 
 
 void post_schur(const Dims& d, int e0, int eb, const double* aug, long augs, double* DinvF, double* RuS,
                 hipStream_t s) {
-  long n1 = static_cast<long>(d.n) * (d.m + 1) * eb;
-  k_post_schur<double, double><<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
+  const long per = static_cast<long>(d.n) * (d.m + 1);
+  if (eb <= 0) return;
+  if (eb <= 65535)
+    k_post_schur_2d<double, double><<<dim3(nblocks(per, 256), eb), 256, 0, s>>>(d, e0, aug, augs, DinvF, RuS);
+  else
+    k_post_schur<double, double><<<nblocks(per * eb, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
   EXA_HIP(hipGetLastError());
 }
 
 void post_schur(const Dims& d, int e0, int eb, const float* aug, long augs, float* DinvF, double* RuS,
                 hipStream_t s) {
-  long n1 = static_cast<long>(d.n) * (d.m + 1) * eb;
-  k_post_schur<float, float><<<nblocks(n1, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
+  const long per = static_cast<long>(d.n) * (d.m + 1);
+  if (eb <= 0) return;
+  if (eb <= 65535)
+    k_post_schur_2d<float, float><<<dim3(nblocks(per, 256), eb), 256, 0, s>>>(d, e0, aug, augs, DinvF, RuS);
+  else
+    k_post_schur<float, float><<<nblocks(per * eb, 256), 256, 0, s>>>(d, e0, eb, aug, augs, DinvF, RuS);
   EXA_HIP(hipGetLastError());
 }
 
@@ -3135,7 +3165,11 @@ void to_float(float* dst, const double* src, int rows, int cols, long ld, long s
               hipStream_t s) {
   long n1 = static_cast<long>(rows) * cols * batch;
   if (n1 == 0) return;
-  k_to_float<double><<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
+  const long per = static_cast<long>(rows) * cols;
+  if (ld == rows && batch <= 65535)
+    k_to_float_2d<double><<<dim3(nblocks(per, 256), batch), 256, 0, s>>>(dst, src, per, sstride, dstride);
+  else
+    k_to_float<double><<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
   EXA_HIP(hipGetLastError());
 }
 
@@ -3143,7 +3177,11 @@ void to_float(float* dst, const float* src, int rows, int cols, long ld, long ss
               hipStream_t s) {
   long n1 = static_cast<long>(rows) * cols * batch;
   if (n1 == 0) return;
-  k_to_float<float><<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
+  const long per = static_cast<long>(rows) * cols;
+  if (ld == rows && batch <= 65535)
+    k_to_float_2d<float><<<dim3(nblocks(per, 256), batch), 256, 0, s>>>(dst, src, per, sstride, dstride);
+  else
+    k_to_float<float><<<nblocks(n1, 256), 256, 0, s>>>(dst, src, rows, cols, ld, sstride, dstride, batch);
   EXA_HIP(hipGetLastError());
 }
 
